@@ -1,0 +1,91 @@
+/*
+ * cfd_oracle.h -- CPU FP64 restatement of the reference RK3/CN substep.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the correctness oracle for the
+ * HIP product path.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * CPU-baseline leg may load it, and only as the checker / the timed CPU
+ * baseline, never as the thing measured or shipped.
+ *
+ * Every routine cites the reference file:line it restates (reference tree:
+ * Mel222/CFD, Fortran).  Deliberate deviations are listed here:
+ *   D1  the RK loop is restored (kRK = 1..3, littleharsh.f90:159-163).
+ *   D2  the x/z nonlinear derivatives are assembled per plane
+ *       (intent of FOU3D.f90:1553-1613), not from the last plane.
+ *   D8  y-grid: the polynomial grid of y_grid_canopy.f90 (ygrid = 0) and a
+ *       tanh tip-to-tip grid (ygrid = 1) with the same canopy extension.
+ * One spectral band resolution (Ngal equal in all bands), i.e. the whole
+ * column set of proc_lims_columns is "long" (start.f90:1420-1527), which is
+ * the case for every configuration this project runs.
+ */
+#ifndef CFD_ORACLE_H
+#define CFD_ORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  /* integers first, then doubles: the ctypes mirror in tests/ depends on it */
+  int ngalx, ngalz, ny;        /* Ngal(1,1), Ngal(2,1), ny (input.in:15-19) */
+  int dsty, dsty1, ppp;        /* canopy points in y; polynomial exponent */
+  int ygrid;                   /* 0 polynomial, 1 tanh */
+  int geometry_type;           /* 0 smooth, 3 canopies (start.f90:201-226) */
+  int ntilex, ntilez;
+  int extras24;                /* geom_circ_rough_mixed_forcing_24pt.f90:219-259 */
+  int flag_ctpress;            /* 0 constant flow rate, 1 constant mpgx */
+  double Re, alp, bet, dyq;
+  double posth, posth1, ytanh_beta;
+  double CFL, mpgx;
+  double r, c_i, c_k, stag;    /* geom_circ_rough.txt */
+} or_params;
+
+typedef struct or_state or_state;
+
+or_state* or_create(const or_params* p, char* err, int errlen);
+void      or_destroy(or_state* s);
+
+/* dims[0..15]: N1 N2 Mx Ncol nyu nyv nyp jlo_u jlo_v jlo_p ngx ngz ny dsty nib_u nib_v */
+void or_dims(const or_state* s, int* dims);
+/* y grid, index j - jlo: yu/dthdyu/dyu2i(3,.) over nyu, yv/dthdyv/dyv2i over nyv */
+void or_get_ygrid(const or_state* s, double* yu, double* yv, double* dthdyu,
+                  double* dthdyv, double* dyu2i, double* dyv2i);
+/* gw[0..1] gridweighting(mid,1..2); gwi[0..1] gridweighting_interp(mid,1..2); sc: dtv */
+void or_get_metrics(const or_state* s, double* gw, double* gwi, double* dtv);
+/* IB list of one grid (0 = u/w, 1 = v) and wall (0 bottom, 1 top): returns n;
+   fills l9[9*n] (i,k,j,i2,k2,j2,i3,k3,j3; column-major like the reference)
+   and w3[3*n] (Lap, w2, w3) when the pointers are non-null */
+int  or_get_iblist(const or_state* s, int grid, int wall, int* l9, double* w3);
+/* pressure LU (3, nyp, Ncol) after LU_decP */
+void or_get_dg(const or_state* s, double* dg);
+
+/* fields in reference column layout: complex(8) f(j, column), y fastest,
+   interleaved (re, im).  which: 0 u1 1 u2 2 u3 3 p 4 Nu1 5 Nu2 6 Nu3
+   7 du1 8 du2 9 du3 10 psi 11 div */
+int  or_set_field(or_state* s, int which, const double* src);
+int  or_get_field(const or_state* s, int which, double* dst);
+/* scalars: t dt dti mpgx dgx QxT Qx dtc */
+void or_set_scalars(or_state* s, const double* v);
+void or_get_scalars(const or_state* s, double* v);
+
+/* operators (reference names) */
+void or_rhs0(or_state* s, int kRK);                 /* RHS0_u1/u2/u3 */
+void or_nonlinear(or_state* s, int kRK);            /* nonlinear (NL part) */
+void or_solve_u(or_state* s, int comp, int kRK);    /* solveU(u1|u3) incl. IB */
+void or_solve_v(or_state* s, int kRK);              /* solveV(u2) incl. IB */
+void or_meanflow_ctU(or_state* s, int kRK);
+void or_solve_p(or_state* s, int kRK);
+void or_vcorr(or_state* s, int kRK);
+void or_divergence(or_state* s);
+double or_flowrate(const or_state* s);              /* flowrateIm on (0,0) u1 */
+void or_substep(or_state* s, int kRK);
+void or_step(or_state* s, int nsteps);
+void or_set_threads(int n);
+
+/* FFT known-answer hooks with the reference calling conventions
+   (cft_buff.f:1-65, rft_buff.f:24-86) */
+void or_cft(int n, double* c, int ise, int isa, int m, int iopt);
+void or_rft(int n, double* c, int isa, int m, int iopt);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,117 @@
+! ref_glue.f90 -- project-written C bindings around the MPI-free reference
+! routines (FFTPACK wrappers cft/rft, y_grid_canopy, boundary_circ_rough).
+! Built only into oracle/_ref/ by oracle/Makefile.ref, from the reference
+! sources where they lie.  Test infrastructure: used to pin the oracle.
+module ref_glue
+  use iso_c_binding
+  use declaration
+  implicit none
+contains
+
+  subroutine ref_cft(n, c, ise, isa, m, iopt) bind(C, name="ref_cft")
+    integer(c_int), value :: n, ise, isa, m, iopt
+    real(c_double) :: c(*)
+    real(8), allocatable :: aux(:)
+    allocate(aux(10*n+19))
+    aux = 0d0
+    call cfti(n, aux)
+    call cft(c, ise, isa, m, iopt, aux)
+    deallocate(aux)
+  end subroutine
+
+  subroutine ref_rft(n, c, isa, m, iopt) bind(C, name="ref_rft")
+    integer(c_int), value :: n, isa, m, iopt
+    real(c_double) :: c(*)
+    real(8), allocatable :: aux(:)
+    allocate(aux(2*n+18))
+    aux = 0d0
+    call rfti(n, aux)
+    call rft(c, isa, m, iopt, aux)
+    deallocate(aux)
+  end subroutine
+
+  ! y_grid_canopy with the module state it reads set from the arguments.
+  ! Outputs are copied into caller arrays of length nyu = ny+2*dsty+1 and
+  ! nyv = ny+2*dsty; the d2 coefficient arrays are (3, .) column-major.
+  subroutine ref_ygrid(nyin, dstyin, dsty1in, dyqin, pppin, posthin, posth1in, outdir, &
+                       yuo, yvo, dthuo, dthvo, d2uo, d2vo) bind(C, name="ref_ygrid")
+    integer(c_int), value :: nyin, dstyin, dsty1in, pppin
+    real(c_double), value :: dyqin, posthin, posth1in
+    character(kind=c_char), dimension(*) :: outdir
+    real(c_double) :: yuo(*), yvo(*), dthuo(*), dthvo(*), d2uo(*), d2vo(*)
+    integer :: i, nyu, nyv
+    if (associated(N)) deallocate(N)
+    if (associated(Ngal)) deallocate(Ngal)
+    if (associated(Ny)) deallocate(Ny)
+    if (associated(yu)) deallocate(yu, yv, dthdyu, dthdyv, dyu2i, dyv2i)
+    if (associated(h_ny)) deallocate(h_ny)
+    allocate(N(4,0:nband+1), Ngal(4,0:nband+1), Ny(3,0:nband+1), h_ny(1))
+    N = 0
+    Ngal = 0
+    Ny = 0
+    N(3,nband) = nyin - 2
+    Ngal(3,nband) = nyin - 2
+    dsty = dstyin
+    dsty1 = dsty1in
+    dyq = dyqin
+    ppp = pppin
+    posth = posthin
+    posth1 = posth1in
+    h_ny(1) = 0.1d0
+    dirout = ''
+    i = 1
+    do while (outdir(i) /= c_null_char)
+      dirout(i:i) = outdir(i)
+      i = i + 1
+    end do
+    call y_grid_canopy
+    nyu = nyin + 2*dstyin + 1
+    nyv = nyin + 2*dstyin
+    do i = 1, nyu
+      yuo(i) = yu(-dstyin + i - 1)
+      dthuo(i) = dthdyu(-dstyin + i - 1)
+      d2uo(3*(i-1)+1) = dyu2i(1, -dstyin + i - 1)
+      d2uo(3*(i-1)+2) = dyu2i(2, -dstyin + i - 1)
+      d2uo(3*(i-1)+3) = dyu2i(3, -dstyin + i - 1)
+    end do
+    do i = 1, nyv
+      yvo(i) = yv(-dstyin + i - 1)
+      dthvo(i) = dthdyv(-dstyin + i - 1)
+      d2vo(3*(i-1)+1) = dyv2i(1, -dstyin + i - 1)
+      d2vo(3*(i-1)+2) = dyv2i(2, -dstyin + i - 1)
+      d2vo(3*(i-1)+3) = dyv2i(3, -dstyin + i - 1)
+    end do
+  end subroutine
+
+  ! boundary_circ_rough: reads geom_circ_rough.txt from the working directory
+  ! and writes outdir//'boundary_'//ext1//'x'//ext2//'x'//ext3//'.dat'.
+  subroutine ref_geom(ngx, ngz, nyin, dstyin, ntx, ntz, outdir) bind(C, name="ref_geom")
+    integer(c_int), value :: ngx, ngz, nyin, dstyin, ntx, ntz
+    character(kind=c_char), dimension(*) :: outdir
+    integer :: i
+    if (associated(Ngal)) deallocate(Ngal)
+    allocate(Ngal(4,0:nband+1))
+    Ngal = 0
+    Ngal(1,1:nband) = ngx
+    Ngal(2,1:nband) = ngz
+    Ngal(3,3) = nyin - 2 + dstyin
+    Ngal(4,3) = nyin - 1 + dstyin
+    dsty = dstyin
+    ntilex = ntx
+    ntilez = ntz
+    Lx = 1d0
+    Ly = 2d0
+    Lz = 1d0
+    ext1 = '0000'
+    ext2 = '0000'
+    ext3 = '0000'
+    dirout = ''
+    i = 1
+    do while (outdir(i) /= c_null_char)
+      dirout(i:i) = outdir(i)
+      i = i + 1
+    end do
+    call boundary_circ_rough
+  end subroutine
+
+end module ref_glue
