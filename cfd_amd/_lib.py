@@ -1,0 +1,81 @@
+"""ctypes binding of libcfd_amd.so (the C ABI declared in include/cfd_amd.h).
+
+The library is built in-tree by ``cfd_amd/csrc/Makefile``; importing this
+module without it raises, so no code path can silently fall back to a CPU
+implementation.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcfd_amd.so")
+
+INT_FIELDS = ("ngalx ngalz ny dsty dsty1 ppp ygrid geometry_type ntilex ntilez "
+              "extras24 flag_ctpress precision device").split()
+DBL_FIELDS = "Re alp bet dyq posth posth1 ytanh_beta CFL mpgx r c_i c_k stag".split()
+
+
+class CParams(ctypes.Structure):
+    """Mirror of ``cfd_params``."""
+    _fields_ = [(n, ctypes.c_int) for n in INT_FIELDS] + [(n, ctypes.c_double) for n in DBL_FIELDS]
+
+
+# symbol -> (restype, argtypes)
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+_H = ctypes.c_void_p
+SIGNATURES = {
+    "cfd_create": (ctypes.c_int, [ctypes.POINTER(CParams), ctypes.POINTER(ctypes.c_void_p)]),
+    "cfd_destroy": (ctypes.c_int, [_H]),
+    "cfd_last_error": (ctypes.c_char_p, [_H]),
+    "cfd_build_ygrid": (ctypes.c_int, [ctypes.POINTER(CParams)] + [_D] * 6),
+    "cfd_build_iblist": (ctypes.c_int, [ctypes.POINTER(CParams), ctypes.c_int, ctypes.c_int, _I, _D, _I]),
+    "cfd_dims": (ctypes.c_int, [_H, _I]),
+    "cfd_get_ygrid": (ctypes.c_int, [_H] + [_D] * 6),
+    "cfd_get_iblist": (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int, _I, _D, _I]),
+    "cfd_set_field": (ctypes.c_int, [_H, ctypes.c_int, _D]),
+    "cfd_get_field": (ctypes.c_int, [_H, ctypes.c_int, _D]),
+    "cfd_set_scalars": (ctypes.c_int, [_H, _D]),
+    "cfd_get_scalars": (ctypes.c_int, [_H, _D]),
+    "cfd_rhs0": (ctypes.c_int, [_H, ctypes.c_int]),
+    "cfd_nonlinear": (ctypes.c_int, [_H, ctypes.c_int]),
+    "cfd_solve_u": (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int]),
+    "cfd_solve_v": (ctypes.c_int, [_H, ctypes.c_int]),
+    "cfd_meanflow_ctU": (ctypes.c_int, [_H, ctypes.c_int]),
+    "cfd_solve_p": (ctypes.c_int, [_H, ctypes.c_int]),
+    "cfd_vcorr": (ctypes.c_int, [_H, ctypes.c_int]),
+    "cfd_substep": (ctypes.c_int, [_H, ctypes.c_int]),
+    "cfd_step": (ctypes.c_int, [_H, ctypes.c_int]),
+    "cfd_sync": (ctypes.c_int, [_H]),
+    "cfd_profile": (ctypes.c_int, [_H, ctypes.c_int]),
+    "cfd_profile_read": (ctypes.c_int, [_H, _D, _I]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libcfd_amd.so once and attach the signatures; raise if absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                "libcfd_amd.so is not built (%s); run `make -C cfd_amd/csrc` "
+                "or __graft_entry__.build()" % LIB_PATH)
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+class CfdError(RuntimeError):
+    """Nonzero status from the C ABI."""
+
+
+def check(rc, handle=None):
+    if rc != 0:
+        msg = load().cfd_last_error(handle)
+        raise CfdError("cfd_amd error %d: %s" % (rc, msg.decode() if msg else "?"))

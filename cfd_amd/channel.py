@@ -1,0 +1,177 @@
+"""Host-side mirror of the reference hot path on one GPU.
+
+``Channel`` owns one ``cfd_handle`` and exposes the reference's hot-path
+subroutines under their own names (littleharsh.f90:163-189):
+``RHS0`` (RHS0_u1/u2/u3), ``nonlinear``, ``solveU``, ``solveV``,
+``meanflow_ctU``, ``solveP``, ``v_corr``, plus ``substep`` (one pass of the
+RK loop body) and ``step`` (whole time steps, the time loop restored).
+
+Fields cross the boundary in the reference layout: complex128 arrays of shape
+(Ncol, ny_grid), i.e. ``f(j, column)`` with y fastest, columns ordered k
+outer / i inner (start.f90:1467-1481).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .params import Params
+
+FIELDS = ("u1", "u2", "u3", "p", "Nu1", "Nu2", "Nu3", "du1", "du2", "du3", "psi", "div")
+GRID = {"u1": "u", "u2": "v", "u3": "u", "p": "p", "Nu1": "u", "Nu2": "v", "Nu3": "u",
+        "du1": "u", "du2": "v", "du3": "u", "psi": "p", "div": "p"}
+SCALARS = ("t", "dt", "dti", "mpgx", "dgx", "QxT", "Qx", "dtc")
+KCLASSES = ("zinv", "xpass", "zfwd", "elem", "ysolve", "press", "ib")
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D)
+
+
+def build_ygrid(params):
+    """Host-only y grid (cfd_build_ygrid): dict of yu, yv, dthdyu, dthdyv,
+    dyu2i (nyu,3), dyv2i (nyv,3)."""
+    lib = _lib.load()
+    p = params.to_c()
+    nyu, nyv = params.nyu, params.nyv
+    out = [np.zeros(nyu), np.zeros(nyv), np.zeros(nyu), np.zeros(nyv), np.zeros(3 * nyu), np.zeros(3 * nyv)]
+    _lib.check(lib.cfd_build_ygrid(ctypes.byref(p), *[_dp(a) for a in out]))
+    return dict(yu=out[0], yv=out[1], dthdyu=out[2], dthdyv=out[3],
+                dyu2i=out[4].reshape(nyu, 3), dyv2i=out[5].reshape(nyv, 3))
+
+
+def build_iblist(params, grid, wall):
+    """Host-only IB list (cfd_build_iblist): (int32 (n,9), float64 (n,3))."""
+    lib = _lib.load()
+    p = params.to_c()
+    n = ctypes.c_int(0)
+    _lib.check(lib.cfd_build_iblist(ctypes.byref(p), grid, wall, None, None, ctypes.byref(n)))
+    l9 = np.zeros((n.value, 9), dtype=np.int32)
+    w3 = np.zeros((n.value, 3))
+    _lib.check(lib.cfd_build_iblist(ctypes.byref(p), grid, wall, l9.ctypes.data_as(_I), _dp(w3), ctypes.byref(n)))
+    return l9, w3
+
+
+class Channel:
+    """One rough channel resident on one GPU."""
+
+    def __init__(self, params=None, **kw):
+        self.params = (params or Params()).replace(**kw) if kw else (params or Params())
+        self._lib = _lib.load()
+        self._c = self.params.to_c()
+        h = ctypes.c_void_p()
+        rc = self._lib.cfd_create(ctypes.byref(self._c), ctypes.byref(h))
+        _lib.check(rc, None)
+        self._h = h
+        d = (ctypes.c_int * 16)()
+        self._call("cfd_dims", d)
+        d = list(d)
+        self.N1, self.N2, self.Mx, self.Ncol = d[0:4]
+        self.nyu, self.nyv, self.nyp = d[4:7]
+        self.jlo_u, self.jlo_v, self.jlo_p = d[7:10]
+        self.nib_u, self.nib_v = d[14], d[15]
+        self._ny = {"u": self.nyu, "v": self.nyv, "p": self.nyp}
+
+    # -- plumbing ---------------------------------------------------------
+    def _call(self, name, *args):
+        if self._h is None:
+            raise _lib.CfdError("channel is closed")
+        rc = getattr(self._lib, name)(self._h, *args)
+        _lib.check(rc, self._h)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            self._lib.cfd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- state --------------------------------------------------------------
+    def shape(self, name):
+        return (self.Ncol, self._ny[GRID[name]])
+
+    def set_field(self, name, arr):
+        arr = np.ascontiguousarray(arr, dtype=np.complex128)
+        if arr.shape != self.shape(name):
+            raise ValueError("%s: expected shape %s, got %s" % (name, self.shape(name), arr.shape))
+        self._call("cfd_set_field", FIELDS.index(name), arr.view(np.float64).ctypes.data_as(_D))
+
+    def get_field(self, name):
+        out = np.zeros(self.shape(name), dtype=np.complex128)
+        self._call("cfd_get_field", FIELDS.index(name), out.view(np.float64).ctypes.data_as(_D))
+        return out
+
+    def scalars(self):
+        v = np.zeros(8)
+        self._call("cfd_get_scalars", _dp(v))
+        return dict(zip(SCALARS, v.tolist()))
+
+    def set_scalars(self, **kw):
+        cur = self.scalars()
+        for k, val in kw.items():
+            if k not in cur:
+                raise KeyError(k)
+            cur[k] = float(val)
+        v = np.array([cur[k] for k in SCALARS])
+        self._call("cfd_set_scalars", _dp(v))
+
+    def ygrid(self):
+        out = [np.zeros(self.nyu), np.zeros(self.nyv), np.zeros(self.nyu), np.zeros(self.nyv),
+               np.zeros(3 * self.nyu), np.zeros(3 * self.nyv)]
+        self._call("cfd_get_ygrid", *[_dp(a) for a in out])
+        return dict(yu=out[0], yv=out[1], dthdyu=out[2], dthdyv=out[3],
+                    dyu2i=out[4].reshape(-1, 3), dyv2i=out[5].reshape(-1, 3))
+
+    # -- reference operators ---------------------------------------------------
+    def RHS0(self, kRK):
+        self._call("cfd_rhs0", kRK)
+
+    def nonlinear(self, kRK):
+        self._call("cfd_nonlinear", kRK)
+
+    def solveU(self, comp, kRK):
+        self._call("cfd_solve_u", comp, kRK)
+
+    def solveV(self, kRK):
+        self._call("cfd_solve_v", kRK)
+
+    def meanflow_ctU(self, kRK):
+        self._call("cfd_meanflow_ctU", kRK)
+
+    def solveP(self, kRK):
+        self._call("cfd_solve_p", kRK)
+
+    def v_corr(self, kRK):
+        self._call("cfd_vcorr", kRK)
+
+    def substep(self, kRK):
+        self._call("cfd_substep", kRK)
+
+    def step(self, nsteps=1):
+        self._call("cfd_step", int(nsteps))
+
+    def sync(self):
+        self._call("cfd_sync")
+
+    # -- timing ----------------------------------------------------------------
+    def profile(self, enable=True):
+        self._call("cfd_profile", 1 if enable else 0)
+
+    def profile_read(self):
+        ms = np.zeros(len(KCLASSES))
+        cnt = np.zeros(len(KCLASSES), dtype=np.int32)
+        self._call("cfd_profile_read", _dp(ms), cnt.ctypes.data_as(_I))
+        return {k: (float(m), int(c)) for k, m, c in zip(KCLASSES, ms, cnt)}

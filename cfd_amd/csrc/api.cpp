@@ -1,0 +1,752 @@
+// api.cpp -- C ABI of the rough-channel substep (include/cfd_amd.h): setup,
+// device state, layout conversion at the boundary and operator sequencing.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "common.hpp"
+
+using namespace cfd;
+
+struct cfd_solver : public Solver {
+  // event pool for kernel-class timing
+  std::vector<hipEvent_t> evpool;
+  std::vector<int> evclass;
+  size_t evn = 0;
+  // per-field offsets into the IB job tables
+  int ibz_off[4] = {0, 0, 0, 0}, ibx_off[4] = {0, 0, 0, 0};
+  std::vector<void*> allocs;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(cfd_solver* s, int code, const std::string& msg) {
+  if (s) s->err = msg;
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(s, x)                                                                        \
+  do {                                                                                      \
+    hipError_t e_ = (x);                                                                    \
+    if (e_ != hipSuccess) return fail((s), CFD_EHIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+int dalloc(cfd_solver* s, T** p, size_t n) {
+  void* q = nullptr;
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc(&q, n * sizeof(T));
+  if (e != hipSuccess) return fail(s, CFD_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  hipMemset(q, 0, n * sizeof(T));
+  s->allocs.push_back(q);
+  *p = static_cast<T*>(q);
+  return 0;
+}
+
+template <typename T>
+int upload(cfd_solver* s, T** p, const std::vector<T>& h) {
+  if (int rc = dalloc(s, p, h.size())) return rc;
+  if (!h.empty()) HIPCHK(s, hipMemcpy(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+bool make_plan(int L, FftPlan& P) {
+  P.L = L;
+  P.nr = 0;
+  int m = L;
+  const int tryr[4] = {4, 2, 3, 5};
+  for (int t = 0; t < 4 && m > 1;) {
+    if (m % tryr[t] == 0) {
+      if (P.nr >= kMaxRadix) return false;
+      P.rad[P.nr++] = tryr[t];
+      m /= tryr[t];
+    } else {
+      ++t;
+    }
+  }
+  return m == 1;
+}
+
+std::vector<double2> twiddles(int L) {
+  std::vector<double2> w(L);
+  for (int m = 0; m < L; ++m) {
+    const long double a = -2.0L * 3.14159265358979323846264338327950288L * m / (long double)L;
+    w[m] = make_double2((double)cosl(a), (double)sinl(a));
+  }
+  return w;
+}
+
+int grid_of(int which) {
+  static const int g[CFD_NFIELDS] = {0, 1, 0, 2, 0, 1, 0, 0, 1, 0, 2, 2};
+  return g[which];
+}
+
+int ny_of(const Solver& s, int which) {
+  const int g = grid_of(which);
+  return g == 0 ? s.nyu : g == 1 ? s.nyv : s.nyp;
+}
+
+// kernel-class timing
+struct Timed {
+  cfd_solver* s;
+  int c;
+  Timed(cfd_solver* s_, int c_) : s(s_), c(c_) {
+    if (!s->prof) return;
+    if (s->evn + 2 > s->evpool.size()) {
+      // drain the pool
+      hipStreamSynchronize(s->stream);
+      for (size_t q = 0; q + 1 < s->evn; q += 2) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, s->evpool[q], s->evpool[q + 1]);
+        s->prof_ms[s->evclass[q / 2]] += ms;
+        s->prof_cnt[s->evclass[q / 2]] += 1;
+      }
+      s->evn = 0;
+    }
+    hipEventRecord(s->evpool[s->evn], s->stream);
+  }
+  ~Timed() {
+    if (!s->prof) return;
+    hipEventRecord(s->evpool[s->evn + 1], s->stream);
+    s->evclass[s->evn / 2] = c;
+    s->evn += 2;
+  }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// create / destroy
+// ---------------------------------------------------------------------------
+extern "C" int cfd_create(const cfd_params* pp, cfd_handle* out) {
+  if (!pp || !out) return fail(nullptr, CFD_EINVAL, "null argument");
+  *out = nullptr;
+  const cfd_params& p = *pp;
+  if (p.ngalx < 6 || p.ngalz < 6 || p.ny < 4 || p.dsty < 0) return fail(nullptr, CFD_EINVAL, "grid sizes out of range");
+  if (p.ngalx % 2 || p.ngalz % 2) return fail(nullptr, CFD_EINVAL, "Ngalx and Ngalz must be even");
+  if (p.precision != 0) return fail(nullptr, CFD_EINVAL, "only precision = 0 (FP64) is available");
+  if (p.Re <= 0 || p.alp <= 0 || p.bet <= 0) return fail(nullptr, CFD_EINVAL, "Re, alp, bet must be positive");
+  cfd_solver* s = new (std::nothrow) cfd_solver();
+  if (!s) return fail(nullptr, CFD_ENOMEM, "host allocation failed");
+  auto bail = [&](int rc) {
+    g_err = s->err;
+    cfd_destroy(s);
+    return rc;
+  };
+  s->P = p;
+  s->dev = p.device;
+  if (hipSetDevice(p.device) != hipSuccess) {
+    s->err = "hipSetDevice failed";
+    return bail(CFD_EHIP);
+  }
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+    s->err = "hipStreamCreate failed";
+    return bail(CFD_EHIP);
+  }
+  s->ngx = p.ngalx;
+  s->ngz = p.ngalz;
+  s->ny = p.ny;
+  s->dsty = p.dsty;
+  s->N1 = 2 * (p.ngalx / 3);  // start.f90:108-111
+  s->N2 = 2 * (p.ngalz / 3);
+  s->Mx = s->N1 / 2 + 1;
+  s->Ncol = s->Mx * s->N2;
+  s->nn = p.ny - 2;
+  std::string err;
+  if (!build_ygrid(p, s->yg, err) || !build_iblists(p, s->ib, err)) {
+    s->err = err;
+    return bail(CFD_EINVAL);
+  }
+  s->nyu = s->yg.nyu;
+  s->nyv = s->yg.nyv;
+  s->nyp = s->nyu - 2;
+  s->jlo_u = s->yg.jlo_u;
+  s->jlo_v = s->yg.jlo_v;
+  s->jlo_p = s->jlo_u + 1;
+  const YGrid& g = s->yg;
+  // ghost weights (start.f90:494-508), band 2 = the whole column here
+  const int N40 = -p.dsty, N30 = -p.dsty, N43 = p.ny - 1 + p.dsty, N33 = p.ny - 2 + p.dsty;
+  s->gw[0] = -(g.YV(N30) - g.YU(N40)) / (g.YV(N30) - g.YU(N40 + 1));
+  s->gw[1] = (g.YU(N43 + 1) - g.YV(N33 + 1)) / (g.YV(N33 + 1) - g.YU(N43));
+  s->gwi[0] = (g.YU(N40) - g.YU(N40 + 1)) / (g.YV(N30) - g.YU(N40 + 1));
+  s->gwi[1] = (g.YU(N43 + 1) - g.YU(N43)) / (g.YV(N33 + 1) - g.YU(N43));
+  // wavenumbers (def_k, start.f90:965-986); first derivative stored as i*k1
+  s->k1x.resize(s->Mx);
+  s->k2x.resize(s->Mx);
+  s->k1z.resize(s->N2);
+  s->k2z.resize(s->N2);
+  for (int i = 0; i < s->Mx; ++i) {
+    s->k1x[i] = p.alp * i;
+    s->k2x[i] = -((p.alp * i) * (p.alp * i));
+  }
+  for (int k = 0; k < s->N2; ++k) {
+    const double kz = (k < s->N2 / 2) ? p.bet * k : -(p.bet * (s->N2 - k));
+    s->k1z[k] = kz;
+    s->k2z[k] = -(kz * kz);
+  }
+  // RK3 (start.f90:395-409)
+  const double a[3] = {4.0 / 15.0, 1.0 / 15.0, 1.0 / 6.0}, gg[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
+  const double c[3] = {0.0, -17.0 / 60.0, -5.0 / 12.0}, d[3] = {8.0 / 15.0, 2.0 / 15.0, 1.0 / 3.0};
+  for (int q = 0; q < 3; ++q) {
+    s->aRK[q] = a[q];
+    s->bRK[q] = a[q];
+    s->gRK[q] = gg[q];
+    s->cRK[q] = c[q];
+    s->dRK[q] = d[q];
+  }
+  const double h0 = g.YV(N30 + 1) - g.YV(N30);
+  s->dtv = p.Re / (-s->k2x[s->N1 / 2] - s->k2z[s->N2 / 2 - 1] + 4.0 / (h0 * h0));  // start.f90:432
+
+  // FFT plans
+  if (!make_plan(s->ngz, s->pz) || !make_plan(s->ngx, s->px)) {
+    s->err = "Ngalx and Ngalz must factor into 2, 3 and 5";
+    return bail(CFD_EINVAL);
+  }
+  int rc;
+  if ((rc = upload(s, &s->d_twz, twiddles(s->ngz))) || (rc = upload(s, &s->d_twx, twiddles(s->ngx)))) return bail(rc);
+
+  // metrics
+  std::vector<double> plo(s->nyp), pup(s->nyp), pdg(s->nyp);
+  {
+    // pressure operator D_y G_y rows (LU_buildP, tridLU_3D_IB.f90:417-466)
+    const int ns = s->jlo_p, ne = s->jlo_p + s->nyp - 1;
+    auto DU = [&](int j) { return g.dthdyu[j - g.jlo_u]; };
+    auto DV = [&](int j) { return g.dthdyv[j - g.jlo_v]; };
+    auto Dvb = [&](int j) { return -1.0 * DU(j + 1); };
+    auto Gvb = [&](int j) { return -1.0 * DV(j); };
+    auto Dvt = [&](int j) { return 1.0 * DU(j); };
+    auto Gvt = [&](int j) { return 1.0 * DV(j - 1); };
+    for (int j = ns; j <= ne; ++j) {
+      const int q = j - ns;
+      if (j == ns) {
+        pdg[q] = Dvt(ns) * Gvb(ns);
+        pup[q] = Dvt(ns) * Gvt(ns + 1);
+        plo[q] = 0.0;
+      } else if (j == ne) {
+        pdg[q] = Dvb(ne - 1) * Gvt(ne);
+        plo[q] = Dvb(ne - 1) * Gvb(ne - 1);
+        pup[q] = 0.0;
+      } else {
+        pdg[q] = (Dvb(j - 1) * Gvt(j)) + (Dvt(j) * Gvb(j));
+        plo[q] = Dvb(j - 1) * Gvb(j - 1);
+        pup[q] = Dvt(j) * Gvt(j + 1);
+      }
+    }
+  }
+  if ((rc = upload(s, &s->d_dthdyu, g.dthdyu)) || (rc = upload(s, &s->d_dthdyv, g.dthdyv)) ||
+      (rc = upload(s, &s->d_d2u, g.d2u)) || (rc = upload(s, &s->d_d2v, g.d2v)) || (rc = upload(s, &s->d_yv, g.yv)) ||
+      (rc = upload(s, &s->d_k1x, s->k1x)) || (rc = upload(s, &s->d_k2x, s->k2x)) ||
+      (rc = upload(s, &s->d_k1z, s->k1z)) || (rc = upload(s, &s->d_k2z, s->k2z)) || (rc = upload(s, &s->d_plo, plo)) ||
+      (rc = upload(s, &s->d_pup, pup)) || (rc = upload(s, &s->d_pdg, pdg)) ||
+      (rc = dalloc(s, &s->d_mf, (size_t)4 * (s->ny + 2))) || (rc = dalloc(s, &s->d_sc, 1)))
+    return bail(rc);
+
+  // fields and scratch
+  const size_t N = s->Ncol;
+  for (int f = 0; f < CFD_NFIELDS; ++f)
+    if ((rc = dalloc(s, &s->F[f], (size_t)ny_of(*s, f) * N))) return bail(rc);
+  if ((rc = dalloc(s, &s->nxz1, s->nyu * N)) || (rc = dalloc(s, &s->nxz3, s->nyu * N)) ||
+      (rc = dalloc(s, &s->fvv, s->nyu * N)) || (rc = dalloc(s, &s->nxz2, s->nyv * N)) ||
+      (rc = dalloc(s, &s->fuv, s->nyv * N)) || (rc = dalloc(s, &s->fvw, s->nyv * N)) ||
+      (rc = dalloc(s, &s->cscr, (size_t)3 * s->nyu * N)))
+    return bail(rc);
+  const size_t zp = (size_t)s->ngz * s->Mx;
+  const int zplanes[12] = {s->nyu, s->nyu, s->nyu, s->nyv, s->nyv, s->nyv,
+                           s->nyu, s->nyu, s->nyu, s->nyu, s->nyv, s->nyv};
+  for (int q = 0; q < 12; ++q)
+    if ((rc = dalloc(s, &s->Z[q], zplanes[q] * zp))) return bail(rc);
+
+  // nonlinear z-inverse jobs (interp_u / interp_v folded in, FOU3D.f90:261-308)
+  {
+    std::vector<ZInvJob> J;
+    auto job = [&](const double2* src, double2* dst, int pd, int mode, int pa, int pb, int pc, double wa, double wb,
+                   double gw) {
+      ZInvJob z{};
+      z.src = src;
+      z.dst = dst;
+      z.pa = pa;
+      z.pb = pb;
+      z.pc = pc;
+      z.pd = pd;
+      z.mode = mode;
+      z.wa = wa;
+      z.wb = wb;
+      z.g = gw;
+      J.push_back(z);
+    };
+    auto vw = [&](int j, double& wa, double& wb) {  // centre j from faces j (wa) and j-1 (wb)
+      const double den = g.YV(j) - g.YV(j - 1);
+      wa = (g.YU(j) - g.YV(j - 1)) / den;
+      wb = (g.YV(j) - g.YU(j)) / den;
+    };
+    for (int q = 0; q < s->nyu; ++q) {
+      const int j = s->jlo_u + q;
+      job(s->F[CFD_U1], s->Z[0], q, 0, q, 0, 0, 0, 0, 0);
+      job(s->F[CFD_U3], s->Z[2], q, 0, q, 0, 0, 0, 0, 0);
+      double wa, wb;
+      if (q == 0) {
+        vw(j + 1, wa, wb);
+        job(s->F[CFD_U2], s->Z[1], q, 2, 1, 0, 0, wa, wb, s->gwi[0]);
+      } else if (q == s->nyu - 1) {
+        vw(j - 1, wa, wb);
+        job(s->F[CFD_U2], s->Z[1], q, 2, s->nyv - 1, s->nyv - 2, s->nyv - 1, wa, wb, s->gwi[1]);
+      } else {
+        vw(j, wa, wb);
+        job(s->F[CFD_U2], s->Z[1], q, 1, q, q - 1, 0, wa, wb, 0);
+      }
+    }
+    for (int q = 0; q < s->nyv; ++q) {
+      const int j = s->jlo_v + q;
+      const double den = g.YU(j + 1) - g.YU(j);
+      const double wa = (g.YV(j) - g.YU(j)) / den, wb = (g.YU(j + 1) - g.YV(j)) / den;
+      job(s->F[CFD_U1], s->Z[3], q, 1, q + 1, q, 0, wa, wb, 0);
+      job(s->F[CFD_U2], s->Z[4], q, 0, q, 0, 0, 0, 0, 0);
+      job(s->F[CFD_U3], s->Z[5], q, 1, q + 1, q, 0, wa, wb, 0);
+    }
+    s->n_nljobs = (int)J.size();
+    if ((rc = upload(s, &s->d_nljobs, J))) return bail(rc);
+  }
+
+  // IB lists and transform jobs
+  for (int gr = 0; gr < 2; ++gr) {
+    std::vector<IBPoint> all = s->ib[gr][0];
+    all.insert(all.end(), s->ib[gr][1].begin(), s->ib[gr][1].end());
+    const int jlo = gr == 0 ? s->jlo_u : s->jlo_v, ny_g = gr == 0 ? s->nyu : s->nyv;
+    std::vector<int>& pl = s->ibplanes[gr];
+    for (const IBPoint& e : all) pl.push_back(e.j);
+    std::sort(pl.begin(), pl.end());
+    pl.erase(std::unique(pl.begin(), pl.end()), pl.end());
+    std::vector<IBDev> dl;
+    std::vector<int> seg;
+    for (size_t q = 0; q < all.size(); ++q) {
+      const IBPoint& e = all[q];
+      auto inr = [&](int i, int k) { return i >= 1 && i <= s->ngx && k >= 1 && k <= s->ngz; };
+      if (e.j2 != e.j || e.j3 != e.j || !inr(e.i, e.k) || !inr(e.i2, e.k2) || !inr(e.i3, e.k3) || e.j - jlo < 1 ||
+          e.j - jlo > ny_g - 2) {
+        s->err = "IB list entry outside the grid";
+        return bail(CFD_EINVAL);
+      }
+      IBDev d{};
+      d.o = (e.k - 1) * s->ngx + (e.i - 1);
+      d.o2 = (e.k2 - 1) * s->ngx + (e.i2 - 1);
+      d.o3 = (e.k3 - 1) * s->ngx + (e.i3 - 1);
+      d.plane = int(std::lower_bound(pl.begin(), pl.end(), e.j) - pl.begin());
+      d.lap = e.lap;
+      d.w2 = e.w2;
+      d.w3 = e.w3;
+      if (q == 0 || e.i != all[q - 1].i || e.k != all[q - 1].k || e.j != all[q - 1].j) seg.push_back((int)q);
+      dl.push_back(d);
+    }
+    seg.push_back((int)all.size());
+    s->nibseg[gr] = (int)seg.size() - 1;
+    if ((rc = upload(s, &s->d_ibl[gr], dl)) || (rc = upload(s, &s->d_ibseg[gr], seg))) return bail(rc);
+  }
+  {
+    std::vector<ZInvJob> zj;
+    std::vector<XJob> xi, xf;
+    std::vector<ZFwdJob> zf;
+    for (int f = 0; f < 3; ++f) {
+      const int gr = f == 1 ? 1 : 0;
+      const int jlo = gr == 0 ? s->jlo_u : s->jlo_v;
+      const int nb = (int)s->ibplanes[gr].size();
+      s->nibp[f] = nb;
+      if ((rc = dalloc(s, &s->zib_r[f], (size_t)nb * zp)) || (rc = dalloc(s, &s->zib_u[f], (size_t)nb * zp)) ||
+          (rc = dalloc(s, &s->pib_r[f], (size_t)nb * s->ngz * s->ngx)) ||
+          (rc = dalloc(s, &s->pib_u[f], (size_t)nb * s->ngz * s->ngx)))
+        return bail(rc);
+      s->ibz_off[f] = (int)zj.size() / 2;  // jobs come in (r, u) pairs
+      s->ibx_off[f] = (int)xi.size();
+      for (int b = 0; b < nb; ++b) {
+        const int plane = s->ibplanes[gr][b] - jlo;
+        ZInvJob r{}, u{};
+        r.src = s->F[CFD_DU1 + f];
+        r.dst = s->zib_r[f];
+        r.pa = plane;
+        r.pd = b;
+        u.src = s->F[CFD_U1 + f];
+        u.dst = s->zib_u[f];
+        u.pa = plane;
+        u.pd = b;
+        zj.push_back(r);
+        zj.push_back(u);
+        XJob x{};
+        x.za = s->zib_r[f];
+        x.zb = s->zib_u[f];
+        x.pa = s->pib_r[f];
+        x.pb = s->pib_u[f];
+        x.plane = b;
+        xi.push_back(x);
+        XJob y{};
+        y.pa = s->pib_r[f];
+        y.zo = s->zib_r[f];
+        y.plane = b;
+        xf.push_back(y);
+        ZFwdJob z{};
+        z.src = s->zib_r[f];
+        z.dst = s->F[CFD_DU1 + f];
+        z.ps = b;
+        z.pd = plane;
+        zf.push_back(z);
+      }
+    }
+    s->ibz_off[3] = (int)zj.size() / 2;
+    s->ibx_off[3] = (int)xi.size();
+    s->n_ibz = (int)zj.size();
+    s->n_ibxi = (int)xi.size();
+    s->n_ibxf = (int)xf.size();
+    s->n_ibzf = (int)zf.size();
+    if ((rc = upload(s, &s->d_ibz, zj)) || (rc = upload(s, &s->d_ibxi, xi)) || (rc = upload(s, &s->d_ibxf, xf)) ||
+        (rc = upload(s, &s->d_ibzf, zf)))
+      return bail(rc);
+  }
+  // scalars
+  {
+    DevScalars h{};
+    h.mpgx = p.mpgx;
+    HIPCHK(s, hipMemcpy(s->d_sc, &h, sizeof h, hipMemcpyHostToDevice));
+  }
+  s->evpool.resize(4096);
+  s->evclass.resize(2048);
+  for (auto& e : s->evpool)
+    if (hipEventCreate(&e) != hipSuccess) {
+      s->err = "hipEventCreate failed";
+      return bail(CFD_EHIP);
+    }
+  HIPCHK(s, hipDeviceSynchronize());
+  *out = s;
+  return CFD_OK;
+}
+
+extern "C" int cfd_build_ygrid(const cfd_params* p, double* yu, double* yv, double* du, double* dv, double* d2u,
+                               double* d2v) {
+  if (!p) return fail(nullptr, CFD_EINVAL, "null argument");
+  YGrid g;
+  std::string err;
+  if (!build_ygrid(*p, g, err)) return fail(nullptr, CFD_EINVAL, err);
+  if (yu) std::memcpy(yu, g.yu.data(), sizeof(double) * g.nyu);
+  if (yv) std::memcpy(yv, g.yv.data(), sizeof(double) * g.nyv);
+  if (du) std::memcpy(du, g.dthdyu.data(), sizeof(double) * g.nyu);
+  if (dv) std::memcpy(dv, g.dthdyv.data(), sizeof(double) * g.nyv);
+  if (d2u) std::memcpy(d2u, g.d2u.data(), sizeof(double) * 3 * g.nyu);
+  if (d2v) std::memcpy(d2v, g.d2v.data(), sizeof(double) * 3 * g.nyv);
+  return CFD_OK;
+}
+
+extern "C" int cfd_build_iblist(const cfd_params* p, int grid, int wall, int* l9, double* w3, int* n) {
+  if (!p || grid < 0 || grid > 1 || wall < 0 || wall > 1) return fail(nullptr, CFD_EINVAL, "bad argument");
+  std::vector<IBPoint> L[2][2];
+  std::string err;
+  if (!build_iblists(*p, L, err)) return fail(nullptr, CFD_EINVAL, err);
+  const std::vector<IBPoint>& v = L[grid][wall];
+  if (n) *n = (int)v.size();
+  for (size_t q = 0; q < v.size(); ++q) {
+    const IBPoint& e = v[q];
+    if (l9) {
+      const int a[9] = {e.i, e.k, e.j, e.i2, e.k2, e.j2, e.i3, e.k3, e.j3};
+      std::memcpy(l9 + 9 * q, a, sizeof a);
+    }
+    if (w3) {
+      w3[3 * q] = e.lap;
+      w3[3 * q + 1] = e.w2;
+      w3[3 * q + 2] = e.w3;
+    }
+  }
+  return CFD_OK;
+}
+
+extern "C" int cfd_destroy(cfd_handle h) {
+  cfd_solver* s = static_cast<cfd_solver*>(h);
+  if (!s) return CFD_OK;
+  hipSetDevice(s->dev);
+  if (s->stream) hipStreamSynchronize(s->stream);
+  for (void* q : s->allocs) hipFree(q);
+  for (auto& e : s->evpool)
+    if (e) hipEventDestroy(e);
+  if (s->stream) hipStreamDestroy(s->stream);
+  delete s;
+  return CFD_OK;
+}
+
+extern "C" const char* cfd_last_error(cfd_handle h) {
+  cfd_solver* s = static_cast<cfd_solver*>(h);
+  return s ? s->err.c_str() : g_err.c_str();
+}
+
+extern "C" int cfd_dims(cfd_handle h, int* d) {
+  cfd_solver* s = static_cast<cfd_solver*>(h);
+  if (!s || !d) return fail(s, CFD_EINVAL, "null argument");
+  const int v[16] = {s->N1, s->N2, s->Mx, s->Ncol, s->nyu, s->nyv, s->nyp, s->jlo_u,
+                     s->jlo_v, s->jlo_p, s->ngx, s->ngz, s->ny, s->dsty,
+                     (int)(s->ib[0][0].size() + s->ib[0][1].size()), (int)(s->ib[1][0].size() + s->ib[1][1].size())};
+  std::memcpy(d, v, sizeof v);
+  return CFD_OK;
+}
+
+extern "C" int cfd_get_ygrid(cfd_handle h, double* yu, double* yv, double* du, double* dv, double* d2u, double* d2v) {
+  cfd_solver* s = static_cast<cfd_solver*>(h);
+  if (!s) return fail(s, CFD_EINVAL, "null handle");
+  const YGrid& g = s->yg;
+  if (yu) std::memcpy(yu, g.yu.data(), sizeof(double) * g.nyu);
+  if (yv) std::memcpy(yv, g.yv.data(), sizeof(double) * g.nyv);
+  if (du) std::memcpy(du, g.dthdyu.data(), sizeof(double) * g.nyu);
+  if (dv) std::memcpy(dv, g.dthdyv.data(), sizeof(double) * g.nyv);
+  if (d2u) std::memcpy(d2u, g.d2u.data(), sizeof(double) * 3 * g.nyu);
+  if (d2v) std::memcpy(d2v, g.d2v.data(), sizeof(double) * 3 * g.nyv);
+  return CFD_OK;
+}
+
+extern "C" int cfd_get_iblist(cfd_handle h, int grid, int wall, int* l9, double* w3, int* n) {
+  cfd_solver* s = static_cast<cfd_solver*>(h);
+  if (!s || grid < 0 || grid > 1 || wall < 0 || wall > 1) return fail(s, CFD_EINVAL, "bad IB list selector");
+  const std::vector<IBPoint>& L = s->ib[grid][wall];
+  if (n) *n = (int)L.size();
+  for (size_t q = 0; q < L.size(); ++q) {
+    const IBPoint& e = L[q];
+    if (l9) {
+      const int v[9] = {e.i, e.k, e.j, e.i2, e.k2, e.j2, e.i3, e.k3, e.j3};
+      std::memcpy(l9 + 9 * q, v, sizeof v);
+    }
+    if (w3) {
+      w3[3 * q] = e.lap;
+      w3[3 * q + 1] = e.w2;
+      w3[3 * q + 2] = e.w3;
+    }
+  }
+  return CFD_OK;
+}
+
+// reference column layout f(j, column) <-> device f[plane][column]
+extern "C" int cfd_set_field(cfd_handle h, int which, const double* src) {
+  cfd_solver* s = static_cast<cfd_solver*>(h);
+  if (!s || !src || which < 0 || which >= CFD_NFIELDS) return fail(s, CFD_EINVAL, "bad field argument");
+  const int ny = ny_of(*s, which);
+  const size_t N = s->Ncol;
+  std::vector<double2> t((size_t)ny * N);
+  for (size_t c = 0; c < N; ++c)
+    for (int q = 0; q < ny; ++q) t[q * N + c] = make_double2(src[2 * (c * ny + q)], src[2 * (c * ny + q) + 1]);
+  HIPCHK(s, hipSetDevice(s->dev));
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  HIPCHK(s, hipMemcpy(s->F[which], t.data(), t.size() * sizeof(double2), hipMemcpyHostToDevice));
+  return CFD_OK;
+}
+
+extern "C" int cfd_get_field(cfd_handle h, int which, double* dst) {
+  cfd_solver* s = static_cast<cfd_solver*>(h);
+  if (!s || !dst || which < 0 || which >= CFD_NFIELDS) return fail(s, CFD_EINVAL, "bad field argument");
+  const int ny = ny_of(*s, which);
+  const size_t N = s->Ncol;
+  std::vector<double2> t((size_t)ny * N);
+  HIPCHK(s, hipSetDevice(s->dev));
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  HIPCHK(s, hipMemcpy(t.data(), s->F[which], t.size() * sizeof(double2), hipMemcpyDeviceToHost));
+  for (size_t c = 0; c < N; ++c)
+    for (int q = 0; q < ny; ++q) {
+      dst[2 * (c * ny + q)] = t[q * N + c].x;
+      dst[2 * (c * ny + q) + 1] = t[q * N + c].y;
+    }
+  return CFD_OK;
+}
+
+extern "C" int cfd_set_scalars(cfd_handle h, const double* v) {
+  cfd_solver* s = static_cast<cfd_solver*>(h);
+  if (!s || !v) return fail(s, CFD_EINVAL, "null argument");
+  DevScalars d{};
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  HIPCHK(s, hipMemcpy(&d, s->d_sc, sizeof d, hipMemcpyDeviceToHost));
+  d.t = v[CFD_T];
+  d.dt = v[CFD_DT];
+  d.dti = v[CFD_DTI];
+  d.mpgx = v[CFD_MPGX];
+  d.dgx = v[CFD_DGX];
+  d.QxT = v[CFD_QXT];
+  d.Qx = v[CFD_QX];
+  d.dtc = v[CFD_DTC];
+  HIPCHK(s, hipMemcpy(s->d_sc, &d, sizeof d, hipMemcpyHostToDevice));
+  return CFD_OK;
+}
+
+extern "C" int cfd_get_scalars(cfd_handle h, double* v) {
+  cfd_solver* s = static_cast<cfd_solver*>(h);
+  if (!s || !v) return fail(s, CFD_EINVAL, "null argument");
+  DevScalars d{};
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  HIPCHK(s, hipMemcpy(&d, s->d_sc, sizeof d, hipMemcpyDeviceToHost));
+  const double o[CFD_NSCALARS] = {d.t, d.dt, d.dti, d.mpgx, d.dgx, d.QxT, d.Qx, d.dtc};
+  std::memcpy(v, o, sizeof o);
+  return CFD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// operators
+// ---------------------------------------------------------------------------
+namespace {
+
+int check(cfd_solver* s, int kRK) {
+  if (!s) return fail(s, CFD_EINVAL, "null handle");
+  if (kRK < 1 || kRK > 3) return fail(s, CFD_EINVAL, "kRK must be 1, 2 or 3");
+  if (hipSetDevice(s->dev) != hipSuccess) return fail(s, CFD_EHIP, "hipSetDevice failed");
+  return CFD_OK;
+}
+
+int launched(cfd_solver* s) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(s, CFD_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
+  return CFD_OK;
+}
+
+void op_nonlinear(cfd_solver* s, int kRK) {
+  if (kRK == 1) launch_reset_max(*s);
+  { Timed t(s, KC_ZINV); launch_zinv(*s, s->d_nljobs, s->n_nljobs); }
+  { Timed t(s, KC_XPASS); launch_xpass_nl(*s, kRK == 1); }
+  { Timed t(s, KC_ZFWD); launch_zfwd_nl(*s); }
+  if (kRK == 1) launch_cfl(*s);
+  { Timed t(s, KC_ELEM); launch_nlcombine(*s, kRK); }
+}
+
+// IB correction of the predictor of fields [f0, f1) (0 u1, 1 u2, 2 u3)
+void op_ib(cfd_solver* s, int f0, int f1) {
+  const int z0 = s->ibz_off[f0], z1 = s->ibz_off[f1];
+  const int x0 = s->ibx_off[f0], x1 = s->ibx_off[f1];
+  if (x1 <= x0) return;
+  Timed t(s, KC_IB);
+  launch_zinv(*s, s->d_ibz + 2 * z0, 2 * (z1 - z0));
+  launch_xpass_inv2(*s, s->d_ibxi + x0, x1 - x0);
+  for (int f = f0; f < f1; ++f) launch_ib_apply(*s, f);
+  launch_xpass_fwd2(*s, s->d_ibxf + x0, x1 - x0);
+  launch_zfwd_plain(*s, s->d_ibzf + x0, x1 - x0);
+}
+
+void op_solve(cfd_solver* s, int f, int kRK) {
+  { Timed t(s, KC_ELEM); launch_predictor(*s, f); }
+  op_ib(s, f, f + 1);
+  { Timed t(s, KC_YSOLVE); launch_ysolve(*s, f, kRK); }
+}
+
+void op_substep(cfd_solver* s, int kRK) {
+  { Timed t(s, KC_ELEM); launch_rhs0(*s, kRK); }
+  op_nonlinear(s, kRK);
+  {
+    Timed t(s, KC_ELEM);
+    for (int f = 0; f < 3; ++f) launch_predictor(*s, f);
+  }
+  op_ib(s, 0, 3);
+  {
+    Timed t(s, KC_YSOLVE);
+    for (int f = 0; f < 3; ++f) launch_ysolve(*s, f, kRK);
+  }
+  if (s->P.flag_ctpress == 0) launch_meanflow(*s, kRK);
+  {
+    Timed t(s, KC_PRESS);
+    launch_solve_p(*s, kRK);
+    launch_vcorr(*s, kRK);
+    launch_divergence(*s);
+  }
+}
+
+}  // namespace
+
+extern "C" int cfd_rhs0(cfd_handle h, int kRK) {
+  cfd_solver* s = static_cast<cfd_solver*>(h);
+  if (int rc = check(s, kRK)) return rc;
+  launch_rhs0(*s, kRK);
+  return launched(s);
+}
+extern "C" int cfd_nonlinear(cfd_handle h, int kRK) {
+  cfd_solver* s = static_cast<cfd_solver*>(h);
+  if (int rc = check(s, kRK)) return rc;
+  op_nonlinear(s, kRK);
+  return launched(s);
+}
+extern "C" int cfd_solve_u(cfd_handle h, int comp, int kRK) {
+  cfd_solver* s = static_cast<cfd_solver*>(h);
+  if (int rc = check(s, kRK)) return rc;
+  if (comp != 1 && comp != 3) return fail(s, CFD_EINVAL, "comp must be 1 or 3");
+  op_solve(s, comp - 1, kRK);
+  return launched(s);
+}
+extern "C" int cfd_solve_v(cfd_handle h, int kRK) {
+  cfd_solver* s = static_cast<cfd_solver*>(h);
+  if (int rc = check(s, kRK)) return rc;
+  op_solve(s, 1, kRK);
+  return launched(s);
+}
+extern "C" int cfd_meanflow_ctU(cfd_handle h, int kRK) {
+  cfd_solver* s = static_cast<cfd_solver*>(h);
+  if (int rc = check(s, kRK)) return rc;
+  launch_meanflow(*s, kRK);
+  return launched(s);
+}
+extern "C" int cfd_solve_p(cfd_handle h, int kRK) {
+  cfd_solver* s = static_cast<cfd_solver*>(h);
+  if (int rc = check(s, kRK)) return rc;
+  launch_solve_p(*s, kRK);
+  return launched(s);
+}
+extern "C" int cfd_vcorr(cfd_handle h, int kRK) {
+  cfd_solver* s = static_cast<cfd_solver*>(h);
+  if (int rc = check(s, kRK)) return rc;
+  launch_vcorr(*s, kRK);
+  launch_divergence(*s);
+  return launched(s);
+}
+extern "C" int cfd_substep(cfd_handle h, int kRK) {
+  cfd_solver* s = static_cast<cfd_solver*>(h);
+  if (int rc = check(s, kRK)) return rc;
+  op_substep(s, kRK);
+  return launched(s);
+}
+extern "C" int cfd_step(cfd_handle h, int nsteps) {
+  cfd_solver* s = static_cast<cfd_solver*>(h);
+  if (int rc = check(s, 1)) return rc;
+  if (nsteps < 0) return fail(s, CFD_EINVAL, "nsteps must be >= 0");
+  for (int it = 0; it < nsteps; ++it) {
+    for (int kRK = 1; kRK <= 3; ++kRK) op_substep(s, kRK);
+    if (s->P.flag_ctpress == 1) launch_flowrate(*s);
+    if (int rc = launched(s)) return rc;
+  }
+  return CFD_OK;
+}
+extern "C" int cfd_sync(cfd_handle h) {
+  cfd_solver* s = static_cast<cfd_solver*>(h);
+  if (!s) return fail(s, CFD_EINVAL, "null handle");
+  HIPCHK(s, hipSetDevice(s->dev));
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  return CFD_OK;
+}
+
+extern "C" int cfd_profile(cfd_handle h, int enable) {
+  cfd_solver* s = static_cast<cfd_solver*>(h);
+  if (!s) return fail(s, CFD_EINVAL, "null handle");
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  s->prof = enable != 0;
+  s->evn = 0;
+  for (int c = 0; c < KC_N; ++c) {
+    s->prof_ms[c] = 0;
+    s->prof_cnt[c] = 0;
+  }
+  return CFD_OK;
+}
+
+extern "C" int cfd_profile_read(cfd_handle h, double* ms, int* cnt) {
+  cfd_solver* s = static_cast<cfd_solver*>(h);
+  if (!s) return fail(s, CFD_EINVAL, "null handle");
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  for (size_t q = 0; q + 1 < s->evn; q += 2) {
+    float m = 0.f;
+    HIPCHK(s, hipEventElapsedTime(&m, s->evpool[q], s->evpool[q + 1]));
+    s->prof_ms[s->evclass[q / 2]] += m;
+    s->prof_cnt[s->evclass[q / 2]] += 1;
+  }
+  s->evn = 0;
+  for (int c = 0; c < KC_N; ++c) {
+    if (ms) ms[c] = s->prof_ms[c];
+    if (cnt) cnt[c] = s->prof_cnt[c];
+  }
+  return CFD_OK;
+}

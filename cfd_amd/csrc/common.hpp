@@ -1,0 +1,179 @@
+// common.hpp -- shared declarations of the HIP rough-channel substep.
+//
+// Device data layout (one GPU owns the whole domain):
+//   spectral field  f[plane][k][i]  double2, plane = j - jlo of its grid,
+//                   k = 0..N2-1 in the reference column order (kz >= 0 first,
+//                   then the negative modes), i = 0..Mx-1 (Mx = N1/2+1).
+//                   Element (j, column) sits at plane*Ncol + column, column =
+//                   k*Mx + i: the reference column index, so a y-line of one
+//                   (kx,kz) mode is strided by Ncol and a wavefront of adjacent
+//                   columns reads one contiguous run per y row.
+//   z-transformed   Z[plane][z][i]  double2, z = 0..Ngz-1 (FFT order), the
+//                   half-transformed planes between the z and x passes.
+//   physical        P[plane][z][x]  double, x = 0..Ngx-1 (IB planes only; the
+//                   nonlinear products never leave the chip in physical form).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "cfd_amd.h"
+#include "setup.hpp"
+
+namespace cfd {
+
+constexpr int kMaxRadix = 16;
+
+struct FftPlan {
+  int L;             // transform length
+  int nr;            // number of radix passes
+  int rad[kMaxRadix];
+};
+
+// z-inverse job: one plane of a source field (optionally interpolated in y)
+// to one plane of a Z buffer.
+//   mode 0: v = S[pa]
+//   mode 1: v = wa*S[pa] + wb*S[pb]
+//   mode 2: t = wa*S[pa] + wb*S[pb]; v = g*(S[pc] - t) + t
+struct ZInvJob {
+  const double2* src;
+  double2* dst;
+  int pa, pb, pc, pd;
+  int mode, pad;
+  double wa, wb, g;
+};
+
+// x-pass job of the IB transforms: row block of one plane of a pair of Z
+// buffers (inverse) or of a physical buffer (forward).
+struct XJob {
+  const double2* za;   // inverse: spectral rows of field a
+  const double2* zb;   // inverse: spectral rows of field b
+  double* pa;          // inverse: physical rows of field a / forward: rows
+  double* pb;          // inverse: physical rows of field b
+  double2* zo;         // forward: output spectral rows
+  int plane;           // plane index inside the buffers
+  int pad;
+};
+
+// z-forward job of the IB transforms: Z plane -> spectral plane (N2 modes).
+struct ZFwdJob {
+  const double2* src;
+  double2* dst;
+  int ps, pd;
+};
+
+// one entry of a device IB list (0-based in-plane indices)
+struct IBDev {
+  int o, o2, o3;       // z*Ngx + x of the point and of its two neighbours
+  int plane;           // IB-plane index inside the physical buffers
+  double lap, w2, w3, pad;
+};
+
+// device-resident scalars of the time stepping (declaration.f90:49-52, 119)
+struct DevScalars {
+  double t, dt, dti, mpgx, dgx, QxT, Qx, dtc;
+  unsigned long long m1, m2, m3;  // CFL maxima as IEEE bit patterns (>= 0)
+  unsigned long long pad;
+};
+
+enum KClass { KC_ZINV = 0, KC_XPASS, KC_ZFWD, KC_ELEM, KC_YSOLVE, KC_PRESS, KC_IB, KC_N };
+
+struct Solver {
+  cfd_params P{};
+  std::string err;
+  int dev = 0;
+  hipStream_t stream = nullptr;
+
+  // sizes
+  int ngx, ngz, ny, dsty, N1, N2, Mx, Ncol;
+  int nyu, nyv, nyp, jlo_u, jlo_v, jlo_p;
+  int nn;
+
+  // host copies of setup products
+  YGrid yg;
+  std::vector<IBPoint> ib[2][2];
+  double gw[2], gwi[2], dtv;
+  double aRK[3], bRK[3], gRK[3], cRK[3], dRK[3];
+  std::vector<double> k1x, k2x, k1z, k2z;
+
+  // device metrics
+  double *d_dthdyu = nullptr, *d_dthdyv = nullptr, *d_d2u = nullptr, *d_d2v = nullptr;
+  double *d_k1x = nullptr, *d_k2x = nullptr, *d_k1z = nullptr, *d_k2z = nullptr;
+  double *d_plo = nullptr, *d_pup = nullptr, *d_pdg = nullptr;  // pressure operator rows
+  double* d_yv = nullptr;                                       // flow-rate weights
+  double* d_mf = nullptr;                                       // mean-flow scratch (4*(ny+1))
+
+  // device fields
+  double2* F[CFD_NFIELDS] = {};
+  // nonlinear scratch: x/z parts and fluxes (spectral)
+  double2 *nxz1 = nullptr, *nxz2 = nullptr, *nxz3 = nullptr, *fuv = nullptr, *fvw = nullptr, *fvv = nullptr;
+  // Z buffers: inverse inputs 0..5 (u1,u2i,u3 on u planes; u1i,v,u3i on v planes)
+  // and products 6..11 (uu,uw,ww,vv on u planes; uv,vw on v planes)
+  double2* Z[12] = {};
+  double* cscr = nullptr;  // per-column factor scratch for the y solves (3 fields)
+  DevScalars* d_sc = nullptr;
+
+  // FFT
+  FftPlan pz{}, px{};
+  double2 *d_twz = nullptr, *d_twx = nullptr;
+
+  // job tables
+  ZInvJob* d_nljobs = nullptr;
+  int n_nljobs = 0;
+
+  // IB
+  int nibp[3] = {0, 0, 0};            // IB planes per field (u1, u2, u3)
+  std::vector<int> ibplanes[2];       // global j of each IB plane, per grid
+  double2 *zib_r[3] = {}, *zib_u[3] = {};
+  double *pib_r[3] = {}, *pib_u[3] = {};
+  ZInvJob* d_ibz = nullptr;
+  int n_ibz = 0;
+  XJob *d_ibxi = nullptr, *d_ibxf = nullptr;
+  int n_ibxi = 0, n_ibxf = 0;
+  ZFwdJob* d_ibzf = nullptr;
+  int n_ibzf = 0;
+  IBDev* d_ibl[2] = {};
+  int* d_ibseg[2] = {};
+  int nibseg[2] = {0, 0};
+
+  // profiling
+  bool prof = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double prof_ms[KC_N] = {};
+  int prof_cnt[KC_N] = {};
+};
+
+// launchers (kernels_fft.hip)
+void launch_zinv(Solver& s, const ZInvJob* jobs, int njobs);
+void launch_xpass_nl(Solver& s, int cfl);
+void launch_zfwd_nl(Solver& s);
+void launch_xpass_inv2(Solver& s, const XJob* jobs, int njobs);
+void launch_xpass_fwd2(Solver& s, const XJob* jobs, int njobs);
+void launch_zfwd_plain(Solver& s, const ZFwdJob* jobs, int njobs);
+// launchers (kernels_spec.hip)
+void launch_rhs0(Solver& s, int kRK);
+void launch_nlcombine(Solver& s, int kRK);
+void launch_reset_max(Solver& s);
+void launch_cfl(Solver& s);
+void launch_predictor(Solver& s, int field);
+void launch_ib_apply(Solver& s, int field);
+void launch_ysolve(Solver& s, int field, int kRK);
+void launch_meanflow(Solver& s, int kRK);
+void launch_solve_p(Solver& s, int kRK);
+void launch_vcorr(Solver& s, int kRK);
+void launch_divergence(Solver& s);
+void launch_flowrate(Solver& s);
+
+// device complex helpers
+__host__ __device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline double2 cscl(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+__host__ __device__ inline double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__host__ __device__ inline double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+// (0,k) * u
+__host__ __device__ inline double2 imulk(double k, double2 u) { return make_double2(-k * u.y, k * u.x); }
+
+}  // namespace cfd

@@ -1,0 +1,124 @@
+/*
+ * cfd_amd.h -- C ABI of the MI355X-native RK3/Crank-Nicolson substep of the
+ * Fourier(x,z) / finite-difference(y) rough-channel DNS.
+ *
+ * The reference (Mel222/CFD, Fortran + MPI) has no plugin API: its hot path
+ * is a set of external subroutines called from the main loop
+ * (littleharsh.f90:163-189) that share state through `module declaration`.
+ * Each entry point below replaces one of those subroutines; the state they
+ * share lives behind a cfd_handle on one GPU.  A Fortran caller binds them
+ * with ISO_C_BINDING (see cfd_amd/fortran/cfd_amd_mod.f90).
+ *
+ * Conventions
+ *  - Every function returns 0 on success, >0 on error (CFD_E*); the message
+ *    is available from cfd_last_error(handle) (handle may be NULL for
+ *    cfd_create failures).  The Fortran wrapper turns a nonzero status into
+ *    `error stop`, in place of the reference's `stop`.
+ *  - Spectral fields cross the boundary in the reference layout
+ *    complex(8) f(jlim1:jlim2, column): y fastest, columns ordered
+ *    k = 1..N2 outer and i = 0..N1/2 inner (start.f90:1467-1481), as
+ *    interleaved (re, im) doubles.  Copies are explicit and synchronous.
+ *  - Handles are not thread-safe; several handles may coexist.
+ */
+#ifndef CFD_AMD_H
+#define CFD_AMD_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CFD_OK 0
+#define CFD_EINVAL 1  /* invalid argument / configuration */
+#define CFD_EHIP 2    /* HIP runtime error */
+#define CFD_ENOMEM 3  /* device allocation failed */
+
+typedef struct cfd_solver* cfd_handle;
+
+/* Parameters of input.in (start.f90:62-227) and geom_circ_rough.txt
+   (geom_circ_rough_mixed_forcing_24pt.f90:113-123).  Integers first, then
+   doubles.  One spectral resolution in all y-bands: Ngalx1 == Ngalx2 and
+   Ngalz1 == Ngalz2. */
+typedef struct cfd_params {
+  int ngalx, ngalz;     /* physical grid points in x and z (Ngalx1, Ngalz1)      */
+  int ny;               /* y points tip to tip, y(0)=-1 .. y(ny-1)=1 (input.in:19) */
+  int dsty, dsty1;      /* canopy points in y; points in the upper canopy layer   */
+  int ppp;              /* polynomial exponent of the y map (input.in:21)         */
+  int ygrid;            /* 0 = polynomial (y_grid_canopy.f90), 1 = tanh          */
+  int geometry_type;    /* 0 smooth, 3 canopies                                   */
+  int ntilex, ntilez;   /* post tiles per wall                                    */
+  int extras24;         /* 1 = the 4 fixed forcing points of the 24pt variant     */
+  int flag_ctpress;     /* 0 constant flow rate, 1 constant mean pressure grad.  */
+  int precision;        /* 0 = FP64 everywhere                                    */
+  int device;           /* HIP device ordinal                                     */
+  double Re, alp, bet;  /* bulk Reynolds number; Lx = 2 pi/alp, Lz = 2 pi/bet     */
+  double dyq;           /* dymax/dymin of the polynomial map                      */
+  double posth, posth1; /* canopy depth (outer units); turning point             */
+  double ytanh_beta;    /* stretching of the tanh map                             */
+  double CFL, mpgx;     /* CFL number; mean pressure gradient (flag_ctpress=1)    */
+  double r, c_i, c_k, stag; /* post radius, centre (grid units in a tile), stagger */
+} cfd_params;
+
+/* field ids for cfd_set_field / cfd_get_field */
+enum {
+  CFD_U1 = 0, CFD_U2, CFD_U3, CFD_P, CFD_NU1, CFD_NU2, CFD_NU3,
+  CFD_DU1, CFD_DU2, CFD_DU3, CFD_PSI, CFD_DIV, CFD_NFIELDS
+};
+/* scalar ids for cfd_set_scalars / cfd_get_scalars (8 doubles) */
+enum { CFD_T = 0, CFD_DT, CFD_DTI, CFD_MPGX, CFD_DGX, CFD_QXT, CFD_QX, CFD_DTC, CFD_NSCALARS };
+
+/* Setup: replaces start (start.f90:36-649): y_grid_canopy, def_k,
+   proc_lims_columns (one device owns every column), getbounds /
+   boundary_circ_rough (IB lists), gridweighting, LU_buildP. */
+int cfd_create(const cfd_params* p, cfd_handle* out);
+
+/* Host-only setup products (no GPU needed): the y grid and IB lists that
+   cfd_create builds, for a caller that keeps its own copy or checks them.
+   Array sizes: nyu = ny+2*dsty+1, nyv = ny+2*dsty; see cfd_get_ygrid and
+   cfd_get_iblist. */
+int cfd_build_ygrid(const cfd_params* p, double* yu, double* yv, double* dthdyu,
+                    double* dthdyv, double* dyu2i, double* dyv2i);
+int cfd_build_iblist(const cfd_params* p, int grid, int wall, int* l9, double* w3, int* n);
+int cfd_destroy(cfd_handle h);
+const char* cfd_last_error(cfd_handle h);
+
+/* dims[16]: N1 N2 Mx Ncol nyu nyv nyp jlo_u jlo_v jlo_p ngx ngz ny dsty nib_u nib_v */
+int cfd_dims(cfd_handle h, int* dims);
+/* y grid (y_grid_canopy.f90): yu, dthdyu, dyu2i(3,.) over nyu points from
+   j = jlo_u; yv, dthdyv, dyv2i over nyv points from jlo_v */
+int cfd_get_ygrid(cfd_handle h, double* yu, double* yv, double* dthdyu,
+                  double* dthdyv, double* dyu2i, double* dyv2i);
+/* IB list of one grid (0 = u/w, 1 = v) and wall (0 bottom, 1 top) as written
+   by boundary_circ_rough: l9 (9,n) = i,k,j,i2,k2,j2,i3,k3,j3 and w3 (3,n) =
+   Lap, w2, w3.  *n receives the count; l9/w3 may be NULL to query it. */
+int cfd_get_iblist(cfd_handle h, int grid, int wall, int* l9, double* w3, int* n);
+
+int cfd_set_field(cfd_handle h, int which, const double* src);
+int cfd_get_field(cfd_handle h, int which, double* dst);
+int cfd_set_scalars(cfd_handle h, const double* v);
+int cfd_get_scalars(cfd_handle h, double* v);
+
+/* Hot-path operators.  kRK = 1, 2, 3. */
+int cfd_rhs0(cfd_handle h, int kRK);             /* RHS0_u1/u2/u3   littleharsh.f90:613-730 */
+int cfd_nonlinear(cfd_handle h, int kRK);        /* nonlinear       FOU3D.f90:41-259        */
+int cfd_solve_u(cfd_handle h, int comp, int kRK);/* solveU(u1|u3)   littleharsh.f90:732-762 */
+int cfd_solve_v(cfd_handle h, int kRK);          /* solveV(u2)      littleharsh.f90:764-794 */
+int cfd_meanflow_ctU(cfd_handle h, int kRK);     /* meanflow_ctU    littleharsh.f90:522-537 */
+int cfd_solve_p(cfd_handle h, int kRK);          /* solveP          littleharsh.f90:556-611 */
+int cfd_vcorr(cfd_handle h, int kRK);            /* v_corr          littleharsh.f90:796-907 */
+int cfd_substep(cfd_handle h, int kRK);          /* body of the RK loop, littleharsh.f90:163-189 */
+/* nsteps time steps of 3 substeps each (the time loop of littleharsh.f90:159-200
+   with the RK loop restored), plus meanflow_ctP when flag_ctpress = 1.
+   Asynchronous: returns once the work is queued on the handle's stream. */
+int cfd_step(cfd_handle h, int nsteps);
+int cfd_sync(cfd_handle h);
+
+/* Kernel-class timing with HIP events on the handle's stream.  classes:
+   0 z-inverse FFT, 1 x-pass FFT+products, 2 z-forward FFT, 3 spectral
+   elementwise, 4 y-solves (u,v,w), 5 pressure solve + correction, 6 IB.
+   ms[c] = accumulated milliseconds, cnt[c] = launches since enable. */
+int cfd_profile(cfd_handle h, int enable);
+int cfd_profile_read(cfd_handle h, double* ms, int* cnt);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
