@@ -30,6 +30,9 @@ def _pair(p, seed=0):
         f = random_field(p, name, rng, amp=1.0 if name in ("u1", "u2", "u3") else 0.3)
         if name == "u1":
             f[0, :] += 1.0  # mean flow
+        if name == "u2":
+            f[:, 0] = 0.0  # no flow through the walls
+            f[:, -1] = 0.0
         ch.set_field(name, f)
         orc.set_field(name, f)
     sc = dict(t=0.0, dt=2e-3, dti=500.0, mpgx=-1e-3, QxT=orc.flowrate() + 0.01)
