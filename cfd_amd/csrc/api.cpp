@@ -130,6 +130,8 @@ extern "C" int cfd_create(const cfd_params* pp, cfd_handle* out) {
   const cfd_params& p = *pp;
   if (p.ngalx < 6 || p.ngalz < 6 || p.ny < 4 || p.dsty < 0) return fail(nullptr, CFD_EINVAL, "grid sizes out of range");
   if (p.ngalx % 2 || p.ngalz % 2) return fail(nullptr, CFD_EINVAL, "Ngalx and Ngalz must be even");
+  if (p.ngalx > 1024 || p.ngalz > 768)
+    return fail(nullptr, CFD_EINVAL, "Ngalx <= 1024 and Ngalz <= 768 (LDS-resident transforms)");
   if (p.precision != 0) return fail(nullptr, CFD_EINVAL, "only precision = 0 (FP64) is available");
   if (p.Re <= 0 || p.alp <= 0 || p.bet <= 0) return fail(nullptr, CFD_EINVAL, "Re, alp, bet must be positive");
   cfd_solver* s = new (std::nothrow) cfd_solver();

@@ -113,3 +113,104 @@ __device__ inline void load_twiddles(double2* lds, const double2* g, int L) {
 }
 
 }  // namespace cfd
+
+// ---------------------------------------------------------------------------
+// Compile-time plans.  A transform length known at compile time gets its
+// radix sequence (8s first, then 4, 2, 3, 5) folded into the code: constant
+// strides and divisors, unrolled butterfly loops, and an LDS layout padded by
+// one element in eight so the stride-R stores of the early Stockham passes
+// spread over all banks.
+// ---------------------------------------------------------------------------
+namespace cfd {
+
+struct CPlan {
+  int nr;
+  int rad[kMaxRadix];
+};
+
+constexpr CPlan make_cplan(int L) {
+  CPlan p{0, {}};
+  int m = L;
+  while (m % 8 == 0 && m > 1) { p.rad[p.nr++] = 8; m /= 8; }
+  while (m % 4 == 0 && m > 1) { p.rad[p.nr++] = 4; m /= 4; }
+  while (m % 2 == 0 && m > 1) { p.rad[p.nr++] = 2; m /= 2; }
+  while (m % 3 == 0 && m > 1) { p.rad[p.nr++] = 3; m /= 3; }
+  while (m % 5 == 0 && m > 1) { p.rad[p.nr++] = 5; m /= 5; }
+  if (m != 1) p.nr = -1;
+  return p;
+}
+
+// padded LDS index
+__device__ __forceinline__ constexpr int pidx(int i) { return i + (i >> 3); }
+constexpr int padded(int n) { return n + (n >> 3) + 1; }
+
+template <bool INV>
+__device__ __forceinline__ void bfly8(double2* v) {
+  // radix-8 as 2 x radix-4 with the internal twiddles w8^k
+  const double s = 0.70710678118654752440;
+  double2 e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+  bfly4<INV>(e);
+  bfly4<INV>(o);
+  // o[k] *= w8^k  (w8 = e^{-i pi/4} forward, e^{+i pi/4} inverse)
+  o[1] = INV ? make_double2(s * (o[1].x - o[1].y), s * (o[1].x + o[1].y))
+             : make_double2(s * (o[1].x + o[1].y), s * (o[1].y - o[1].x));
+  o[2] = rot<INV>(o[2]);
+  o[3] = INV ? make_double2(-s * (o[3].x + o[3].y), s * (o[3].x - o[3].y))
+             : make_double2(s * (o[3].y - o[3].x), -s * (o[3].x + o[3].y));
+  for (int k = 0; k < 4; ++k) {
+    v[k] = cadd(e[k], o[k]);
+    v[k + 4] = csub(e[k], o[k]);
+  }
+}
+
+template <bool INV, int R>
+__device__ __forceinline__ void bfly(double2* v) {
+  if constexpr (R == 2) bfly2<INV>(v);
+  else if constexpr (R == 3) bfly3<INV>(v);
+  else if constexpr (R == 4) bfly4<INV>(v);
+  else if constexpr (R == 5) bfly5<INV>(v);
+  else bfly8<INV>(v);
+}
+
+template <bool INV, int L, int B, int S, int Ns>
+__device__ __forceinline__ void fft_pass(double2*& a, double2*& b, const double2* tw) {
+  constexpr CPlan P = make_cplan(L);
+  if constexpr (S < P.nr) {
+    constexpr int R = P.rad[S];
+    constexpr int nb = L / R;
+    constexpr int ts = L / (Ns * R);
+    for (int w = threadIdx.x; w < B * nb; w += blockDim.x) {
+      const int t = w / nb, j = w - t * nb;
+      const double2* src = a + t * padded(L);
+      double2* dst = b + t * padded(L);
+      const int k = j % Ns;
+      double2 v[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[r] = src[pidx(j + r * nb)];
+      if (Ns > 1 && k) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[r] = cmul(v[r], twid<INV>(tw, r * k * ts));
+      }
+      bfly<INV, R>(v);
+      const int base = (j - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) dst[pidx(base + r * Ns)] = v[r];
+    }
+    __syncthreads();
+    double2* tmp = a;
+    a = b;
+    b = tmp;
+    fft_pass<INV, L, B, S + 1, Ns * R>(a, b, tw);
+  }
+}
+
+// B transforms of compile-time length L in padded LDS rows a[t*padded(L) +
+// pidx(n)]; b is scratch.  Returns the buffer holding the result.
+template <bool INV, int L, int B>
+__device__ double2* fft_lds_c(double2* a, double2* b, const double2* tw) {
+  static_assert(make_cplan(L).nr > 0, "length must factor into 2, 3, 5");
+  fft_pass<INV, L, B, 0, 1>(a, b, tw);
+  return a;
+}
+
+}  // namespace cfd

@@ -25,22 +25,52 @@ __device__ inline double dabs(double x) { return x < 0 ? -x : x; }
 // slot of reference mode k (0-based) in the Ngz-point z transform
 __device__ __forceinline__ int zslot(int k, int N2, int ngz) { return k < N2 / 2 ? k : k - N2 + ngz; }
 
+// LDS row layout: padded rows for compile-time lengths (LC > 0), dense rows
+// for the runtime-plan fallback (LC = 0)
+template <int LC>
+__device__ __forceinline__ int rstride(int L) {
+  if constexpr (LC > 0) return padded(LC);
+  else return L;
+}
+template <int LC>
+__device__ __forceinline__ int ix(int i) {
+  if constexpr (LC > 0) return pidx(i);
+  else return i;
+}
+template <bool INV, int LC, int B>
+__device__ __forceinline__ double2* run_fft(double2* a, double2* b, const double2* tw, const FftPlan& P) {
+  if constexpr (LC > 0) return fft_lds_c<INV, LC, B>(a, b, tw);
+  else return fft_lds<INV>(a, b, tw, P, B);
+}
+
+// transforms per block of the z passes
+constexpr int ti_for(int L) { return L <= 128 ? 8 : (L <= 256 ? 4 : 2); }
+constexpr int kTIFallback = 4;
+
+template <int LC>
+size_t lds_bytes(int L, int B) {
+  const int st = LC > 0 ? padded(LC) : L;
+  return sizeof(double2) * (size_t)(L + 2 * B * st);
+}
+
 // ---------------------------------------------------------------------------
 // z-inverse: grid (ceil(Mx/TI), njobs)
 // ---------------------------------------------------------------------------
+template <int LC, int TI>
 __global__ void __launch_bounds__(kThreads) k_zinv(const ZInvJob* __restrict__ jobs, FftPlan P,
-                                                   const double2* __restrict__ twg, int N2, int Mx, int TI) {
+                                                   const double2* __restrict__ twg, int N2, int Mx) {
   extern __shared__ double2 lds[];
-  const int L = P.L;
+  const int L = LC > 0 ? LC : P.L;
+  const int st = rstride<LC>(L);
   double2* tw = lds;
   double2* A = lds + L;
-  double2* B = A + TI * L;
+  double2* B = A + TI * st;
   const ZInvJob J = jobs[blockIdx.y];
   const int i0 = blockIdx.x * TI;
   const int ncol = N2 * Mx;
   load_twiddles(tw, twg, L);
   // zero the padding slots, then scatter the N2 dealiased modes
-  for (int w = threadIdx.x; w < TI * L; w += blockDim.x) A[w] = make_double2(0.0, 0.0);
+  for (int w = threadIdx.x; w < TI * st; w += blockDim.x) A[w] = make_double2(0.0, 0.0);
   __syncthreads();
   for (int w = threadIdx.x; w < N2 * TI; w += blockDim.x) {
     const int k = w / TI, t = w - k * TI;
@@ -58,15 +88,15 @@ __global__ void __launch_bounds__(kThreads) k_zinv(const ZInvJob* __restrict__ j
         v = make_double2(J.g * (c.x - v.x) + v.x, J.g * (c.y - v.y) + v.y);
       }
     }
-    A[t * L + zslot(k, N2, L)] = v;
+    A[t * st + ix<LC>(zslot(k, N2, L))] = v;
   }
   __syncthreads();
-  double2* R = fft_lds<true>(A, B, tw, P, TI);
+  double2* R = run_fft<true, LC, TI>(A, B, tw, P);
   for (int w = threadIdx.x; w < L * TI; w += blockDim.x) {
     const int z = w / TI, t = w - z * TI;
     const int i = i0 + t;
     if (i >= Mx) continue;
-    J.dst[((size_t)J.pd * L + z) * Mx + i] = R[t * L + z];
+    J.dst[((size_t)J.pd * L + z) * Mx + i] = R[t * st + ix<LC>(z)];
   }
 }
 
@@ -95,8 +125,9 @@ __device__ __forceinline__ double2 pair_mode(const double2* __restrict__ A, cons
 }
 
 // split the forward transform F of a + i b into the modes of a and b, scaled
+template <int LC>
 __device__ __forceinline__ void split_mode(const double2* F, int m, int L, double sc, double2& a, double2& b) {
-  const double2 f = F[m], g = F[m == 0 ? 0 : L - m];
+  const double2 f = F[ix<LC>(m)], g = F[ix<LC>(m == 0 ? 0 : L - m)];
   a = make_double2(0.5 * sc * (f.x + g.x), 0.5 * sc * (f.y - g.y));
   b = make_double2(0.5 * sc * (f.y + g.y), -0.5 * sc * (f.x - g.x));
 }
@@ -131,12 +162,14 @@ struct XNLArgs {
   int nyu, nyv, Mx, cfl;
 };
 
+template <int LC>
 __global__ void __launch_bounds__(kThreads) k_xpass_nl(XNLArgs a, FftPlan P, const double2* __restrict__ twg) {
   extern __shared__ double2 lds[];
-  const int L = P.L, Mx = a.Mx;
+  const int L = LC > 0 ? LC : P.L, Mx = a.Mx;
+  const int st = rstride<LC>(L);
   double2* tw = lds;
   double2* A = lds + L;
-  double2* B = A + 3 * L;
+  double2* B = A + 3 * st;
   __shared__ double red[kThreads / 64];
   const int z = blockIdx.x, q = blockIdx.y;
   const int ngz = gridDim.x;
@@ -152,22 +185,23 @@ __global__ void __launch_bounds__(kThreads) k_xpass_nl(XNLArgs a, FftPlan P, con
   const double2* v = hasv ? a.zi[4] + ru : nullptr;
   const double2* u3i = hasv ? a.zi[5] + ru : nullptr;
   for (int m = threadIdx.x; m < L; m += blockDim.x) {
-    A[m] = pair_mode(u1, u3, m, L, Mx);
-    A[L + m] = pair_mode(u2i, v, m, L, Mx);
-    A[2 * L + m] = pair_mode(u1i, u3i, m, L, Mx);
+    A[ix<LC>(m)] = pair_mode(u1, u3, m, L, Mx);
+    A[st + ix<LC>(m)] = pair_mode(u2i, v, m, L, Mx);
+    A[2 * st + ix<LC>(m)] = pair_mode(u1i, u3i, m, L, Mx);
   }
   __syncthreads();
-  double2* X = fft_lds<true>(A, B, tw, P, 3);
+  double2* X = run_fft<true, LC, 3>(A, B, tw, P);
   double2* Y = (X == A) ? B : A;
   // products in physical space; CFL maxima (dtc_calc, FOU3D.f90:516-544)
   const double dv = vint ? a.dthdyv[q] : 0.0;
   double m1 = 0.0, m2 = 0.0, m3 = 0.0;
   for (int x = threadIdx.x; x < L; x += blockDim.x) {
-    const double2 p0 = X[x], p1 = X[L + x], p2 = X[2 * L + x];
+    const int o = ix<LC>(x);
+    const double2 p0 = X[o], p1 = X[st + o], p2 = X[2 * st + o];
     const double uu = p0.x, ww = p0.y, vc = p1.x, vf = p1.y, uf = p2.x, wf = p2.y;
-    Y[x] = make_double2(uu * uu, uu * ww);
-    Y[L + x] = make_double2(vc * vc, ww * ww);
-    Y[2 * L + x] = make_double2(uf * vf, wf * vf);
+    Y[o] = make_double2(uu * uu, uu * ww);
+    Y[st + o] = make_double2(vc * vc, ww * ww);
+    Y[2 * st + o] = make_double2(uf * vf, wf * vf);
     m1 = fmax(m1, dabs(uu));
     m3 = fmax(m3, dabs(ww));
     m2 = fmax(m2, dabs(vf) * dv);
@@ -181,20 +215,20 @@ __global__ void __launch_bounds__(kThreads) k_xpass_nl(XNLArgs a, FftPlan P, con
     if (threadIdx.x == 0) atomic_max_pos(&a.sc->m2, m2);
   }
   __syncthreads();
-  const double2* F = fft_lds<false>(Y, X, tw, P, 3);
+  const double2* F = run_fft<false, LC, 3>(Y, X, tw, P);
   const double sc = 1.0 / L;
   for (int m = threadIdx.x; m < Mx; m += blockDim.x) {
     double2 ca, cb;
     if (uint) {
-      split_mode(F, m, L, sc, ca, cb);
+      split_mode<LC>(F, m, L, sc, ca, cb);
       a.zo[0][ru + m] = ca;  // uu
       a.zo[1][ru + m] = cb;  // uw
-      split_mode(F + L, m, L, sc, ca, cb);
+      split_mode<LC>(F + st, m, L, sc, ca, cb);
       a.zo[3][ru + m] = ca;  // vv
       a.zo[2][ru + m] = cb;  // ww
     }
     if (hasv) {
-      split_mode(F + 2 * L, m, L, sc, ca, cb);
+      split_mode<LC>(F + 2 * st, m, L, sc, ca, cb);
       a.zo[4][ru + m] = ca;  // uv
       a.zo[5][ru + m] = cb;  // vw
     }
@@ -209,17 +243,19 @@ struct ZNLArgs {
   const double2* z[6];  // uu uw ww vv (u planes), uv vw (v planes)
   double2 *nxz1, *nxz3, *fvv, *nxz2, *fuv, *fvw;
   const double *k1x, *k1z;
-  int nyu, nyv, N2, Mx, TI;
+  int nyu, nyv, N2, Mx;
 };
 
 constexpr int kMaxOut = 8;  // outputs (k,t) per thread: N2*TI <= kMaxOut*kThreads
 
+template <int LC, int TI>
 __global__ void __launch_bounds__(kThreads) k_zfwd_nl(ZNLArgs a, FftPlan P, const double2* __restrict__ twg) {
   extern __shared__ double2 lds[];
-  const int L = P.L, Mx = a.Mx, N2 = a.N2, TI = a.TI;
+  const int L = LC > 0 ? LC : P.L, Mx = a.Mx, N2 = a.N2;
+  const int st = rstride<LC>(L);
   double2* tw = lds;
   double2* A = lds + L;
-  double2* B = A + TI * L;
+  double2* B = A + TI * st;
   const int i0 = blockIdx.x * TI;
   const bool isu = (int)blockIdx.y < a.nyu;
   const int q = isu ? blockIdx.y : blockIdx.y - a.nyu;
@@ -230,6 +266,7 @@ __global__ void __launch_bounds__(kThreads) k_zfwd_nl(ZNLArgs a, FftPlan P, cons
   const int ncol = N2 * Mx;
   const double sc = 1.0 / L;
   double2 acc1[kMaxOut], acc2[kMaxOut];
+#pragma unroll
   for (int c = 0; c < kMaxOut; ++c) acc1[c] = acc2[c] = make_double2(0, 0);
   const int nprod = isu ? 4 : 2;
   for (int pr = 0; pr < nprod; ++pr) {
@@ -237,10 +274,10 @@ __global__ void __launch_bounds__(kThreads) k_zfwd_nl(ZNLArgs a, FftPlan P, cons
     for (int w = threadIdx.x; w < TI * L; w += blockDim.x) {
       const int zz = w / TI, t = w - zz * TI;
       const int i = i0 + t;
-      A[t * L + zz] = (i < Mx) ? src[((size_t)q * L + zz) * Mx + i] : make_double2(0, 0);
+      A[t * st + ix<LC>(zz)] = (i < Mx) ? src[((size_t)q * L + zz) * Mx + i] : make_double2(0, 0);
     }
     __syncthreads();
-    const double2* R = fft_lds<false>(A, B, tw, P, TI);
+    const double2* R = run_fft<false, LC, TI>(A, B, tw, P);
 #pragma unroll
     for (int c = 0; c < kMaxOut; ++c) {
       const int w = threadIdx.x + c * blockDim.x;
@@ -249,7 +286,7 @@ __global__ void __launch_bounds__(kThreads) k_zfwd_nl(ZNLArgs a, FftPlan P, cons
       const int i = i0 + t;
       if (i >= Mx) continue;
       const int sl = zslot(k, N2, L);
-      const double2 X = cscl(R[t * L + sl], sc);
+      const double2 X = cscl(R[t * st + ix<LC>(sl)], sc);
       const bool zero = (sl == L - N2 / 2);  // der_x/der_z zero band (FOU3D.f90:329-333)
       const double kx = a.k1x[i], kz = a.k1z[k];
       const size_t o = (size_t)q * ncol + (size_t)k * Mx + i;
@@ -280,84 +317,85 @@ __global__ void __launch_bounds__(kThreads) k_zfwd_nl(ZNLArgs a, FftPlan P, cons
 // IB transforms
 // ---------------------------------------------------------------------------
 // inverse pair: spectral rows (a, b) of one plane -> physical rows; grid (Ngz, njobs)
+template <int LC>
 __global__ void __launch_bounds__(kThreads) k_xpass_inv2(const XJob* __restrict__ jobs, FftPlan P,
                                                          const double2* __restrict__ twg, int Mx) {
   extern __shared__ double2 lds[];
-  const int L = P.L;
+  const int L = LC > 0 ? LC : P.L;
+  const int st = rstride<LC>(L);
   double2* tw = lds;
   double2* A = lds + L;
-  double2* B = A + L;
+  double2* B = A + st;
   const XJob J = jobs[blockIdx.y];
   const int z = blockIdx.x, ngz = gridDim.x;
   load_twiddles(tw, twg, L);
   const size_t r = ((size_t)J.plane * ngz + z) * Mx;
-  for (int m = threadIdx.x; m < L; m += blockDim.x) A[m] = pair_mode(J.za + r, J.zb + r, m, L, Mx);
+  for (int m = threadIdx.x; m < L; m += blockDim.x) A[ix<LC>(m)] = pair_mode(J.za + r, J.zb + r, m, L, Mx);
   __syncthreads();
-  const double2* X = fft_lds<true>(A, B, tw, P, 1);
+  const double2* X = run_fft<true, LC, 1>(A, B, tw, P);
   const size_t o = ((size_t)J.plane * ngz + z) * L;
   for (int x = threadIdx.x; x < L; x += blockDim.x) {
-    J.pa[o + x] = X[x].x;
-    J.pb[o + x] = X[x].y;
+    const double2 v = X[ix<LC>(x)];
+    J.pa[o + x] = v.x;
+    J.pb[o + x] = v.y;
   }
 }
 
 // forward: physical rows z, z+1 of one plane -> spectral rows; grid (Ngz/2, njobs)
+template <int LC>
 __global__ void __launch_bounds__(kThreads) k_xpass_fwd2(const XJob* __restrict__ jobs, FftPlan P,
                                                          const double2* __restrict__ twg, int Mx) {
   extern __shared__ double2 lds[];
-  const int L = P.L;
+  const int L = LC > 0 ? LC : P.L;
+  const int st = rstride<LC>(L);
   double2* tw = lds;
   double2* A = lds + L;
-  double2* B = A + L;
+  double2* B = A + st;
   const XJob J = jobs[blockIdx.y];
   const int z = 2 * blockIdx.x, ngz = 2 * gridDim.x;
   load_twiddles(tw, twg, L);
   const size_t o = ((size_t)J.plane * ngz + z) * L;
-  for (int x = threadIdx.x; x < L; x += blockDim.x) A[x] = make_double2(J.pa[o + x], J.pa[o + L + x]);
+  for (int x = threadIdx.x; x < L; x += blockDim.x) A[ix<LC>(x)] = make_double2(J.pa[o + x], J.pa[o + L + x]);
   __syncthreads();
-  const double2* F = fft_lds<false>(A, B, tw, P, 1);
+  const double2* F = run_fft<false, LC, 1>(A, B, tw, P);
   const size_t r = ((size_t)J.plane * ngz + z) * Mx;
   const double sc = 1.0 / L;
   for (int m = threadIdx.x; m < Mx; m += blockDim.x) {
     double2 ca, cb;
-    split_mode(F, m, L, sc, ca, cb);
+    split_mode<LC>(F, m, L, sc, ca, cb);
     J.zo[r + m] = ca;
     J.zo[r + Mx + m] = cb;
   }
 }
 
 // plain z-forward: Z plane -> N2 dealiased spectral modes; grid (ceil(Mx/TI), njobs)
+template <int LC, int TI>
 __global__ void __launch_bounds__(kThreads) k_zfwd_plain(const ZFwdJob* __restrict__ jobs, FftPlan P,
-                                                         const double2* __restrict__ twg, int N2, int Mx, int TI) {
+                                                         const double2* __restrict__ twg, int N2, int Mx) {
   extern __shared__ double2 lds[];
-  const int L = P.L;
+  const int L = LC > 0 ? LC : P.L;
+  const int st = rstride<LC>(L);
   double2* tw = lds;
   double2* A = lds + L;
-  double2* B = A + TI * L;
+  double2* B = A + TI * st;
   const ZFwdJob J = jobs[blockIdx.y];
   const int i0 = blockIdx.x * TI;
   load_twiddles(tw, twg, L);
   for (int w = threadIdx.x; w < TI * L; w += blockDim.x) {
     const int zz = w / TI, t = w - zz * TI;
     const int i = i0 + t;
-    A[t * L + zz] = (i < Mx) ? J.src[((size_t)J.ps * L + zz) * Mx + i] : make_double2(0, 0);
+    A[t * st + ix<LC>(zz)] = (i < Mx) ? J.src[((size_t)J.ps * L + zz) * Mx + i] : make_double2(0, 0);
   }
   __syncthreads();
-  const double2* R = fft_lds<false>(A, B, tw, P, TI);
+  const double2* R = run_fft<false, LC, TI>(A, B, tw, P);
   const double sc = 1.0 / L;
   const int ncol = N2 * Mx;
   for (int w = threadIdx.x; w < N2 * TI; w += blockDim.x) {
     const int k = w / TI, t = w - k * TI;
     const int i = i0 + t;
     if (i >= Mx) continue;
-    J.dst[(size_t)J.pd * ncol + (size_t)k * Mx + i] = cscl(R[t * L + zslot(k, N2, L)], sc);
+    J.dst[(size_t)J.pd * ncol + (size_t)k * Mx + i] = cscl(R[t * st + ix<LC>(zslot(k, N2, L))], sc);
   }
-}
-
-int tile_for(int L) {
-  int TI = 8;
-  while (TI > 1 && (size_t)TI * L * 2 * sizeof(double2) > 48 * 1024) TI /= 2;
-  return TI;
 }
 
 // raise the dynamic-LDS limit of a kernel once per size
@@ -369,15 +407,74 @@ void set_lds(const void* kern, size_t bytes) {
   done.push_back({kern, bytes});
 }
 
+// ---------------------------------------------------------------------------
+// launch helpers per compile-time length
+// ---------------------------------------------------------------------------
+template <int LC>
+void zinv_t(Solver& s, const ZInvJob* jobs, int njobs) {
+  constexpr int TI = LC > 0 ? ti_for(LC) : kTIFallback;
+  const size_t lds = lds_bytes<LC>(s.pz.L, TI);
+  set_lds((const void*)k_zinv<LC, TI>, lds);
+  dim3 grid((s.Mx + TI - 1) / TI, njobs);
+  hipLaunchKernelGGL((k_zinv<LC, TI>), grid, dim3(kThreads), lds, s.stream, jobs, s.pz, s.d_twz, s.N2, s.Mx);
+}
+
+template <int LC>
+void xpass_nl_t(Solver& s, const XNLArgs& a) {
+  const size_t lds = lds_bytes<LC>(s.px.L, 3);
+  set_lds((const void*)k_xpass_nl<LC>, lds);
+  hipLaunchKernelGGL((k_xpass_nl<LC>), dim3(s.ngz, s.nyu), dim3(kThreads), lds, s.stream, a, s.px, s.d_twx);
+}
+
+template <int LC>
+void zfwd_nl_t(Solver& s, const ZNLArgs& a) {
+  constexpr int TI = LC > 0 ? ti_for(LC) : kTIFallback;
+  static_assert(TI <= 16, "tile");
+  const size_t lds = lds_bytes<LC>(s.pz.L, TI);
+  set_lds((const void*)k_zfwd_nl<LC, TI>, lds);
+  dim3 grid((s.Mx + TI - 1) / TI, s.nyu + s.nyv);
+  hipLaunchKernelGGL((k_zfwd_nl<LC, TI>), grid, dim3(kThreads), lds, s.stream, a, s.pz, s.d_twz);
+}
+
+template <int LC>
+void xinv2_t(Solver& s, const XJob* jobs, int njobs) {
+  const size_t lds = lds_bytes<LC>(s.px.L, 1);
+  set_lds((const void*)k_xpass_inv2<LC>, lds);
+  hipLaunchKernelGGL((k_xpass_inv2<LC>), dim3(s.ngz, njobs), dim3(kThreads), lds, s.stream, jobs, s.px, s.d_twx,
+                     s.Mx);
+}
+
+template <int LC>
+void xfwd2_t(Solver& s, const XJob* jobs, int njobs) {
+  const size_t lds = lds_bytes<LC>(s.px.L, 1);
+  set_lds((const void*)k_xpass_fwd2<LC>, lds);
+  hipLaunchKernelGGL((k_xpass_fwd2<LC>), dim3(s.ngz / 2, njobs), dim3(kThreads), lds, s.stream, jobs, s.px,
+                     s.d_twx, s.Mx);
+}
+
+template <int LC>
+void zfwdp_t(Solver& s, const ZFwdJob* jobs, int njobs) {
+  constexpr int TI = LC > 0 ? ti_for(LC) : kTIFallback;
+  const size_t lds = lds_bytes<LC>(s.pz.L, TI);
+  set_lds((const void*)k_zfwd_plain<LC, TI>, lds);
+  dim3 grid((s.Mx + TI - 1) / TI, njobs);
+  hipLaunchKernelGGL((k_zfwd_plain<LC, TI>), grid, dim3(kThreads), lds, s.stream, jobs, s.pz, s.d_twz, s.N2,
+                     s.Mx);
+}
+
+// transform lengths with compile-time plans; other lengths use the runtime plan
+#define CFD_FFT_SIZES(X) X(48) X(64) X(72) X(96) X(128) X(192) X(256) X(384) X(768)
+
 }  // namespace
 
 void launch_zinv(Solver& s, const ZInvJob* jobs, int njobs) {
   if (njobs <= 0) return;
-  const int L = s.pz.L, TI = tile_for(L);
-  const size_t lds = sizeof(double2) * (L + 2 * TI * L);
-  set_lds((const void*)k_zinv, lds);
-  dim3 grid((s.Mx + TI - 1) / TI, njobs);
-  hipLaunchKernelGGL(k_zinv, grid, dim3(kThreads), lds, s.stream, jobs, s.pz, s.d_twz, s.N2, s.Mx, TI);
+  switch (s.pz.L) {
+#define C(n) case n: zinv_t<n>(s, jobs, njobs); break;
+    CFD_FFT_SIZES(C)
+#undef C
+    default: zinv_t<0>(s, jobs, njobs);
+  }
 }
 
 void launch_xpass_nl(Solver& s, int cfl) {
@@ -392,10 +489,12 @@ void launch_xpass_nl(Solver& s, int cfl) {
   a.nyv = s.nyv;
   a.Mx = s.Mx;
   a.cfl = cfl;
-  const int L = s.px.L;
-  const size_t lds = sizeof(double2) * (L + 6 * L);
-  set_lds((const void*)k_xpass_nl, lds);
-  hipLaunchKernelGGL(k_xpass_nl, dim3(s.ngz, s.nyu), dim3(kThreads), lds, s.stream, a, s.px, s.d_twx);
+  switch (s.px.L) {
+#define C(n) case n: xpass_nl_t<n>(s, a); break;
+    CFD_FFT_SIZES(C)
+#undef C
+    default: xpass_nl_t<0>(s, a);
+  }
 }
 
 void launch_zfwd_nl(Solver& s) {
@@ -413,40 +512,42 @@ void launch_zfwd_nl(Solver& s) {
   a.nyv = s.nyv;
   a.N2 = s.N2;
   a.Mx = s.Mx;
-  const int L = s.pz.L;
-  int TI = tile_for(L);
-  while (TI > 1 && s.N2 * TI > kMaxOut * kThreads) TI /= 2;
-  a.TI = TI;
-  const size_t lds = sizeof(double2) * (L + 2 * TI * L);
-  set_lds((const void*)k_zfwd_nl, lds);
-  dim3 grid((s.Mx + TI - 1) / TI, s.nyu + s.nyv);
-  hipLaunchKernelGGL(k_zfwd_nl, grid, dim3(kThreads), lds, s.stream, a, s.pz, s.d_twz);
+  switch (s.pz.L) {
+#define C(n) case n: zfwd_nl_t<n>(s, a); break;
+    CFD_FFT_SIZES(C)
+#undef C
+    default: zfwd_nl_t<0>(s, a);
+  }
 }
 
 void launch_xpass_inv2(Solver& s, const XJob* jobs, int njobs) {
   if (njobs <= 0) return;
-  const int L = s.px.L;
-  const size_t lds = sizeof(double2) * 3 * L;
-  set_lds((const void*)k_xpass_inv2, lds);
-  hipLaunchKernelGGL(k_xpass_inv2, dim3(s.ngz, njobs), dim3(kThreads), lds, s.stream, jobs, s.px, s.d_twx, s.Mx);
+  switch (s.px.L) {
+#define C(n) case n: xinv2_t<n>(s, jobs, njobs); break;
+    CFD_FFT_SIZES(C)
+#undef C
+    default: xinv2_t<0>(s, jobs, njobs);
+  }
 }
 
 void launch_xpass_fwd2(Solver& s, const XJob* jobs, int njobs) {
   if (njobs <= 0) return;
-  const int L = s.px.L;
-  const size_t lds = sizeof(double2) * 3 * L;
-  set_lds((const void*)k_xpass_fwd2, lds);
-  hipLaunchKernelGGL(k_xpass_fwd2, dim3(s.ngz / 2, njobs), dim3(kThreads), lds, s.stream, jobs, s.px, s.d_twx,
-                     s.Mx);
+  switch (s.px.L) {
+#define C(n) case n: xfwd2_t<n>(s, jobs, njobs); break;
+    CFD_FFT_SIZES(C)
+#undef C
+    default: xfwd2_t<0>(s, jobs, njobs);
+  }
 }
 
 void launch_zfwd_plain(Solver& s, const ZFwdJob* jobs, int njobs) {
   if (njobs <= 0) return;
-  const int L = s.pz.L, TI = tile_for(L);
-  const size_t lds = sizeof(double2) * (L + 2 * TI * L);
-  set_lds((const void*)k_zfwd_plain, lds);
-  dim3 grid((s.Mx + TI - 1) / TI, njobs);
-  hipLaunchKernelGGL(k_zfwd_plain, grid, dim3(kThreads), lds, s.stream, jobs, s.pz, s.d_twz, s.N2, s.Mx, TI);
+  switch (s.pz.L) {
+#define C(n) case n: zfwdp_t<n>(s, jobs, njobs); break;
+    CFD_FFT_SIZES(C)
+#undef C
+    default: zfwdp_t<0>(s, jobs, njobs);
+  }
 }
 
 }  // namespace cfd

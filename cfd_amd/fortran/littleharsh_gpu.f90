@@ -1,0 +1,185 @@
+! littleharsh_gpu.f90 -- Fortran driver of the GPU substep.  Mirrors the
+! reference main program (littleharsh.f90:63-207) with the time loop and the
+! RK loop restored: it reads input.in in the layout `start` reads
+! (start.f90:62-227, formats 10/20/21/30 of start.f90:434-437, so a real
+! written without a decimal point keeps the implied-decimal semantics),
+! reads geom_circ_rough.txt, sets up the device solver, starts from the
+! parabolic profile of flag_init = 3 (start.f90:2109-2249) and calls the hot
+! path through the C ABI in the reference order.
+program littleharsh_gpu
+  use iso_c_binding
+  use cfd_amd_mod
+  implicit none
+  type(cfd_params) :: p
+  type(c_ptr) :: h
+  integer(c_int) :: rc, d(16), kRK
+  integer :: i, Ngalx2, Ngalz2, flag_init, nwrite, iter, maxiter, npeakx, npeakz, dstx, dstz
+  integer :: physlim_bot, physlim_top, nyu, nyv, nyp, ncol, jlo, nn, j
+  real(8) :: Kib, maxerr, maxt, maxA, hny, Lfracx, Lfracz, sc(8), Qx
+  character(len=90) :: fnameimb, dirin, dirout
+  character(len=32) :: arg
+  real(8), allocatable :: yu(:), yv(:), du(:), dv(:), d2u(:), d2v(:)
+  complex(8), allocatable :: u1(:,:), u2(:,:), u3(:,:), pp(:,:)
+
+10 format(7X,D10.1)
+20 format(7X,I10)
+21 format(12X,I5)
+30 format(7X,A90)
+
+  maxiter = huge(1)
+  if (command_argument_count() >= 1) then
+    call get_command_argument(1, arg)
+    read(arg, *) maxiter
+  end if
+
+  open(40, file='input.in', form='formatted', status='old')
+  do i = 1, 9
+    read(40,10)
+  end do
+  read(40,10) p%Re
+  read(40,10)
+  read(40,10)
+  read(40,10) p%alp
+  read(40,10) p%bet
+  read(40,20) p%ngalx
+  read(40,20) Ngalx2
+  read(40,20) p%ngalz
+  read(40,20) Ngalz2
+  read(40,20) p%ny
+  read(40,10) p%dyq
+  read(40,20) p%ppp
+  read(40,10)
+  read(40,10)
+  read(40,20) flag_init
+  read(40,10)
+  read(40,10)
+  read(40,10)
+  read(40,30) fnameimb
+  read(40,10)
+  read(40,30) dirin
+  read(40,30) dirout
+  read(40,10)
+  read(40,10)
+  read(40,10) p%CFL
+  read(40,20) nwrite
+  read(40,21) p%flag_ctpress
+  read(40,10)
+  p%mpgx = 0d0
+  if (p%flag_ctpress == 1) then
+    read(40,10) p%mpgx
+  else if (p%flag_ctpress == 0) then
+    read(40,10)
+  else
+    write(*,*) 'Wrong presure flag'
+    error stop 1
+  end if
+  read(40,10)
+  read(40,10)
+  read(40,10)
+  read(40,10) Kib
+  read(40,10) maxerr
+  read(40,10) maxt
+  read(40,10) maxA
+  read(40,20) physlim_bot
+  read(40,20) physlim_top
+  read(40,10)
+  read(40,10)
+  read(40,20) p%geometry_type
+  read(40,10)
+  read(40,10)
+  read(40,10)
+  read(40,10)
+  read(40,20) p%ntilex
+  read(40,20) p%ntilez
+  read(40,10) p%posth
+  read(40,10) p%posth1
+  read(40,10) hny
+  read(40,20) npeakx
+  read(40,20) npeakz
+  read(40,10) Lfracx
+  read(40,10) Lfracz
+  read(40,20) p%dsty
+  read(40,20) p%dsty1
+  read(40,20) dstx
+  read(40,20) dstz
+  close(40)
+  if (Ngalx2 /= p%ngalx .or. Ngalz2 /= p%ngalz) then
+    write(*,*) 'one spectral resolution in all bands: Ngalx1 = Ngalx2, Ngalz1 = Ngalz2'
+    error stop 1
+  end if
+  if (p%geometry_type == 3) then
+    open(69, file='geom_circ_rough.txt', form='formatted', status='old')
+    read(69,10) p%r
+    read(69,10) p%c_i
+    read(69,10) p%c_k
+    read(69,10) p%stag
+    close(69)
+  else
+    p%r = 1d0; p%c_i = 2d0; p%c_k = 2d0; p%stag = 0d0
+  end if
+  p%ygrid = 0
+  p%ytanh_beta = 2d0
+  p%extras24 = 1
+  p%precision = 0
+  p%device = 0
+
+  rc = cfd_create(p, h);               call cfd_check(rc, h, 'cfd_create')
+  rc = cfd_dims(h, d);                 call cfd_check(rc, h, 'cfd_dims')
+  ncol = d(4); nyu = d(5); nyv = d(6); nyp = d(7); jlo = d(8)
+  allocate(yu(nyu), yv(nyv), du(nyu), dv(nyv), d2u(3*nyu), d2v(3*nyv))
+  rc = cfd_get_ygrid(h, yu, yv, du, dv, d2u, d2v)
+  call cfd_check(rc, h, 'cfd_get_ygrid')
+
+  ! flag_init = 3: parabolic mean profile and constant mean of v, w, p
+  ! (mblock_ini_parabolic_profile, start.f90:2109-2249)
+  allocate(u1(nyu, ncol), u2(nyv, ncol), u3(nyu, ncol), pp(nyp, ncol))
+  u1 = 0d0; u2 = 0d0; u3 = 0d0; pp = 0d0
+  do j = 1, nyu
+    u1(j, 1) = 0.5d0 * p%Re * p%mpgx * (yu(j)**2 - 1d0)
+    u3(j, 1) = 1d-5
+  end do
+  u2(:, 1) = 1d-6
+  pp(:, 1) = 1d0
+  rc = cfd_set_field(h, CFD_U1, u1);   call cfd_check(rc, h, 'set u1')
+  rc = cfd_set_field(h, CFD_U2, u2);   call cfd_check(rc, h, 'set u2')
+  rc = cfd_set_field(h, CFD_U3, u3);   call cfd_check(rc, h, 'set u3')
+  rc = cfd_set_field(h, CFD_P, pp);    call cfd_check(rc, h, 'set p')
+  ! flowrateIm tip to tip (littleharsh.f90:411-428) -> QxT
+  nn = p%ny - 2
+  Qx = 0d0
+  do j = 1, nn + 1
+    Qx = Qx + (yv(j - jlo + 1) - yv(j - jlo)) * real(u1(j - jlo + 1, 1), 8)
+  end do
+  sc = 0d0
+  sc(CFD_MPGX) = p%mpgx
+  sc(CFD_QXT) = Qx
+  sc(CFD_QX) = Qx
+  rc = cfd_set_scalars(h, sc);         call cfd_check(rc, h, 'set scalars')
+
+  write(*,'(a,3i6,a,i8,a,es12.5)') ' grid', p%ngalx, p%ny, p%ngalz, '  columns', ncol, '  QxT', Qx
+  iter = 0
+  rc = cfd_get_scalars(h, sc);         call cfd_check(rc, h, 'get scalars')
+  do while (sc(CFD_T) < maxt .and. iter < maxiter)
+    do kRK = 1, 3
+      rc = cfd_rhs0(h, kRK);           call cfd_check(rc, h, 'RHS0')
+      rc = cfd_nonlinear(h, kRK);      call cfd_check(rc, h, 'nonlinear')
+      rc = cfd_solve_u(h, 1, kRK);     call cfd_check(rc, h, 'solveU(u1)')
+      rc = cfd_solve_v(h, kRK);        call cfd_check(rc, h, 'solveV(u2)')
+      rc = cfd_solve_u(h, 3, kRK);     call cfd_check(rc, h, 'solveU(u3)')
+      if (p%flag_ctpress == 0) then
+        rc = cfd_meanflow_ctU(h, kRK); call cfd_check(rc, h, 'meanflow_ctU')
+      end if
+      rc = cfd_solve_p(h, kRK);        call cfd_check(rc, h, 'solveP')
+      rc = cfd_vcorr(h, kRK);          call cfd_check(rc, h, 'v_corr')
+    end do
+    iter = iter + 1
+    rc = cfd_get_scalars(h, sc);       call cfd_check(rc, h, 'get scalars')
+    if (mod(iter, max(nwrite, 1)) == 0 .or. iter == maxiter) then
+      write(*,'(a,i8,a,es14.7,a,es12.5,a,es14.7)') ' iter', iter, '  t', sc(CFD_T), '  dt', sc(CFD_DT), &
+                                                   '  mpgx', sc(CFD_MPGX)
+    end if
+  end do
+  rc = cfd_get_field(h, CFD_U1, u1);   call cfd_check(rc, h, 'get u1')
+  write(*,'(a,es22.15)') ' u1(0,0) at the centreline ', real(u1(nyu/2 + 1, 1), 8)
+  rc = cfd_destroy(h)
+end program littleharsh_gpu
