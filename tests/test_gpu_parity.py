@@ -124,9 +124,12 @@ def test_solve_p_and_vcorr(case):
     _cmp(ch, orc, ("div", "psi", "p"))
     ch.v_corr(1)
     orc.v_corr(1)
-    _cmp(ch, orc, ("u1", "u2", "u3", "div"), tol=1e-10)
+    _cmp(ch, orc, ("u1", "u2", "u3"), tol=1e-10)
+    # the divergence left by the projection is round-off on both sides: compare
+    # it against the velocity scale, not relative to itself
     u = ch.get_field("u1")
     assert np.abs(ch.get_field("div")).max() <= 1e-11 * np.abs(u).max()
+    assert np.abs(orc.get_field("div")).max() <= 1e-11 * np.abs(u).max()
 
 
 @pytest.mark.parametrize("case", list(CASES))
