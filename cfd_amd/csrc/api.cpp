@@ -258,6 +258,8 @@ extern "C" int cfd_create(const cfd_params* pp, cfd_handle* out) {
       (rc = dalloc(s, &s->fuv, s->nyv * N)) || (rc = dalloc(s, &s->fvw, s->nyv * N)) ||
       (rc = dalloc(s, &s->cscr, (size_t)3 * s->nyu * N)))
     return bail(rc);
+  for (int f = 0; f < 3; ++f)
+    if ((rc = dalloc(s, &s->ysc[f], (size_t)s->nyu * N))) return bail(rc);
   const size_t zp = (size_t)s->ngz * s->Mx;
   const int zplanes[12] = {s->nyu, s->nyu, s->nyu, s->nyv, s->nyv, s->nyv,
                            s->nyu, s->nyu, s->nyu, s->nyu, s->nyv, s->nyv};
@@ -406,6 +408,18 @@ extern "C" int cfd_create(const cfd_params* pp, cfd_handle* out) {
     if ((rc = upload(s, &s->d_ibz, zj)) || (rc = upload(s, &s->d_ibxi, xi)) || (rc = upload(s, &s->d_ibxf, xf)) ||
         (rc = upload(s, &s->d_ibzf, zf)))
       return bail(rc);
+  }
+  // IB plane masks and plane-index lists of the fused sweeps
+  for (int f = 0; f < 3; ++f) {
+    const int gr = f == 1 ? 1 : 0;
+    const int jlo = gr == 0 ? s->jlo_u : s->jlo_v;
+    std::vector<unsigned char> mask(s->nyu, 0);
+    std::vector<int> q;
+    for (int j : s->ibplanes[gr]) {
+      mask[j - jlo] = 1;
+      q.push_back(j - jlo);
+    }
+    if ((rc = upload(s, &s->d_ibmask[f], mask)) || (rc = upload(s, &s->d_ibq[f], q))) return bail(rc);
   }
   // scalars
   {
@@ -630,25 +644,29 @@ void op_solve(cfd_solver* s, int f, int kRK) {
   { Timed t(s, KC_YSOLVE); launch_ysolve(*s, f, kRK); }
 }
 
+// The substep with fused column sweeps: the nonlinear transforms, the
+// predictor on the IB planes and its IB correction, then one sweep per
+// column that builds RHS0 + nonlinear assembly + predictor and solves the
+// Helmholtz system, the (0,0) flow-rate correction, and one sweep per column
+// for solveP + v_corr.  Same arithmetic as the operator-by-operator path;
+// du holds r only on the IB planes afterwards.
 void op_substep(cfd_solver* s, int kRK) {
-  { Timed t(s, KC_ELEM); launch_rhs0(*s, kRK); }
-  op_nonlinear(s, kRK);
+  if (kRK == 1) launch_reset_max(*s);
+  { Timed t(s, KC_ZINV); launch_zinv(*s, s->d_nljobs, s->n_nljobs); }
+  { Timed t(s, KC_XPASS); launch_xpass_nl(*s, kRK == 1); }
+  { Timed t(s, KC_ZFWD); launch_zfwd_nl(*s); }
+  if (kRK == 1) launch_cfl(*s);
   {
-    Timed t(s, KC_ELEM);
-    for (int f = 0; f < 3; ++f) launch_predictor(*s, f);
+    Timed t(s, KC_IB);
+    launch_ib_rhs(*s, kRK);
   }
   op_ib(s, 0, 3);
-  {
-    Timed t(s, KC_YSOLVE);
-    for (int f = 0; f < 3; ++f) launch_ysolve(*s, f, kRK);
+  { Timed t(s, KC_YSOLVE); launch_rhs_solve(*s, kRK); }
+  if (s->P.flag_ctpress == 0) {
+    Timed t(s, KC_ELEM);
+    launch_meanflow_lds(*s, kRK);
   }
-  if (s->P.flag_ctpress == 0) launch_meanflow(*s, kRK);
-  {
-    Timed t(s, KC_PRESS);
-    launch_solve_p(*s, kRK);
-    launch_vcorr(*s, kRK);
-    launch_divergence(*s);
-  }
+  { Timed t(s, KC_PRESS); launch_pressure(*s, kRK); }
 }
 
 }  // namespace
@@ -681,7 +699,7 @@ extern "C" int cfd_solve_v(cfd_handle h, int kRK) {
 extern "C" int cfd_meanflow_ctU(cfd_handle h, int kRK) {
   cfd_solver* s = static_cast<cfd_solver*>(h);
   if (int rc = check(s, kRK)) return rc;
-  launch_meanflow(*s, kRK);
+  launch_meanflow_lds(*s, kRK);
   return launched(s);
 }
 extern "C" int cfd_solve_p(cfd_handle h, int kRK) {

@@ -112,6 +112,9 @@ struct Solver {
   // and products 6..11 (uu,uw,ww,vv on u planes; uv,vw on v planes)
   double2* Z[12] = {};
   double* cscr = nullptr;  // per-column factor scratch for the y solves (3 fields)
+  double2* ysc[3] = {};    // forward-sweep scratch of the fused solves
+  unsigned char* d_ibmask[3] = {};  // 1 on the IB planes of field f
+  int* d_ibq[3] = {};               // plane indices of the IB planes of field f
   DevScalars* d_sc = nullptr;
 
   // FFT
@@ -164,6 +167,11 @@ void launch_solve_p(Solver& s, int kRK);
 void launch_vcorr(Solver& s, int kRK);
 void launch_divergence(Solver& s);
 void launch_flowrate(Solver& s);
+// launchers (kernels_fused.hip)
+void launch_ib_rhs(Solver& s, int kRK);
+void launch_rhs_solve(Solver& s, int kRK);
+void launch_pressure(Solver& s, int kRK);
+void launch_meanflow_lds(Solver& s, int kRK);
 
 // device complex helpers
 __host__ __device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }

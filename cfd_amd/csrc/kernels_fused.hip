@@ -1,0 +1,514 @@
+// kernels_fused.hip -- column-sweep kernels of the fused substep.
+//
+// One thread owns one (kx,kz) column and walks it in y.  Only Ncol threads
+// exist per field (16 512 at 384x192x192, about three wavefronts per CU for
+// the three velocity components), so each sweep loads its inputs CH rows at a
+// time into registers before using them: the loads of a chunk are in flight
+// together, which keeps enough bytes outstanding to stream from HBM.
+//
+//  k_ib_rhs     predictor r = u + dt du on the IB planes only, du built from
+//               RHS0 (littleharsh.f90:613-730) and the nonlinear assembly
+//               (FOU3D.f90:158-173, 232-244); the IB forcing then corrects it.
+//  k_rhs_solve  the same du and r on every plane (the IB-corrected r taken
+//               where the IB acts), Nu stored for the next substep, and the
+//               Helmholtz solve (LU_buildU/V + LU_dec + LUsolU/V,
+//               tridLU_3D_IB.f90:50-160, 349-413, 521-559) in the same sweep.
+//  k_pressure   solveP (littleharsh.f90:556-611, LUsolP tridLU_3D_IB.f90:284-318)
+//               and v_corr (littleharsh.f90:796-907) with the divergence of the
+//               corrected field, in one forward/backward sweep.
+#include <algorithm>
+
+#include "common.hpp"
+
+namespace cfd {
+
+namespace {
+
+constexpr int CH = 8;
+
+struct FusedArgs {
+  const double2* u[3];                  // u1, u2, u3 (old)
+  double2* __restrict__ nu[3];          // Nu1..3: old in, new out
+  const double2* __restrict__ nxz[3];   // x/z part of the new nonlinear term
+  const double2* __restrict__ flux[3];  // uv (v planes), vv (u planes), vw (v planes)
+  const double2* __restrict__ p;
+  double2* __restrict__ r[3];           // du1..3: IB-corrected r on IB planes
+  double2* __restrict__ ysc[3];         // forward-sweep scratch
+  double* __restrict__ cs[3];           // modified upper diagonals
+  double2* uo[3];                       // new velocity (same arrays as u)
+  const unsigned char* __restrict__ ibmask[3];  // 1 on IB planes
+  const double *d2u, *d2v, *dthdyu, *dthdyv, *k1x, *k2x, *k1z, *k2z;
+  const DevScalars* sc;
+  double C1, C2, C1g, dRK, bRK, Re, ddthetavi, g0, g1;
+  int Mx, Ncol, nyu, nyv;
+  int ib_only;  // 1: k_ib_rhs mode
+};
+
+// pointers of one field, selected without indexing the kernel-argument
+// arrays by a runtime value (that would place the arguments in scratch)
+struct FieldPtrs {
+  const double2* u;
+  double2* nu;
+  const double2* nxz;
+  const double2* flux;
+  double2* r;
+  double2* ys;
+  double* cs;
+  double2* uo;
+  const unsigned char* ib;
+};
+
+template <typename T>
+__device__ __forceinline__ T pick(int f, T a0, T a1, T a2) {
+  return f == 0 ? a0 : (f == 1 ? a1 : a2);
+}
+
+__device__ __forceinline__ FieldPtrs field_ptrs(const FusedArgs& a, int f) {
+  FieldPtrs p;
+  p.u = pick(f, a.u[0], a.u[1], a.u[2]);
+  p.nu = pick(f, a.nu[0], a.nu[1], a.nu[2]);
+  p.nxz = pick(f, a.nxz[0], a.nxz[1], a.nxz[2]);
+  p.flux = pick(f, a.flux[0], a.flux[1], a.flux[2]);
+  p.r = pick(f, a.r[0], a.r[1], a.r[2]);
+  p.ys = pick(f, a.ysc[0], a.ysc[1], a.ysc[2]);
+  p.cs = pick(f, a.cs[0], a.cs[1], a.cs[2]);
+  p.uo = pick(f, a.uo[0], a.uo[1], a.uo[2]);
+  p.ib = pick(f, a.ibmask[0], a.ibmask[1], a.ibmask[2]);
+  return p;
+}
+
+// du and r of row q (interior) of field f for column col, from the loaded
+// values; returns r, stores the new Nu through nnew
+__device__ __forceinline__ double2 row_r(const FusedArgs& a, int f, int q, int col, double k2x, double k2z,
+                                         double kk, double dt, double2 um, double2 u0, double2 up, double2 nold,
+                                         double2 nx, double2 fa, double2 fb, double2 pa, double2 pb,
+                                         double2& nnew) {
+  struct {
+    double2 um, u0, up, nold, nx, fa, fb, pa, pb;
+  } v = {um, u0, up, nold, nx, fa, fb, pa, pb};
+  const double* d = (f == 1 ? a.d2v : a.d2u) + 3 * q;
+  const double c2 = k2x + k2z + d[1];
+  const double2 L = make_double2(d[0] * v.um.x + c2 * v.u0.x + d[2] * v.up.x, d[0] * v.um.y + c2 * v.u0.y + d[2] * v.up.y);
+  double2 du;
+  if (f == 1) {
+    const double C3 = -a.dRK * a.ddthetavi * a.dthdyv[q];
+    du = make_double2(a.C1 * L.x + a.C2 * v.nold.x + C3 * (v.pa.x - v.pb.x),
+                      a.C1 * L.y + a.C2 * v.nold.y + C3 * (v.pa.y - v.pb.y));
+  } else {
+    const double2 gp = imulk(-a.dRK * kk, v.pa);
+    du = make_double2(a.C1 * L.x + a.C2 * v.nold.x + gp.x, a.C1 * L.y + a.C2 * v.nold.y + gp.y);
+    if (f == 0 && col == 0) du.x += -a.dRK * a.sc->mpgx;
+  }
+  const double dd = (f == 1 ? a.dthdyv[q] : a.dthdyu[q]);
+  nnew = make_double2(v.nx.x + (v.fa.x - v.fb.x) * dd * a.ddthetavi, v.nx.y + (v.fa.y - v.fb.y) * dd * a.ddthetavi);
+  du = make_double2(du.x + a.C1g * nnew.x, du.y + a.C1g * nnew.y);
+  return make_double2(v.u0.x + dt * du.x, v.u0.y + dt * du.y);
+}
+
+// load the inputs of row q (interior) of field f
+__device__ __forceinline__ void row_load(const FusedArgs& a, const FieldPtrs& F, int f, int q, size_t N, int col,
+                                         double2& up, double2& nold, double2& nx, double2& fa, double2& fb,
+                                         double2& pa, double2& pb) {
+  up = F.u[(size_t)(q + 1) * N + col];
+  nold = F.nu[(size_t)q * N + col];
+  nx = F.nxz[(size_t)q * N + col];
+  if (f == 1) {
+    fa = F.flux[(size_t)(q + 1) * N + col];
+    fb = F.flux[(size_t)q * N + col];
+    pa = a.p[(size_t)q * N + col];
+    pb = a.p[(size_t)(q - 1) * N + col];
+  } else {
+    fa = F.flux[(size_t)q * N + col];
+    fb = F.flux[(size_t)(q - 1) * N + col];
+    pa = a.p[(size_t)(q - 1) * N + col];
+    pb = pa;
+  }
+}
+
+// grid (ceil(Ncol/64), 3)
+__global__ void __launch_bounds__(64) k_rhs_solve(FusedArgs a) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = blockIdx.y;
+  if (col >= a.Ncol) return;
+  const int i = col % a.Mx, k = col / a.Mx;
+  const double k2x = a.k2x[i], k2z = a.k2z[k];
+  const double kk = (f == 0) ? a.k1x[i] : a.k1z[k];
+  const size_t N = a.Ncol;
+  const int n = (f == 1) ? a.nyv : a.nyu;
+  const double dt = a.sc->dt;
+  const double beta = a.bRK * dt / a.Re;
+  const double* d2 = (f == 1) ? a.d2v : a.d2u;
+  const double g0 = (f == 1) ? 0.0 : a.g0, g1 = (f == 1) ? 0.0 : a.g1;
+  const FieldPtrs F = field_ptrs(a, f);
+  const double2* u = F.u;
+  double2* __restrict__ ys = F.ys;
+  double* __restrict__ cs = F.cs;
+  const unsigned char* __restrict__ ib = F.ib;
+  // row 0: BC row (1, g0), r = 0
+  double d = 1.0, a3p = g0;
+  double2 x = make_double2(0.0, 0.0);
+  ys[col] = x;
+  double2 um = u[col], u0 = u[N + col];
+  for (int q0 = 1; q0 < n - 1; q0 += CH) {
+    double2 UP[CH], NO[CH], NX[CH], FA[CH], FB[CH], PA[CH], PB[CH], RB[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int q = q0 + c;
+      if (q < n - 1) {
+        row_load(a, F, f, q, N, col, UP[c], NO[c], NX[c], FA[c], FB[c], PA[c], PB[c]);
+        RB[c] = ib[q] ? F.r[(size_t)q * N + col] : make_double2(0.0, 0.0);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int q = q0 + c;
+      if (q >= n - 1) break;
+      double2 nnew;
+      double2 rq = row_r(a, f, q, col, k2x, k2z, kk, dt, um, u0, UP[c], NO[c], NX[c], FA[c], FB[c], PA[c], PB[c],
+                         nnew);
+      F.nu[(size_t)q * N + col] = nnew;
+      if (ib[q]) rq = RB[c];
+      um = u0;
+      u0 = UP[c];
+      // Thomas forward step (reciprocal pivots, LU_dec order)
+      const double* dq = d2 + 3 * q;
+      const double a1 = -beta * dq[0];
+      const double a2 = 1.0 - beta * (dq[1] + k2x + k2z);
+      const double c_ = a3p * d;
+      cs[(size_t)(q - 1) * N + col] = c_;
+      d = 1.0 / (a2 - a1 * c_);
+      x = make_double2((rq.x - a1 * x.x) * d, (rq.y - a1 * x.y) * d);
+      ys[(size_t)q * N + col] = x;
+      a3p = -beta * dq[2];
+    }
+  }
+  // last row: BC row (g1, 1), r = 0
+  {
+    const int q = n - 1;
+    const double c_ = a3p * d;
+    cs[(size_t)(q - 1) * N + col] = c_;
+    d = 1.0 / (1.0 - g1 * c_);
+    x = make_double2((0.0 - g1 * x.x) * d, (0.0 - g1 * x.y) * d);
+  }
+  // back substitution
+  double2* uo = F.uo;
+  uo[(size_t)(n - 1) * N + col] = x;
+  for (int q0 = n - 2; q0 >= 0; q0 -= CH) {
+    double2 yv[CH];
+    double cv[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int q = q0 - c;
+      if (q >= 0) {
+        yv[c] = ys[(size_t)q * N + col];
+        cv[c] = cs[(size_t)q * N + col];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int q = q0 - c;
+      if (q < 0) break;
+      x = make_double2(yv[c].x - cv[c] * x.x, yv[c].y - cv[c] * x.y);
+      uo[(size_t)q * N + col] = x;
+    }
+  }
+}
+
+// predictor on the IB planes only: grid (ceil(Ncol/kTB), max IB planes, 3)
+__global__ void __launch_bounds__(256) k_ib_rhs(FusedArgs a, const int* __restrict__ planes0,
+                                                const int* __restrict__ planes1, const int* __restrict__ planes2,
+                                                int n0, int n1, int n2) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = blockIdx.z;
+  const int b = blockIdx.y;
+  const int nb = f == 0 ? n0 : f == 1 ? n1 : n2;
+  if (col >= a.Ncol || b >= nb) return;
+  const int q = (f == 0 ? planes0 : f == 1 ? planes1 : planes2)[b];
+  const int i = col % a.Mx, k = col / a.Mx;
+  const double kk = (f == 0) ? a.k1x[i] : a.k1z[k];
+  const size_t N = a.Ncol;
+  const FieldPtrs F = field_ptrs(a, f);
+  const double2 um = F.u[(size_t)(q - 1) * N + col];
+  const double2 u0 = F.u[(size_t)q * N + col];
+  double2 up, nold, nx, fa, fb, pa, pb;
+  row_load(a, F, f, q, N, col, up, nold, nx, fa, fb, pa, pb);
+  double2 nnew;
+  F.r[(size_t)q * N + col] =
+      row_r(a, f, q, col, a.k2x[i], a.k2z[k], kk, a.sc->dt, um, u0, up, nold, nx, fa, fb, pa, pb, nnew);
+}
+
+// ---------------------------------------------------------------------------
+// pressure: divergence, psi, LU sweep, p += psi, projection, ghosts, walls,
+// divergence of the corrected field.  grid (ceil(Ncol/64))
+// ---------------------------------------------------------------------------
+struct PressArgs {
+  double2* __restrict__ u1;
+  double2* __restrict__ u2;
+  double2* __restrict__ u3;
+  double2* __restrict__ p;
+  double2* __restrict__ psi;
+  double2* __restrict__ div;
+  double* __restrict__ cs;
+  const double *plo, *pup, *pdg, *dthdyu, *dthdyv, *k1x, *k2x, *k1z, *k2z;
+  const DevScalars* sc;
+  double apb, ddthetavi, pin, g0, g1;
+  int Mx, Ncol, nyp;
+};
+
+__global__ void __launch_bounds__(64) k_pressure(PressArgs a) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= a.Ncol) return;
+  const int i = col % a.Mx, k = col / a.Mx;
+  const double kx = a.k1x[i], kz = a.k1z[k], k2x = a.k2x[i], k2z = a.k2z[k];
+  const double dtirk = a.sc->dti / a.apb;
+  const double dtrk = a.sc->dt * a.apb;
+  const size_t N = a.Ncol;
+  const int n = a.nyp;
+  double d = 0.0, up = 0.0;
+  double2 x = make_double2(0, 0);
+  double2 vm = a.u2[col];  // v at the wall face below p row 0
+  for (int q0 = 0; q0 < n; q0 += CH) {
+    double2 U[CH], W[CH], V[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int q = q0 + c;
+      if (q < n) {
+        U[c] = a.u1[(size_t)(q + 1) * N + col];
+        W[c] = a.u3[(size_t)(q + 1) * N + col];
+        V[c] = a.u2[(size_t)(q + 1) * N + col];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int q = q0 + c;
+      if (q >= n) break;
+      const double2 du = imulk(kx, U[c]), dw = imulk(kz, W[c]);
+      const double m = a.ddthetavi * a.dthdyu[q + 1];
+      const double2 dv = make_double2(du.x + dw.x + m * (V[c].x - vm.x), du.y + dw.y + m * (V[c].y - vm.y));
+      vm = V[c];
+      double2 rhs = make_double2(dtirk * dv.x, dtirk * dv.y);
+      double lo = a.plo[q], dg = a.pdg[q] + k2x + k2z, uq = a.pup[q];
+      if (col == 0 && q == 0) {
+        rhs = make_double2(0.0, 0.0);
+        lo = 0.0;
+        dg = a.pin;
+        uq = 0.0;
+      }
+      if (q == 0) {
+        d = 1.0 / dg;
+        x = make_double2(rhs.x * d, rhs.y * d);
+      } else {
+        const double c_ = up * d;
+        a.cs[(size_t)(q - 1) * N + col] = c_;
+        d = 1.0 / (dg - lo * c_);
+        x = make_double2((rhs.x - lo * x.x) * d, (rhs.y - lo * x.y) * d);
+      }
+      a.psi[(size_t)q * N + col] = x;
+      up = uq;
+    }
+  }
+  // backward: psi, p, projection of the u/w row above each p row and of the
+  // v face between p rows q and q+1; divergence of the corrected field one
+  // row behind
+  const double kxd = kx * dtrk, kzd = kz * dtrk;
+  const int nyu = n + 2, nyv = n + 1;
+  double2 pn = x;                            // psi(q+1)
+  double2 un1 = make_double2(0, 0), wn1 = un1;  // corrected u, w of p row q+1
+  double2 vn1 = make_double2(0.0, 0.0);      // corrected v at face q+2 (top wall: 0)
+  a.u2[(size_t)(nyv - 1) * N + col] = vn1;
+  for (int q0 = n - 1; q0 >= 0; q0 -= CH) {
+    double2 Y[CH], P[CH], U[CH], W[CH], V[CH];
+    double C[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int q = q0 - c;
+      if (q >= 0) {
+        Y[c] = a.psi[(size_t)q * N + col];
+        C[c] = (q < n - 1) ? a.cs[(size_t)q * N + col] : 0.0;
+        P[c] = a.p[(size_t)q * N + col];
+        U[c] = a.u1[(size_t)(q + 1) * N + col];
+        W[c] = a.u3[(size_t)(q + 1) * N + col];
+        V[c] = a.u2[(size_t)(q + 1) * N + col];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int q = q0 - c;
+      if (q < 0) break;
+      double2 ps;
+      if (q == n - 1) ps = Y[c];
+      else ps = make_double2(Y[c].x - C[c] * pn.x, Y[c].y - C[c] * pn.y);
+      a.psi[(size_t)q * N + col] = ps;
+      a.p[(size_t)q * N + col] = make_double2(P[c].x + ps.x, P[c].y + ps.y);
+      const double2 gx = imulk(kxd, ps), gz = imulk(kzd, ps);
+      const double2 un = make_double2(U[c].x - gx.x, U[c].y - gx.y);
+      const double2 wn = make_double2(W[c].x - gz.x, W[c].y - gz.y);
+      a.u1[(size_t)(q + 1) * N + col] = un;
+      a.u3[(size_t)(q + 1) * N + col] = wn;
+      if (q == n - 1) {
+        a.u1[(size_t)(nyu - 1) * N + col] = make_double2(-a.g1 * un.x, -a.g1 * un.y);
+        a.u3[(size_t)(nyu - 1) * N + col] = make_double2(-a.g1 * wn.x, -a.g1 * wn.y);
+      }
+      // v face q+1 lies between p rows q and q+1
+      double2 vn;
+      if (q == n - 1) {
+        vn = make_double2(0.0, 0.0);  // top wall face, set above
+      } else {
+        const double cc = dtrk * a.ddthetavi * a.dthdyv[q + 1];
+        vn = make_double2(V[c].x - cc * (pn.x - ps.x), V[c].y - cc * (pn.y - ps.y));
+        a.u2[(size_t)(q + 1) * N + col] = vn;
+        // divergence of the corrected field at p row q+1
+        const double2 du = imulk(kx, un1), dw = imulk(kz, wn1);
+        const double m = a.ddthetavi * a.dthdyu[q + 2];
+        a.div[(size_t)(q + 1) * N + col] =
+            make_double2(du.x + dw.x + m * (vn1.x - vn.x), du.y + dw.y + m * (vn1.y - vn.y));
+      }
+      un1 = un;
+      wn1 = wn;
+      vn1 = vn;
+      pn = ps;
+    }
+  }
+  // bottom: wall face, ghosts, divergence at p row 0
+  a.u2[col] = make_double2(0.0, 0.0);
+  a.u1[col] = make_double2(-a.g0 * un1.x, -a.g0 * un1.y);
+  a.u3[col] = make_double2(-a.g0 * wn1.x, -a.g0 * wn1.y);
+  {
+    const double2 du = imulk(kx, un1), dw = imulk(kz, wn1);
+    const double m = a.ddthetavi * a.dthdyu[1];
+    a.div[col] = make_double2(du.x + dw.x + m * (vn1.x - 0.0), du.y + dw.y + m * (vn1.y - 0.0));
+  }
+}
+
+// (0,0) flow-rate correction staged through LDS: the column, metrics and
+// scratch live in LDS; one thread runs the sequential recurrences and sums
+// in the reference order (flowrate_corr, littleharsh.f90:452-484).
+__global__ void __launch_bounds__(256) k_meanflow_lds(double2* __restrict__ u1, size_t Ncol, int jlo_u, int nn,
+                                                      const double* __restrict__ d2u, const double* __restrict__ yv,
+                                                      int jlo_v, DevScalars* sc, double bRK, double dRK, double Re,
+                                                      double g0, double g1) {
+  extern __shared__ double sm[];
+  const int n = nn + 3;
+  double *a1 = sm, *a2 = sm + n, *a3 = sm + 2 * n, *x = sm + 3 * n, *dy = sm + 4 * n, *ur = sm + 5 * n;
+  const double dt = sc->dt;
+  const double beta = bRK * dt / Re;
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    ur[j] = u1[(size_t)(j - jlo_u) * Ncol].x;
+    dy[j] = (j >= 1 && j <= nn + 1) ? (yv[j - jlo_v] - yv[j - 1 - jlo_v]) : 0.0;
+    if (j == 0) {
+      a1[j] = 0.0; a2[j] = 1.0; a3[j] = g0; x[j] = 0.0;
+    } else if (j == nn + 2) {
+      a1[j] = g1; a2[j] = 1.0; a3[j] = 0.0; x[j] = 0.0;
+    } else {
+      const double* d = d2u + 3 * (j - jlo_u);
+      a1[j] = -beta * d[0];
+      a2[j] = 1.0 - beta * d[1];
+      a3[j] = -beta * d[2];
+      x[j] = -dRK * dt;
+    }
+  }
+  __syncthreads();
+  __shared__ double gsh;
+  if (threadIdx.x == 0) {
+    a2[0] = 1.0 / a2[0];
+    for (int j = 1; j < n; ++j) {
+      a3[j - 1] = a3[j - 1] * a2[j - 1];
+      a2[j] = 1.0 / (a2[j] - a1[j] * a3[j - 1]);
+    }
+    x[0] = x[0] * a2[0];
+    for (int j = 1; j < n; ++j) x[j] = (x[j] - a1[j] * x[j - 1]) * a2[j];
+    for (int j = n - 2; j >= 0; --j) x[j] = x[j] - a3[j] * x[j + 1];
+    double Qx1 = 0.0, Qx0 = 0.0;
+    for (int j = 1; j <= nn + 1; ++j) Qx1 = Qx1 + dy[j] * x[j];
+    for (int j = 1; j <= nn + 1; ++j) Qx0 = Qx0 + dy[j] * ur[j];
+    const double g = (sc->QxT - Qx0) / Qx1;
+    sc->mpgx = sc->mpgx + g;
+    sc->dgx = g;
+    gsh = g;
+  }
+  __syncthreads();
+  const double g = gsh;
+  for (int j = threadIdx.x; j < n; j += blockDim.x) u1[(size_t)(j - jlo_u) * Ncol].x = ur[j] + g * x[j];
+}
+
+}  // namespace
+
+void fused_args(Solver& s, int kRK, void* out) {
+  FusedArgs& a = *static_cast<FusedArgs*>(out);
+  for (int f = 0; f < 3; ++f) {
+    a.u[f] = s.F[CFD_U1 + f];
+    a.uo[f] = s.F[CFD_U1 + f];
+    a.nu[f] = s.F[CFD_NU1 + f];
+    a.r[f] = s.F[CFD_DU1 + f];
+    a.ysc[f] = s.ysc[f];
+    a.cs[f] = s.cscr + (size_t)f * s.nyu * s.Ncol;
+    a.ibmask[f] = s.d_ibmask[f];
+  }
+  a.nxz[0] = s.nxz1; a.nxz[1] = s.nxz2; a.nxz[2] = s.nxz3;
+  a.flux[0] = s.fuv; a.flux[1] = s.fvv; a.flux[2] = s.fvw;
+  a.p = s.F[CFD_P];
+  a.d2u = s.d_d2u; a.d2v = s.d_d2v; a.dthdyu = s.d_dthdyu; a.dthdyv = s.d_dthdyv;
+  a.k1x = s.d_k1x; a.k2x = s.d_k2x; a.k1z = s.d_k1z; a.k2z = s.d_k2z;
+  a.sc = s.d_sc;
+  a.C1 = s.aRK[kRK - 1] / s.P.Re;
+  a.C2 = -s.cRK[kRK - 1];
+  a.C1g = -s.gRK[kRK - 1];
+  a.dRK = s.dRK[kRK - 1];
+  a.bRK = s.bRK[kRK - 1];
+  a.Re = s.P.Re;
+  a.ddthetavi = 1.0;
+  a.g0 = s.gw[0];
+  a.g1 = s.gw[1];
+  a.Mx = s.Mx;
+  a.Ncol = s.Ncol;
+  a.nyu = s.nyu;
+  a.nyv = s.nyv;
+  a.ib_only = 0;
+}
+
+void launch_ib_rhs(Solver& s, int kRK) {
+  const int n0 = s.nibp[0], n1 = s.nibp[1], n2 = s.nibp[2];
+  const int nmax = std::max(n0, std::max(n1, n2));
+  if (nmax == 0) return;
+  FusedArgs a;
+  fused_args(s, kRK, &a);
+  dim3 grid((s.Ncol + 255) / 256, nmax, 3);
+  hipLaunchKernelGGL(k_ib_rhs, grid, dim3(256), 0, s.stream, a, (const int*)s.d_ibq[0], (const int*)s.d_ibq[1],
+                     (const int*)s.d_ibq[2], n0, n1, n2);
+}
+
+void launch_rhs_solve(Solver& s, int kRK) {
+  FusedArgs a;
+  fused_args(s, kRK, &a);
+  hipLaunchKernelGGL(k_rhs_solve, dim3((s.Ncol + 63) / 64, 3), dim3(64), 0, s.stream, a);
+}
+
+void launch_pressure(Solver& s, int kRK) {
+  PressArgs a;
+  a.u1 = s.F[CFD_U1]; a.u2 = s.F[CFD_U2]; a.u3 = s.F[CFD_U3];
+  a.p = s.F[CFD_P]; a.psi = s.F[CFD_PSI]; a.div = s.F[CFD_DIV];
+  a.cs = s.cscr;
+  a.plo = s.d_plo; a.pup = s.d_pup; a.pdg = s.d_pdg; a.dthdyu = s.d_dthdyu; a.dthdyv = s.d_dthdyv;
+  a.k1x = s.d_k1x; a.k2x = s.d_k2x; a.k1z = s.d_k1z; a.k2z = s.d_k2z;
+  a.sc = s.d_sc;
+  a.apb = s.aRK[kRK - 1] + s.bRK[kRK - 1];
+  a.ddthetavi = 1.0;
+  const double h = s.yg.YU(2) - s.yg.YU(1);
+  a.pin = 1.0 / (h * h);
+  a.g0 = s.gw[0];
+  a.g1 = s.gw[1];
+  a.Mx = s.Mx;
+  a.Ncol = s.Ncol;
+  a.nyp = s.nyp;
+  hipLaunchKernelGGL(k_pressure, dim3((s.Ncol + 63) / 64), dim3(64), 0, s.stream, a);
+}
+
+void launch_meanflow_lds(Solver& s, int kRK) {
+  const int n = s.nn + 3;
+  const size_t lds = sizeof(double) * 6 * n;
+  hipLaunchKernelGGL(k_meanflow_lds, dim3(1), dim3(256), lds, s.stream, s.F[CFD_U1], (size_t)s.Ncol, s.jlo_u, s.nn,
+                     (const double*)s.d_d2u, (const double*)s.d_yv, s.jlo_v, s.d_sc, s.bRK[kRK - 1],
+                     s.dRK[kRK - 1], s.P.Re, s.gw[0], s.gw[1]);
+}
+
+}  // namespace cfd
