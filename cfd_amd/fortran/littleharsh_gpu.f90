@@ -26,10 +26,13 @@ program littleharsh_gpu
 21 format(12X,I5)
 30 format(7X,A90)
 
+  ! argument: number of time steps, or "parse" to print the parameters read
+  ! from input.in and stop
   maxiter = huge(1)
+  arg = ''
   if (command_argument_count() >= 1) then
     call get_command_argument(1, arg)
-    read(arg, *) maxiter
+    if (trim(arg) /= 'parse') read(arg, *) maxiter
   end if
 
   open(40, file='input.in', form='formatted', status='old')
@@ -123,6 +126,13 @@ program littleharsh_gpu
   p%precision = 0
   p%device = 0
 
+  if (trim(arg) == 'parse') then
+    write(*,'(a,9i8)') ' ints', p%ngalx, p%ngalz, p%ny, p%dsty, p%dsty1, p%ppp, p%geometry_type, p%ntilex, p%ntilez
+    write(*,'(a,i4)') ' flag_ctpress', p%flag_ctpress
+    write(*,'(a,7es24.16)') ' reals', p%Re, p%alp, p%bet, p%dyq, p%posth, p%CFL, p%mpgx
+    write(*,'(a,4es24.16)') ' post', p%r, p%c_i, p%c_k, p%stag
+    stop
+  end if
   rc = cfd_create(p, h);               call cfd_check(rc, h, 'cfd_create')
   rc = cfd_dims(h, d);                 call cfd_check(rc, h, 'cfd_dims')
   ncol = d(4); nyu = d(5); nyv = d(6); nyp = d(7); jlo = d(8)

@@ -20,6 +20,9 @@ def _ensure_built():
         subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
     if not os.path.exists(os.path.join(ROOT, "cfd_amd", "libcfd_amd.so")):
         subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "cfd_amd", "csrc")], check=True)
+    drv = os.path.join(ROOT, "cfd_amd", "fortran", "littleharsh_gpu")
+    if not os.path.exists(drv) and os.path.exists("/opt/rocm/bin/amdflang"):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "cfd_amd", "fortran")], check=False)
 
 
 _ensure_built()
