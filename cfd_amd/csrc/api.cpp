@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -209,6 +210,10 @@ extern "C" int cfd_create(const cfd_params* pp, cfd_handle* out) {
   if (!make_plan(s->ngz, s->pz) || !make_plan(s->ngx, s->px)) {
     s->err = "Ngalx and Ngalz must factor into 2, 3 and 5";
     return bail(CFD_EINVAL);
+  }
+  {
+    const char* e = getenv("CFD_AMD_FFT");
+    s->fft_stockham = e && std::string(e) == "stockham";
   }
   int rc;
   if ((rc = upload(s, &s->d_twz, twiddles(s->ngz))) || (rc = upload(s, &s->d_twx, twiddles(s->ngx)))) return bail(rc);

@@ -119,6 +119,7 @@ struct Solver {
 
   // FFT
   FftPlan pz{}, px{};
+  bool fft_stockham = false;  // force the LDS Stockham kernels (CFD_AMD_FFT=stockham)
   double2 *d_twz = nullptr, *d_twx = nullptr;
 
   // job tables

@@ -12,6 +12,7 @@
 //             complex transform of length Ngx: a + i b.
 //  z-forward  Z plane -> spectral plane, keeping the N2 dealiased z-modes and
 //             applying the x/z derivatives (der_x/der_z, FOU3D.f90:310-414).
+#include "dft.hpp"
 #include "fft.hpp"
 
 namespace cfd {
@@ -159,7 +160,7 @@ struct XNLArgs {
   double2* zo[6];
   const double* dthdyv;
   DevScalars* sc;
-  int nyu, nyv, Mx, cfl;
+  int nyu, nyv, Mx, ngz, cfl;
 };
 
 template <int LC>
@@ -398,6 +399,338 @@ __global__ void __launch_bounds__(kThreads) k_zfwd_plain(const ZFwdJob* __restri
   }
 }
 
+// ---------------------------------------------------------------------------
+// Four-step kernels.  A length L = P x Q transform is done as P- and Q-point
+// register DFTs (dft.hpp) with one exchange through LDS in between: input
+// n = Q p + q, output k = k1 + P k2 (the identity of the dft.hpp header applied
+// at length L, with the L-point twiddles w_L^{q k1} read from the table).
+// The z kernels load and store HBM straight from registers, TI x-modes per
+// block side by side (lanes of consecutive i touch one 16*TI-byte segment).
+// Sizes follow the dealiasing rule of the reference: N = 2*(L/3) modes kept
+// (N2 along z, N1 along x with Mx = N1/2 + 1), FOU3D.f90:546-564.
+// ---------------------------------------------------------------------------
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+constexpr int rup64(int n) { return (n + 63) / 64 * 64; }
+
+template <int L>
+struct Dealias {
+  static constexpr int N = 2 * (L / 3);  // kept modes
+  static constexpr int H = N / 2;        // slots [H, L-H) carry no mode
+  static constexpr int Mx = N / 2 + 1;   // kept x-modes of a real row
+};
+
+__device__ __forceinline__ double2 zinv_load(const ZInvJob& J, size_t off, size_t ncol) {
+  if (J.mode == 0) return J.src[(size_t)J.pa * ncol + off];
+  const double2 a = J.src[(size_t)J.pa * ncol + off], b = J.src[(size_t)J.pb * ncol + off];
+  double2 v = make_double2(J.wa * a.x + J.wb * b.x, J.wa * a.y + J.wb * b.y);
+  if (J.mode == 2) {
+    const double2 c = J.src[(size_t)J.pc * ncol + off];
+    v = make_double2(J.g * (c.x - v.x) + v.x, J.g * (c.y - v.y) + v.y);
+  }
+  return v;
+}
+
+// z-inverse, four-step: grid (ceil(Mx/TI), njobs), TI*max(P,Q) threads.
+// LDS T[k1][q][t].
+template <int L, int P, int Q, int TI>
+__global__ void __launch_bounds__(TI * cmax(P, Q)) k_zinv4(const ZInvJob* __restrict__ jobs,
+                                                          const double2* __restrict__ twg, int Mx) {
+  static_assert(P * Q == L, "P*Q");
+  constexpr int N2 = Dealias<L>::N, H = Dealias<L>::H;
+  __shared__ double2 T[P * Q * TI];
+  const ZInvJob J = jobs[blockIdx.y];
+  const int tid = threadIdx.x, i0 = blockIdx.x * TI;
+  const size_t ncol = (size_t)N2 * Mx;
+  if (tid < Q * TI) {
+    const int t = tid % TI, q = tid / TI, i = i0 + t;
+    double2 v[P];
+    sfor<0, P>([&](auto pc) {
+      constexpr int p = decltype(pc)::value;
+      if constexpr (Q * p >= H && Q * p + Q <= L - H) {
+        v[p] = make_double2(0.0, 0.0);  // the whole group lies in the empty band
+      } else {
+        const int n = Q * p + q;
+        const int k = n < H ? n : (n >= L - H ? n - L + N2 : -1);
+        v[p] = (k >= 0 && i < Mx) ? zinv_load(J, (size_t)k * Mx + i, ncol) : make_double2(0.0, 0.0);
+      }
+    });
+    dft<true, P>(v);
+    T[q * TI + t] = v[0];
+    sfor<1, P>([&](auto kc) {
+      constexpr int k1 = decltype(kc)::value;
+      T[(k1 * Q + q) * TI + t] = mul_tw<true>(v[k1], twg[q * k1]);
+    });
+  }
+  __syncthreads();
+  if (tid < P * TI) {
+    const int t = tid % TI, k1 = tid / TI, i = i0 + t;
+    double2 u[Q];
+    sfor<0, Q>([&](auto qc) { u[decltype(qc)::value] = T[(k1 * Q + decltype(qc)::value) * TI + t]; });
+    dft<true, Q>(u);
+    if (i < Mx) {
+      double2* d = J.dst + (size_t)J.pd * L * Mx + i;
+      sfor<0, Q>([&](auto c) { d[(size_t)(k1 + P * decltype(c)::value) * Mx] = u[decltype(c)::value]; });
+    }
+  }
+}
+
+// z-forward of the nonlinear products, four-step: grid (ceil(Mx/TI), nyu+nyv).
+// Output slot k1 + P k2; groups of k2 wholly in the empty band are skipped at
+// compile time.  Same outputs as k_zfwd_nl.
+template <int L, int P, int Q, int TI>
+__global__ void __launch_bounds__(TI * cmax(P, Q)) k_zfwd4(ZNLArgs a, const double2* __restrict__ twg) {
+  static_assert(P * Q == L, "P*Q");
+  constexpr int N2 = Dealias<L>::N, H = Dealias<L>::H;
+  __shared__ double2 T[P * Q * TI];
+  const int Mx = a.Mx;
+  const int tid = threadIdx.x, i0 = blockIdx.x * TI;
+  const bool isu = (int)blockIdx.y < a.nyu;
+  const int pl = isu ? blockIdx.y : blockIdx.y - a.nyu;
+  const int ny = isu ? a.nyu : a.nyv;
+  const bool inter = pl > 0 && pl < ny - 1;
+  if (isu && !inter) return;  // ghost u planes: no product is needed there
+  const size_t ncol = (size_t)N2 * Mx;
+  const double sc = 1.0 / L;
+  const int t2 = tid % TI, k1 = tid / TI, i2 = i0 + t2;
+  double2 acc1[Q], acc2[Q];
+  const int nprod = isu ? 4 : 2;
+  for (int pr = 0; pr < nprod; ++pr) {
+    const double2* src = a.z[isu ? pr : 4 + pr] + (size_t)pl * L * Mx;
+    if (tid < Q * TI) {
+      const int t = tid % TI, q = tid / TI, i = i0 + t;
+      double2 v[P];
+      sfor<0, P>([&](auto pc) {
+        constexpr int p = decltype(pc)::value;
+        v[p] = i < Mx ? src[(size_t)(Q * p + q) * Mx + i] : make_double2(0.0, 0.0);
+      });
+      dft<false, P>(v);
+      T[q * TI + t] = v[0];
+      sfor<1, P>([&](auto kc) {
+        constexpr int kk = decltype(kc)::value;
+        T[(kk * Q + q) * TI + t] = mul_tw<false>(v[kk], twg[q * kk]);
+      });
+    }
+    __syncthreads();
+    if (tid < P * TI) {
+      double2 u[Q];
+      sfor<0, Q>([&](auto qc) { u[decltype(qc)::value] = T[(k1 * Q + decltype(qc)::value) * TI + t2]; });
+      dft<false, Q>(u);
+      if (i2 < Mx) {
+        const double kx = a.k1x[i2];
+        sfor<0, Q>([&](auto c) {
+          constexpr int k2 = decltype(c)::value;
+          if constexpr (!(P * k2 >= H && P * k2 + P <= L - H)) {
+            const int sl = k1 + P * k2;
+            if (sl < H || sl >= L - H) {
+              const int k = sl < H ? sl : sl - L + N2;
+              const double2 X = cscl(u[k2], sc);
+              const bool zero = (sl == L - H);  // der_x/der_z zero band (FOU3D.f90:329-333)
+              const double kz = a.k1z[k];
+              const size_t o = (size_t)pl * ncol + (size_t)k * Mx + i2;
+              if (isu) {
+                // Nu1 = ikx uu + ikz uw ; Nu3 = ikx uw + ikz ww ; keep vv
+                if (pr == 0) acc1[k2] = imulk(kx, X);
+                if (pr == 1) {
+                  acc1[k2] = cadd(acc1[k2], imulk(kz, X));
+                  acc2[k2] = imulk(kx, X);
+                }
+                if (pr == 2) acc2[k2] = cadd(acc2[k2], imulk(kz, X));
+                if (pr == 3) {
+                  a.fvv[o] = X;
+                  a.nxz1[o] = zero ? make_double2(0, 0) : acc1[k2];
+                  a.nxz3[o] = zero ? make_double2(0, 0) : acc2[k2];
+                }
+              } else {
+                // Nu2 = ikx uv + ikz vw ; keep uv, vw
+                if (pr == 0) {
+                  acc1[k2] = imulk(kx, X);
+                  a.fuv[o] = X;
+                }
+                if (pr == 1) {
+                  a.fvw[o] = X;
+                  if (inter) a.nxz2[o] = zero ? make_double2(0, 0) : cadd(acc1[k2], imulk(kz, X));
+                }
+              }
+            }
+          }
+        });
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// one Hermitian pair mode of a + i b (pair_mode) with the range of m known
+// at compile time where the whole group of Q consecutive m shares it
+template <int L, int Q, int P0>
+__device__ __forceinline__ double2 pair_mode_q(const double2* __restrict__ A, const double2* __restrict__ B,
+                                               int m) {
+  constexpr int Mx = Dealias<L>::Mx;
+  constexpr int lo = Q * P0, hi = Q * P0 + Q - 1;
+  if constexpr (lo >= Mx && hi <= L - Mx) {
+    return make_double2(0.0, 0.0);
+  } else if constexpr (lo > 0 && hi < Mx) {
+    const double2 a = A ? A[m] : make_double2(0, 0), b = B ? B[m] : make_double2(0, 0);
+    return make_double2(a.x - b.y, a.y + b.x);
+  } else if constexpr (lo > L - Mx) {
+    const double2 a = A ? A[L - m] : make_double2(0, 0), b = B ? B[L - m] : make_double2(0, 0);
+    return make_double2(a.x + b.y, -a.y + b.x);
+  } else {
+    return pair_mode(A, B, m, L, Mx);
+  }
+}
+
+// nonlinear x-pass, four-step: grid (ceil(Ngz/R), nyu), rup64(3 P R)
+// threads.  Block (zb, q) takes rows z = R zb .. R zb + R-1 of u-plane q and,
+// when q < nyv, of v-plane q; three complex transforms per row (pairs
+// (u1,u3), (u2i,v), (u1i,u3i)).  Per transform the LDS slots hold the
+// exchange [k1][q] (row stride Q+1), then the physical values in place, then
+// the forward exchange [m2][k1] (row stride P+1) and the spectrum in place.
+template <int L, int P, int Q, int R>
+__global__ void __launch_bounds__(rup64(3 * P * R)) k_xpass4(XNLArgs a, const double2* __restrict__ twg) {
+  static_assert(P * Q == L && Q <= P, "P*Q, Q <= P");
+  constexpr int Mx = Dealias<L>::Mx;
+  constexpr int NT = rup64(3 * P * R);
+  constexpr int SP = P * (Q + 1);  // >= Q * (P + 1) as Q <= P
+  __shared__ double2 S[R * 3 * SP];
+  __shared__ double red[NT / 64];
+  const int tid = threadIdx.x;
+  const int q = blockIdx.y, z0 = blockIdx.x * R, ngz = a.ngz;
+  const bool hasv = q < a.nyv;
+  const bool uint = q > 0 && q < a.nyu - 1;
+  const bool vint = hasv && q > 0 && q < a.nyv - 1;
+
+  // phase 1: tasks (qq, tr, r); inverse P-point DFTs over p of c[Q p + qq]
+  for (int task = tid; task < 3 * Q * R; task += NT) {
+    const int qq = task % Q, tr = (task / Q) % 3, r = task / (3 * Q);
+    const int z = z0 + r;
+    double2* s = S + (r * 3 + tr) * SP;
+    const double2 *A = nullptr, *B = nullptr;
+    if (z < ngz) {
+      const size_t ru = ((size_t)q * ngz + z) * Mx;
+      if (tr == 0) {
+        A = a.zi[0] + ru;
+        B = a.zi[2] + ru;
+      } else if (tr == 1) {
+        A = a.zi[1] + ru;
+        B = hasv ? a.zi[4] + ru : nullptr;
+      } else if (hasv) {
+        A = a.zi[3] + ru;
+        B = a.zi[5] + ru;
+      }
+    }
+    double2 v[P];
+    sfor<0, P>([&](auto pc) {
+      constexpr int p = decltype(pc)::value;
+      v[p] = pair_mode_q<L, Q, p>(A, B, Q * p + qq);
+    });
+    dft<true, P>(v);
+    s[qq] = v[0];
+    sfor<1, P>([&](auto kc) {
+      constexpr int k1 = decltype(kc)::value;
+      s[k1 * (Q + 1) + qq] = mul_tw<true>(v[k1], twg[qq * k1]);
+    });
+  }
+  __syncthreads();
+
+  // phase 2: task (k1, tr, r) per thread; inverse Q-point DFT -> physical
+  // values at x = k1 + P k2, kept in registers and written back in place
+  const bool act = tid < 3 * P * R;
+  const int k1 = tid % P, tr = (tid / P) % 3, r = tid / (3 * P);
+  const int rr = act ? r : 0;
+  double2* s = S + (rr * 3 + tr) * SP + k1 * (Q + 1);
+  double2 u[Q];
+  if (act) {
+    sfor<0, Q>([&](auto c) { u[decltype(c)::value] = s[decltype(c)::value]; });
+    dft<true, Q>(u);
+    sfor<0, Q>([&](auto c) { s[decltype(c)::value] = u[decltype(c)::value]; });
+  }
+  __syncthreads();
+
+  // products in physical space (ops_in_planes, FOU3D.f90:1529-1617) and the
+  // CFL maxima (dtc_calc, FOU3D.f90:516-544)
+  double m1 = 0.0, m2 = 0.0, m3 = 0.0;
+  const bool live = act && z0 + r < ngz;
+  if (act) {
+    const double dv = vint ? a.dthdyv[q] : 0.0;
+    const double2* sp = S + (rr * 3 + (tr > 0 ? tr - 1 : 0)) * SP + k1 * (Q + 1);  // transform tr-1
+    sfor<0, Q>([&](auto c) {
+      constexpr int k2 = decltype(c)::value;
+      const double2 x = u[k2];
+      if (tr == 0) {
+        u[k2] = make_double2(x.x * x.x, x.x * x.y);  // uu*uu, uu*ww
+        m1 = fmax(m1, dabs(x.x));
+        m3 = fmax(m3, dabs(x.y));
+      } else if (tr == 1) {
+        const double ww = sp[k2].y;
+        u[k2] = make_double2(x.x * x.x, ww * ww);  // vc*vc, ww*ww
+        m2 = fmax(m2, dabs(x.y) * dv);
+      } else {
+        const double vf = sp[k2].y;
+        u[k2] = make_double2(x.x * vf, x.y * vf);  // uf*vf, wf*vf
+      }
+    });
+    if (!live) m1 = m2 = m3 = 0.0;
+  }
+  if (a.cfl) {
+    m1 = block_max(m1, red);
+    if (tid == 0) atomic_max_pos(&a.sc->m1, m1);
+    m3 = block_max(m3, red);
+    if (tid == 0) atomic_max_pos(&a.sc->m3, m3);
+    m2 = block_max(m2, red);
+    if (tid == 0) atomic_max_pos(&a.sc->m2, m2);
+  }
+  __syncthreads();
+
+  // phase 3: forward Q-point DFT over k2 -> m2, twiddle w_L^{k1 m2}
+  double2* sf = S + (rr * 3 + tr) * SP;
+  if (act) {
+    dft<false, Q>(u);
+    sf[k1] = u[0];
+    sfor<1, Q>([&](auto c) {
+      constexpr int m2c = decltype(c)::value;
+      sf[m2c * (P + 1) + k1] = mul_tw<false>(u[m2c], twg[k1 * m2c]);
+    });
+  }
+  __syncthreads();
+
+  // phase 4: tasks (m2, tr, r); forward P-point DFTs over k1 -> F[m2 + Q m1]
+  for (int task = tid; task < 3 * Q * R; task += NT) {
+    const int m2 = task % Q, t4 = (task / Q) % 3, r4 = task / (3 * Q);
+    double2* s4 = S + (r4 * 3 + t4) * SP + m2 * (P + 1);
+    double2 v[P];
+    sfor<0, P>([&](auto c) { v[decltype(c)::value] = s4[decltype(c)::value]; });
+    dft<false, P>(v);
+    sfor<0, P>([&](auto c) { s4[decltype(c)::value] = v[decltype(c)::value]; });
+  }
+  __syncthreads();
+
+  // split the spectra of a + i b into the modes of a and b, scaled by 1/L
+  const double sc = 1.0 / L;
+  for (int task = tid; task < 3 * Mx * R; task += NT) {
+    const int m = task % Mx, t5 = (task / Mx) % 3, r5 = task / (3 * Mx);
+    const int z = z0 + r5;
+    if (z >= ngz) continue;
+    if (t5 < 2 ? !uint : !hasv) continue;
+    const double2* F = S + (r5 * 3 + t5) * SP;
+    const int mb = m == 0 ? 0 : L - m;
+    const double2 f = F[(m % Q) * (P + 1) + m / Q], g = F[(mb % Q) * (P + 1) + mb / Q];
+    const double2 ca = make_double2(0.5 * sc * (f.x + g.x), 0.5 * sc * (f.y - g.y));
+    const double2 cb = make_double2(0.5 * sc * (f.y + g.y), -0.5 * sc * (f.x - g.x));
+    const size_t ro = ((size_t)q * ngz + z) * Mx + m;
+    if (t5 == 0) {
+      a.zo[0][ro] = ca;  // uu
+      a.zo[1][ro] = cb;  // uw
+    } else if (t5 == 1) {
+      a.zo[3][ro] = ca;  // vv
+      a.zo[2][ro] = cb;  // ww
+    } else {
+      a.zo[4][ro] = ca;  // uv
+      a.zo[5][ro] = cb;  // vw
+    }
+  }
+}
+
 // raise the dynamic-LDS limit of a kernel once per size
 void set_lds(const void* kern, size_t bytes) {
   static std::vector<std::pair<const void*, size_t>> done;
@@ -462,6 +795,59 @@ void zfwdp_t(Solver& s, const ZFwdJob* jobs, int njobs) {
                      s.Mx);
 }
 
+// four-step specializations: (L, P, Q, TI) of the z kernels, (L, P, Q, R)
+// of the x-pass.  The z-inverse takes Q | N2/2 and the z-forward P | N2/2
+// where the length allows, so whole groups of the empty band drop out.
+#define CFD_ZINV4(X) X(48, 6, 8, 16) X(64, 8, 8, 16) X(192, 12, 16, 16) X(384, 24, 16, 8)
+#define CFD_ZFWD4(X) X(48, 8, 6, 16) X(64, 8, 8, 16) X(192, 16, 12, 16)
+#define CFD_XPASS4(X) X(48, 8, 6, 8) X(64, 8, 8, 8) X(72, 12, 6, 4) X(384, 24, 16, 2)
+
+bool four_step_z(const Solver& s) {
+  return !s.fft_stockham && s.N2 == 2 * (s.ngz / 3) && s.pz.L == s.ngz;
+}
+
+bool try_zinv4(Solver& s, const ZInvJob* jobs, int njobs) {
+  if (!four_step_z(s)) return false;
+  switch (s.pz.L) {
+#define C(L, P, Q, TI)                                                                                   \
+  case L:                                                                                                \
+    hipLaunchKernelGGL((k_zinv4<L, P, Q, TI>), dim3((s.Mx + TI - 1) / TI, njobs), dim3(TI * cmax(P, Q)), \
+                       0, s.stream, jobs, s.d_twz, s.Mx);                                               \
+    return true;
+    CFD_ZINV4(C)
+#undef C
+    default: return false;
+  }
+}
+
+bool try_zfwd4(Solver& s, const ZNLArgs& a) {
+  if (!four_step_z(s)) return false;
+  switch (s.pz.L) {
+#define C(L, P, Q, TI)                                                                                \
+  case L:                                                                                             \
+    hipLaunchKernelGGL((k_zfwd4<L, P, Q, TI>), dim3((s.Mx + TI - 1) / TI, s.nyu + s.nyv),            \
+                       dim3(TI * cmax(P, Q)), 0, s.stream, a, s.d_twz);                              \
+    return true;
+    CFD_ZFWD4(C)
+#undef C
+    default: return false;
+  }
+}
+
+bool try_xpass4(Solver& s, const XNLArgs& a) {
+  if (s.fft_stockham || s.px.L != s.ngx || s.N1 != 2 * (s.ngx / 3) || s.Mx != s.N1 / 2 + 1) return false;
+  switch (s.px.L) {
+#define C(L, P, Q, R)                                                                                    \
+  case L:                                                                                                \
+    hipLaunchKernelGGL((k_xpass4<L, P, Q, R>), dim3((s.ngz + R - 1) / R, s.nyu), dim3(rup64(3 * P * R)), \
+                       0, s.stream, a, s.d_twx);                                                        \
+    return true;
+    CFD_XPASS4(C)
+#undef C
+    default: return false;
+  }
+}
+
 // transform lengths with compile-time plans; other lengths use the runtime plan
 #define CFD_FFT_SIZES(X) X(48) X(64) X(72) X(96) X(128) X(192) X(256) X(384) X(768)
 
@@ -469,6 +855,7 @@ void zfwdp_t(Solver& s, const ZFwdJob* jobs, int njobs) {
 
 void launch_zinv(Solver& s, const ZInvJob* jobs, int njobs) {
   if (njobs <= 0) return;
+  if (try_zinv4(s, jobs, njobs)) return;
   switch (s.pz.L) {
 #define C(n) case n: zinv_t<n>(s, jobs, njobs); break;
     CFD_FFT_SIZES(C)
@@ -488,7 +875,9 @@ void launch_xpass_nl(Solver& s, int cfl) {
   a.nyu = s.nyu;
   a.nyv = s.nyv;
   a.Mx = s.Mx;
+  a.ngz = s.ngz;
   a.cfl = cfl;
+  if (try_xpass4(s, a)) return;
   switch (s.px.L) {
 #define C(n) case n: xpass_nl_t<n>(s, a); break;
     CFD_FFT_SIZES(C)
@@ -512,6 +901,7 @@ void launch_zfwd_nl(Solver& s) {
   a.nyv = s.nyv;
   a.N2 = s.N2;
   a.Mx = s.Mx;
+  if (try_zfwd4(s, a)) return;
   switch (s.pz.L) {
 #define C(n) case n: zfwd_nl_t<n>(s, a); break;
     CFD_FFT_SIZES(C)

@@ -15,6 +15,12 @@ SMALLPOSTS = Params(ngalx=72, ngalz=48, ny=33, dsty=3, ygrid=1, ytanh_beta=2.0, 
                     ntilez=4, extras24=0, flag_ctpress=0, Re=2800.0, alp=1.0, bet=2.0, posth=0.06, r=1.6,
                     c_i=6.0, c_k=6.0, stag=0.5, CFL=0.5)
 
+# the lateral grid and posts of the 384 x 192 x 192 workload on a short
+# channel (tanh grid, 16 x 8 staggered posts per wall)
+WIDE = Params(ngalx=384, ngalz=192, ny=24, dsty=3, ygrid=1, ytanh_beta=2.0, geometry_type=3, ntilex=16,
+              ntilez=8, extras24=0, flag_ctpress=0, Re=6887.0, alp=1.0, bet=2.0, posth=0.12, r=1.2378,
+              c_i=12.0, c_k=12.0, stag=0.5, CFL=0.5)
+
 SMOOTH = Params(ngalx=36, ngalz=24, ny=25, dsty=0, ygrid=0, geometry_type=0, ntilex=2, ntilez=2, Re=2000.0,
                 alp=1.0, bet=2.0, posth=0.0, CFL=0.5)
 

@@ -6,17 +6,19 @@ algorithms and FMA contraction on the GPU): relative L2 <= 1e-11 per field for
 one operator, <= 1e-10 per velocity component after 10 RK3 steps (the bar the
 project states for the GPU FP64 path).  IB lists compare bit-exactly.
 """
+import os
+
 import numpy as np
 import pytest
 
 from cfd_amd import Channel
 from oracle.oracle import Oracle
 
-from .helpers import SHIPPED, SMALLPOSTS, SMOOTH, random_field, rel_l2, max_rel
+from .helpers import SHIPPED, SMALLPOSTS, SMOOTH, WIDE, random_field, rel_l2, max_rel
 
 pytestmark = pytest.mark.gpu
 
-CASES = {"shipped": SHIPPED, "smallposts": SMALLPOSTS, "smooth": SMOOTH}
+CASES = {"shipped": SHIPPED, "smallposts": SMALLPOSTS, "smooth": SMOOTH, "wide": WIDE}
 STATE = ("u1", "u2", "u3", "p", "Nu1", "Nu2", "Nu3", "du1", "du2", "du3")
 TOL_OP = 1e-11
 
@@ -185,6 +187,51 @@ def test_fused_substep_equals_operator_sequence(case):
         x, y = a.get_field(n), b.get_field(n)
         if n.startswith("Nu"):
             x, y = x[:, 1:-1], y[:, 1:-1]
+        assert rel_l2(x, y) <= 1e-13, (n, rel_l2(x, y))
+    sa, sb = a.scalars(), b.scalars()
+    for k in ("t", "dt", "mpgx"):
+        assert abs(sa[k] - sb[k]) <= 1e-13 * abs(sb[k]), k
+
+
+def _channel(p, fft=None):
+    """A Channel whose FFT kernels are chosen by CFD_AMD_FFT at creation."""
+    if fft:
+        os.environ["CFD_AMD_FFT"] = fft
+    try:
+        return Channel(p)
+    finally:
+        os.environ.pop("CFD_AMD_FFT", None)
+
+
+def test_four_step_fft_equals_stockham_at_workload_size():
+    """The four-step register FFT kernels and the LDS Stockham kernels give
+    the same three substeps on the full 384 x 192 x 192 workload."""
+    from cfd_amd import ic, workloads
+    p, _ = workloads.get("rt395_384x192x192")
+    print("case rt395_384x192x192 four-step vs stockham", flush=True)
+    a = _channel(p)
+    b = _channel(p, "stockham")
+    f = ic.initial_fields(p, a.ygrid(), seed=3, amp=0.1)
+    f = ic.project(a, f)
+    q0 = ic.flowrate(f["u1"], a.ygrid(), p.ny)
+    for ch in (a, b):
+        for k in ("u1", "u2", "u3", "p"):
+            ch.set_field(k, f[k])
+        ch.set_scalars(t=0.0, dt=0.0, mpgx=0.0, QxT=q0)
+    a.nonlinear(1)
+    b.nonlinear(1)
+    for n in ("Nu1", "Nu2", "Nu3"):
+        x, y = a.get_field(n)[:, 1:-1], b.get_field(n)[:, 1:-1]
+        assert rel_l2(x, y) <= 1e-13, (n, rel_l2(x, y))
+    for ch in (a, b):
+        for k in ("u1", "u2", "u3", "p"):
+            ch.set_field(k, f[k])
+        ch.set_scalars(t=0.0, dt=0.0, mpgx=0.0, QxT=q0)
+    for kRK in (1, 2, 3):
+        a.substep(kRK)
+        b.substep(kRK)
+    for n in ("u1", "u2", "u3", "p"):
+        x, y = a.get_field(n), b.get_field(n)
         assert rel_l2(x, y) <= 1e-13, (n, rel_l2(x, y))
     sa, sb = a.scalars(), b.scalars()
     for k in ("t", "dt", "mpgx"):
