@@ -214,6 +214,185 @@ __global__ void __launch_bounds__(64) k_rhs_solve(FusedArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Segmented column solve.  The same rows and Helmholtz system as k_rhs_solve,
+// solved by substructuring so that a column is shared by S threads: rows are
+// cut into segments of SEG rows, each followed by one separator row.  Thread
+// (column, segment) builds r for its rows and eliminates its segment in
+// registers, giving x_q = Y_q + G_q x_R + H_q x_L in terms of the separator
+// values either side; each separator row then couples only its two
+// neighbouring separators, a tridiagonal system of S-1 unknowns per column
+// solved through LDS, after which every thread finishes its own rows.  No
+// factor scratch goes to HBM and a column's 200-odd rows are walked by ~15
+// threads instead of one.  Block (CB columns, SY >= S segments), grid
+// (ceil(Ncol/CB), 3); rows past the end of the column are identity rows.
+// ---------------------------------------------------------------------------
+// one row of the Helmholtz system of field f: lower, diagonal and upper
+// coefficients and right-hand side; rows past the column are identity rows
+struct RowSys {
+  double lo, dg, up;
+  double2 r;
+};
+
+struct RowCtx {
+  double k2x, k2z, kk, dt, beta, g0, g1;
+  const double* d2;
+  int n, col;
+  size_t N;
+};
+
+// row q of the segmented solve; um/u0 hold u(q-1), u(q) and are advanced;
+// interior rows store the new Nu
+__device__ __forceinline__ RowSys seg_row(const FusedArgs& a, const FieldPtrs& F, int f, const RowCtx& c, int q,
+                                          double2& um, double2& u0) {
+  RowSys s;
+  if (q >= c.n) {
+    s.lo = 0.0; s.dg = 1.0; s.up = 0.0; s.r = make_double2(0.0, 0.0);
+    return s;
+  }
+  double2 un = make_double2(0.0, 0.0);
+  if (q + 1 < c.n) un = F.u[(size_t)(q + 1) * c.N + c.col];
+  if (q == 0) {
+    s.lo = 0.0; s.dg = 1.0; s.up = c.g0; s.r = make_double2(0.0, 0.0);
+  } else if (q == c.n - 1) {
+    s.lo = c.g1; s.dg = 1.0; s.up = 0.0; s.r = make_double2(0.0, 0.0);
+  } else {
+    double2 up, nold, nx, fa, fb, pa, pb, nnew;
+    row_load(a, F, f, q, c.N, c.col, up, nold, nx, fa, fb, pa, pb);
+    s.r = row_r(a, f, q, c.col, c.k2x, c.k2z, c.kk, c.dt, um, u0, un, nold, nx, fa, fb, pa, pb, nnew);
+    F.nu[(size_t)q * c.N + c.col] = nnew;
+    if (F.ib[q]) s.r = F.r[(size_t)q * c.N + c.col];
+    const double* dq = c.d2 + 3 * q;
+    s.lo = -c.beta * dq[0];
+    s.dg = 1.0 - c.beta * (dq[1] + c.k2x + c.k2z);
+    s.up = -c.beta * dq[2];
+  }
+  um = u0;
+  u0 = un;
+  return s;
+}
+
+template <int SEG>
+__global__ void __launch_bounds__(256) k_rhs_seg(FusedArgs a) {
+  extern __shared__ double sm[];
+  const int CB = blockDim.x, NT = blockDim.x * blockDim.y;
+  const int cl = threadIdx.x, sg = threadIdx.y, me = sg * CB + cl;
+  const int f = blockIdx.y;
+  const int col = blockIdx.x * CB + cl;
+  const int n = (f == 1) ? a.nyv : a.nyu;
+  const int S = (n + SEG + 1) / (SEG + 1);
+  double2* Yb = reinterpret_cast<double2*>(sm);
+  double2* Ye = Yb + NT;
+  double2* Rs = Ye + NT;
+  double2* Xs = Rs + NT;
+  double* Gb = reinterpret_cast<double*>(Xs + NT);
+  double* Hb = Gb + NT;
+  double* Ge = Hb + NT;
+  double* He = Ge + NT;
+  double* As = He + NT;
+  double* Ds = As + NT;
+  double* Cs = Ds + NT;
+  const bool act = col < a.Ncol && sg < S;
+  const size_t N = a.Ncol;
+  const FieldPtrs F = field_ptrs(a, f);
+  const double2* u = F.u;
+  const int b = sg * (SEG + 1);
+  double2 y[SEG];
+  double g[SEG], h[SEG];
+  if (act) {
+    const int i = col % a.Mx, k = col / a.Mx;
+    RowCtx c;
+    c.k2x = a.k2x[i];
+    c.k2z = a.k2z[k];
+    c.kk = (f == 0) ? a.k1x[i] : a.k1z[k];
+    c.dt = a.sc->dt;
+    c.beta = a.bRK * c.dt / a.Re;
+    c.d2 = (f == 1) ? a.d2v : a.d2u;
+    c.g0 = (f == 1) ? 0.0 : a.g0;
+    c.g1 = (f == 1) ? 0.0 : a.g1;
+    c.n = n;
+    c.col = col;
+    c.N = N;
+    double2 um = b >= 1 ? u[(size_t)(b - 1) * N + col] : make_double2(0.0, 0.0);
+    double2 u0 = u[(size_t)b * N + col];
+    // forward elimination inside the segment
+    double2 yp = make_double2(0.0, 0.0);
+    double gp = 0.0, hp = 1.0;
+#pragma unroll
+    for (int j = 0; j < SEG; ++j) {
+      const RowSys rs = seg_row(a, F, f, c, b + j, um, u0);
+      const double inv = 1.0 / (rs.dg - rs.lo * gp);
+      y[j] = make_double2((rs.r.x - rs.lo * yp.x) * inv, (rs.r.y - rs.lo * yp.y) * inv);
+      g[j] = rs.up * inv;
+      h[j] = -rs.lo * hp * inv;
+      yp = y[j];
+      gp = g[j];
+      hp = h[j];
+    }
+    // separator row after the segment
+    if (sg < S - 1) {
+      const RowSys rs = seg_row(a, F, f, c, b + SEG, um, u0);
+      As[me] = rs.lo;
+      Ds[me] = rs.dg;
+      Cs[me] = rs.up;
+      Rs[me] = rs.r;
+    }
+    // back substitution inside the segment: x_q = Y_q + G_q x_R + H_q x_L
+    g[SEG - 1] = -g[SEG - 1];
+#pragma unroll
+    for (int j = SEG - 2; j >= 0; --j) {
+      const double gj = g[j];
+      y[j] = make_double2(y[j].x - gj * y[j + 1].x, y[j].y - gj * y[j + 1].y);
+      h[j] = h[j] - gj * h[j + 1];
+      g[j] = -gj * g[j + 1];
+    }
+    Yb[me] = y[0];
+    Gb[me] = g[0];
+    Hb[me] = h[0];
+    Ye[me] = y[SEG - 1];
+    Ge[me] = g[SEG - 1];
+    He[me] = h[SEG - 1];
+  }
+  __syncthreads();
+  // separators: tridiagonal in x_s, s = 0..S-2, one thread per column
+  if (sg == 0 && col < a.Ncol && S > 1) {
+    double cp = 0.0;
+    double2 x = make_double2(0.0, 0.0);
+    for (int s = 0; s < S - 1; ++s) {
+      const int A = s * CB + cl, B = (s + 1) * CB + cl;
+      const double as = As[A], cs = Cs[A];
+      const double lo = as * He[A];
+      const double dg = Ds[A] + as * Ge[A] + cs * Hb[B];
+      const double2 rh = make_double2(Rs[A].x - as * Ye[A].x - cs * Yb[B].x, Rs[A].y - as * Ye[A].y - cs * Yb[B].y);
+      const double den = 1.0 / (dg - lo * cp);
+      x = make_double2((rh.x - lo * x.x) * den, (rh.y - lo * x.y) * den);
+      cp = cs * Gb[B] * den;
+      Xs[A] = x;
+      Cs[A] = cp;  // reused as the modified upper coefficient
+    }
+    for (int s = S - 3; s >= 0; --s) {
+      const int A = s * CB + cl;
+      const double2 xn = Xs[A + CB];
+      x = make_double2(Xs[A].x - Cs[A] * xn.x, Xs[A].y - Cs[A] * xn.y);
+      Xs[A] = x;
+    }
+  }
+  __syncthreads();
+  if (act) {
+    const double2 xl = sg > 0 ? Xs[me - CB] : make_double2(0.0, 0.0);
+    const double2 xr = sg < S - 1 ? Xs[me] : make_double2(0.0, 0.0);
+    double2* uo = F.uo;
+#pragma unroll
+    for (int j = 0; j < SEG; ++j) {
+      const int q = b + j;
+      if (q < n)
+        uo[(size_t)q * N + col] =
+            make_double2(y[j].x + g[j] * xr.x + h[j] * xl.x, y[j].y + g[j] * xr.y + h[j] * xl.y);
+    }
+    if (sg < S - 1) uo[(size_t)(b + SEG) * N + col] = xr;
+  }
+}
+
 // predictor on the IB planes only: grid (ceil(Ncol/kTB), max IB planes, 3)
 __global__ void __launch_bounds__(256) k_ib_rhs(FusedArgs a, const int* __restrict__ planes0,
                                                 const int* __restrict__ planes1, const int* __restrict__ planes2,
@@ -477,10 +656,20 @@ void launch_ib_rhs(Solver& s, int kRK) {
                      (const int*)s.d_ibq[2], n0, n1, n2);
 }
 
+constexpr int kSeg = 13;  // rows per segment of the segmented solve
+
 void launch_rhs_solve(Solver& s, int kRK) {
   FusedArgs a;
   fused_args(s, kRK, &a);
-  hipLaunchKernelGGL(k_rhs_solve, dim3((s.Ncol + 63) / 64, 3), dim3(64), 0, s.stream, a);
+  const int sy = (s.nyu + kSeg + 1) / (kSeg + 1);  // segments of the longest column
+  if (sy <= 256) {
+    int cb = 1;
+    while (cb * 2 * sy <= 256) cb *= 2;
+    const size_t lds = sizeof(double) * 15 * (size_t)cb * sy;
+    hipLaunchKernelGGL(k_rhs_seg<kSeg>, dim3((s.Ncol + cb - 1) / cb, 3), dim3(cb, sy), lds, s.stream, a);
+  } else {
+    hipLaunchKernelGGL(k_rhs_solve, dim3((s.Ncol + 63) / 64, 3), dim3(64), 0, s.stream, a);
+  }
 }
 
 void launch_pressure(Solver& s, int kRK) {

@@ -24,6 +24,12 @@ WIDE = Params(ngalx=384, ngalz=192, ny=24, dsty=3, ygrid=1, ytanh_beta=2.0, geom
 SMOOTH = Params(ngalx=36, ngalz=24, ny=25, dsty=0, ygrid=0, geometry_type=0, ntilex=2, ntilez=2, Re=2000.0,
                 alp=1.0, bet=2.0, posth=0.0, CFL=0.5)
 
+# smooth walls with nyu = 28 = 2 x (13 + 1): the segmented column solve
+# places a separator on the top boundary row; 24-point transforms take the
+# runtime-plan FFT kernels
+EDGE = Params(ngalx=24, ngalz=24, ny=27, dsty=0, ygrid=0, geometry_type=0, ntilex=2, ntilez=2, Re=1500.0,
+              alp=1.0, bet=2.0, posth=0.0, CFL=0.5)
+
 FIELD_GRID = {"u1": "u", "u2": "v", "u3": "u", "p": "p", "Nu1": "u", "Nu2": "v", "Nu3": "u",
               "du1": "u", "du2": "v", "du3": "u", "psi": "p", "div": "p"}
 

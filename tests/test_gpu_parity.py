@@ -14,11 +14,11 @@ import pytest
 from cfd_amd import Channel
 from oracle.oracle import Oracle
 
-from .helpers import SHIPPED, SMALLPOSTS, SMOOTH, WIDE, random_field, rel_l2, max_rel
+from .helpers import EDGE, SHIPPED, SMALLPOSTS, SMOOTH, WIDE, random_field, rel_l2, max_rel
 
 pytestmark = pytest.mark.gpu
 
-CASES = {"shipped": SHIPPED, "smallposts": SMALLPOSTS, "smooth": SMOOTH, "wide": WIDE}
+CASES = {"shipped": SHIPPED, "smallposts": SMALLPOSTS, "smooth": SMOOTH, "wide": WIDE, "edge": EDGE}
 STATE = ("u1", "u2", "u3", "p", "Nu1", "Nu2", "Nu3", "du1", "du2", "du3")
 TOL_OP = 1e-11
 
@@ -169,7 +169,12 @@ def test_ten_steps(case):
 @pytest.mark.parametrize("case", list(CASES))
 def test_fused_substep_equals_operator_sequence(case):
     """cfd_substep (fused column sweeps) == the reference call sequence of the
-    individual entry points, on the GPU."""
+    individual entry points, on the GPU.  The fused sweep solves the same
+    Helmholtz systems by substructuring instead of one Thomas sweep, so the
+    two differ by the round-off of the elimination order: fields to 1e-12
+    relative L2; mpgx, which divides a flow-rate difference by a quantity
+    proportional to dt (flowrate_corr, littleharsh.f90:452-484), to
+    1e-13 * QxT / dt absolute."""
     p = CASES[case]
     a, _ = _pair(p, seed=60)
     b, _ = _pair(p, seed=60)
@@ -187,10 +192,11 @@ def test_fused_substep_equals_operator_sequence(case):
         x, y = a.get_field(n), b.get_field(n)
         if n.startswith("Nu"):
             x, y = x[:, 1:-1], y[:, 1:-1]
-        assert rel_l2(x, y) <= 1e-13, (n, rel_l2(x, y))
+        assert rel_l2(x, y) <= 1e-12, (n, rel_l2(x, y))
     sa, sb = a.scalars(), b.scalars()
-    for k in ("t", "dt", "mpgx"):
+    for k in ("t", "dt"):
         assert abs(sa[k] - sb[k]) <= 1e-13 * abs(sb[k]), k
+    assert abs(sa["mpgx"] - sb["mpgx"]) <= 1e-13 * abs(sb["QxT"]) / sb["dt"], (sa["mpgx"], sb["mpgx"])
 
 
 def _channel(p, fft=None):
