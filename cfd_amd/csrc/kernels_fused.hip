@@ -559,6 +559,208 @@ __global__ void __launch_bounds__(64) k_pressure(PressArgs a) {
   }
 }
 
+// Segmented pressure sweep: the system, outputs and order of k_pressure,
+// solved by substructuring as in k_rhs_seg (SEG p rows per thread plus one
+// separator row, separators through LDS).  Thread (column, segment) owns p
+// rows b..last, the u/w rows above them and the v faces b..last; the face
+// above its last row (the next segment's first face) is recomputed from that
+// segment's LDS record to close the divergence of its last row, with the old
+// v of that face read before the first barrier.
+template <int SEG>
+__global__ void __launch_bounds__(256) k_press_seg(PressArgs a) {
+  extern __shared__ double sm[];
+  const int CB = blockDim.x, NT = blockDim.x * blockDim.y;
+  const int cl = threadIdx.x, sg = threadIdx.y, me = sg * CB + cl;
+  const int col = blockIdx.x * CB + cl;
+  const int n = a.nyp;
+  const int S = (n + SEG + 1) / (SEG + 1);
+  double2* Yb = reinterpret_cast<double2*>(sm);
+  double2* Ye = Yb + NT;
+  double2* Rs = Ye + NT;
+  double2* Xs = Rs + NT;
+  double* Gb = reinterpret_cast<double*>(Xs + NT);
+  double* Hb = Gb + NT;
+  double* Ge = Hb + NT;
+  double* He = Ge + NT;
+  double* As = He + NT;
+  double* Ds = As + NT;
+  double* Cs = Ds + NT;
+  const bool act = col < a.Ncol && sg < S;
+  const size_t N = a.Ncol;
+  const int b = sg * (SEG + 1);
+  const int i = col % a.Mx, k = col / a.Mx;
+  double kx = 0.0, kz = 0.0;
+  double2 y[SEG];
+  double g[SEG], h[SEG];
+  double2 vnext = make_double2(0.0, 0.0);  // old v of the face above the separator
+  if (act) {
+    kx = a.k1x[i];
+    kz = a.k1z[k];
+    const double k2 = a.k2x[i] + a.k2z[k];
+    const double dtirk = a.sc->dti / a.apb;
+    double2 vq = a.u2[(size_t)b * N + col];  // old v at the face below row b
+    // row q: divergence of the old field as right-hand side
+    auto row = [&](int q, double& lo, double& dg, double& up, double2& rhs) {
+      if (q >= n) {
+        lo = 0.0; dg = 1.0; up = 0.0; rhs = make_double2(0.0, 0.0);
+        return;
+      }
+      const double2 U = a.u1[(size_t)(q + 1) * N + col], W = a.u3[(size_t)(q + 1) * N + col];
+      const double2 V1 = a.u2[(size_t)(q + 1) * N + col];
+      const double2 du = imulk(kx, U), dw = imulk(kz, W);
+      const double m = a.ddthetavi * a.dthdyu[q + 1];
+      const double2 dv = make_double2(du.x + dw.x + m * (V1.x - vq.x), du.y + dw.y + m * (V1.y - vq.y));
+      vq = V1;
+      rhs = make_double2(dtirk * dv.x, dtirk * dv.y);
+      lo = a.plo[q];
+      dg = a.pdg[q] + k2;
+      up = a.pup[q];
+      if (col == 0 && q == 0) {
+        rhs = make_double2(0.0, 0.0);
+        lo = 0.0;
+        dg = a.pin;
+        up = 0.0;
+      }
+    };
+    double2 yp = make_double2(0.0, 0.0);
+    double gp = 0.0, hp = 1.0;
+#pragma unroll
+    for (int j = 0; j < SEG; ++j) {
+      double lo, dg, up;
+      double2 r;
+      row(b + j, lo, dg, up, r);
+      const double inv = 1.0 / (dg - lo * gp);
+      y[j] = make_double2((r.x - lo * yp.x) * inv, (r.y - lo * yp.y) * inv);
+      g[j] = up * inv;
+      h[j] = -lo * hp * inv;
+      yp = y[j];
+      gp = g[j];
+      hp = h[j];
+    }
+    if (sg < S - 1) {
+      double lo, dg, up;
+      double2 r;
+      row(b + SEG, lo, dg, up, r);
+      vnext = vq;  // old v of face b+SEG+1
+      As[me] = lo;
+      Ds[me] = dg;
+      Cs[me] = up;
+      Rs[me] = r;
+    }
+    g[SEG - 1] = -g[SEG - 1];
+#pragma unroll
+    for (int j = SEG - 2; j >= 0; --j) {
+      const double gj = g[j];
+      y[j] = make_double2(y[j].x - gj * y[j + 1].x, y[j].y - gj * y[j + 1].y);
+      h[j] = h[j] - gj * h[j + 1];
+      g[j] = -gj * g[j + 1];
+    }
+    Yb[me] = y[0];
+    Gb[me] = g[0];
+    Hb[me] = h[0];
+    Ye[me] = y[SEG - 1];
+    Ge[me] = g[SEG - 1];
+    He[me] = h[SEG - 1];
+  }
+  __syncthreads();
+  if (sg == 0 && col < a.Ncol && S > 1) {
+    double cp = 0.0;
+    double2 x = make_double2(0.0, 0.0);
+    for (int s = 0; s < S - 1; ++s) {
+      const int A = s * CB + cl, B = (s + 1) * CB + cl;
+      const double as = As[A], cs = Cs[A];
+      const double lo = as * He[A];
+      const double dg = Ds[A] + as * Ge[A] + cs * Hb[B];
+      const double2 rh = make_double2(Rs[A].x - as * Ye[A].x - cs * Yb[B].x, Rs[A].y - as * Ye[A].y - cs * Yb[B].y);
+      const double den = 1.0 / (dg - lo * cp);
+      x = make_double2((rh.x - lo * x.x) * den, (rh.y - lo * x.y) * den);
+      cp = cs * Gb[B] * den;
+      Xs[A] = x;
+      Cs[A] = cp;
+    }
+    for (int s = S - 3; s >= 0; --s) {
+      const int A = s * CB + cl;
+      const double2 xn = Xs[A + CB];
+      x = make_double2(Xs[A].x - Cs[A] * xn.x, Xs[A].y - Cs[A] * xn.y);
+      Xs[A] = x;
+    }
+  }
+  __syncthreads();
+  if (!act || b >= n) return;
+  const double2 xl = sg > 0 ? Xs[me - CB] : make_double2(0.0, 0.0);
+  const double2 xr = sg < S - 1 ? Xs[me] : make_double2(0.0, 0.0);
+  const double dtrk = a.sc->dt * a.apb;
+  const double kxd = kx * dtrk, kzd = kz * dtrk;
+  const int nyu = n + 2;
+  double2 psm = xl;                             // psi of the row below
+  double2 unp = make_double2(0.0, 0.0), wnp = unp, vprev = unp;  // corrected u, w, v of the row below
+  // psi, p, projection of the u/w row above p row q and of face q, and the
+  // divergence of p row q-1 once both its faces are known
+  auto proc = [&](int q, double2 ps) {
+    a.psi[(size_t)q * N + col] = ps;
+    const double2 P = a.p[(size_t)q * N + col];
+    a.p[(size_t)q * N + col] = make_double2(P.x + ps.x, P.y + ps.y);
+    const double2 U = a.u1[(size_t)(q + 1) * N + col], W = a.u3[(size_t)(q + 1) * N + col];
+    const double2 gx = imulk(kxd, ps), gz = imulk(kzd, ps);
+    const double2 un = make_double2(U.x - gx.x, U.y - gx.y);
+    const double2 wn = make_double2(W.x - gz.x, W.y - gz.y);
+    a.u1[(size_t)(q + 1) * N + col] = un;
+    a.u3[(size_t)(q + 1) * N + col] = wn;
+    if (q == 0) {
+      a.u1[col] = make_double2(-a.g0 * un.x, -a.g0 * un.y);
+      a.u3[col] = make_double2(-a.g0 * wn.x, -a.g0 * wn.y);
+    }
+    if (q == n - 1) {
+      a.u1[(size_t)(nyu - 1) * N + col] = make_double2(-a.g1 * un.x, -a.g1 * un.y);
+      a.u3[(size_t)(nyu - 1) * N + col] = make_double2(-a.g1 * wn.x, -a.g1 * wn.y);
+    }
+    double2 vf = make_double2(0.0, 0.0);  // bottom wall face
+    if (q > 0) {
+      const double2 V = a.u2[(size_t)q * N + col];
+      const double cc = dtrk * a.ddthetavi * a.dthdyv[q];
+      vf = make_double2(V.x - cc * (ps.x - psm.x), V.y - cc * (ps.y - psm.y));
+    }
+    a.u2[(size_t)q * N + col] = vf;
+    if (q > b) {
+      const double2 du = imulk(kx, unp), dw = imulk(kz, wnp);
+      const double m = a.ddthetavi * a.dthdyu[q];
+      a.div[(size_t)(q - 1) * N + col] =
+          make_double2(du.x + dw.x + m * (vf.x - vprev.x), du.y + dw.y + m * (vf.y - vprev.y));
+    }
+    unp = un;
+    wnp = wn;
+    vprev = vf;
+    psm = ps;
+  };
+#pragma unroll
+  for (int j = 0; j < SEG; ++j) {
+    const int q = b + j;
+    if (q < n)
+      proc(q, make_double2(y[j].x + g[j] * xr.x + h[j] * xl.x, y[j].y + g[j] * xr.y + h[j] * xl.y));
+  }
+  int last;
+  if (sg < S - 1) {
+    last = b + SEG;
+    proc(last, xr);
+  } else {
+    last = n - 1;
+  }
+  // face above the last row: the top wall, or the next segment's first face
+  double2 vf = make_double2(0.0, 0.0);
+  if (last == n - 1) {
+    a.u2[(size_t)n * N + col] = vf;
+  } else {
+    const int B = me + CB;
+    const double2 xr2 = sg + 1 < S - 1 ? Xs[B] : make_double2(0.0, 0.0);
+    const double2 ps1 = make_double2(Yb[B].x + Gb[B] * xr2.x + Hb[B] * xr.x, Yb[B].y + Gb[B] * xr2.y + Hb[B] * xr.y);
+    const double cc = dtrk * a.ddthetavi * a.dthdyv[last + 1];
+    vf = make_double2(vnext.x - cc * (ps1.x - psm.x), vnext.y - cc * (ps1.y - psm.y));
+  }
+  const double2 du = imulk(kx, unp), dw = imulk(kz, wnp);
+  const double m = a.ddthetavi * a.dthdyu[last + 1];
+  a.div[(size_t)last * N + col] = make_double2(du.x + dw.x + m * (vf.x - vprev.x), du.y + dw.y + m * (vf.y - vprev.y));
+}
+
 // (0,0) flow-rate correction staged through LDS: the column, metrics and
 // scratch live in LDS; one thread runs the sequential recurrences and sums
 // in the reference order (flowrate_corr, littleharsh.f90:452-484).
@@ -689,7 +891,15 @@ void launch_pressure(Solver& s, int kRK) {
   a.Mx = s.Mx;
   a.Ncol = s.Ncol;
   a.nyp = s.nyp;
-  hipLaunchKernelGGL(k_pressure, dim3((s.Ncol + 63) / 64), dim3(64), 0, s.stream, a);
+  const int sy = (s.nyp + kSeg + 1) / (kSeg + 1);
+  if (sy <= 256) {
+    int cb = 1;
+    while (cb * 2 * sy <= 256) cb *= 2;
+    const size_t lds = sizeof(double) * 15 * (size_t)cb * sy;
+    hipLaunchKernelGGL(k_press_seg<kSeg>, dim3((s.Ncol + cb - 1) / cb), dim3(cb, sy), lds, s.stream, a);
+  } else {
+    hipLaunchKernelGGL(k_pressure, dim3((s.Ncol + 63) / 64), dim3(64), 0, s.stream, a);
+  }
 }
 
 void launch_meanflow_lds(Solver& s, int kRK) {
