@@ -580,29 +580,40 @@ __device__ __forceinline__ double2 pair_mode_q(const double2* __restrict__ A, co
   }
 }
 
-// nonlinear x-pass, four-step: grid (ceil(Ngz/R), nyu), rup64(3 P R)
-// threads.  Block (zb, q) takes rows z = R zb .. R zb + R-1 of u-plane q and,
-// when q < nyv, of v-plane q; three complex transforms per row (pairs
-// (u1,u3), (u2i,v), (u1i,u3i)).  Per transform the LDS slots hold the
-// exchange [k1][q] (row stride Q+1), then the physical values in place, then
-// the forward exchange [m2][k1] (row stride P+1) and the spectrum in place.
+// lanes per transform of the x-pass block: one group of waves per transform
+constexpr int xp_width(int P, int Q, int R) { return rup64(cmax(P * R, Q * R)); }
+
+__device__ inline double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  return v;
+}
+
+// nonlinear x-pass, four-step: grid (ceil(Ngz/R), nyu), 3 W threads with
+// W = xp_width(P, Q, R).  Block (zb, q) takes rows z = R zb .. R zb + R-1 of
+// u-plane q and, when q < nyv, of v-plane q; three complex transforms per row
+// (pairs (u1,u3), (u2i,v), (u1i,u3i)), transform tr handled by threads
+// [tr W, tr W + W) so every branch on tr is uniform across a wavefront.  Per
+// transform the LDS slots hold the exchange [k1][q] (row stride Q+1), then the
+// physical values in place, then the forward exchange [m2][k1] (row stride
+// P+1) and the spectrum in place.
 template <int L, int P, int Q, int R>
-__global__ void __launch_bounds__(rup64(3 * P * R)) k_xpass4(XNLArgs a, const double2* __restrict__ twg) {
+__global__ void __launch_bounds__(3 * xp_width(P, Q, R)) k_xpass4(XNLArgs a, const double2* __restrict__ twg) {
   static_assert(P * Q == L && Q <= P, "P*Q, Q <= P");
   constexpr int Mx = Dealias<L>::Mx;
-  constexpr int NT = rup64(3 * P * R);
+  constexpr int W = xp_width(P, Q, R);
   constexpr int SP = P * (Q + 1);  // >= Q * (P + 1) as Q <= P
   __shared__ double2 S[R * 3 * SP];
-  __shared__ double red[NT / 64];
   const int tid = threadIdx.x;
+  const int tr = tid / W, lane = tid - tr * W;
   const int q = blockIdx.y, z0 = blockIdx.x * R, ngz = a.ngz;
   const bool hasv = q < a.nyv;
   const bool uint = q > 0 && q < a.nyu - 1;
   const bool vint = hasv && q > 0 && q < a.nyv - 1;
 
-  // phase 1: tasks (qq, tr, r); inverse P-point DFTs over p of c[Q p + qq]
-  for (int task = tid; task < 3 * Q * R; task += NT) {
-    const int qq = task % Q, tr = (task / Q) % 3, r = task / (3 * Q);
+  // phase 1: tasks (qq, r) of transform tr; inverse P-point DFTs over p of
+  // c[Q p + qq]
+  for (int j = lane; j < Q * R; j += W) {
+    const int qq = j % Q, r = j / Q;
     const int z = z0 + r;
     double2* s = S + (r * 3 + tr) * SP;
     const double2 *A = nullptr, *B = nullptr;
@@ -633,12 +644,12 @@ __global__ void __launch_bounds__(rup64(3 * P * R)) k_xpass4(XNLArgs a, const do
   }
   __syncthreads();
 
-  // phase 2: task (k1, tr, r) per thread; inverse Q-point DFT -> physical
-  // values at x = k1 + P k2, kept in registers and written back in place
-  const bool act = tid < 3 * P * R;
-  const int k1 = tid % P, tr = (tid / P) % 3, r = tid / (3 * P);
-  const int rr = act ? r : 0;
-  double2* s = S + (rr * 3 + tr) * SP + k1 * (Q + 1);
+  // phase 2: task (k1, r) of transform tr per thread; inverse Q-point DFT ->
+  // physical values at x = k1 + P k2, kept in registers and written back in
+  // place for the products of the next transform
+  const bool act = lane < P * R;
+  const int k1 = lane % P, r = act ? lane / P : 0;
+  double2* s = S + (r * 3 + tr) * SP + k1 * (Q + 1);
   double2 u[Q];
   if (act) {
     sfor<0, Q>([&](auto c) { u[decltype(c)::value] = s[decltype(c)::value]; });
@@ -647,43 +658,58 @@ __global__ void __launch_bounds__(rup64(3 * P * R)) k_xpass4(XNLArgs a, const do
   }
   __syncthreads();
 
-  // products in physical space (ops_in_planes, FOU3D.f90:1529-1617) and the
-  // CFL maxima (dtc_calc, FOU3D.f90:516-544)
-  double m1 = 0.0, m2 = 0.0, m3 = 0.0;
-  const bool live = act && z0 + r < ngz;
+  // products in physical space (ops_in_planes, FOU3D.f90:1529-1617); on CFL
+  // substeps the maxima of |u|, |w| and |v| dtheta/dy (dtc_calc,
+  // FOU3D.f90:516-544)
+  double ma = 0.0, mb = 0.0;
   if (act) {
-    const double dv = vint ? a.dthdyv[q] : 0.0;
-    const double2* sp = S + (rr * 3 + (tr > 0 ? tr - 1 : 0)) * SP + k1 * (Q + 1);  // transform tr-1
-    sfor<0, Q>([&](auto c) {
-      constexpr int k2 = decltype(c)::value;
-      const double2 x = u[k2];
-      if (tr == 0) {
-        u[k2] = make_double2(x.x * x.x, x.x * x.y);  // uu*uu, uu*ww
-        m1 = fmax(m1, dabs(x.x));
-        m3 = fmax(m3, dabs(x.y));
-      } else if (tr == 1) {
-        const double ww = sp[k2].y;
-        u[k2] = make_double2(x.x * x.x, ww * ww);  // vc*vc, ww*ww
-        m2 = fmax(m2, dabs(x.y) * dv);
-      } else {
-        const double vf = sp[k2].y;
-        u[k2] = make_double2(x.x * vf, x.y * vf);  // uf*vf, wf*vf
-      }
-    });
-    if (!live) m1 = m2 = m3 = 0.0;
+    const double2* sp = S + (r * 3 + (tr > 0 ? tr - 1 : 0)) * SP + k1 * (Q + 1);  // transform tr-1
+    if (a.cfl) {
+      sfor<0, Q>([&](auto c) {
+        const double2 x = u[decltype(c)::value];
+        ma = fmax(ma, fabs(x.x));
+        mb = fmax(mb, fabs(x.y));
+      });
+    }
+    if (tr == 0) {
+      sfor<0, Q>([&](auto c) {
+        const double2 x = u[decltype(c)::value];
+        u[decltype(c)::value] = make_double2(x.x * x.x, x.x * x.y);  // uu*uu, uu*ww
+      });
+    } else if (tr == 1) {
+      sfor<0, Q>([&](auto c) {
+        const double2 x = u[decltype(c)::value];
+        const double ww = sp[decltype(c)::value].y;
+        u[decltype(c)::value] = make_double2(x.x * x.x, ww * ww);  // vc*vc, ww*ww
+      });
+    } else {
+      sfor<0, Q>([&](auto c) {
+        const double2 x = u[decltype(c)::value];
+        const double vf = sp[decltype(c)::value].y;
+        u[decltype(c)::value] = make_double2(x.x * vf, x.y * vf);  // uf*vf, wf*vf
+      });
+    }
   }
-  if (a.cfl) {
-    m1 = block_max(m1, red);
-    if (tid == 0) atomic_max_pos(&a.sc->m1, m1);
-    m3 = block_max(m3, red);
-    if (tid == 0) atomic_max_pos(&a.sc->m3, m3);
-    m2 = block_max(m2, red);
-    if (tid == 0) atomic_max_pos(&a.sc->m2, m2);
+  // W is a multiple of 64, so a.cfl and tr are uniform across each wavefront
+  // and every lane takes part in its reduction
+  if (a.cfl && tr < 2) {
+    if (!act || z0 + r >= ngz) ma = mb = 0.0;
+    if (tr == 1) mb *= vint ? a.dthdyv[q] : 0.0;
+    ma = wave_max(ma);
+    mb = wave_max(mb);
+    if ((lane & 63) == 0) {
+      if (tr == 0) {
+        atomic_max_pos(&a.sc->m1, ma);
+        atomic_max_pos(&a.sc->m3, mb);
+      } else {
+        atomic_max_pos(&a.sc->m2, mb);
+      }
+    }
   }
   __syncthreads();
 
   // phase 3: forward Q-point DFT over k2 -> m2, twiddle w_L^{k1 m2}
-  double2* sf = S + (rr * 3 + tr) * SP;
+  double2* sf = S + (r * 3 + tr) * SP;
   if (act) {
     dft<false, Q>(u);
     sf[k1] = u[0];
@@ -694,10 +720,11 @@ __global__ void __launch_bounds__(rup64(3 * P * R)) k_xpass4(XNLArgs a, const do
   }
   __syncthreads();
 
-  // phase 4: tasks (m2, tr, r); forward P-point DFTs over k1 -> F[m2 + Q m1]
-  for (int task = tid; task < 3 * Q * R; task += NT) {
-    const int m2 = task % Q, t4 = (task / Q) % 3, r4 = task / (3 * Q);
-    double2* s4 = S + (r4 * 3 + t4) * SP + m2 * (P + 1);
+  // phase 4: tasks (m2, r) of transform tr; forward P-point DFTs over k1 ->
+  // F[m2 + Q m1]
+  for (int j = lane; j < Q * R; j += W) {
+    const int m2 = j % Q, r4 = j / Q;
+    double2* s4 = S + (r4 * 3 + tr) * SP + m2 * (P + 1);
     double2 v[P];
     sfor<0, P>([&](auto c) { v[decltype(c)::value] = s4[decltype(c)::value]; });
     dft<false, P>(v);
@@ -706,28 +733,20 @@ __global__ void __launch_bounds__(rup64(3 * P * R)) k_xpass4(XNLArgs a, const do
   __syncthreads();
 
   // split the spectra of a + i b into the modes of a and b, scaled by 1/L
+  if (tr < 2 ? !uint : !hasv) return;
   const double sc = 1.0 / L;
-  for (int task = tid; task < 3 * Mx * R; task += NT) {
-    const int m = task % Mx, t5 = (task / Mx) % 3, r5 = task / (3 * Mx);
+  double2* oa = tr == 0 ? a.zo[0] : (tr == 1 ? a.zo[3] : a.zo[4]);  // uu, vv, uv
+  double2* ob = tr == 0 ? a.zo[1] : (tr == 1 ? a.zo[2] : a.zo[5]);  // uw, ww, vw
+  for (int j = lane; j < Mx * R; j += W) {
+    const int m = j % Mx, r5 = j / Mx;
     const int z = z0 + r5;
     if (z >= ngz) continue;
-    if (t5 < 2 ? !uint : !hasv) continue;
-    const double2* F = S + (r5 * 3 + t5) * SP;
+    const double2* F = S + (r5 * 3 + tr) * SP;
     const int mb = m == 0 ? 0 : L - m;
     const double2 f = F[(m % Q) * (P + 1) + m / Q], g = F[(mb % Q) * (P + 1) + mb / Q];
-    const double2 ca = make_double2(0.5 * sc * (f.x + g.x), 0.5 * sc * (f.y - g.y));
-    const double2 cb = make_double2(0.5 * sc * (f.y + g.y), -0.5 * sc * (f.x - g.x));
     const size_t ro = ((size_t)q * ngz + z) * Mx + m;
-    if (t5 == 0) {
-      a.zo[0][ro] = ca;  // uu
-      a.zo[1][ro] = cb;  // uw
-    } else if (t5 == 1) {
-      a.zo[3][ro] = ca;  // vv
-      a.zo[2][ro] = cb;  // ww
-    } else {
-      a.zo[4][ro] = ca;  // uv
-      a.zo[5][ro] = cb;  // vw
-    }
+    oa[ro] = make_double2(0.5 * sc * (f.x + g.x), 0.5 * sc * (f.y - g.y));
+    ob[ro] = make_double2(0.5 * sc * (f.y + g.y), -0.5 * sc * (f.x - g.x));
   }
 }
 
@@ -839,8 +858,8 @@ bool try_xpass4(Solver& s, const XNLArgs& a) {
   switch (s.px.L) {
 #define C(L, P, Q, R)                                                                                    \
   case L:                                                                                                \
-    hipLaunchKernelGGL((k_xpass4<L, P, Q, R>), dim3((s.ngz + R - 1) / R, s.nyu), dim3(rup64(3 * P * R)), \
-                       0, s.stream, a, s.d_twx);                                                        \
+    hipLaunchKernelGGL((k_xpass4<L, P, Q, R>), dim3((s.ngz + R - 1) / R, s.nyu),                     \
+                       dim3(3 * xp_width(P, Q, R)), 0, s.stream, a, s.d_twx);                          \
     return true;
     CFD_XPASS4(C)
 #undef C
