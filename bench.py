@@ -4,12 +4,18 @@ A step is one RK3 substep of the rough-channel DNS (RHS, nonlinear term with
 its FFTs, IB forcing, the three implicit y-solves, flow-rate correction,
 pressure solve and projection) over the whole device-resident field.  The
 workload is Re_tau = 395 on 384 x 192 x 192 (x, y, z) with 16 x 8 posts per
-wall, FP64.  Fields are generated on the host (seeded), projected and uploaded
+wall.  Fields are generated on the host (seeded), projected and uploaded
 before timing; the timed region runs on device-resident data only.
+
+`value` is the FP64 path.  The same run times the mixed path (precision 1:
+FP32 transforms, products and nonlinear scratch; FP64 fields, solves,
+projection and flow-rate correction) over the same substeps from the same
+field and reports its throughput and its error against the FP64 run.
 
 Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
 """
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -23,50 +29,35 @@ sys.path.insert(0, ROOT)
 PEAK_HBM_GBS = 8000.0
 
 
-def xpass_bytes(ch):
-    """Algorithmic bytes of one x-pass launch: each Z plane row read or
-    written once (3 u-plane + 3 v-plane inputs; 4 interior u-plane + 2
-    v-plane product outputs), double2 elements."""
-    zrow = ch.params.ngalz * ch.Mx * 16
-    return zrow * (3 * ch.nyu + 3 * ch.nyv + 4 * (ch.nyu - 2) + 2 * ch.nyv)
-
-
-def class_bytes(ch):
-    """Algorithmic bytes per launch of each kernel class."""
-    S = lambda ny: ny * ch.Ncol * 16  # noqa: E731
-    Z = lambda ny: ny * ch.params.ngalz * ch.Mx * 16  # noqa: E731
-    nyu, nyv, nyp = ch.nyu, ch.nyv, ch.nyp
+def class_bytes(ch, zb):
+    """Algorithmic bytes per launch of the three FFT kernel classes: every
+    element of a kernel's inputs and outputs counted once.  S = one spectral
+    plane stack (Ncol complex per plane), Z = one half-transformed stack (Ngz x
+    Mx complex per plane); spectral velocities are FP64 (16 B), Z and
+    nonlinear scratch elements are zb bytes (16 FP64, 8 mixed)."""
+    S = lambda ny, b=16: ny * ch.Ncol * b  # noqa: E731
+    Z = lambda ny: ny * ch.params.ngalz * ch.Mx * zb  # noqa: E731
+    nyu, nyv = ch.nyu, ch.nyv
     return {
-        # 6 jobs families: read spectral planes once (u1,u3,v + interpolation
-        # reuse), write 6 Z stacks
+        # read u1, u3 (u planes) and v (v planes) once, write 6 Z stacks
+        # (u1, u2 interpolated, u3 on u planes; u1, u3 interpolated, v on v planes)
         "zinv": 2 * S(nyu) + S(nyv) + 3 * Z(nyu) + 3 * Z(nyv),
-        "xpass": xpass_bytes(ch),
-        # read 4 interior u-plane + 2 v-plane product stacks, write 6 spectral
-        "zfwd": 4 * Z(nyu - 2) + 2 * Z(nyv) + 3 * S(nyu - 2) + 2 * S(nyv) + S(nyv - 2),
+        # read the 6 input stacks, write 4 interior u-plane + 2 v-plane product stacks
+        "xpass": 3 * Z(nyu) + 3 * Z(nyv) + 4 * Z(nyu - 2) + 2 * Z(nyv),
+        # read 4 interior u-plane + 2 v-plane product stacks, write nxz1, nxz3,
+        # vv (interior u planes), uv, vw (v planes) and nxz2 (interior v planes)
+        "zfwd": 4 * Z(nyu - 2) + 2 * Z(nyv) + 3 * S(nyu - 2, zb) + 2 * S(nyv, zb) + S(nyv - 2, zb),
     }
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30, help="timed RK3 substeps")
-    ap.add_argument("--warmup", type=int, default=6, help="untimed RK3 substeps")
-    ap.add_argument("--workload", default="rt395_384x192x192")
-    ap.add_argument("--cpu-substeps", type=int, default=3, help="oracle substeps for the CPU baseline (0: skip)")
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs the substep on one GPU; --gpus %d is not supported" % args.gpus,
-              file=sys.stderr)
-        sys.exit(2)
+def timed(params, f, q0, args, nprof):
+    """Create a Channel, load the field, run warmup + timed + profiled
+    substeps; returns (substeps/s, ms per substep, per-class profile, final state)."""
+    from cfd_amd import Channel
 
-    from cfd_amd import Channel, ic, workloads
-
-    params, _ = workloads.get(args.workload)
     ch = Channel(params)
-    g = ch.ygrid()
-    f = ic.initial_fields(params, g, seed=0, amp=0.1)
-    f = ic.project(ch, f)
-    q0 = ic.flowrate(f["u1"], g, params.ny)
+    for k in ("u1", "u2", "u3", "p"):
+        ch.set_field(k, f[k])
     ch.set_scalars(t=0.0, dt=0.0, mpgx=0.0, QxT=q0)
     k = [0]
 
@@ -82,27 +73,79 @@ def main():
     ch.sync()
     t1 = time.perf_counter()
     dt_wall = t1 - t0
-    value = args.steps / dt_wall
-
     # per-kernel-class durations with HIP events on the solver's stream
-    nprof = 6
     ch.profile(True)
     run(nprof)
     prof = ch.profile_read()
     ch.profile(False)
-    cb = class_bytes(ch)
+    sc = ch.scalars()
+    state = {key: ch.get_field(key) for key in ("u1", "u2", "u3")}
+    state["mpgx"] = sc["mpgx"]
+    state["scalars"] = sc
+    return ch, args.steps / dt_wall, 1e3 * dt_wall / args.steps, prof, state
+
+
+def roofline(ch, prof, zb, workload, label):
+    cb = class_bytes(ch, zb)
     per_launch = {c: (ms / cnt if cnt else 0.0) for c, (ms, cnt) in prof.items()}
     dom = max(("zinv", "xpass", "zfwd"), key=lambda c: prof[c][0])
     avg_ms = per_launch[dom]
     achieved = cb[dom] / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_%s_%s.json" % (dom, args.workload))
+    tf = os.path.join(ROOT, "profiles", "traffic_%s_%s%s.json" % (dom, workload, label))
     if os.path.exists(tf):
         with open(tf) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
-    sc = ch.scalars()
-    u1 = ch.get_field("u1")
-    finite = bool(np.isfinite(u1).all())
+    return {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": achieved / PEAK_HBM_GBS, "traffic": traffic, "avg_launch_ms": avg_ms,
+            "algorithmic_bytes": cb[dom]}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30, help="timed RK3 substeps")
+    ap.add_argument("--warmup", type=int, default=6, help="untimed RK3 substeps")
+    ap.add_argument("--workload", default="rt395_384x192x192")
+    ap.add_argument("--cpu-substeps", type=int, default=3, help="oracle substeps for the CPU baseline (0: skip)")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs the substep on one GPU; --gpus %d is not supported" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    from cfd_amd import Channel, diagnostics, ic, workloads
+
+    params, _ = workloads.get(args.workload)
+    probe = Channel(params)
+    g = probe.ygrid()
+    f = ic.initial_fields(params, g, seed=0, amp=0.1)
+    f = ic.project(probe, f)
+    probe.close()
+    q0 = ic.flowrate(f["u1"], g, params.ny)
+    nprof = 6
+
+    ch, value, ms_step, prof, st64 = timed(params, f, q0, args, nprof)
+    rl = roofline(ch, prof, 16, args.workload, "")
+    sc = st64["scalars"]
+    finite = bool(np.isfinite(st64["u1"]).all())
+    kms = {c: ms / nprof for c, (ms, cnt) in prof.items()}
+    ch.close()
+
+    mixed = None
+    if not args.no_mixed:
+        pm = dataclasses.replace(params, precision=1)
+        chm, vm, msm, profm, stm = timed(pm, f, q0, args, nprof)
+        mixed = {
+            "value": vm, "unit": "substeps/s", "ms_per_step": msm, "speedup_vs_fp64": vm / value,
+            "dtype": "f32 transforms/products/nonlinear scratch; f64 fields, solves, projection, flow rate",
+            "roofline": roofline(chm, profm, 8, args.workload, "_mixed"),
+            "kernel_ms_per_substep": {c: ms / nprof for c, (ms, cnt) in profm.items()},
+            "err_vs_fp64": dict(diagnostics.compare(stm, st64, g, params.Re),
+                                substeps=args.warmup + args.steps + nprof),
+        }
+        chm.close()
 
     # CPU baseline: the FP64 oracle on the host cores, a bounded sample
     cpu = None
@@ -130,7 +173,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * dt_wall / args.steps,
+        "ms_per_step": ms_step,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -139,15 +182,13 @@ def main():
         "config": {"workload": args.workload, "grid": [params.ngalx, params.ny, params.ngalz],
                    "dsty": params.dsty, "posts_per_wall": [params.ntilex, params.ntilez], "Re_bulk": params.Re,
                    "precision": "fp64", "parallelism": "single-gpu"},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic, "avg_launch_ms": avg_ms,
-                     "algorithmic_bytes": cb[dom]},
-        "kernel_ms_per_substep": {c: ms / nprof for c, (ms, cnt) in prof.items()},
+        "roofline": rl,
+        "kernel_ms_per_substep": kms,
+        "mixed": mixed,
         "cpu_baseline": cpu,
         "sim": {"t": sc["t"], "dt": sc["dt"], "mpgx": sc["mpgx"], "finite": finite},
     }
     print(json.dumps(out))
-    ch.close()
 
 
 if __name__ == "__main__":

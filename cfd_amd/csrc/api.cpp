@@ -83,6 +83,23 @@ std::vector<double2> twiddles(int L) {
   return w;
 }
 
+std::vector<float2> twiddles_f(int L) {
+  std::vector<float2> w(L);
+  for (int m = 0; m < L; ++m) {
+    const long double a = -2.0L * 3.14159265358979323846264338327950288L * m / (long double)L;
+    w[m] = make_float2((float)cosl(a), (float)sinl(a));
+  }
+  return w;
+}
+
+// untyped device buffer of n elements of esz bytes, zeroed
+int balloc(cfd_solver* s, void** p, size_t n, size_t esz) {
+  char* q = nullptr;
+  if (int rc = dalloc(s, &q, (n ? n : 1) * esz)) return rc;
+  *p = q;
+  return 0;
+}
+
 int grid_of(int which) {
   static const int g[CFD_NFIELDS] = {0, 1, 0, 2, 0, 1, 0, 0, 1, 0, 2, 2};
   return g[which];
@@ -133,7 +150,10 @@ extern "C" int cfd_create(const cfd_params* pp, cfd_handle* out) {
   if (p.ngalx % 2 || p.ngalz % 2) return fail(nullptr, CFD_EINVAL, "Ngalx and Ngalz must be even");
   if (p.ngalx > 1024 || p.ngalz > 768)
     return fail(nullptr, CFD_EINVAL, "Ngalx <= 1024 and Ngalz <= 768 (LDS-resident transforms)");
-  if (p.precision != 0) return fail(nullptr, CFD_EINVAL, "only precision = 0 (FP64) is available");
+  if (p.precision != 0 && p.precision != 1)
+    return fail(nullptr, CFD_EINVAL, "precision must be 0 (FP64) or 1 (mixed: FP32 nonlinear term)");
+  if (p.precision == 1 && !(cfd::fft4_x_supported(p.ngalx) && cfd::fft4_z_supported(p.ngalz)))
+    return fail(nullptr, CFD_EINVAL, "the mixed path needs Ngalx and Ngalz with four-step FFT kernels");
   if (p.Re <= 0 || p.alp <= 0 || p.bet <= 0) return fail(nullptr, CFD_EINVAL, "Re, alp, bet must be positive");
   cfd_solver* s = new (std::nothrow) cfd_solver();
   if (!s) return fail(nullptr, CFD_ENOMEM, "host allocation failed");
@@ -215,8 +235,12 @@ extern "C" int cfd_create(const cfd_params* pp, cfd_handle* out) {
     const char* e = getenv("CFD_AMD_FFT");
     s->fft_stockham = e && std::string(e) == "stockham";
   }
+  s->mixed = p.precision == 1;
+  if (s->mixed) s->fft_stockham = false;
   int rc;
-  if ((rc = upload(s, &s->d_twz, twiddles(s->ngz))) || (rc = upload(s, &s->d_twx, twiddles(s->ngx)))) return bail(rc);
+  if ((rc = upload(s, &s->d_twz, twiddles(s->ngz))) || (rc = upload(s, &s->d_twx, twiddles(s->ngx))) ||
+      (rc = upload(s, &s->d_twzf, twiddles_f(s->ngz))) || (rc = upload(s, &s->d_twxf, twiddles_f(s->ngx))))
+    return bail(rc);
 
   // metrics
   std::vector<double> plo(s->nyp), pup(s->nyp), pdg(s->nyp);
@@ -258,9 +282,10 @@ extern "C" int cfd_create(const cfd_params* pp, cfd_handle* out) {
   const size_t N = s->Ncol;
   for (int f = 0; f < CFD_NFIELDS; ++f)
     if ((rc = dalloc(s, &s->F[f], (size_t)ny_of(*s, f) * N))) return bail(rc);
-  if ((rc = dalloc(s, &s->nxz1, s->nyu * N)) || (rc = dalloc(s, &s->nxz3, s->nyu * N)) ||
-      (rc = dalloc(s, &s->fvv, s->nyu * N)) || (rc = dalloc(s, &s->nxz2, s->nyv * N)) ||
-      (rc = dalloc(s, &s->fuv, s->nyv * N)) || (rc = dalloc(s, &s->fvw, s->nyv * N)) ||
+  const size_t esz = s->mixed ? sizeof(float2) : sizeof(double2);  // nonlinear-term storage
+  if ((rc = balloc(s, &s->nxz1, s->nyu * N, esz)) || (rc = balloc(s, &s->nxz3, s->nyu * N, esz)) ||
+      (rc = balloc(s, &s->fvv, s->nyu * N, esz)) || (rc = balloc(s, &s->nxz2, s->nyv * N, esz)) ||
+      (rc = balloc(s, &s->fuv, s->nyv * N, esz)) || (rc = balloc(s, &s->fvw, s->nyv * N, esz)) ||
       (rc = dalloc(s, &s->cscr, (size_t)3 * s->nyu * N)))
     return bail(rc);
   for (int f = 0; f < 3; ++f)
@@ -269,12 +294,12 @@ extern "C" int cfd_create(const cfd_params* pp, cfd_handle* out) {
   const int zplanes[12] = {s->nyu, s->nyu, s->nyu, s->nyv, s->nyv, s->nyv,
                            s->nyu, s->nyu, s->nyu, s->nyu, s->nyv, s->nyv};
   for (int q = 0; q < 12; ++q)
-    if ((rc = dalloc(s, &s->Z[q], zplanes[q] * zp))) return bail(rc);
+    if ((rc = balloc(s, &s->Z[q], zplanes[q] * zp, esz))) return bail(rc);
 
   // nonlinear z-inverse jobs (interp_u / interp_v folded in, FOU3D.f90:261-308)
   {
     std::vector<ZInvJob> J;
-    auto job = [&](const double2* src, double2* dst, int pd, int mode, int pa, int pb, int pc, double wa, double wb,
+    auto job = [&](const double2* src, void* dst, int pd, int mode, int pa, int pb, int pc, double wa, double wb,
                    double gw) {
       ZInvJob z{};
       z.src = src;
@@ -623,7 +648,7 @@ int launched(cfd_solver* s) {
 
 void op_nonlinear(cfd_solver* s, int kRK) {
   if (kRK == 1) launch_reset_max(*s);
-  { Timed t(s, KC_ZINV); launch_zinv(*s, s->d_nljobs, s->n_nljobs); }
+  { Timed t(s, KC_ZINV); launch_zinv(*s, s->d_nljobs, s->n_nljobs, s->mixed); }
   { Timed t(s, KC_XPASS); launch_xpass_nl(*s, kRK == 1); }
   { Timed t(s, KC_ZFWD); launch_zfwd_nl(*s); }
   if (kRK == 1) launch_cfl(*s);
@@ -657,7 +682,7 @@ void op_solve(cfd_solver* s, int f, int kRK) {
 // du holds r only on the IB planes afterwards.
 void op_substep(cfd_solver* s, int kRK) {
   if (kRK == 1) launch_reset_max(*s);
-  { Timed t(s, KC_ZINV); launch_zinv(*s, s->d_nljobs, s->n_nljobs); }
+  { Timed t(s, KC_ZINV); launch_zinv(*s, s->d_nljobs, s->n_nljobs, s->mixed); }
   { Timed t(s, KC_XPASS); launch_xpass_nl(*s, kRK == 1); }
   { Timed t(s, KC_ZFWD); launch_zfwd_nl(*s); }
   if (kRK == 1) launch_cfl(*s);

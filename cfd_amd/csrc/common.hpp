@@ -38,7 +38,7 @@ struct FftPlan {
 //   mode 2: t = wa*S[pa] + wb*S[pb]; v = g*(S[pc] - t) + t
 struct ZInvJob {
   const double2* src;
-  double2* dst;
+  void* dst;  // double2, or float2 for the nonlinear planes of the mixed path
   int pa, pb, pc, pd;
   int mode, pad;
   double wa, wb, g;
@@ -106,11 +106,15 @@ struct Solver {
 
   // device fields
   double2* F[CFD_NFIELDS] = {};
-  // nonlinear scratch: x/z parts and fluxes (spectral)
-  double2 *nxz1 = nullptr, *nxz2 = nullptr, *nxz3 = nullptr, *fuv = nullptr, *fvw = nullptr, *fvv = nullptr;
+  // precision of the nonlinear term: false = FP64 everywhere; true = the
+  // mixed path, whose Z planes, transforms, products and nonlinear scratch
+  // (nxz*, fluxes) are FP32 (float2) while fields and solves stay FP64
+  bool mixed = false;
+  // nonlinear scratch: x/z parts and fluxes (spectral), double2 or float2
+  void *nxz1 = nullptr, *nxz2 = nullptr, *nxz3 = nullptr, *fuv = nullptr, *fvw = nullptr, *fvv = nullptr;
   // Z buffers: inverse inputs 0..5 (u1,u2i,u3 on u planes; u1i,v,u3i on v planes)
-  // and products 6..11 (uu,uw,ww,vv on u planes; uv,vw on v planes)
-  double2* Z[12] = {};
+  // and products 6..11 (uu,uw,ww,vv on u planes; uv,vw on v planes), double2 or float2
+  void* Z[12] = {};
   double* cscr = nullptr;  // per-column factor scratch for the y solves (3 fields)
   double2* ysc[3] = {};    // forward-sweep scratch of the fused solves
   unsigned char* d_ibmask[3] = {};  // 1 on the IB planes of field f
@@ -121,6 +125,7 @@ struct Solver {
   FftPlan pz{}, px{};
   bool fft_stockham = false;  // force the LDS Stockham kernels (CFD_AMD_FFT=stockham)
   double2 *d_twz = nullptr, *d_twx = nullptr;
+  float2 *d_twzf = nullptr, *d_twxf = nullptr;  // the same tables rounded to FP32
 
   // job tables
   ZInvJob* d_nljobs = nullptr;
@@ -149,7 +154,10 @@ struct Solver {
 };
 
 // launchers (kernels_fft.hip)
-void launch_zinv(Solver& s, const ZInvJob* jobs, int njobs);
+void launch_zinv(Solver& s, const ZInvJob* jobs, int njobs, bool f32 = false);
+// transform lengths served by the four-step kernels (the FP32 path needs them)
+bool fft4_z_supported(int L);
+bool fft4_x_supported(int L);
 void launch_xpass_nl(Solver& s, int cfl);
 void launch_zfwd_nl(Solver& s);
 void launch_xpass_inv2(Solver& s, const XJob* jobs, int njobs);

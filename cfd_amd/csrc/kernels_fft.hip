@@ -97,7 +97,7 @@ __global__ void __launch_bounds__(kThreads) k_zinv(const ZInvJob* __restrict__ j
     const int z = w / TI, t = w - z * TI;
     const int i = i0 + t;
     if (i >= Mx) continue;
-    J.dst[((size_t)J.pd * L + z) * Mx + i] = R[t * st + ix<LC>(z)];
+    ((double2*)J.dst)[((size_t)J.pd * L + z) * Mx + i] = R[t * st + ix<LC>(z)];
   }
 }
 
@@ -156,8 +156,8 @@ __device__ inline double block_max(double v, double* red) {
 // Z[6..9]: uu, uw, ww, vv (u planes)   Z[10..11]: uv, vw (v planes)
 // ---------------------------------------------------------------------------
 struct XNLArgs {
-  const double2* zi[6];
-  double2* zo[6];
+  const void* zi[6];  // double2 (FP64) or float2 (mixed) Z planes
+  void* zo[6];
   const double* dthdyv;
   DevScalars* sc;
   int nyu, nyv, Mx, ngz, cfl;
@@ -179,12 +179,12 @@ __global__ void __launch_bounds__(kThreads) k_xpass_nl(XNLArgs a, FftPlan P, con
   const bool vint = hasv && q > 0 && q < a.nyv - 1;
   load_twiddles(tw, twg, L);
   const size_t ru = ((size_t)q * ngz + z) * Mx;
-  const double2* u1 = a.zi[0] + ru;
-  const double2* u2i = a.zi[1] + ru;
-  const double2* u3 = a.zi[2] + ru;
-  const double2* u1i = hasv ? a.zi[3] + ru : nullptr;
-  const double2* v = hasv ? a.zi[4] + ru : nullptr;
-  const double2* u3i = hasv ? a.zi[5] + ru : nullptr;
+  const double2* u1 = (const double2*)a.zi[0] + ru;
+  const double2* u2i = (const double2*)a.zi[1] + ru;
+  const double2* u3 = (const double2*)a.zi[2] + ru;
+  const double2* u1i = hasv ? (const double2*)a.zi[3] + ru : nullptr;
+  const double2* v = hasv ? (const double2*)a.zi[4] + ru : nullptr;
+  const double2* u3i = hasv ? (const double2*)a.zi[5] + ru : nullptr;
   for (int m = threadIdx.x; m < L; m += blockDim.x) {
     A[ix<LC>(m)] = pair_mode(u1, u3, m, L, Mx);
     A[st + ix<LC>(m)] = pair_mode(u2i, v, m, L, Mx);
@@ -222,16 +222,16 @@ __global__ void __launch_bounds__(kThreads) k_xpass_nl(XNLArgs a, FftPlan P, con
     double2 ca, cb;
     if (uint) {
       split_mode<LC>(F, m, L, sc, ca, cb);
-      a.zo[0][ru + m] = ca;  // uu
-      a.zo[1][ru + m] = cb;  // uw
+      ((double2*)a.zo[0])[ru + m] = ca;  // uu
+      ((double2*)a.zo[1])[ru + m] = cb;  // uw
       split_mode<LC>(F + st, m, L, sc, ca, cb);
-      a.zo[3][ru + m] = ca;  // vv
-      a.zo[2][ru + m] = cb;  // ww
+      ((double2*)a.zo[3])[ru + m] = ca;  // vv
+      ((double2*)a.zo[2])[ru + m] = cb;  // ww
     }
     if (hasv) {
       split_mode<LC>(F + 2 * st, m, L, sc, ca, cb);
-      a.zo[4][ru + m] = ca;  // uv
-      a.zo[5][ru + m] = cb;  // vw
+      ((double2*)a.zo[4])[ru + m] = ca;  // uv
+      ((double2*)a.zo[5])[ru + m] = cb;  // vw
     }
   }
 }
@@ -241,8 +241,8 @@ __global__ void __launch_bounds__(kThreads) k_xpass_nl(XNLArgs a, FftPlan P, con
 // grid (ceil(Mx/TI), nyu + nyv): y-block q < nyu is u-plane q, else v-plane.
 // ---------------------------------------------------------------------------
 struct ZNLArgs {
-  const double2* z[6];  // uu uw ww vv (u planes), uv vw (v planes)
-  double2 *nxz1, *nxz3, *fvv, *nxz2, *fuv, *fvw;
+  const void* z[6];  // uu uw ww vv (u planes), uv vw (v planes); double2 or float2
+  void *nxz1, *nxz3, *fvv, *nxz2, *fuv, *fvw;
   const double *k1x, *k1z;
   int nyu, nyv, N2, Mx;
 };
@@ -271,7 +271,7 @@ __global__ void __launch_bounds__(kThreads) k_zfwd_nl(ZNLArgs a, FftPlan P, cons
   for (int c = 0; c < kMaxOut; ++c) acc1[c] = acc2[c] = make_double2(0, 0);
   const int nprod = isu ? 4 : 2;
   for (int pr = 0; pr < nprod; ++pr) {
-    const double2* src = a.z[isu ? pr : 4 + pr];
+    const double2* src = (const double2*)a.z[isu ? pr : 4 + pr];
     for (int w = threadIdx.x; w < TI * L; w += blockDim.x) {
       const int zz = w / TI, t = w - zz * TI;
       const int i = i0 + t;
@@ -297,16 +297,16 @@ __global__ void __launch_bounds__(kThreads) k_zfwd_nl(ZNLArgs a, FftPlan P, cons
         if (pr == 1) { acc1[c] = cadd(acc1[c], imulk(kz, X)); acc2[c] = imulk(kx, X); }
         if (pr == 2) acc2[c] = cadd(acc2[c], imulk(kz, X));
         if (pr == 3) {
-          a.fvv[o] = X;
-          a.nxz1[o] = zero ? make_double2(0, 0) : acc1[c];
-          a.nxz3[o] = zero ? make_double2(0, 0) : acc2[c];
+          ((double2*)a.fvv)[o] = X;
+          ((double2*)a.nxz1)[o] = zero ? make_double2(0, 0) : acc1[c];
+          ((double2*)a.nxz3)[o] = zero ? make_double2(0, 0) : acc2[c];
         }
       } else {
         // Nu2 = ikx uv + ikz vw ; keep uv, vw
-        if (pr == 0) { acc1[c] = imulk(kx, X); a.fuv[o] = X; }
+        if (pr == 0) { acc1[c] = imulk(kx, X); ((double2*)a.fuv)[o] = X; }
         if (pr == 1) {
-          a.fvw[o] = X;
-          if (inter) a.nxz2[o] = zero ? make_double2(0, 0) : cadd(acc1[c], imulk(kz, X));
+          ((double2*)a.fvw)[o] = X;
+          if (inter) ((double2*)a.nxz2)[o] = zero ? make_double2(0, 0) : cadd(acc1[c], imulk(kz, X));
         }
       }
     }
@@ -405,9 +405,12 @@ __global__ void __launch_bounds__(kThreads) k_zfwd_plain(const ZFwdJob* __restri
 // n = Q p + q, output k = k1 + P k2 (the identity of the dft.hpp header applied
 // at length L, with the L-point twiddles w_L^{q k1} read from the table).
 // The z kernels load and store HBM straight from registers, TI x-modes per
-// block side by side (lanes of consecutive i touch one 16*TI-byte segment).
+// block side by side (lanes of consecutive i touch one TI-element segment).
 // Sizes follow the dealiasing rule of the reference: N = 2*(L/3) modes kept
 // (N2 along z, N1 along x with Mx = N1/2 + 1), FOU3D.f90:546-564.
+// T is the arithmetic and storage type of the Z planes and of the nonlinear
+// outputs: double for the FP64 path, float for the mixed path (whose
+// spectral velocity inputs stay FP64 and are rounded once on load).
 // ---------------------------------------------------------------------------
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
 constexpr int rup64(int n) { return (n + 63) / 64 * 64; }
@@ -418,6 +421,11 @@ struct Dealias {
   static constexpr int H = N / 2;        // slots [H, L-H) carry no mode
   static constexpr int Mx = N / 2 + 1;   // kept x-modes of a real row
 };
+
+template <typename C>
+__device__ __forceinline__ C from_d2(double2 v) {
+  return mk<C>((real_t<C>)v.x, (real_t<C>)v.y);
+}
 
 __device__ __forceinline__ double2 zinv_load(const ZInvJob& J, size_t off, size_t ncol) {
   if (J.mode == 0) return J.src[(size_t)J.pa * ncol + off];
@@ -431,57 +439,65 @@ __device__ __forceinline__ double2 zinv_load(const ZInvJob& J, size_t off, size_
 }
 
 // z-inverse, four-step: grid (ceil(Mx/TI), njobs), TI*max(P,Q) threads.
-// LDS T[k1][q][t].
-template <int L, int P, int Q, int TI>
+// LDS X[k1][q][t].
+template <typename T, int L, int P, int Q, int TI>
 __global__ void __launch_bounds__(TI * cmax(P, Q)) k_zinv4(const ZInvJob* __restrict__ jobs,
-                                                          const double2* __restrict__ twg, int Mx) {
+                                                          const cplx_t<T>* __restrict__ twg, int Mx) {
+  using C = cplx_t<T>;
   static_assert(P * Q == L, "P*Q");
   constexpr int N2 = Dealias<L>::N, H = Dealias<L>::H;
-  __shared__ double2 T[P * Q * TI];
+  __shared__ C X[P * Q * TI];
   const ZInvJob J = jobs[blockIdx.y];
   const int tid = threadIdx.x, i0 = blockIdx.x * TI;
   const size_t ncol = (size_t)N2 * Mx;
   if (tid < Q * TI) {
     const int t = tid % TI, q = tid / TI, i = i0 + t;
-    double2 v[P];
+    C v[P];
     sfor<0, P>([&](auto pc) {
       constexpr int p = decltype(pc)::value;
       if constexpr (Q * p >= H && Q * p + Q <= L - H) {
-        v[p] = make_double2(0.0, 0.0);  // the whole group lies in the empty band
+        v[p] = mk<C>(0, 0);  // the whole group lies in the empty band
       } else {
         const int n = Q * p + q;
         const int k = n < H ? n : (n >= L - H ? n - L + N2 : -1);
-        v[p] = (k >= 0 && i < Mx) ? zinv_load(J, (size_t)k * Mx + i, ncol) : make_double2(0.0, 0.0);
+        v[p] = (k >= 0 && i < Mx) ? from_d2<C>(zinv_load(J, (size_t)k * Mx + i, ncol)) : mk<C>(0, 0);
       }
     });
     dft<true, P>(v);
-    T[q * TI + t] = v[0];
+    X[q * TI + t] = v[0];
     sfor<1, P>([&](auto kc) {
       constexpr int k1 = decltype(kc)::value;
-      T[(k1 * Q + q) * TI + t] = mul_tw<true>(v[k1], twg[q * k1]);
+      X[(k1 * Q + q) * TI + t] = mul_tw<true>(v[k1], twg[q * k1]);
     });
   }
   __syncthreads();
   if (tid < P * TI) {
     const int t = tid % TI, k1 = tid / TI, i = i0 + t;
-    double2 u[Q];
-    sfor<0, Q>([&](auto qc) { u[decltype(qc)::value] = T[(k1 * Q + decltype(qc)::value) * TI + t]; });
+    C u[Q];
+    sfor<0, Q>([&](auto qc) { u[decltype(qc)::value] = X[(k1 * Q + decltype(qc)::value) * TI + t]; });
     dft<true, Q>(u);
     if (i < Mx) {
-      double2* d = J.dst + (size_t)J.pd * L * Mx + i;
+      C* d = (C*)J.dst + (size_t)J.pd * L * Mx + i;
       sfor<0, Q>([&](auto c) { d[(size_t)(k1 + P * decltype(c)::value) * Mx] = u[decltype(c)::value]; });
     }
   }
 }
 
+// (0,k) * u
+template <typename C>
+__device__ __forceinline__ C imk(real_t<C> k, C u) {
+  return mk<C>(-k * u.y, k * u.x);
+}
+
 // z-forward of the nonlinear products, four-step: grid (ceil(Mx/TI), nyu+nyv).
 // Output slot k1 + P k2; groups of k2 wholly in the empty band are skipped at
 // compile time.  Same outputs as k_zfwd_nl.
-template <int L, int P, int Q, int TI>
-__global__ void __launch_bounds__(TI * cmax(P, Q)) k_zfwd4(ZNLArgs a, const double2* __restrict__ twg) {
+template <typename T, int L, int P, int Q, int TI>
+__global__ void __launch_bounds__(TI * cmax(P, Q)) k_zfwd4(ZNLArgs a, const cplx_t<T>* __restrict__ twg) {
+  using C = cplx_t<T>;
   static_assert(P * Q == L, "P*Q");
   constexpr int N2 = Dealias<L>::N, H = Dealias<L>::H;
-  __shared__ double2 T[P * Q * TI];
+  __shared__ C X[P * Q * TI];
   const int Mx = a.Mx;
   const int tid = threadIdx.x, i0 = blockIdx.x * TI;
   const bool isu = (int)blockIdx.y < a.nyu;
@@ -490,65 +506,71 @@ __global__ void __launch_bounds__(TI * cmax(P, Q)) k_zfwd4(ZNLArgs a, const doub
   const bool inter = pl > 0 && pl < ny - 1;
   if (isu && !inter) return;  // ghost u planes: no product is needed there
   const size_t ncol = (size_t)N2 * Mx;
-  const double sc = 1.0 / L;
+  const T sc = (T)(1.0 / L);
   const int t2 = tid % TI, k1 = tid / TI, i2 = i0 + t2;
-  double2 acc1[Q], acc2[Q];
+  C* const nxz1 = (C*)a.nxz1;
+  C* const nxz2 = (C*)a.nxz2;
+  C* const nxz3 = (C*)a.nxz3;
+  C* const fuv = (C*)a.fuv;
+  C* const fvw = (C*)a.fvw;
+  C* const fvv = (C*)a.fvv;
+  C acc1[Q], acc2[Q];
   const int nprod = isu ? 4 : 2;
   for (int pr = 0; pr < nprod; ++pr) {
-    const double2* src = a.z[isu ? pr : 4 + pr] + (size_t)pl * L * Mx;
+    const C* src = (const C*)a.z[isu ? pr : 4 + pr] + (size_t)pl * L * Mx;
     if (tid < Q * TI) {
       const int t = tid % TI, q = tid / TI, i = i0 + t;
-      double2 v[P];
+      C v[P];
       sfor<0, P>([&](auto pc) {
         constexpr int p = decltype(pc)::value;
-        v[p] = i < Mx ? src[(size_t)(Q * p + q) * Mx + i] : make_double2(0.0, 0.0);
+        v[p] = i < Mx ? src[(size_t)(Q * p + q) * Mx + i] : mk<C>(0, 0);
       });
       dft<false, P>(v);
-      T[q * TI + t] = v[0];
+      X[q * TI + t] = v[0];
       sfor<1, P>([&](auto kc) {
         constexpr int kk = decltype(kc)::value;
-        T[(kk * Q + q) * TI + t] = mul_tw<false>(v[kk], twg[q * kk]);
+        X[(kk * Q + q) * TI + t] = mul_tw<false>(v[kk], twg[q * kk]);
       });
     }
     __syncthreads();
     if (tid < P * TI) {
-      double2 u[Q];
-      sfor<0, Q>([&](auto qc) { u[decltype(qc)::value] = T[(k1 * Q + decltype(qc)::value) * TI + t2]; });
+      C u[Q];
+      sfor<0, Q>([&](auto qc) { u[decltype(qc)::value] = X[(k1 * Q + decltype(qc)::value) * TI + t2]; });
       dft<false, Q>(u);
       if (i2 < Mx) {
-        const double kx = a.k1x[i2];
+        const T kx = (T)a.k1x[i2];
         sfor<0, Q>([&](auto c) {
           constexpr int k2 = decltype(c)::value;
           if constexpr (!(P * k2 >= H && P * k2 + P <= L - H)) {
             const int sl = k1 + P * k2;
             if (sl < H || sl >= L - H) {
               const int k = sl < H ? sl : sl - L + N2;
-              const double2 X = cscl(u[k2], sc);
+              const C Xs = mk<C>(u[k2].x * sc, u[k2].y * sc);
               const bool zero = (sl == L - H);  // der_x/der_z zero band (FOU3D.f90:329-333)
-              const double kz = a.k1z[k];
+              const T kz = (T)a.k1z[k];
               const size_t o = (size_t)pl * ncol + (size_t)k * Mx + i2;
               if (isu) {
                 // Nu1 = ikx uu + ikz uw ; Nu3 = ikx uw + ikz ww ; keep vv
-                if (pr == 0) acc1[k2] = imulk(kx, X);
+                if (pr == 0) acc1[k2] = imk(kx, Xs);
                 if (pr == 1) {
-                  acc1[k2] = cadd(acc1[k2], imulk(kz, X));
-                  acc2[k2] = imulk(kx, X);
+                  acc1[k2] = vadd(acc1[k2], imk(kz, Xs));
+                  acc2[k2] = imk(kx, Xs);
                 }
-                if (pr == 2) acc2[k2] = cadd(acc2[k2], imulk(kz, X));
+                if (pr == 2) acc2[k2] = vadd(acc2[k2], imk(kz, Xs));
                 if (pr == 3) {
-                  a.fvv[o] = X;
-                  a.nxz1[o] = zero ? make_double2(0, 0) : acc1[k2];
-                  a.nxz3[o] = zero ? make_double2(0, 0) : acc2[k2];
+                  fvv[o] = Xs;
+                  nxz1[o] = zero ? mk<C>(0, 0) : acc1[k2];
+                  nxz3[o] = zero ? mk<C>(0, 0) : acc2[k2];
                 }
               } else {
                 // Nu2 = ikx uv + ikz vw ; keep uv, vw
                 if (pr == 0) {
-                  acc1[k2] = imulk(kx, X);
-                  a.fuv[o] = X;
+                  acc1[k2] = imk(kx, Xs);
+                  fuv[o] = Xs;
                 }
                 if (pr == 1) {
-                  a.fvw[o] = X;
-                  if (inter) a.nxz2[o] = zero ? make_double2(0, 0) : cadd(acc1[k2], imulk(kz, X));
+                  fvw[o] = Xs;
+                  if (inter) nxz2[o] = zero ? mk<C>(0, 0) : vadd(acc1[k2], imk(kz, Xs));
                 }
               }
             }
@@ -560,23 +582,41 @@ __global__ void __launch_bounds__(TI * cmax(P, Q)) k_zfwd4(ZNLArgs a, const doub
   }
 }
 
-// one Hermitian pair mode of a + i b (pair_mode) with the range of m known
-// at compile time where the whole group of Q consecutive m shares it
-template <int L, int Q, int P0>
-__device__ __forceinline__ double2 pair_mode_q(const double2* __restrict__ A, const double2* __restrict__ B,
-                                               int m) {
+// Hermitian pair spectrum of a + i b (as pair_mode), any precision
+template <typename C>
+__device__ __forceinline__ C pair_mode_t(const C* __restrict__ A, const C* __restrict__ B, int m, int L, int Mx) {
+  if (m == 0) {
+    const real_t<C> x = A ? A[0].x : 0, y = B ? B[0].x : 0;
+    return mk<C>(x, y);
+  }
+  if (m < Mx) {
+    const C a = A ? A[m] : mk<C>(0, 0), b = B ? B[m] : mk<C>(0, 0);
+    return mk<C>(a.x - b.y, a.y + b.x);
+  }
+  if (m > L - Mx) {
+    const int q = L - m;
+    const C a = A ? A[q] : mk<C>(0, 0), b = B ? B[q] : mk<C>(0, 0);
+    return mk<C>(a.x + b.y, -a.y + b.x);
+  }
+  return mk<C>(0, 0);
+}
+
+// one Hermitian pair mode with the range of m known at compile time where
+// the whole group of Q consecutive m shares it
+template <int L, int Q, int P0, typename C>
+__device__ __forceinline__ C pair_mode_q(const C* __restrict__ A, const C* __restrict__ B, int m) {
   constexpr int Mx = Dealias<L>::Mx;
   constexpr int lo = Q * P0, hi = Q * P0 + Q - 1;
   if constexpr (lo >= Mx && hi <= L - Mx) {
-    return make_double2(0.0, 0.0);
+    return mk<C>(0, 0);
   } else if constexpr (lo > 0 && hi < Mx) {
-    const double2 a = A ? A[m] : make_double2(0, 0), b = B ? B[m] : make_double2(0, 0);
-    return make_double2(a.x - b.y, a.y + b.x);
+    const C a = A ? A[m] : mk<C>(0, 0), b = B ? B[m] : mk<C>(0, 0);
+    return mk<C>(a.x - b.y, a.y + b.x);
   } else if constexpr (lo > L - Mx) {
-    const double2 a = A ? A[L - m] : make_double2(0, 0), b = B ? B[L - m] : make_double2(0, 0);
-    return make_double2(a.x + b.y, -a.y + b.x);
+    const C a = A ? A[L - m] : mk<C>(0, 0), b = B ? B[L - m] : mk<C>(0, 0);
+    return mk<C>(a.x + b.y, -a.y + b.x);
   } else {
-    return pair_mode(A, B, m, L, Mx);
+    return pair_mode_t<C>(A, B, m, L, Mx);
   }
 }
 
@@ -596,41 +636,43 @@ __device__ inline double wave_max(double v) {
 // transform the LDS slots hold the exchange [k1][q] (row stride Q+1), then the
 // physical values in place, then the forward exchange [m2][k1] (row stride
 // P+1) and the spectrum in place.
-template <int L, int P, int Q, int R>
-__global__ void __launch_bounds__(3 * xp_width(P, Q, R)) k_xpass4(XNLArgs a, const double2* __restrict__ twg) {
+template <typename T, int L, int P, int Q, int R>
+__global__ void __launch_bounds__(3 * xp_width(P, Q, R)) k_xpass4(XNLArgs a, const cplx_t<T>* __restrict__ twg) {
+  using C = cplx_t<T>;
   static_assert(P * Q == L && Q <= P, "P*Q, Q <= P");
   constexpr int Mx = Dealias<L>::Mx;
   constexpr int W = xp_width(P, Q, R);
   constexpr int SP = P * (Q + 1);  // >= Q * (P + 1) as Q <= P
-  __shared__ double2 S[R * 3 * SP];
+  __shared__ C S[R * 3 * SP];
   const int tid = threadIdx.x;
   const int tr = tid / W, lane = tid - tr * W;
   const int q = blockIdx.y, z0 = blockIdx.x * R, ngz = a.ngz;
   const bool hasv = q < a.nyv;
   const bool uint = q > 0 && q < a.nyu - 1;
   const bool vint = hasv && q > 0 && q < a.nyv - 1;
+  const C* const* zi = reinterpret_cast<const C* const*>(a.zi);
 
   // phase 1: tasks (qq, r) of transform tr; inverse P-point DFTs over p of
   // c[Q p + qq]
   for (int j = lane; j < Q * R; j += W) {
     const int qq = j % Q, r = j / Q;
     const int z = z0 + r;
-    double2* s = S + (r * 3 + tr) * SP;
-    const double2 *A = nullptr, *B = nullptr;
+    C* s = S + (r * 3 + tr) * SP;
+    const C *A = nullptr, *B = nullptr;
     if (z < ngz) {
       const size_t ru = ((size_t)q * ngz + z) * Mx;
       if (tr == 0) {
-        A = a.zi[0] + ru;
-        B = a.zi[2] + ru;
+        A = zi[0] + ru;
+        B = zi[2] + ru;
       } else if (tr == 1) {
-        A = a.zi[1] + ru;
-        B = hasv ? a.zi[4] + ru : nullptr;
+        A = zi[1] + ru;
+        B = hasv ? zi[4] + ru : nullptr;
       } else if (hasv) {
-        A = a.zi[3] + ru;
-        B = a.zi[5] + ru;
+        A = zi[3] + ru;
+        B = zi[5] + ru;
       }
     }
-    double2 v[P];
+    C v[P];
     sfor<0, P>([&](auto pc) {
       constexpr int p = decltype(pc)::value;
       v[p] = pair_mode_q<L, Q, p>(A, B, Q * p + qq);
@@ -649,8 +691,8 @@ __global__ void __launch_bounds__(3 * xp_width(P, Q, R)) k_xpass4(XNLArgs a, con
   // place for the products of the next transform
   const bool act = lane < P * R;
   const int k1 = lane % P, r = act ? lane / P : 0;
-  double2* s = S + (r * 3 + tr) * SP + k1 * (Q + 1);
-  double2 u[Q];
+  C* s = S + (r * 3 + tr) * SP + k1 * (Q + 1);
+  C u[Q];
   if (act) {
     sfor<0, Q>([&](auto c) { u[decltype(c)::value] = s[decltype(c)::value]; });
     dft<true, Q>(u);
@@ -661,55 +703,56 @@ __global__ void __launch_bounds__(3 * xp_width(P, Q, R)) k_xpass4(XNLArgs a, con
   // products in physical space (ops_in_planes, FOU3D.f90:1529-1617); on CFL
   // substeps the maxima of |u|, |w| and |v| dtheta/dy (dtc_calc,
   // FOU3D.f90:516-544)
-  double ma = 0.0, mb = 0.0;
+  T ma = 0, mb = 0;
   if (act) {
-    const double2* sp = S + (r * 3 + (tr > 0 ? tr - 1 : 0)) * SP + k1 * (Q + 1);  // transform tr-1
+    const C* sp = S + (r * 3 + (tr > 0 ? tr - 1 : 0)) * SP + k1 * (Q + 1);  // transform tr-1
     if (a.cfl) {
       sfor<0, Q>([&](auto c) {
-        const double2 x = u[decltype(c)::value];
+        const C x = u[decltype(c)::value];
         ma = fmax(ma, fabs(x.x));
         mb = fmax(mb, fabs(x.y));
       });
     }
     if (tr == 0) {
       sfor<0, Q>([&](auto c) {
-        const double2 x = u[decltype(c)::value];
-        u[decltype(c)::value] = make_double2(x.x * x.x, x.x * x.y);  // uu*uu, uu*ww
+        const C x = u[decltype(c)::value];
+        u[decltype(c)::value] = mk<C>(x.x * x.x, x.x * x.y);  // uu*uu, uu*ww
       });
     } else if (tr == 1) {
       sfor<0, Q>([&](auto c) {
-        const double2 x = u[decltype(c)::value];
-        const double ww = sp[decltype(c)::value].y;
-        u[decltype(c)::value] = make_double2(x.x * x.x, ww * ww);  // vc*vc, ww*ww
+        const C x = u[decltype(c)::value];
+        const T ww = sp[decltype(c)::value].y;
+        u[decltype(c)::value] = mk<C>(x.x * x.x, ww * ww);  // vc*vc, ww*ww
       });
     } else {
       sfor<0, Q>([&](auto c) {
-        const double2 x = u[decltype(c)::value];
-        const double vf = sp[decltype(c)::value].y;
-        u[decltype(c)::value] = make_double2(x.x * vf, x.y * vf);  // uf*vf, wf*vf
+        const C x = u[decltype(c)::value];
+        const T vf = sp[decltype(c)::value].y;
+        u[decltype(c)::value] = mk<C>(x.x * vf, x.y * vf);  // uf*vf, wf*vf
       });
     }
   }
   // W is a multiple of 64, so a.cfl and tr are uniform across each wavefront
   // and every lane takes part in its reduction
   if (a.cfl && tr < 2) {
-    if (!act || z0 + r >= ngz) ma = mb = 0.0;
-    if (tr == 1) mb *= vint ? a.dthdyv[q] : 0.0;
-    ma = wave_max(ma);
-    mb = wave_max(mb);
+    double da = ma, db = mb;
+    if (!act || z0 + r >= ngz) da = db = 0.0;
+    if (tr == 1) db *= vint ? a.dthdyv[q] : 0.0;
+    da = wave_max(da);
+    db = wave_max(db);
     if ((lane & 63) == 0) {
       if (tr == 0) {
-        atomic_max_pos(&a.sc->m1, ma);
-        atomic_max_pos(&a.sc->m3, mb);
+        atomic_max_pos(&a.sc->m1, da);
+        atomic_max_pos(&a.sc->m3, db);
       } else {
-        atomic_max_pos(&a.sc->m2, mb);
+        atomic_max_pos(&a.sc->m2, db);
       }
     }
   }
   __syncthreads();
 
   // phase 3: forward Q-point DFT over k2 -> m2, twiddle w_L^{k1 m2}
-  double2* sf = S + (r * 3 + tr) * SP;
+  C* sf = S + (r * 3 + tr) * SP;
   if (act) {
     dft<false, Q>(u);
     sf[k1] = u[0];
@@ -724,8 +767,8 @@ __global__ void __launch_bounds__(3 * xp_width(P, Q, R)) k_xpass4(XNLArgs a, con
   // F[m2 + Q m1]
   for (int j = lane; j < Q * R; j += W) {
     const int m2 = j % Q, r4 = j / Q;
-    double2* s4 = S + (r4 * 3 + tr) * SP + m2 * (P + 1);
-    double2 v[P];
+    C* s4 = S + (r4 * 3 + tr) * SP + m2 * (P + 1);
+    C v[P];
     sfor<0, P>([&](auto c) { v[decltype(c)::value] = s4[decltype(c)::value]; });
     dft<false, P>(v);
     sfor<0, P>([&](auto c) { s4[decltype(c)::value] = v[decltype(c)::value]; });
@@ -734,19 +777,19 @@ __global__ void __launch_bounds__(3 * xp_width(P, Q, R)) k_xpass4(XNLArgs a, con
 
   // split the spectra of a + i b into the modes of a and b, scaled by 1/L
   if (tr < 2 ? !uint : !hasv) return;
-  const double sc = 1.0 / L;
-  double2* oa = tr == 0 ? a.zo[0] : (tr == 1 ? a.zo[3] : a.zo[4]);  // uu, vv, uv
-  double2* ob = tr == 0 ? a.zo[1] : (tr == 1 ? a.zo[2] : a.zo[5]);  // uw, ww, vw
+  const T hs = (T)(0.5 / L);
+  C* oa = (C*)(tr == 0 ? a.zo[0] : (tr == 1 ? a.zo[3] : a.zo[4]));  // uu, vv, uv
+  C* ob = (C*)(tr == 0 ? a.zo[1] : (tr == 1 ? a.zo[2] : a.zo[5]));  // uw, ww, vw
   for (int j = lane; j < Mx * R; j += W) {
     const int m = j % Mx, r5 = j / Mx;
     const int z = z0 + r5;
     if (z >= ngz) continue;
-    const double2* F = S + (r5 * 3 + tr) * SP;
+    const C* F = S + (r5 * 3 + tr) * SP;
     const int mb = m == 0 ? 0 : L - m;
-    const double2 f = F[(m % Q) * (P + 1) + m / Q], g = F[(mb % Q) * (P + 1) + mb / Q];
+    const C f = F[(m % Q) * (P + 1) + m / Q], g = F[(mb % Q) * (P + 1) + mb / Q];
     const size_t ro = ((size_t)q * ngz + z) * Mx + m;
-    oa[ro] = make_double2(0.5 * sc * (f.x + g.x), 0.5 * sc * (f.y - g.y));
-    ob[ro] = make_double2(0.5 * sc * (f.y + g.y), -0.5 * sc * (f.x - g.x));
+    oa[ro] = mk<C>(hs * (f.x + g.x), hs * (f.y - g.y));
+    ob[ro] = mk<C>(hs * (f.y + g.y), -hs * (f.x - g.x));
   }
 }
 
@@ -822,16 +865,16 @@ void zfwdp_t(Solver& s, const ZFwdJob* jobs, int njobs) {
 #define CFD_XPASS4(X) X(48, 8, 6, 8) X(64, 8, 8, 8) X(72, 12, 6, 4) X(384, 24, 16, 2)
 
 bool four_step_z(const Solver& s) {
-  return !s.fft_stockham && s.N2 == 2 * (s.ngz / 3) && s.pz.L == s.ngz;
+  return (s.mixed || !s.fft_stockham) && s.N2 == 2 * (s.ngz / 3) && s.pz.L == s.ngz;
 }
 
-bool try_zinv4(Solver& s, const ZInvJob* jobs, int njobs) {
-  if (!four_step_z(s)) return false;
+template <typename T>
+bool zinv4_t(Solver& s, const ZInvJob* jobs, int njobs, const cplx_t<T>* tw) {
   switch (s.pz.L) {
-#define C(L, P, Q, TI)                                                                                   \
-  case L:                                                                                                \
-    hipLaunchKernelGGL((k_zinv4<L, P, Q, TI>), dim3((s.Mx + TI - 1) / TI, njobs), dim3(TI * cmax(P, Q)), \
-                       0, s.stream, jobs, s.d_twz, s.Mx);                                               \
+#define C(L, P, Q, TI)                                                                                      \
+  case L:                                                                                                   \
+    hipLaunchKernelGGL((k_zinv4<T, L, P, Q, TI>), dim3((s.Mx + TI - 1) / TI, njobs), dim3(TI * cmax(P, Q)), \
+                       0, s.stream, jobs, tw, s.Mx);                                                       \
     return true;
     CFD_ZINV4(C)
 #undef C
@@ -839,13 +882,18 @@ bool try_zinv4(Solver& s, const ZInvJob* jobs, int njobs) {
   }
 }
 
-bool try_zfwd4(Solver& s, const ZNLArgs& a) {
+bool try_zinv4(Solver& s, const ZInvJob* jobs, int njobs, bool f32) {
   if (!four_step_z(s)) return false;
+  return f32 ? zinv4_t<float>(s, jobs, njobs, s.d_twzf) : zinv4_t<double>(s, jobs, njobs, s.d_twz);
+}
+
+template <typename T>
+bool zfwd4_t(Solver& s, const ZNLArgs& a, const cplx_t<T>* tw) {
   switch (s.pz.L) {
-#define C(L, P, Q, TI)                                                                                \
-  case L:                                                                                             \
-    hipLaunchKernelGGL((k_zfwd4<L, P, Q, TI>), dim3((s.Mx + TI - 1) / TI, s.nyu + s.nyv),            \
-                       dim3(TI * cmax(P, Q)), 0, s.stream, a, s.d_twz);                              \
+#define C(L, P, Q, TI)                                                                                   \
+  case L:                                                                                                \
+    hipLaunchKernelGGL((k_zfwd4<T, L, P, Q, TI>), dim3((s.Mx + TI - 1) / TI, s.nyu + s.nyv),            \
+                       dim3(TI * cmax(P, Q)), 0, s.stream, a, tw);                                      \
     return true;
     CFD_ZFWD4(C)
 #undef C
@@ -853,13 +901,18 @@ bool try_zfwd4(Solver& s, const ZNLArgs& a) {
   }
 }
 
-bool try_xpass4(Solver& s, const XNLArgs& a) {
-  if (s.fft_stockham || s.px.L != s.ngx || s.N1 != 2 * (s.ngx / 3) || s.Mx != s.N1 / 2 + 1) return false;
+bool try_zfwd4(Solver& s, const ZNLArgs& a) {
+  if (!four_step_z(s)) return false;
+  return s.mixed ? zfwd4_t<float>(s, a, s.d_twzf) : zfwd4_t<double>(s, a, s.d_twz);
+}
+
+template <typename T>
+bool xpass4_t(Solver& s, const XNLArgs& a, const cplx_t<T>* tw) {
   switch (s.px.L) {
-#define C(L, P, Q, R)                                                                                    \
-  case L:                                                                                                \
-    hipLaunchKernelGGL((k_xpass4<L, P, Q, R>), dim3((s.ngz + R - 1) / R, s.nyu),                     \
-                       dim3(3 * xp_width(P, Q, R)), 0, s.stream, a, s.d_twx);                          \
+#define C(L, P, Q, R)                                                                                       \
+  case L:                                                                                                   \
+    hipLaunchKernelGGL((k_xpass4<T, L, P, Q, R>), dim3((s.ngz + R - 1) / R, s.nyu),                     \
+                       dim3(3 * xp_width(P, Q, R)), 0, s.stream, a, tw);                                  \
     return true;
     CFD_XPASS4(C)
 #undef C
@@ -867,14 +920,47 @@ bool try_xpass4(Solver& s, const XNLArgs& a) {
   }
 }
 
+bool try_xpass4(Solver& s, const XNLArgs& a) {
+  if ((s.fft_stockham && !s.mixed) || s.px.L != s.ngx || s.N1 != 2 * (s.ngx / 3) || s.Mx != s.N1 / 2 + 1)
+    return false;
+  return s.mixed ? xpass4_t<float>(s, a, s.d_twxf) : xpass4_t<double>(s, a, s.d_twx);
+}
+
 // transform lengths with compile-time plans; other lengths use the runtime plan
 #define CFD_FFT_SIZES(X) X(48) X(64) X(72) X(96) X(128) X(192) X(256) X(384) X(768)
 
 }  // namespace
 
-void launch_zinv(Solver& s, const ZInvJob* jobs, int njobs) {
+bool fft4_z_supported(int L) {
+  switch (L) {
+#define C(L, P, Q, TI) case L:
+    CFD_ZINV4(C)
+#undef C
+    break;
+    default: return false;
+  }
+  switch (L) {
+#define C(L, P, Q, TI) case L:
+    CFD_ZFWD4(C)
+#undef C
+    return true;
+    default: return false;
+  }
+}
+
+bool fft4_x_supported(int L) {
+  switch (L) {
+#define C(L, P, Q, R) case L:
+    CFD_XPASS4(C)
+#undef C
+    return true;
+    default: return false;
+  }
+}
+
+void launch_zinv(Solver& s, const ZInvJob* jobs, int njobs, bool f32) {
   if (njobs <= 0) return;
-  if (try_zinv4(s, jobs, njobs)) return;
+  if (try_zinv4(s, jobs, njobs, f32)) return;
   switch (s.pz.L) {
 #define C(n) case n: zinv_t<n>(s, jobs, njobs); break;
     CFD_FFT_SIZES(C)

@@ -29,8 +29,8 @@ constexpr int CH = 8;
 struct FusedArgs {
   const double2* u[3];                  // u1, u2, u3 (old)
   double2* __restrict__ nu[3];          // Nu1..3: old in, new out
-  const double2* __restrict__ nxz[3];   // x/z part of the new nonlinear term
-  const double2* __restrict__ flux[3];  // uv (v planes), vv (u planes), vw (v planes)
+  const void* nxz[3];   // x/z part of the new nonlinear term (TN: double2 or float2)
+  const void* flux[3];  // uv (v planes), vv (u planes), vw (v planes) (TN)
   const double2* __restrict__ p;
   double2* __restrict__ r[3];           // du1..3: IB-corrected r on IB planes
   double2* __restrict__ ysc[3];         // forward-sweep scratch
@@ -49,8 +49,8 @@ struct FusedArgs {
 struct FieldPtrs {
   const double2* u;
   double2* nu;
-  const double2* nxz;
-  const double2* flux;
+  const void* nxz;
+  const void* flux;
   double2* r;
   double2* ys;
   double* cs;
@@ -105,27 +105,35 @@ __device__ __forceinline__ double2 row_r(const FusedArgs& a, int f, int q, int c
   return make_double2(v.u0.x + dt * du.x, v.u0.y + dt * du.y);
 }
 
-// load the inputs of row q (interior) of field f
+__device__ __forceinline__ double2 as_d2(double2 v) { return v; }
+__device__ __forceinline__ double2 as_d2(float2 v) { return make_double2((double)v.x, (double)v.y); }
+
+// load the inputs of row q (interior) of field f; TN is the storage type of
+// the nonlinear scratch (nxz, fluxes)
+template <typename TN>
 __device__ __forceinline__ void row_load(const FusedArgs& a, const FieldPtrs& F, int f, int q, size_t N, int col,
                                          double2& up, double2& nold, double2& nx, double2& fa, double2& fb,
                                          double2& pa, double2& pb) {
   up = F.u[(size_t)(q + 1) * N + col];
   nold = F.nu[(size_t)q * N + col];
-  nx = F.nxz[(size_t)q * N + col];
+  const TN* nxz = (const TN*)F.nxz;
+  const TN* flux = (const TN*)F.flux;
+  nx = as_d2(nxz[(size_t)q * N + col]);
   if (f == 1) {
-    fa = F.flux[(size_t)(q + 1) * N + col];
-    fb = F.flux[(size_t)q * N + col];
+    fa = as_d2(flux[(size_t)(q + 1) * N + col]);
+    fb = as_d2(flux[(size_t)q * N + col]);
     pa = a.p[(size_t)q * N + col];
     pb = a.p[(size_t)(q - 1) * N + col];
   } else {
-    fa = F.flux[(size_t)q * N + col];
-    fb = F.flux[(size_t)(q - 1) * N + col];
+    fa = as_d2(flux[(size_t)q * N + col]);
+    fb = as_d2(flux[(size_t)(q - 1) * N + col]);
     pa = a.p[(size_t)(q - 1) * N + col];
     pb = pa;
   }
 }
 
 // grid (ceil(Ncol/64), 3)
+template <typename TN>
 __global__ void __launch_bounds__(64) k_rhs_solve(FusedArgs a) {
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   const int f = blockIdx.y;
@@ -155,7 +163,7 @@ __global__ void __launch_bounds__(64) k_rhs_solve(FusedArgs a) {
     for (int c = 0; c < CH; ++c) {
       const int q = q0 + c;
       if (q < n - 1) {
-        row_load(a, F, f, q, N, col, UP[c], NO[c], NX[c], FA[c], FB[c], PA[c], PB[c]);
+        row_load<TN>(a, F, f, q, N, col, UP[c], NO[c], NX[c], FA[c], FB[c], PA[c], PB[c]);
         RB[c] = ib[q] ? F.r[(size_t)q * N + col] : make_double2(0.0, 0.0);
       }
     }
@@ -243,6 +251,7 @@ struct RowCtx {
 
 // row q of the segmented solve; um/u0 hold u(q-1), u(q) and are advanced;
 // interior rows store the new Nu
+template <typename TN>
 __device__ __forceinline__ RowSys seg_row(const FusedArgs& a, const FieldPtrs& F, int f, const RowCtx& c, int q,
                                           double2& um, double2& u0) {
   RowSys s;
@@ -258,7 +267,7 @@ __device__ __forceinline__ RowSys seg_row(const FusedArgs& a, const FieldPtrs& F
     s.lo = c.g1; s.dg = 1.0; s.up = 0.0; s.r = make_double2(0.0, 0.0);
   } else {
     double2 up, nold, nx, fa, fb, pa, pb, nnew;
-    row_load(a, F, f, q, c.N, c.col, up, nold, nx, fa, fb, pa, pb);
+    row_load<TN>(a, F, f, q, c.N, c.col, up, nold, nx, fa, fb, pa, pb);
     s.r = row_r(a, f, q, c.col, c.k2x, c.k2z, c.kk, c.dt, um, u0, un, nold, nx, fa, fb, pa, pb, nnew);
     F.nu[(size_t)q * c.N + c.col] = nnew;
     if (F.ib[q]) s.r = F.r[(size_t)q * c.N + c.col];
@@ -272,7 +281,7 @@ __device__ __forceinline__ RowSys seg_row(const FusedArgs& a, const FieldPtrs& F
   return s;
 }
 
-template <int SEG>
+template <int SEG, typename TN>
 __global__ void __launch_bounds__(256) k_rhs_seg(FusedArgs a) {
   extern __shared__ double sm[];
   const int CB = blockDim.x, NT = blockDim.x * blockDim.y;
@@ -320,7 +329,7 @@ __global__ void __launch_bounds__(256) k_rhs_seg(FusedArgs a) {
     double gp = 0.0, hp = 1.0;
 #pragma unroll
     for (int j = 0; j < SEG; ++j) {
-      const RowSys rs = seg_row(a, F, f, c, b + j, um, u0);
+      const RowSys rs = seg_row<TN>(a, F, f, c, b + j, um, u0);
       const double inv = 1.0 / (rs.dg - rs.lo * gp);
       y[j] = make_double2((rs.r.x - rs.lo * yp.x) * inv, (rs.r.y - rs.lo * yp.y) * inv);
       g[j] = rs.up * inv;
@@ -331,7 +340,7 @@ __global__ void __launch_bounds__(256) k_rhs_seg(FusedArgs a) {
     }
     // separator row after the segment
     if (sg < S - 1) {
-      const RowSys rs = seg_row(a, F, f, c, b + SEG, um, u0);
+      const RowSys rs = seg_row<TN>(a, F, f, c, b + SEG, um, u0);
       As[me] = rs.lo;
       Ds[me] = rs.dg;
       Cs[me] = rs.up;
@@ -394,6 +403,7 @@ __global__ void __launch_bounds__(256) k_rhs_seg(FusedArgs a) {
 }
 
 // predictor on the IB planes only: grid (ceil(Ncol/kTB), max IB planes, 3)
+template <typename TN>
 __global__ void __launch_bounds__(256) k_ib_rhs(FusedArgs a, const int* __restrict__ planes0,
                                                 const int* __restrict__ planes1, const int* __restrict__ planes2,
                                                 int n0, int n1, int n2) {
@@ -410,7 +420,7 @@ __global__ void __launch_bounds__(256) k_ib_rhs(FusedArgs a, const int* __restri
   const double2 um = F.u[(size_t)(q - 1) * N + col];
   const double2 u0 = F.u[(size_t)q * N + col];
   double2 up, nold, nx, fa, fb, pa, pb;
-  row_load(a, F, f, q, N, col, up, nold, nx, fa, fb, pa, pb);
+  row_load<TN>(a, F, f, q, N, col, up, nold, nx, fa, fb, pa, pb);
   double2 nnew;
   F.r[(size_t)q * N + col] =
       row_r(a, f, q, col, a.k2x[i], a.k2z[k], kk, a.sc->dt, um, u0, up, nold, nx, fa, fb, pa, pb, nnew);
@@ -854,8 +864,12 @@ void launch_ib_rhs(Solver& s, int kRK) {
   FusedArgs a;
   fused_args(s, kRK, &a);
   dim3 grid((s.Ncol + 255) / 256, nmax, 3);
-  hipLaunchKernelGGL(k_ib_rhs, grid, dim3(256), 0, s.stream, a, (const int*)s.d_ibq[0], (const int*)s.d_ibq[1],
-                     (const int*)s.d_ibq[2], n0, n1, n2);
+  if (s.mixed)
+    hipLaunchKernelGGL(k_ib_rhs<float2>, grid, dim3(256), 0, s.stream, a, (const int*)s.d_ibq[0],
+                       (const int*)s.d_ibq[1], (const int*)s.d_ibq[2], n0, n1, n2);
+  else
+    hipLaunchKernelGGL(k_ib_rhs<double2>, grid, dim3(256), 0, s.stream, a, (const int*)s.d_ibq[0],
+                       (const int*)s.d_ibq[1], (const int*)s.d_ibq[2], n0, n1, n2);
 }
 
 constexpr int kSeg = 13;  // rows per segment of the segmented solve
@@ -868,9 +882,15 @@ void launch_rhs_solve(Solver& s, int kRK) {
     int cb = 1;
     while (cb * 2 * sy <= 256) cb *= 2;
     const size_t lds = sizeof(double) * 15 * (size_t)cb * sy;
-    hipLaunchKernelGGL(k_rhs_seg<kSeg>, dim3((s.Ncol + cb - 1) / cb, 3), dim3(cb, sy), lds, s.stream, a);
+    const dim3 grid((s.Ncol + cb - 1) / cb, 3), block(cb, sy);
+    if (s.mixed)
+      hipLaunchKernelGGL((k_rhs_seg<kSeg, float2>), grid, block, lds, s.stream, a);
+    else
+      hipLaunchKernelGGL((k_rhs_seg<kSeg, double2>), grid, block, lds, s.stream, a);
+  } else if (s.mixed) {
+    hipLaunchKernelGGL(k_rhs_solve<float2>, dim3((s.Ncol + 63) / 64, 3), dim3(64), 0, s.stream, a);
   } else {
-    hipLaunchKernelGGL(k_rhs_solve, dim3((s.Ncol + 63) / 64, 3), dim3(64), 0, s.stream, a);
+    hipLaunchKernelGGL(k_rhs_solve<double2>, dim3((s.Ncol + 63) / 64, 3), dim3(64), 0, s.stream, a);
   }
 }
 

@@ -78,13 +78,17 @@ __global__ void __launch_bounds__(kTB) k_rhs0(RhsArgs a) {
 // grid (ceil(Ncol/kTB), nyu-2, 3)
 // ---------------------------------------------------------------------------
 struct CombArgs {
-  const double2 *nxz1, *nxz2, *nxz3, *fuv, *fvw, *fvv;
+  const void *nxz1, *nxz2, *nxz3, *fuv, *fvw, *fvv;  // TN: double2 (FP64) or float2 (mixed)
   double2 *n1, *n2, *n3, *d1, *d2, *d3;
   const double *dthdyu, *dthdyv;
   double C1, ddthetavi;
   Geo g;
 };
 
+__device__ __forceinline__ double2 as_d2(double2 v) { return v; }
+__device__ __forceinline__ double2 as_d2(float2 v) { return make_double2((double)v.x, (double)v.y); }
+
+template <typename TN>
 __global__ void __launch_bounds__(kTB) k_nlcombine(CombArgs a) {
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   const int q = blockIdx.y + 1;
@@ -93,7 +97,8 @@ __global__ void __launch_bounds__(kTB) k_nlcombine(CombArgs a) {
   const size_t N = a.g.Ncol, o = q * N + col;
   if (f == 1) {
     if (q > a.g.nyv - 2) return;
-    const double2 vp = a.fvv[(q + 1) * N + col], v0 = a.fvv[o], x = a.nxz2[o];
+    const TN* fvv = (const TN*)a.fvv;
+    const double2 vp = as_d2(fvv[(q + 1) * N + col]), v0 = as_d2(fvv[o]), x = as_d2(((const TN*)a.nxz2)[o]);
     const double dd = a.dthdyv[q];
     const double2 n = make_double2(x.x + (vp.x - v0.x) * dd * a.ddthetavi, x.y + (vp.y - v0.y) * dd * a.ddthetavi);
     a.n2[o] = n;
@@ -101,11 +106,11 @@ __global__ void __launch_bounds__(kTB) k_nlcombine(CombArgs a) {
     a.d2[o] = make_double2(d.x + a.C1 * n.x, d.y + a.C1 * n.y);
     return;
   }
-  const double2* fl = f == 0 ? a.fuv : a.fvw;
-  const double2* nx = f == 0 ? a.nxz1 : a.nxz3;
+  const TN* fl = (const TN*)(f == 0 ? a.fuv : a.fvw);
+  const TN* nx = (const TN*)(f == 0 ? a.nxz1 : a.nxz3);
   double2* nu = f == 0 ? a.n1 : a.n3;
   double2* du = f == 0 ? a.d1 : a.d3;
-  const double2 fp = fl[o], fm = fl[(q - 1) * N + col], x = nx[o];
+  const double2 fp = as_d2(fl[o]), fm = as_d2(fl[(q - 1) * N + col]), x = as_d2(nx[o]);
   const double dd = a.dthdyu[q];
   const double2 n = make_double2(x.x + (fp.x - fm.x) * dd * a.ddthetavi, x.y + (fp.y - fm.y) * dd * a.ddthetavi);
   nu[o] = n;
@@ -451,7 +456,10 @@ void launch_nlcombine(Solver& s, int kRK) {
   a.ddthetavi = 1.0;
   a.g = geo(s);
   dim3 grid((s.Ncol + kTB - 1) / kTB, s.nyu - 2, 3);
-  hipLaunchKernelGGL(k_nlcombine, grid, dim3(kTB), 0, s.stream, a);
+  if (s.mixed)
+    hipLaunchKernelGGL(k_nlcombine<float2>, grid, dim3(kTB), 0, s.stream, a);
+  else
+    hipLaunchKernelGGL(k_nlcombine<double2>, grid, dim3(kTB), 0, s.stream, a);
 }
 
 void launch_reset_max(Solver& s) { hipLaunchKernelGGL(k_reset_max, dim3(1), dim3(1), 0, s.stream, s.d_sc); }

@@ -47,7 +47,9 @@ typedef struct cfd_params {
   int ntilex, ntilez;   /* post tiles per wall                                    */
   int extras24;         /* 1 = the 4 fixed forcing points of the 24pt variant     */
   int flag_ctpress;     /* 0 constant flow rate, 1 constant mean pressure grad.  */
-  int precision;        /* 0 = FP64 everywhere                                    */
+  int precision;        /* 0 = FP64 everywhere; 1 = mixed: FP32 transforms,      */
+                        /* products and nonlinear scratch, FP64 fields, solves, */
+                        /* projection and flow rate (four-step FFT lengths only) */
   int device;           /* HIP device ordinal                                     */
   double Re, alp, bet;  /* bulk Reynolds number; Lx = 2 pi/alp, Lz = 2 pi/bet     */
   double dyq;           /* dymax/dymin of the polynomial map                      */
