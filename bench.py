@@ -101,6 +101,25 @@ def roofline(ch, prof, zb, workload, label):
             "algorithmic_bytes": cb[dom]}
 
 
+def copy_calibration():
+    """1 GiB device copy, three times: a kernel of known HBM traffic (1 GiB read
+    + 1 GiB written per launch) and the measured copy bandwidth."""
+    import torch
+
+    n = 1 << 27  # doubles
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    print(json.dumps({"copy_bytes_per_launch": 2 * n * 8, "launches": 4,
+                      "copy_GBps": 3 * 2 * n * 8 / (t1 - t0) / 1e9}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,7 +128,13 @@ def main():
     ap.add_argument("--workload", default="rt395_384x192x192")
     ap.add_argument("--cpu-substeps", type=int, default=3, help="oracle substeps for the CPU baseline (0: skip)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device-to-device copy 3 times and print its bandwidth (the reference "
+                         "kernel for scaling HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs the substep on one GPU; --gpus %d is not supported" % args.gpus,
               file=sys.stderr)

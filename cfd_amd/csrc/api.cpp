@@ -290,7 +290,15 @@ extern "C" int cfd_create(const cfd_params* pp, cfd_handle* out) {
     return bail(rc);
   for (int f = 0; f < 3; ++f)
     if ((rc = dalloc(s, &s->ysc[f], (size_t)s->nyu * N))) return bail(rc);
-  const size_t zp = (size_t)s->ngz * s->Mx;
+  // Z rows padded to a multiple of 128 bytes when the four-step kernels own
+  // all three nonlinear passes (the Stockham fallback reads rows of Mx)
+  {
+    const bool four = (s->mixed || !s->fft_stockham) && cfd::fft4_z_supported(s->ngz) &&
+                      cfd::fft4_x_supported(s->ngx) && s->N2 == 2 * (s->ngz / 3) && s->N1 == 2 * (s->ngx / 3);
+    const int per = (int)(128 / esz);
+    s->zs = four ? (s->Mx + per - 1) / per * per : s->Mx;
+  }
+  const size_t zp = (size_t)s->ngz * s->zs;
   const int zplanes[12] = {s->nyu, s->nyu, s->nyu, s->nyv, s->nyv, s->nyv,
                            s->nyu, s->nyu, s->nyu, s->nyu, s->nyv, s->nyv};
   for (int q = 0; q < 12; ++q)
@@ -319,29 +327,33 @@ extern "C" int cfd_create(const cfd_params* pp, cfd_handle* out) {
       wa = (g.YU(j) - g.YV(j - 1)) / den;
       wb = (g.YV(j) - g.YU(j)) / den;
     };
-    for (int q = 0; q < s->nyu; ++q) {
-      const int j = s->jlo_u + q;
-      job(s->F[CFD_U1], s->Z[0], q, 0, q, 0, 0, 0, 0, 0);
-      job(s->F[CFD_U3], s->Z[2], q, 0, q, 0, 0, 0, 0, 0);
-      double wa, wb;
-      if (q == 0) {
-        vw(j + 1, wa, wb);
-        job(s->F[CFD_U2], s->Z[1], q, 2, 1, 0, 0, wa, wb, s->gwi[0]);
-      } else if (q == s->nyu - 1) {
-        vw(j - 1, wa, wb);
-        job(s->F[CFD_U2], s->Z[1], q, 2, s->nyv - 1, s->nyv - 2, s->nyv - 1, wa, wb, s->gwi[1]);
-      } else {
-        vw(j, wa, wb);
-        job(s->F[CFD_U2], s->Z[1], q, 1, q, q - 1, 0, wa, wb, 0);
+    // ordered by y-plane: the jobs of u-plane q and v-plane q read the
+    // velocity planes q-1..q+1, so neighbouring jobs share their sources
+    for (int q = 0; q < std::max(s->nyu, s->nyv); ++q) {
+      if (q < s->nyu) {
+        const int j = s->jlo_u + q;
+        job(s->F[CFD_U1], s->Z[0], q, 0, q, 0, 0, 0, 0, 0);
+        job(s->F[CFD_U3], s->Z[2], q, 0, q, 0, 0, 0, 0, 0);
+        double wa, wb;
+        if (q == 0) {
+          vw(j + 1, wa, wb);
+          job(s->F[CFD_U2], s->Z[1], q, 2, 1, 0, 0, wa, wb, s->gwi[0]);
+        } else if (q == s->nyu - 1) {
+          vw(j - 1, wa, wb);
+          job(s->F[CFD_U2], s->Z[1], q, 2, s->nyv - 1, s->nyv - 2, s->nyv - 1, wa, wb, s->gwi[1]);
+        } else {
+          vw(j, wa, wb);
+          job(s->F[CFD_U2], s->Z[1], q, 1, q, q - 1, 0, wa, wb, 0);
+        }
       }
-    }
-    for (int q = 0; q < s->nyv; ++q) {
-      const int j = s->jlo_v + q;
-      const double den = g.YU(j + 1) - g.YU(j);
-      const double wa = (g.YV(j) - g.YU(j)) / den, wb = (g.YU(j + 1) - g.YV(j)) / den;
-      job(s->F[CFD_U1], s->Z[3], q, 1, q + 1, q, 0, wa, wb, 0);
-      job(s->F[CFD_U2], s->Z[4], q, 0, q, 0, 0, 0, 0, 0);
-      job(s->F[CFD_U3], s->Z[5], q, 1, q + 1, q, 0, wa, wb, 0);
+      if (q < s->nyv) {
+        const int j = s->jlo_v + q;
+        const double den = g.YU(j + 1) - g.YU(j);
+        const double wa = (g.YV(j) - g.YU(j)) / den, wb = (g.YU(j + 1) - g.YV(j)) / den;
+        job(s->F[CFD_U1], s->Z[3], q, 1, q + 1, q, 0, wa, wb, 0);
+        job(s->F[CFD_U2], s->Z[4], q, 0, q, 0, 0, 0, 0, 0);
+        job(s->F[CFD_U3], s->Z[5], q, 1, q + 1, q, 0, wa, wb, 0);
+      }
     }
     s->n_nljobs = (int)J.size();
     if ((rc = upload(s, &s->d_nljobs, J))) return bail(rc);
@@ -648,7 +660,7 @@ int launched(cfd_solver* s) {
 
 void op_nonlinear(cfd_solver* s, int kRK) {
   if (kRK == 1) launch_reset_max(*s);
-  { Timed t(s, KC_ZINV); launch_zinv(*s, s->d_nljobs, s->n_nljobs, s->mixed); }
+  { Timed t(s, KC_ZINV); launch_zinv(*s, s->d_nljobs, s->n_nljobs, true); }
   { Timed t(s, KC_XPASS); launch_xpass_nl(*s, kRK == 1); }
   { Timed t(s, KC_ZFWD); launch_zfwd_nl(*s); }
   if (kRK == 1) launch_cfl(*s);
@@ -682,7 +694,7 @@ void op_solve(cfd_solver* s, int f, int kRK) {
 // du holds r only on the IB planes afterwards.
 void op_substep(cfd_solver* s, int kRK) {
   if (kRK == 1) launch_reset_max(*s);
-  { Timed t(s, KC_ZINV); launch_zinv(*s, s->d_nljobs, s->n_nljobs, s->mixed); }
+  { Timed t(s, KC_ZINV); launch_zinv(*s, s->d_nljobs, s->n_nljobs, true); }
   { Timed t(s, KC_XPASS); launch_xpass_nl(*s, kRK == 1); }
   { Timed t(s, KC_ZFWD); launch_zfwd_nl(*s); }
   if (kRK == 1) launch_cfl(*s);

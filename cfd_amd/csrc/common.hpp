@@ -115,6 +115,7 @@ struct Solver {
   // Z buffers: inverse inputs 0..5 (u1,u2i,u3 on u planes; u1i,v,u3i on v planes)
   // and products 6..11 (uu,uw,ww,vv on u planes; uv,vw on v planes), double2 or float2
   void* Z[12] = {};
+  int zs = 0;  // Z row stride in elements: Mx rounded up to 128 B when the four-step kernels own Z
   double* cscr = nullptr;  // per-column factor scratch for the y solves (3 fields)
   double2* ysc[3] = {};    // forward-sweep scratch of the fused solves
   unsigned char* d_ibmask[3] = {};  // 1 on the IB planes of field f
@@ -154,7 +155,9 @@ struct Solver {
 };
 
 // launchers (kernels_fft.hip)
-void launch_zinv(Solver& s, const ZInvJob* jobs, int njobs, bool f32 = false);
+// nl: the nonlinear planes (precision of the path, rows of zs, XCD-aware
+// order); otherwise FP64 IB planes with rows of Mx
+void launch_zinv(Solver& s, const ZInvJob* jobs, int njobs, bool nl = false);
 // transform lengths served by the four-step kernels (the FP32 path needs them)
 bool fft4_z_supported(int L);
 bool fft4_x_supported(int L);

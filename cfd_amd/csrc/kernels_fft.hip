@@ -161,6 +161,7 @@ struct XNLArgs {
   const double* dthdyv;
   DevScalars* sc;
   int nyu, nyv, Mx, ngz, cfl;
+  int zs;  // Z row stride (elements)
 };
 
 template <int LC>
@@ -245,6 +246,7 @@ struct ZNLArgs {
   void *nxz1, *nxz3, *fvv, *nxz2, *fuv, *fvw;
   const double *k1x, *k1z;
   int nyu, nyv, N2, Mx;
+  int zs;  // Z row stride (elements)
 };
 
 constexpr int kMaxOut = 8;  // outputs (k,t) per thread: N2*TI <= kMaxOut*kThreads
@@ -438,17 +440,34 @@ __device__ __forceinline__ double2 zinv_load(const ZInvJob& J, size_t off, size_
   return v;
 }
 
-// z-inverse, four-step: grid (ceil(Mx/TI), njobs), TI*max(P,Q) threads.
-// LDS X[k1][q][t].
+// z-inverse, four-step: TI*max(P,Q) threads; LDS X[k1][q][t].  Destination
+// rows have stride zs elements.  Block -> (job, x-tile): with jper = 0 the
+// grid is (ntiles, njobs); with jper > 0 it is one-dimensional and XCD-aware:
+// blocks are dealt to the 8 XCDs round-robin by block id, so block id b runs
+// on XCD b % 8, and XCD g takes the contiguous job range [g jper, (g+1) jper).
+// With the jobs ordered by y-plane, the jobs that read the same source
+// planes (a plane and its interpolation neighbours) then share one L2.
 template <typename T, int L, int P, int Q, int TI>
 __global__ void __launch_bounds__(TI * cmax(P, Q)) k_zinv4(const ZInvJob* __restrict__ jobs,
-                                                          const cplx_t<T>* __restrict__ twg, int Mx) {
+                                                          const cplx_t<T>* __restrict__ twg, int Mx, int zs,
+                                                          int njobs, int ntiles, int jper) {
   using C = cplx_t<T>;
   static_assert(P * Q == L, "P*Q");
   constexpr int N2 = Dealias<L>::N, H = Dealias<L>::H;
   __shared__ C X[P * Q * TI];
-  const ZInvJob J = jobs[blockIdx.y];
-  const int tid = threadIdx.x, i0 = blockIdx.x * TI;
+  int job, tile;
+  if (jper > 0) {
+    const int g = blockIdx.x & 7, kk = blockIdx.x >> 3;
+    const int jl = kk / ntiles;
+    job = g * jper + jl;
+    tile = kk - jl * ntiles;
+    if (jl >= jper || job >= njobs) return;
+  } else {
+    job = blockIdx.y;
+    tile = blockIdx.x;
+  }
+  const ZInvJob J = jobs[job];
+  const int tid = threadIdx.x, i0 = tile * TI;
   const size_t ncol = (size_t)N2 * Mx;
   if (tid < Q * TI) {
     const int t = tid % TI, q = tid / TI, i = i0 + t;
@@ -477,8 +496,8 @@ __global__ void __launch_bounds__(TI * cmax(P, Q)) k_zinv4(const ZInvJob* __rest
     sfor<0, Q>([&](auto qc) { u[decltype(qc)::value] = X[(k1 * Q + decltype(qc)::value) * TI + t]; });
     dft<true, Q>(u);
     if (i < Mx) {
-      C* d = (C*)J.dst + (size_t)J.pd * L * Mx + i;
-      sfor<0, Q>([&](auto c) { d[(size_t)(k1 + P * decltype(c)::value) * Mx] = u[decltype(c)::value]; });
+      C* d = (C*)J.dst + (size_t)J.pd * L * zs + i;
+      sfor<0, Q>([&](auto c) { d[(size_t)(k1 + P * decltype(c)::value) * zs] = u[decltype(c)::value]; });
     }
   }
 }
@@ -517,13 +536,13 @@ __global__ void __launch_bounds__(TI * cmax(P, Q)) k_zfwd4(ZNLArgs a, const cplx
   C acc1[Q], acc2[Q];
   const int nprod = isu ? 4 : 2;
   for (int pr = 0; pr < nprod; ++pr) {
-    const C* src = (const C*)a.z[isu ? pr : 4 + pr] + (size_t)pl * L * Mx;
+    const C* src = (const C*)a.z[isu ? pr : 4 + pr] + (size_t)pl * L * a.zs;
     if (tid < Q * TI) {
       const int t = tid % TI, q = tid / TI, i = i0 + t;
       C v[P];
       sfor<0, P>([&](auto pc) {
         constexpr int p = decltype(pc)::value;
-        v[p] = i < Mx ? src[(size_t)(Q * p + q) * Mx + i] : mk<C>(0, 0);
+        v[p] = i < Mx ? src[(size_t)(Q * p + q) * a.zs + i] : mk<C>(0, 0);
       });
       dft<false, P>(v);
       X[q * TI + t] = v[0];
@@ -660,7 +679,7 @@ __global__ void __launch_bounds__(3 * xp_width(P, Q, R)) k_xpass4(XNLArgs a, con
     C* s = S + (r * 3 + tr) * SP;
     const C *A = nullptr, *B = nullptr;
     if (z < ngz) {
-      const size_t ru = ((size_t)q * ngz + z) * Mx;
+      const size_t ru = ((size_t)q * ngz + z) * a.zs;
       if (tr == 0) {
         A = zi[0] + ru;
         B = zi[2] + ru;
@@ -787,7 +806,7 @@ __global__ void __launch_bounds__(3 * xp_width(P, Q, R)) k_xpass4(XNLArgs a, con
     const C* F = S + (r5 * 3 + tr) * SP;
     const int mb = m == 0 ? 0 : L - m;
     const C f = F[(m % Q) * (P + 1) + m / Q], g = F[(mb % Q) * (P + 1) + mb / Q];
-    const size_t ro = ((size_t)q * ngz + z) * Mx + m;
+    const size_t ro = ((size_t)q * ngz + z) * a.zs + m;
     oa[ro] = mk<C>(hs * (f.x + g.x), hs * (f.y - g.y));
     ob[ro] = mk<C>(hs * (f.y + g.y), -hs * (f.x - g.x));
   }
@@ -869,22 +888,28 @@ bool four_step_z(const Solver& s) {
 }
 
 template <typename T>
-bool zinv4_t(Solver& s, const ZInvJob* jobs, int njobs, const cplx_t<T>* tw) {
+bool zinv4_t(Solver& s, const ZInvJob* jobs, int njobs, const cplx_t<T>* tw, int zs, bool xcd) {
   switch (s.pz.L) {
-#define C(L, P, Q, TI)                                                                                      \
-  case L:                                                                                                   \
-    hipLaunchKernelGGL((k_zinv4<T, L, P, Q, TI>), dim3((s.Mx + TI - 1) / TI, njobs), dim3(TI * cmax(P, Q)), \
-                       0, s.stream, jobs, tw, s.Mx);                                                       \
-    return true;
+#define C(L, P, Q, TI)                                                                                 \
+  case L: {                                                                                            \
+    const int nt = (s.Mx + TI - 1) / TI, jper = xcd ? (njobs + 7) / 8 : 0;                             \
+    const dim3 grid = xcd ? dim3(8 * jper * nt) : dim3(nt, njobs);                                     \
+    hipLaunchKernelGGL((k_zinv4<T, L, P, Q, TI>), grid, dim3(TI * cmax(P, Q)), 0, s.stream, jobs, tw, \
+                       s.Mx, zs, njobs, nt, jper);                                                     \
+    return true;                                                                                       \
+  }
     CFD_ZINV4(C)
 #undef C
     default: return false;
   }
 }
 
-bool try_zinv4(Solver& s, const ZInvJob* jobs, int njobs, bool f32) {
+bool try_zinv4(Solver& s, const ZInvJob* jobs, int njobs, bool nl) {
   if (!four_step_z(s)) return false;
-  return f32 ? zinv4_t<float>(s, jobs, njobs, s.d_twzf) : zinv4_t<double>(s, jobs, njobs, s.d_twz);
+  // nonlinear planes: precision of the path, padded rows, XCD-aware order;
+  // IB planes: FP64, rows of Mx
+  if (nl && s.mixed) return zinv4_t<float>(s, jobs, njobs, s.d_twzf, s.zs, true);
+  return zinv4_t<double>(s, jobs, njobs, s.d_twz, nl ? s.zs : s.Mx, nl);
 }
 
 template <typename T>
@@ -958,9 +983,9 @@ bool fft4_x_supported(int L) {
   }
 }
 
-void launch_zinv(Solver& s, const ZInvJob* jobs, int njobs, bool f32) {
+void launch_zinv(Solver& s, const ZInvJob* jobs, int njobs, bool nl) {
   if (njobs <= 0) return;
-  if (try_zinv4(s, jobs, njobs, f32)) return;
+  if (try_zinv4(s, jobs, njobs, nl)) return;
   switch (s.pz.L) {
 #define C(n) case n: zinv_t<n>(s, jobs, njobs); break;
     CFD_FFT_SIZES(C)
@@ -982,6 +1007,7 @@ void launch_xpass_nl(Solver& s, int cfl) {
   a.Mx = s.Mx;
   a.ngz = s.ngz;
   a.cfl = cfl;
+  a.zs = s.zs;
   if (try_xpass4(s, a)) return;
   switch (s.px.L) {
 #define C(n) case n: xpass_nl_t<n>(s, a); break;
@@ -1006,6 +1032,7 @@ void launch_zfwd_nl(Solver& s) {
   a.nyv = s.nyv;
   a.N2 = s.N2;
   a.Mx = s.Mx;
+  a.zs = s.zs;
   if (try_zfwd4(s, a)) return;
   switch (s.pz.L) {
 #define C(n) case n: zfwd_nl_t<n>(s, a); break;
