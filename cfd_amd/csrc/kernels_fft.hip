@@ -639,8 +639,9 @@ __device__ __forceinline__ C pair_mode_q(const C* __restrict__ A, const C* __res
   }
 }
 
-// lanes per transform of the x-pass block: one group of waves per transform
-constexpr int xp_width(int P, int Q, int R) { return rup64(cmax(P * R, Q * R)); }
+// lanes per transform of the x-pass block: one group of waves per transform,
+// one thread per phase-2 task (k1, r); phases 1 and 4 loop over their Q R tasks
+constexpr int xp_width(int P, int Q, int R) { return rup64(P * R); }
 
 __device__ inline double wave_max(double v) {
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
@@ -658,10 +659,10 @@ __device__ inline double wave_max(double v) {
 template <typename T, int L, int P, int Q, int R>
 __global__ void __launch_bounds__(3 * xp_width(P, Q, R)) k_xpass4(XNLArgs a, const cplx_t<T>* __restrict__ twg) {
   using C = cplx_t<T>;
-  static_assert(P * Q == L && Q <= P, "P*Q, Q <= P");
+  static_assert(P * Q == L, "P*Q");
   constexpr int Mx = Dealias<L>::Mx;
   constexpr int W = xp_width(P, Q, R);
-  constexpr int SP = P * (Q + 1);  // >= Q * (P + 1) as Q <= P
+  constexpr int SP = cmax(P * (Q + 1), Q * (P + 1));  // slots per transform and row
   __shared__ C S[R * 3 * SP];
   const int tid = threadIdx.x;
   const int tr = tid / W, lane = tid - tr * W;
@@ -881,7 +882,13 @@ void zfwdp_t(Solver& s, const ZFwdJob* jobs, int njobs) {
 // where the length allows, so whole groups of the empty band drop out.
 #define CFD_ZINV4(X) X(48, 6, 8, 16) X(64, 8, 8, 16) X(192, 12, 16, 16) X(384, 24, 16, 8)
 #define CFD_ZFWD4(X) X(48, 8, 6, 16) X(64, 8, 8, 16) X(192, 16, 12, 16)
-#define CFD_XPASS4(X) X(48, 8, 6, 8) X(64, 8, 8, 8) X(72, 12, 6, 4) X(384, 24, 16, 2)
+#ifndef CFD_XPASS4_F64
+#define CFD_XPASS4_F64(X) X(48, 8, 6, 8) X(64, 8, 8, 8) X(72, 12, 6, 4) X(384, 16, 24, 4)
+#endif
+#ifndef CFD_XPASS4_F32
+#define CFD_XPASS4_F32(X) X(48, 8, 6, 8) X(64, 8, 8, 8) X(72, 12, 6, 4) X(384, 16, 24, 4)
+#endif
+#define CFD_XPASS4(X) CFD_XPASS4_F64(X)
 
 bool four_step_z(const Solver& s) {
   return (s.mixed || !s.fft_stockham) && s.N2 == 2 * (s.ngz / 3) && s.pz.L == s.ngz;
@@ -931,15 +938,25 @@ bool try_zfwd4(Solver& s, const ZNLArgs& a) {
   return s.mixed ? zfwd4_t<float>(s, a, s.d_twzf) : zfwd4_t<double>(s, a, s.d_twz);
 }
 
-template <typename T>
-bool xpass4_t(Solver& s, const XNLArgs& a, const cplx_t<T>* tw) {
-  switch (s.px.L) {
-#define C(L, P, Q, R)                                                                                       \
-  case L:                                                                                                   \
-    hipLaunchKernelGGL((k_xpass4<T, L, P, Q, R>), dim3((s.ngz + R - 1) / R, s.nyu),                     \
-                       dim3(3 * xp_width(P, Q, R)), 0, s.stream, a, tw);                                  \
+#define CFD_XP_CASE(T, L, P, Q, R)                                                                  \
+  case L:                                                                                         \
+    hipLaunchKernelGGL((k_xpass4<T, L, P, Q, R>), dim3((s.ngz + R - 1) / R, s.nyu),               \
+                       dim3(3 * xp_width(P, Q, R)), 0, s.stream, a, tw);                            \
     return true;
-    CFD_XPASS4(C)
+
+bool xpass4_t(Solver& s, const XNLArgs& a, const double2* tw) {
+  switch (s.px.L) {
+#define C(L, P, Q, R) CFD_XP_CASE(double, L, P, Q, R)
+    CFD_XPASS4_F64(C)
+#undef C
+    default: return false;
+  }
+}
+
+bool xpass4_t(Solver& s, const XNLArgs& a, const float2* tw) {
+  switch (s.px.L) {
+#define C(L, P, Q, R) CFD_XP_CASE(float, L, P, Q, R)
+    CFD_XPASS4_F32(C)
 #undef C
     default: return false;
   }
@@ -948,7 +965,7 @@ bool xpass4_t(Solver& s, const XNLArgs& a, const cplx_t<T>* tw) {
 bool try_xpass4(Solver& s, const XNLArgs& a) {
   if ((s.fft_stockham && !s.mixed) || s.px.L != s.ngx || s.N1 != 2 * (s.ngx / 3) || s.Mx != s.N1 / 2 + 1)
     return false;
-  return s.mixed ? xpass4_t<float>(s, a, s.d_twxf) : xpass4_t<double>(s, a, s.d_twx);
+  return s.mixed ? xpass4_t(s, a, s.d_twxf) : xpass4_t(s, a, s.d_twx);
 }
 
 // transform lengths with compile-time plans; other lengths use the runtime plan
