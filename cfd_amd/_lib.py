@@ -20,6 +20,12 @@ class CParams(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in INT_FIELDS] + [(n, ctypes.c_double) for n in DBL_FIELDS]
 
 
+class CFileHeader(ctypes.Structure):
+    """Mirror of ``cfd_file_header`` (record 1 and 2 of a field file)."""
+    _fields_ = [(n, ctypes.c_double) for n in ("t", "Re", "alp", "bet", "mpgx")] + \
+               [("nband", ctypes.c_int), ("iter", ctypes.c_int), ("N", ctypes.c_int * 20)]
+
+
 # symbol -> (restype, argtypes)
 _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int)
@@ -49,6 +55,13 @@ SIGNATURES = {
     "cfd_sync": (ctypes.c_int, [_H]),
     "cfd_profile": (ctypes.c_int, [_H, ctypes.c_int]),
     "cfd_profile_read": (ctypes.c_int, [_H, _D, _I]),
+    "cfd_field_file_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(CFileHeader), ctypes.c_int,
+                                            ctypes.c_double, _D, _D] + [ctypes.c_int] * 6 + [_D]),
+    "cfd_field_file_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(CFileHeader)] + [ctypes.c_int] * 4
+                            + [_D, _I]),
+    "cfd_record_out": (ctypes.c_int, [_H, ctypes.c_char_p, ctypes.c_int]),
+    "cfd_band_table": (ctypes.c_int, [ctypes.POINTER(CParams), _I]),
+    "cfd_read_in": (ctypes.c_int, [_H, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, _I]),
 }
 
 _lib = None

@@ -11,6 +11,7 @@ Fields cross the boundary in the reference layout: complex128 arrays of shape
 outer / i inner (start.f90:1467-1481).
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -165,6 +166,22 @@ class Channel:
 
     def sync(self):
         self._call("cfd_sync")
+
+    # -- field files ------------------------------------------------------------
+    def record_out(self, dirout, iter=0):
+        """Write u1, u2, u3, p as the reference's record_out does (file names
+        <f>_<N1>x<N2>x<ny>_t<int(10 t)>.dat in dirout); returns the u1 path."""
+        self._call("cfd_record_out", os.fsencode(dirout), int(iter))
+        t = self.scalars()["t"]
+        return os.path.join(dirout, "u1_%04dx%04dx%04d_t%05d.dat" % (self.N1, self.N2, self.params.ny,
+                                                                     int(10.0 * t)))
+
+    def read_in(self, dirin, fnameimb, flag_init=2):
+        """read_in + the flag_init branch of start: reads dirin/u1<tail> ...
+        with tail = fnameimb[2:]; returns iter0."""
+        it = ctypes.c_int(0)
+        self._call("cfd_read_in", os.fsencode(dirin), os.fsencode(fnameimb), int(flag_init), ctypes.byref(it))
+        return it.value
 
     # -- timing ----------------------------------------------------------------
     def profile(self, enable=True):

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "io.hpp"
 
 using namespace cfd;
 
@@ -810,5 +811,112 @@ extern "C" int cfd_profile_read(cfd_handle h, double* ms, int* cnt) {
     if (ms) ms[c] = s->prof_ms[c];
     if (cnt) cnt[c] = s->prof_cnt[c];
   }
+  return CFD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// field files (record_out / read_in)
+// ---------------------------------------------------------------------------
+namespace {
+
+struct FileField {
+  int which, id, grid;  // field, record id, 0 = u/w centres, 1 = v faces, 2 = p centres
+  const char* name;
+};
+const FileField kFileFields[4] = {{CFD_U1, 1, 0, "u1"}, {CFD_U2, 2, 1, "u2"}, {CFD_U3, 3, 0, "u3"}, {CFD_P, 4, 2, "p"}};
+
+}  // namespace
+
+extern "C" int cfd_record_out(cfd_handle h, const char* dirout, int iter) {
+  cfd_solver* s = static_cast<cfd_solver*>(h);
+  if (!s || !dirout) return fail(s, CFD_EINVAL, "null argument");
+  DevScalars d{};
+  HIPCHK(s, hipSetDevice(s->dev));
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  HIPCHK(s, hipMemcpy(&d, s->d_sc, sizeof d, hipMemcpyDeviceToHost));
+  cfd_file_header hd{};
+  hd.t = d.t;
+  hd.Re = s->P.Re;
+  hd.alp = s->P.alp;
+  hd.bet = s->P.bet;
+  hd.mpgx = d.mpgx;
+  hd.nband = 3;
+  hd.iter = iter;
+  band_table(s->P, s->yg, hd.N);
+  char tail[96];
+  snprintf(tail, sizeof tail, "_%04d", s->N1);
+  std::string name_tail = tail;
+  snprintf(tail, sizeof tail, "x%04dx%04d_t%05d.dat", s->N2, s->ny, (int)(10.0 * d.t));
+  name_tail += tail;
+  const int nx = 2 * s->Mx, nz = s->N2;
+  for (const FileField& ff : kFileFields) {
+    const int ny_g = ny_of(*s, ff.which);
+    std::vector<double> planes((size_t)ny_g * nx * nz);
+    HIPCHK(s, hipMemcpy(planes.data(), s->F[ff.which], planes.size() * sizeof(double), hipMemcpyDeviceToHost));
+    // u1, u3, p carry the u/w centres and their metric, u2 the v faces
+    // (FOU3D.f90:2242-2340); dthetavi is never assigned in the reference
+    // (declaration.f90:72), so its zero-initialised module value is written
+    const bool v = ff.grid == 1;
+    const std::vector<double>& y = v ? s->yg.yv : s->yg.yu;
+    const std::vector<double>& dy = v ? s->yg.dthdyv : s->yg.dthdyu;
+    const int jlo = ff.grid == 0 ? s->jlo_u : (ff.grid == 1 ? s->jlo_v : s->jlo_p);
+    const std::string path = std::string(dirout) + "/" + ff.name + name_tail;
+    const std::string e = write_field_file(path.c_str(), hd, ff.id, 0.0, y.data(), dy.data(), (int)y.size(),
+                                           v ? s->jlo_v : s->jlo_u, ny_g, jlo, nx, nz, planes.data());
+    if (!e.empty()) return fail(s, CFD_EINVAL, e);
+  }
+  return CFD_OK;
+}
+
+extern "C" int cfd_read_in(cfd_handle h, const char* dirin, const char* fnameimb, int flag_init, int* iter0) {
+  cfd_solver* s = static_cast<cfd_solver*>(h);
+  if (!s || !dirin || !fnameimb) return fail(s, CFD_EINVAL, "null argument");
+  if (flag_init != 1 && flag_init != 2) return fail(s, CFD_EINVAL, "flag_init must be 1 or 2 to read fields");
+  const std::string name = fnameimb;
+  const std::string tail = name.size() > 2 ? name.substr(2) : std::string();  // start.f90:2274
+  int N[20];
+  band_table(s->P, s->yg, N);
+  const int nx = 2 * s->Mx, nz = s->N2;
+  cfd_file_header h1{};
+  HIPCHK(s, hipSetDevice(s->dev));
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  for (const FileField& ff : kFileFields) {
+    const int ny_g = ny_of(*s, ff.which);
+    std::vector<double> planes((size_t)ny_g * nx * nz);
+    cfd_file_header hd{};
+    int mism = 0;
+    const std::string path = std::string(dirin) + "/" + ff.name + tail;
+    const std::string e = read_field_file(path.c_str(), hd, ny_g, nx, nz, s->ngz, planes.data(), mism);
+    if (!e.empty()) return fail(s, CFD_EINVAL, e);
+    if (ff.which == CFD_U1) {
+      h1 = hd;
+      if (hd.nband != 3) return fail(s, CFD_EINVAL, "wrong startfile? nband in " + path);
+      if (hd.alp != s->P.alp || hd.bet != s->P.bet)
+        fprintf(stderr, "wrong startfile? alp,alp_old= %g %g bet,bet_old= %g %g\n", s->P.alp, hd.alp, s->P.bet,
+                hd.bet);
+      bool diff = false;  // start.f90:2305-2321: N(1:3, bands) and N(3,0), N(3,nband)
+      for (int b = 1; b <= 3; ++b)
+        for (int r = 0; r < 3; ++r) diff = diff || N[b * 4 + r] != hd.N[b * 4 + r];
+      diff = diff || N[2] != hd.N[2] || N[3 * 4 + 2] != hd.N[3 * 4 + 2];
+      if (diff) fprintf(stderr, "WARNING startfile size (%s)\n", path.c_str());
+    }
+    if (mism) fprintf(stderr, "WARNING!: unexpected size of %d planes in %s\n", mism, path.c_str());
+    HIPCHK(s, hipMemcpy(s->F[ff.which], planes.data(), planes.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  // flag_init branches (start.f90:1974-2009): time, iteration, target flow rate
+  DevScalars d{};
+  HIPCHK(s, hipMemcpy(&d, s->d_sc, sizeof d, hipMemcpyDeviceToHost));
+  d.t = flag_init == 2 ? h1.t : 0.0;
+  if (iter0) *iter0 = flag_init == 2 ? h1.iter : 0;
+  if (s->P.flag_ctpress != 1) {
+    // flowrateIm of the (0,0) column, tip to tip (littleharsh.f90:411-428)
+    std::vector<double2> col(s->nyu);
+    HIPCHK(s, hipMemcpy2D(col.data(), sizeof(double2), s->F[CFD_U1], s->Ncol * sizeof(double2), sizeof(double2),
+                          s->nyu, hipMemcpyDeviceToHost));
+    double Q = 0.0;
+    for (int j = 1; j <= s->nn + 1; ++j) Q = Q + (s->yg.YV(j) - s->yg.YV(j - 1)) * col[j - s->jlo_u].x;
+    d.QxT = Q;
+  }
+  HIPCHK(s, hipMemcpy(s->d_sc, &d, sizeof d, hipMemcpyHostToDevice));
   return CFD_OK;
 }

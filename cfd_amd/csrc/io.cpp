@@ -1,0 +1,204 @@
+// io.cpp -- field files in the reference layout (host code).
+//
+// The files are Fortran sequential unformatted: each record is its payload
+// framed by a 4-byte little-endian length before and after.  Layout of the
+// records: record_out (FOU3D.f90:2172-2345) and read_in (start.f90:2251-2700),
+// summarised in cfd_amd.h.  The plane buffer of u_to_buff (FOU3D.f90:2598-2627)
+// is the spectral plane with x-modes fastest and the z-modes in column order,
+// which is also the device layout of one plane, so planes move as one block.
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "io.hpp"
+
+namespace cfd {
+
+namespace {
+
+struct File {
+  FILE* f = nullptr;
+  ~File() {
+    if (f) fclose(f);
+  }
+};
+
+// a record under construction
+struct Rec {
+  std::vector<unsigned char> b;
+  template <typename T>
+  void put(const T* p, size_t n) {
+    const unsigned char* c = reinterpret_cast<const unsigned char*>(p);
+    b.insert(b.end(), c, c + n * sizeof(T));
+  }
+  template <typename T>
+  void put(T v) {
+    put(&v, 1);
+  }
+};
+
+bool write_rec(FILE* f, const Rec& r) {
+  const uint32_t n = (uint32_t)r.b.size();
+  return fwrite(&n, 4, 1, f) == 1 && (n == 0 || fwrite(r.b.data(), 1, n, f) == n) && fwrite(&n, 4, 1, f) == 1;
+}
+
+// read one record's payload; false at end of file or on a broken frame
+bool read_rec(FILE* f, std::vector<unsigned char>& b) {
+  uint32_t n = 0, m = 0;
+  if (fread(&n, 4, 1, f) != 1) return false;
+  b.resize(n);
+  if (n && fread(b.data(), 1, n, f) != n) return false;
+  if (fread(&m, 4, 1, f) != 1 || m != n) return false;
+  return true;
+}
+
+template <typename T>
+T get(const std::vector<unsigned char>& b, size_t off) {
+  T v;
+  std::memcpy(&v, b.data() + off, sizeof(T));
+  return v;
+}
+
+}  // namespace
+
+std::string write_field_file(const char* path, const cfd_file_header& hd, int id, double dthetavi, const double* y,
+                             const double* dthdy, int ny_arr, int jy0, int ny_g, int j0, int nx, int nz,
+                             const double* planes) {
+  if (j0 < jy0 || j0 + ny_g > jy0 + ny_arr) return "plane range outside the grid array";
+  File F;
+  F.f = fopen(path, "wb");
+  if (!F.f) return std::string("cannot open ") + path + " for writing";
+  Rec r1;
+  r1.put(hd.t);
+  r1.put(hd.Re);
+  r1.put(hd.alp);
+  r1.put(hd.bet);
+  r1.put(hd.mpgx);
+  r1.put((int32_t)hd.nband);
+  r1.put((int32_t)hd.iter);
+  const int32_t zeros[88] = {};
+  r1.put(zeros, 88);
+  Rec r2;
+  for (int q = 0; q < 20; ++q) r2.put((int32_t)hd.N[q]);
+  Rec r3;
+  r3.put(y, (size_t)ny_arr);
+  r3.put(dthetavi);
+  r3.put(dthdy, (size_t)ny_arr);
+  if (!write_rec(F.f, r1) || !write_rec(F.f, r2) || !write_rec(F.f, r3)) return std::string("write error on ") + path;
+  const size_t pl = (size_t)nx * nz;
+  for (int q = 0; q < ny_g; ++q) {
+    Rec r;
+    r.put((int32_t)(j0 + q));
+    r.put((int32_t)id);
+    r.put((int32_t)nx);
+    r.put((int32_t)nz);
+    r.put(y[j0 + q - jy0]);
+    r.put(planes + q * pl, pl);
+    if (!write_rec(F.f, r)) return std::string("write error on ") + path;
+  }
+  return "";
+}
+
+std::string read_field_file(const char* path, cfd_file_header& hd, int ny_g, int nx, int nz, int ngz, double* planes,
+                            int& mismatch) {
+  mismatch = 0;
+  File F;
+  F.f = fopen(path, "rb");
+  if (!F.f) return std::string("cannot open ") + path;
+  std::vector<unsigned char> b;
+  if (!read_rec(F.f, b) || b.size() < 5 * 8 + 2 * 4) return std::string("bad header record in ") + path;
+  hd.t = get<double>(b, 0);
+  hd.Re = get<double>(b, 8);
+  hd.alp = get<double>(b, 16);
+  hd.bet = get<double>(b, 24);
+  hd.mpgx = get<double>(b, 32);
+  hd.nband = get<int32_t>(b, 40);
+  hd.iter = get<int32_t>(b, 44);
+  if (!read_rec(F.f, b)) return std::string("bad N record in ") + path;
+  for (int q = 0; q < 20; ++q) hd.N[q] = (size_t)(q + 1) * 4 <= b.size() ? get<int32_t>(b, (size_t)q * 4) : 0;
+  if (!read_rec(F.f, b)) return std::string("bad grid record in ") + path;
+  const size_t pl = (size_t)nx * nz;
+  std::memset(planes, 0, sizeof(double) * pl * ny_g);
+  for (int q = 0; q < ny_g; ++q) {
+    if (!read_rec(F.f, b) || b.size() < 4 * 4 + 8) return std::string("missing plane record in ") + path;
+    const int nxf = get<int32_t>(b, 8), nzf = get<int32_t>(b, 12);
+    if (nxf < 0 || nzf < 0 || b.size() != 4 * 4 + 8 + (size_t)nxf * nzf * 8)
+      return std::string("bad plane record in ") + path;
+    if (nxf != nx || nzf != nz) ++mismatch;
+    const size_t d0 = 4 * 4 + 8;
+    double* u = planes + q * pl;
+    // buff_to_u into a plane of ngz z-slots, then the slots that carry one of
+    // the nz target modes (1..nz/2 at the bottom, the rest at the top)
+    const int kh = std::min(nzf / 2, ngz / 2), ix = std::min(nxf, nx);
+    auto copy = [&](int kf, int slot) {
+      int kt = -1;
+      if (slot <= nz / 2) kt = slot;
+      else if (slot >= ngz - (nz - nz / 2) + 1) kt = slot - ngz + nz;
+      if (kt < 1 || kt > nz) return;
+      for (int i = 0; i < ix; ++i)
+        u[(size_t)(kt - 1) * nx + i] = get<double>(b, d0 + ((size_t)(kf - 1) * nxf + i) * 8);
+    };
+    for (int kf = 1; kf <= kh; ++kf) copy(kf, kf);
+    for (int kf = nzf - kh + 1; kf <= nzf; ++kf) copy(kf, kf + ngz - nzf);
+  }
+  return "";
+}
+
+void band_table(const cfd_params& p, const YGrid& g, int N[20]) {
+  // N(1:4, 0:nband+1), nband = 3 (start.f90:89-111, y_grid_canopy.f90:250-275)
+  const int N1 = 2 * (p.ngalx / 3), N2 = 2 * (p.ngalz / 3), dny = p.dsty, nn = p.ny - 2;
+  // height of the fine bottom band (start.f90:199-226 for the geometry types built here)
+  const double pi = 3.14159265358979323846;
+  const double h = p.geometry_type == 3 ? 2.0 * (2.0 * pi / p.bet) / p.ntilez : 0.25;
+  const double target = g.YU(0) + h;
+  int j = -dny;
+  while (j < g.jlo_u + g.nyu - 1 && g.YU(j) < target) ++j;
+  const int N31 = j, N32 = nn - j, N33 = nn + dny;
+  const int tab[5][4] = {{0, 0, -dny, -dny}, {N1, N2, N31, N31 + 1}, {N1, N2, N32, N32}, {N1, N2, N33, N33 + 1},
+                         {-2, 0, 0, 0}};
+  for (int b = 0; b < 5; ++b)
+    for (int r = 0; r < 4; ++r) N[b * 4 + r] = tab[b][r];
+}
+
+}  // namespace cfd
+
+extern "C" int cfd_field_file_write(const char* path, const cfd_file_header* hd, int id, double dthetavi,
+                                    const double* y, const double* dthdy, int ny_arr, int jy0, int ny_g, int j0,
+                                    int nx, int nz, const double* planes) {
+  if (!path || !hd || !y || !dthdy || !planes || ny_arr <= 0 || ny_g <= 0 || nx <= 0 || nz <= 0) return CFD_EINVAL;
+  const std::string e =
+      cfd::write_field_file(path, *hd, id, dthetavi, y, dthdy, ny_arr, jy0, ny_g, j0, nx, nz, planes);
+  if (!e.empty()) {
+    fprintf(stderr, "cfd_field_file_write: %s\n", e.c_str());
+    return CFD_EINVAL;
+  }
+  return CFD_OK;
+}
+
+extern "C" int cfd_field_file_read(const char* path, cfd_file_header* hd, int ny_g, int nx, int nz, int ngz,
+                                   double* planes, int* mismatch) {
+  if (!path || !hd || !planes || ny_g <= 0 || nx <= 0 || nz <= 0 || ngz < nz) return CFD_EINVAL;
+  int mm = 0;
+  const std::string e = cfd::read_field_file(path, *hd, ny_g, nx, nz, ngz, planes, mm);
+  if (mismatch) *mismatch = mm;
+  if (!e.empty()) {
+    fprintf(stderr, "cfd_field_file_read: %s\n", e.c_str());
+    return CFD_EINVAL;
+  }
+  return CFD_OK;
+}
+
+extern "C" int cfd_band_table(const cfd_params* p, int* N20) {
+  if (!p || !N20) return CFD_EINVAL;
+  cfd::YGrid g;
+  std::string err;
+  if (!cfd::build_ygrid(*p, g, err)) {
+    fprintf(stderr, "cfd_band_table: %s\n", err.c_str());
+    return CFD_EINVAL;
+  }
+  cfd::band_table(*p, g, N20);
+  return CFD_OK;
+}

@@ -113,6 +113,21 @@ module cfd_amd_mod
       import :: c_int, c_ptr
       type(c_ptr), value :: h
     end function
+    ! record_out (FOU3D.f90:2172-2345): u1, u2, u3, p files in dirout
+    integer(c_int) function cfd_record_out(h, dirout, iter) bind(C, name="cfd_record_out")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: h
+      character(kind=c_char) :: dirout(*)
+      integer(c_int), value :: iter
+    end function
+    ! mblock_ini + read_in (start.f90:2061-2097, 2251-2700), flag_init 1 or 2
+    integer(c_int) function cfd_read_in(h, dirin, fnameimb, flag_init, iter0) bind(C, name="cfd_read_in")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: h
+      character(kind=c_char) :: dirin(*), fnameimb(*)
+      integer(c_int), value :: flag_init
+      integer(c_int) :: iter0
+    end function
   end interface
 
 contains

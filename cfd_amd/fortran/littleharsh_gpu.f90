@@ -3,17 +3,20 @@
 ! RK loop restored: it reads input.in in the layout `start` reads
 ! (start.f90:62-227, formats 10/20/21/30 of start.f90:434-437, so a real
 ! written without a decimal point keeps the implied-decimal semantics),
-! reads geom_circ_rough.txt, sets up the device solver, starts from the
-! parabolic profile of flag_init = 3 (start.f90:2109-2249) and calls the hot
-! path through the C ABI in the reference order.
+! reads geom_circ_rough.txt, sets up the device solver, starts from field
+! files (flag_init = 1 new, 2 continue: read_in, start.f90:1974-2009) or from
+! the parabolic profile of flag_init = 3 (start.f90:2109-2249), calls the hot
+! path through the C ABI in the reference order, writes the field files every
+! nwrite steps at the start of a step (record_out, FOU3D.f90:58-70, 130-142)
+! and once more at the end (finalize, littleharsh.f90:353-390).
 program littleharsh_gpu
   use iso_c_binding
   use cfd_amd_mod
   implicit none
   type(cfd_params) :: p
   type(c_ptr) :: h
-  integer(c_int) :: rc, d(16), kRK
-  integer :: i, Ngalx2, Ngalz2, flag_init, nwrite, iter, maxiter, npeakx, npeakz, dstx, dstz
+  integer(c_int) :: rc, d(16), kRK, iter0
+  integer :: i, Ngalx2, Ngalz2, flag_init, nwrite, iter, iwrite, maxiter, npeakx, npeakz, dstx, dstz
   integer :: physlim_bot, physlim_top, nyu, nyv, nyp, ncol, jlo, nn, j
   real(8) :: Kib, maxerr, maxt, maxA, hny, Lfracx, Lfracz, sc(8), Qx
   character(len=90) :: fnameimb, dirin, dirout
@@ -140,36 +143,56 @@ program littleharsh_gpu
   rc = cfd_get_ygrid(h, yu, yv, du, dv, d2u, d2v)
   call cfd_check(rc, h, 'cfd_get_ygrid')
 
-  ! flag_init = 3: parabolic mean profile and constant mean of v, w, p
-  ! (mblock_ini_parabolic_profile, start.f90:2109-2249)
   allocate(u1(nyu, ncol), u2(nyv, ncol), u3(nyu, ncol), pp(nyp, ncol))
-  u1 = 0d0; u2 = 0d0; u3 = 0d0; pp = 0d0
-  do j = 1, nyu
-    u1(j, 1) = 0.5d0 * p%Re * p%mpgx * (yu(j)**2 - 1d0)
-    u3(j, 1) = 1d-5
-  end do
-  u2(:, 1) = 1d-6
-  pp(:, 1) = 1d0
-  rc = cfd_set_field(h, CFD_U1, u1);   call cfd_check(rc, h, 'set u1')
-  rc = cfd_set_field(h, CFD_U2, u2);   call cfd_check(rc, h, 'set u2')
-  rc = cfd_set_field(h, CFD_U3, u3);   call cfd_check(rc, h, 'set u3')
-  rc = cfd_set_field(h, CFD_P, pp);    call cfd_check(rc, h, 'set p')
-  ! flowrateIm tip to tip (littleharsh.f90:411-428) -> QxT
-  nn = p%ny - 2
-  Qx = 0d0
-  do j = 1, nn + 1
-    Qx = Qx + (yv(j - jlo + 1) - yv(j - jlo)) * real(u1(j - jlo + 1, 1), 8)
-  end do
-  sc = 0d0
-  sc(CFD_MPGX) = p%mpgx
-  sc(CFD_QXT) = Qx
-  sc(CFD_QX) = Qx
-  rc = cfd_set_scalars(h, sc);         call cfd_check(rc, h, 'set scalars')
+  iter0 = 0
+  if (flag_init == 1 .or. flag_init == 2) then
+    sc = 0d0
+    sc(CFD_MPGX) = p%mpgx
+    rc = cfd_set_scalars(h, sc);       call cfd_check(rc, h, 'set scalars')
+    rc = cfd_read_in(h, trim(dirin)//c_null_char, trim(fnameimb)//c_null_char, flag_init, iter0)
+    call cfd_check(rc, h, 'read_in')
+    rc = cfd_get_scalars(h, sc);       call cfd_check(rc, h, 'get scalars')
+    Qx = sc(CFD_QXT)
+  else if (flag_init == 3) then
+    ! parabolic mean profile and constant mean of v, w, p
+    ! (mblock_ini_parabolic_profile, start.f90:2109-2249)
+    u1 = 0d0; u2 = 0d0; u3 = 0d0; pp = 0d0
+    do j = 1, nyu
+      u1(j, 1) = 0.5d0 * p%Re * p%mpgx * (yu(j)**2 - 1d0)
+      u3(j, 1) = 1d-5
+    end do
+    u2(:, 1) = 1d-6
+    pp(:, 1) = 1d0
+    rc = cfd_set_field(h, CFD_U1, u1);   call cfd_check(rc, h, 'set u1')
+    rc = cfd_set_field(h, CFD_U2, u2);   call cfd_check(rc, h, 'set u2')
+    rc = cfd_set_field(h, CFD_U3, u3);   call cfd_check(rc, h, 'set u3')
+    rc = cfd_set_field(h, CFD_P, pp);    call cfd_check(rc, h, 'set p')
+    ! flowrateIm tip to tip (littleharsh.f90:411-428) -> QxT
+    nn = p%ny - 2
+    Qx = 0d0
+    do j = 1, nn + 1
+      Qx = Qx + (yv(j - jlo + 1) - yv(j - jlo)) * real(u1(j - jlo + 1, 1), 8)
+    end do
+    sc = 0d0
+    sc(CFD_MPGX) = p%mpgx
+    sc(CFD_QXT) = Qx
+    sc(CFD_QX) = Qx
+    rc = cfd_set_scalars(h, sc);         call cfd_check(rc, h, 'set scalars')
+  else
+    write(*,*) 'INITIAL CONDITIONS NOT IMPLEMENTED YET'
+    error stop 1
+  end if
 
   write(*,'(a,3i6,a,i8,a,es12.5)') ' grid', p%ngalx, p%ny, p%ngalz, '  columns', ncol, '  QxT', Qx
-  iter = 0
+  iter = iter0
+  iwrite = iter
   rc = cfd_get_scalars(h, sc);         call cfd_check(rc, h, 'get scalars')
-  do while (sc(CFD_T) < maxt .and. iter < maxiter)
+  do while (sc(CFD_T) < maxt .and. iter - iter0 < maxiter)
+    if (iter >= iwrite) then
+      rc = cfd_record_out(h, trim(dirout)//c_null_char, iter)
+      call cfd_check(rc, h, 'record_out')
+      iwrite = iwrite + nwrite
+    end if
     do kRK = 1, 3
       rc = cfd_rhs0(h, kRK);           call cfd_check(rc, h, 'RHS0')
       rc = cfd_nonlinear(h, kRK);      call cfd_check(rc, h, 'nonlinear')
@@ -184,11 +207,14 @@ program littleharsh_gpu
     end do
     iter = iter + 1
     rc = cfd_get_scalars(h, sc);       call cfd_check(rc, h, 'get scalars')
-    if (mod(iter, max(nwrite, 1)) == 0 .or. iter == maxiter) then
+    if (mod(iter, max(nwrite, 1)) == 0 .or. iter - iter0 == maxiter) then
       write(*,'(a,i8,a,es14.7,a,es12.5,a,es14.7)') ' iter', iter, '  t', sc(CFD_T), '  dt', sc(CFD_DT), &
                                                    '  mpgx', sc(CFD_MPGX)
     end if
   end do
+  ! finalize: one more record of the final state
+  rc = cfd_record_out(h, trim(dirout)//c_null_char, iter)
+  call cfd_check(rc, h, 'record_out')
   rc = cfd_get_field(h, CFD_U1, u1);   call cfd_check(rc, h, 'get u1')
   write(*,'(a,es22.15)') ' u1(0,0) at the centreline ', real(u1(nyu/2 + 1, 1), 8)
   rc = cfd_destroy(h)

@@ -111,6 +111,60 @@ int cfd_substep(cfd_handle h, int kRK);          /* body of the RK loop, littleh
    with the RK loop restored), plus meanflow_ctP when flag_ctpress = 1.
    Asynchronous: returns once the work is queued on the handle's stream. */
 int cfd_step(cfd_handle h, int nsteps);
+/* ---------------------------------------------------------------------------
+ * Field files in the reference layout: Fortran sequential unformatted, 4-byte
+ * little-endian record markers (record_out, FOU3D.f90:2172-2345; u_to_buff /
+ * buff_to_u, FOU3D.f90:2598-2654; read_in, start.f90:2251-2700):
+ *   record 1  t, Re, alp, bet, mpgx (real8), nband, iter (int4), 88 int4 zeros
+ *   record 2  N(4, 0:nband+1) (int4, column-major)
+ *   record 3  y(j0 .. j0+ny_g-1), dthetavi, dthdy(same range)   (real8)
+ *   then one record per plane j ascending:
+ *             j, id, nx, nz (int4), y(j) (real8), buff(nx, nz) (real8)
+ * buff holds (Re, Im) of kx = 0..N1/2 fastest (nx = N1+2 reals), then the
+ * nz = N2 z-modes in column order (kz >= 0, then the negative modes).
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  double t, Re, alp, bet, mpgx;
+  int nband, iter;
+  int N[20]; /* N(1:4, 0:nband+1) for nband = 3 */
+} cfd_file_header;
+
+/* Write one field file.  y, dthdy: the grid array of the field, ny_arr values
+ * for j = jy0 .. jy0+ny_arr-1 (record 3; the p file carries the u/w centres);
+ * planes: ny_g planes of nx*nz reals numbered j0 .. j0+ny_g-1, each record
+ * with its y(j).  Host-only. */
+int cfd_field_file_write(const char* path, const cfd_file_header* hd, int id, double dthetavi, const double* y,
+                         const double* dthdy, int ny_arr, int jy0, int ny_g, int j0, int nx, int nz,
+                         const double* planes);
+/* Read one field file into ny_g planes of nx*nz reals (zeroed first).  A file
+ * plane of other size is copied as buff_to_u copies it into a plane of ngz
+ * z-slots (FOU3D.f90:2631-2654): x reals up to min(nx, nx_file), the first
+ * min(nz_file, ngz)/2 z-modes from the bottom and the same number of negative
+ * modes from the top, then kept where they fall inside the nz target modes;
+ * *mismatch counts such planes.  Host-only. */
+int cfd_field_file_read(const char* path, cfd_file_header* hd, int ny_g, int nx, int nz, int ngz, double* planes,
+                        int* mismatch);
+
+/* The band table N(1:4, 0:4) that record_out writes (start.f90:89-111;
+ * y_grid_canopy.f90:250-275): x/z mode counts per band and the band limits
+ * in y, the bottom band reaching h_ny above the lower tips (0.25 for smooth
+ * walls, two post spacings for posts, start.f90:199-226).  Host-only. */
+int cfd_band_table(const cfd_params* p, int* N20);
+
+/* record_out (FOU3D.f90:2172-2345): write u1, u2, u3, p of the handle as
+ * dirout/<f>_<N1>x<N2>x<ny>_t<int(10 t)>.dat (fields %04d, time %05d) with
+ * the current t and mpgx and the given iteration number. */
+int cfd_record_out(cfd_handle h, const char* dirout, int iter);
+/* mblock_ini + read_in (start.f90:2061-2097, 2251-2700) with the flag_init
+ * branches of start.f90:1974-2009: reads dirin/u1<tail>, u2<tail>, u3<tail>,
+ * p<tail>, tail = fnameimb without its first two characters.  flag_init 1:
+ * t = 0, *iter0 = 0; flag_init 2: t and *iter0 from the u1 header.  QxT is set
+ * to the flow rate of the read u1 unless flag_ctpress = 1; mpgx is left as
+ * it is (the reference does not restore it).  A different nband stops with
+ * CFD_EINVAL; a different alp, bet or N is reported on stderr and read on,
+ * as the reference warns and reads on. */
+int cfd_read_in(cfd_handle h, const char* dirin, const char* fnameimb, int flag_init, int* iter0);
+
 int cfd_sync(cfd_handle h);
 
 /* Kernel-class timing with HIP events on the handle's stream.  classes:

@@ -114,4 +114,46 @@ contains
     call boundary_circ_rough
   end subroutine
 
+  ! y_grid_canopy with h_ny(1) given: the band table N(4, 0:nband+1) it
+  ! computes (rows 3 and 4; rows 1 and 2 come from start and stay 0 here),
+  ! copied column-major into nout(20).
+  subroutine ref_ygrid_bands(nyin, dstyin, dsty1in, dyqin, pppin, posthin, posth1in, hnyin, outdir, nout) &
+      bind(C, name="ref_ygrid_bands")
+    integer(c_int), value :: nyin, dstyin, dsty1in, pppin
+    real(c_double), value :: dyqin, posthin, posth1in, hnyin
+    character(kind=c_char), dimension(*) :: outdir
+    integer(c_int) :: nout(*)
+    integer :: i, b, r
+    if (associated(N)) deallocate(N)
+    if (associated(Ngal)) deallocate(Ngal)
+    if (associated(Ny)) deallocate(Ny)
+    if (associated(yu)) deallocate(yu, yv, dthdyu, dthdyv, dyu2i, dyv2i)
+    if (associated(h_ny)) deallocate(h_ny)
+    allocate(N(4,0:nband+1), Ngal(4,0:nband+1), Ny(3,0:nband+1), h_ny(1))
+    N = 0
+    Ngal = 0
+    Ny = 0
+    N(3,nband) = nyin - 2
+    Ngal(3,nband) = nyin - 2
+    dsty = dstyin
+    dsty1 = dsty1in
+    dyq = dyqin
+    ppp = pppin
+    posth = posthin
+    posth1 = posth1in
+    h_ny(1) = hnyin
+    dirout = ''
+    i = 1
+    do while (outdir(i) /= c_null_char)
+      dirout(i:i) = outdir(i)
+      i = i + 1
+    end do
+    call y_grid_canopy
+    do b = 0, nband + 1
+      do r = 1, 4
+        nout(4*b + r) = N(r, b)
+      end do
+    end do
+  end subroutine
+
 end module ref_glue

@@ -76,6 +76,25 @@ def ygrids():
     np.savez_compressed(os.path.join(OUT, "ygrid.npz"), **out)
 
 
+# band tables: y grid case and the bottom-band height h_ny
+BANDS = {"shipped_h25": ("shipped", 0.25), "smooth_h25": ("smooth", 0.25),
+         "p7_ntz8": ("p7", 2.0 * (2.0 * np.pi / 2.0) / 8), "twolayer_ntz12": ("twolayer", 2.0 * (2.0 * np.pi / 2.0) / 12)}
+
+
+def bands():
+    ref = O.load_ref()
+    out = {}
+    td = tempfile.mkdtemp()
+    for name, (gname, h) in BANDS.items():
+        ny, dsty, dsty1, dyq, ppp, posth, posth1 = YGRIDS[gname]
+        n = (ctypes.c_int * 20)()
+        ref.ref_ygrid_bands(ny, dsty, dsty1, ctypes.c_double(dyq), ppp, ctypes.c_double(posth),
+                            ctypes.c_double(posth1), ctypes.c_double(h), (td + "/").encode(), n)
+        out["%s_N" % name] = np.array(list(n), dtype=np.int32)
+        out["%s_params" % name] = np.array([ny, dsty, dsty1, dyq, ppp, posth, posth1, h])
+    np.savez_compressed(os.path.join(OUT, "bands.npz"), **out)
+
+
 def read_boundary(fn):
     b = open(fn, "rb").read()
     off = [0]
@@ -122,5 +141,6 @@ if __name__ == "__main__":
         sys.exit("oracle/_ref/libref.so is not built (make -C oracle -f Makefile.ref)")
     fft_kat()
     ygrids()
+    bands()
     geoms()
     print("wrote", sorted(f for f in os.listdir(OUT) if f.endswith(".npz")))

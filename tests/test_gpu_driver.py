@@ -50,3 +50,37 @@ def test_fortran_driver_matches_python(tmp_path):
     ch.step(3)
     py = ch.get_field("u1")[0, ch.nyu // 2].real
     assert abs(py - fort) <= 1e-13 * abs(py), (py, fort)
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran driver not built")
+def test_fortran_driver_restart(tmp_path):
+    """3 steps, then a second run from the files it wrote (flag_init = 2) for 2
+    more: the centreline value equals an uninterrupted 5-step run (prescribed
+    pressure gradient in this input.in), and the resumed run starts at iter 3."""
+    full, part = tmp_path / "full", tmp_path / "part"
+    for d in (full, part):
+        d.mkdir()
+        for f in ("input.in", "geom_circ_rough.txt"):
+            shutil.copy(os.path.join(FDIR, f), d / f)
+    print("fortran driver, 5 steps", flush=True)
+    out = subprocess.run([EXE, "5"], cwd=full, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    ref = [l for l in out.stdout.splitlines() if "centreline" in l][0].split()[-1]
+    print("fortran driver, 3 steps", flush=True)
+    out = subprocess.run([EXE, "3"], cwd=part, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    files = sorted(f for f in os.listdir(part) if f.startswith("u1_") and f.endswith(".dat"))
+    assert files, "no field file written"
+    lines = open(part / "input.in").read().splitlines()
+    for i, l in enumerate(lines):
+        if l.startswith("flgin"):
+            lines[i] = "flgin =         2 ! continue from field files"
+        if l.startswith("fnmeb"):
+            lines[i] = "fnmeb =" + files[-1]
+    open(part / "input.in", "w").write("\n".join(lines) + "\n")
+    print("fortran driver, restart for 2 steps", flush=True)
+    out = subprocess.run([EXE, "2"], cwd=part, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    got = [l for l in out.stdout.splitlines() if "centreline" in l][0].split()[-1]
+    assert got == ref, (got, ref)
+    assert any(" iter       5" in l for l in out.stdout.splitlines()), out.stdout
