@@ -82,6 +82,7 @@ def timed(params, f, q0, args, nprof):
     state = {key: ch.get_field(key) for key in ("u1", "u2", "u3")}
     state["mpgx"] = sc["mpgx"]
     state["scalars"] = sc
+    state["device_bytes"] = ch.device_bytes()
     return ch, args.steps / dt_wall, 1e3 * dt_wall / args.steps, prof, state
 
 
@@ -167,6 +168,7 @@ def main():
             "dtype": "f32 transforms/products/nonlinear scratch; f64 fields, solves, projection, flow rate",
             "roofline": roofline(chm, profm, 8, args.workload, "_mixed"),
             "kernel_ms_per_substep": {c: ms / nprof for c, (ms, cnt) in profm.items()},
+            "device_bytes": stm["device_bytes"],
             "err_vs_fp64": dict(diagnostics.compare(stm, st64, g, params.Re),
                                 substeps=args.warmup + args.steps + nprof),
         }
@@ -192,7 +194,7 @@ def main():
         orc.close()
 
     out = {
-        "metric": "RK3 substeps/s at 384x192x192 FP64",
+        "metric": "RK3 substeps/s at %dx%dx%d FP64" % (params.ngalx, params.ny, params.ngalz),
         "value": value,
         "unit": "substeps/s",
         "n_gpus": 1,
@@ -209,6 +211,7 @@ def main():
                    "precision": "fp64", "parallelism": "single-gpu"},
         "roofline": rl,
         "kernel_ms_per_substep": kms,
+        "device_bytes": st64["device_bytes"],
         "mixed": mixed,
         "cpu_baseline": cpu,
         "sim": {"t": sc["t"], "dt": sc["dt"], "mpgx": sc["mpgx"], "finite": finite},

@@ -53,6 +53,7 @@ SIGNATURES = {
     "cfd_substep": (ctypes.c_int, [_H, ctypes.c_int]),
     "cfd_step": (ctypes.c_int, [_H, ctypes.c_int]),
     "cfd_sync": (ctypes.c_int, [_H]),
+    "cfd_device_bytes": (ctypes.c_int, [_H, _D]),
     "cfd_profile": (ctypes.c_int, [_H, ctypes.c_int]),
     "cfd_profile_read": (ctypes.c_int, [_H, _D, _I]),
     "cfd_field_file_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(CFileHeader), ctypes.c_int,

@@ -167,6 +167,12 @@ class Channel:
     def sync(self):
         self._call("cfd_sync")
 
+    def device_bytes(self):
+        """Device memory held by this solver (bytes)."""
+        v = ctypes.c_double(0.0)
+        self._call("cfd_device_bytes", ctypes.byref(v))
+        return v.value
+
     # -- field files ------------------------------------------------------------
     def record_out(self, dirout, iter=0):
         """Write u1, u2, u3, p as the reference's record_out does (file names

@@ -14,6 +14,7 @@
 using namespace cfd;
 
 struct cfd_solver : public Solver {
+  double alloc_bytes = 0.0;  // device bytes held by this handle
   // event pool for kernel-class timing
   std::vector<hipEvent_t> evpool;
   std::vector<int> evclass;
@@ -45,6 +46,7 @@ int dalloc(cfd_solver* s, T** p, size_t n) {
   if (n == 0) n = 1;
   hipError_t e = hipMalloc(&q, n * sizeof(T));
   if (e != hipSuccess) return fail(s, CFD_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  s->alloc_bytes += (double)n * sizeof(T);
   hipMemset(q, 0, n * sizeof(T));
   s->allocs.push_back(q);
   *p = static_cast<T*>(q);
@@ -153,7 +155,7 @@ extern "C" int cfd_create(const cfd_params* pp, cfd_handle* out) {
     return fail(nullptr, CFD_EINVAL, "Ngalx <= 1024 and Ngalz <= 768 (LDS-resident transforms)");
   if (p.precision != 0 && p.precision != 1)
     return fail(nullptr, CFD_EINVAL, "precision must be 0 (FP64) or 1 (mixed: FP32 nonlinear term)");
-  if (p.precision == 1 && !(cfd::fft4_x_supported(p.ngalx) && cfd::fft4_z_supported(p.ngalz)))
+  if (p.precision == 1 && !(cfd::fft4_x_supported(p.ngalx, true) && cfd::fft4_z_supported(p.ngalz, true)))
     return fail(nullptr, CFD_EINVAL, "the mixed path needs Ngalx and Ngalz with four-step FFT kernels");
   if (p.Re <= 0 || p.alp <= 0 || p.bet <= 0) return fail(nullptr, CFD_EINVAL, "Re, alp, bet must be positive");
   cfd_solver* s = new (std::nothrow) cfd_solver();
@@ -294,8 +296,9 @@ extern "C" int cfd_create(const cfd_params* pp, cfd_handle* out) {
   // Z rows padded to a multiple of 128 bytes when the four-step kernels own
   // all three nonlinear passes (the Stockham fallback reads rows of Mx)
   {
-    const bool four = (s->mixed || !s->fft_stockham) && cfd::fft4_z_supported(s->ngz) &&
-                      cfd::fft4_x_supported(s->ngx) && s->N2 == 2 * (s->ngz / 3) && s->N1 == 2 * (s->ngx / 3);
+    const bool four = (s->mixed || !s->fft_stockham) && cfd::fft4_z_supported(s->ngz, s->mixed) &&
+                      cfd::fft4_x_supported(s->ngx, s->mixed) && s->N2 == 2 * (s->ngz / 3) &&
+                      s->N1 == 2 * (s->ngx / 3);
     const int per = (int)(128 / esz);
     s->zs = four ? (s->Mx + per - 1) / per * per : s->Mx;
   }
@@ -918,5 +921,12 @@ extern "C" int cfd_read_in(cfd_handle h, const char* dirin, const char* fnameimb
     d.QxT = Q;
   }
   HIPCHK(s, hipMemcpy(s->d_sc, &d, sizeof d, hipMemcpyHostToDevice));
+  return CFD_OK;
+}
+
+extern "C" int cfd_device_bytes(cfd_handle h, double* bytes) {
+  cfd_solver* s = static_cast<cfd_solver*>(h);
+  if (!s || !bytes) return fail(s, CFD_EINVAL, "null argument");
+  *bytes = s->alloc_bytes;
   return CFD_OK;
 }

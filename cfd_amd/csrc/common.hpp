@@ -158,9 +158,10 @@ struct Solver {
 // nl: the nonlinear planes (precision of the path, rows of zs, XCD-aware
 // order); otherwise FP64 IB planes with rows of Mx
 void launch_zinv(Solver& s, const ZInvJob* jobs, int njobs, bool nl = false);
-// transform lengths served by the four-step kernels (the FP32 path needs them)
-bool fft4_z_supported(int L);
-bool fft4_x_supported(int L);
+// transform lengths served by the four-step kernels in FP32 (f32) or FP64;
+// the mixed path needs them
+bool fft4_z_supported(int L, bool f32);
+bool fft4_x_supported(int L, bool f32);
 void launch_xpass_nl(Solver& s, int cfl);
 void launch_zfwd_nl(Solver& s);
 void launch_xpass_inv2(Solver& s, const XJob* jobs, int njobs);

@@ -510,94 +510,98 @@ __device__ __forceinline__ C imk(real_t<C> k, C u) {
 
 // z-forward of the nonlinear products, four-step: grid (ceil(Mx/TI), nyu+nyv).
 // Output slot k1 + P k2; groups of k2 wholly in the empty band are skipped at
-// compile time.  Same outputs as k_zfwd_nl.
+// compile time.  Same outputs as k_zfwd_nl.  The products of a plane are
+// transformed one after the other, each output written as soon as it is
+// complete (u planes: uu, uw -> Nu1 = ikx uu + ikz uw; ww -> Nu3 = ikx uw +
+// ikz ww; vv kept; v planes: uv kept, vw kept -> Nu2 = ikx uv + ikz vw), so one
+// accumulator per output slot is live at a time.
+template <typename T, int L, int P, int Q, int TI, bool ISU, int PR>
+__device__ __forceinline__ void zfwd_product(const ZNLArgs& a, const cplx_t<T>* __restrict__ twg, cplx_t<T>* X,
+                                             int pl, bool inter, cplx_t<T>* acc) {
+  using C = cplx_t<T>;
+  constexpr int N2 = Dealias<L>::N, H = Dealias<L>::H;
+  const int Mx = a.Mx;
+  const int tid = threadIdx.x, i0 = blockIdx.x * TI;
+  const size_t ncol = (size_t)N2 * Mx;
+  const T sc = (T)(1.0 / L);
+  const C* src = (const C*)a.z[ISU ? PR : 4 + PR] + (size_t)pl * L * a.zs;
+  if (tid < Q * TI) {
+    const int t = tid % TI, q = tid / TI, i = i0 + t;
+    C v[P];
+    sfor<0, P>([&](auto pc) {
+      constexpr int p = decltype(pc)::value;
+      v[p] = i < Mx ? src[(size_t)(Q * p + q) * a.zs + i] : mk<C>(0, 0);
+    });
+    dft<false, P>(v);
+    X[q * TI + t] = v[0];
+    sfor<1, P>([&](auto kc) {
+      constexpr int kk = decltype(kc)::value;
+      X[(kk * Q + q) * TI + t] = mul_tw<false>(v[kk], twg[q * kk]);
+    });
+  }
+  __syncthreads();
+  const int t2 = tid % TI, k1 = tid / TI, i2 = i0 + t2;
+  if (tid < P * TI && i2 < Mx) {
+    C u[Q];
+    sfor<0, Q>([&](auto qc) { u[decltype(qc)::value] = X[(k1 * Q + decltype(qc)::value) * TI + t2]; });
+    dft<false, Q>(u);
+    const T kx = (T)a.k1x[i2];
+    sfor<0, Q>([&](auto c) {
+      constexpr int k2 = decltype(c)::value;
+      if constexpr (!(P * k2 >= H && P * k2 + P <= L - H)) {
+        const int sl = k1 + P * k2;
+        if (sl < H || sl >= L - H) {
+          const int k = sl < H ? sl : sl - L + N2;
+          const C Xs = mk<C>(u[k2].x * sc, u[k2].y * sc);
+          const bool zero = (sl == L - H);  // der_x/der_z zero band (FOU3D.f90:329-333)
+          const T kz = (T)a.k1z[k];
+          const size_t o = (size_t)pl * ncol + (size_t)k * Mx + i2;
+          if constexpr (ISU) {
+            if constexpr (PR == 0) {
+              acc[k2] = imk(kx, Xs);  // ikx uu
+            } else if constexpr (PR == 1) {
+              ((C*)a.nxz1)[o] = zero ? mk<C>(0, 0) : vadd(acc[k2], imk(kz, Xs));  // + ikz uw
+              acc[k2] = imk(kx, Xs);                                             // ikx uw
+            } else if constexpr (PR == 2) {
+              ((C*)a.nxz3)[o] = zero ? mk<C>(0, 0) : vadd(acc[k2], imk(kz, Xs));  // + ikz ww
+            } else {
+              ((C*)a.fvv)[o] = Xs;
+            }
+          } else {
+            if constexpr (PR == 0) {
+              acc[k2] = imk(kx, Xs);  // ikx uv
+              ((C*)a.fuv)[o] = Xs;
+            } else {
+              ((C*)a.fvw)[o] = Xs;
+              if (inter) ((C*)a.nxz2)[o] = zero ? mk<C>(0, 0) : vadd(acc[k2], imk(kz, Xs));
+            }
+          }
+        }
+      }
+    });
+  }
+  __syncthreads();
+}
+
 template <typename T, int L, int P, int Q, int TI>
 __global__ void __launch_bounds__(TI * cmax(P, Q)) k_zfwd4(ZNLArgs a, const cplx_t<T>* __restrict__ twg) {
   using C = cplx_t<T>;
   static_assert(P * Q == L, "P*Q");
-  constexpr int N2 = Dealias<L>::N, H = Dealias<L>::H;
   __shared__ C X[P * Q * TI];
-  const int Mx = a.Mx;
-  const int tid = threadIdx.x, i0 = blockIdx.x * TI;
   const bool isu = (int)blockIdx.y < a.nyu;
   const int pl = isu ? blockIdx.y : blockIdx.y - a.nyu;
   const int ny = isu ? a.nyu : a.nyv;
   const bool inter = pl > 0 && pl < ny - 1;
   if (isu && !inter) return;  // ghost u planes: no product is needed there
-  const size_t ncol = (size_t)N2 * Mx;
-  const T sc = (T)(1.0 / L);
-  const int t2 = tid % TI, k1 = tid / TI, i2 = i0 + t2;
-  C* const nxz1 = (C*)a.nxz1;
-  C* const nxz2 = (C*)a.nxz2;
-  C* const nxz3 = (C*)a.nxz3;
-  C* const fuv = (C*)a.fuv;
-  C* const fvw = (C*)a.fvw;
-  C* const fvv = (C*)a.fvv;
-  C acc1[Q], acc2[Q];
-  const int nprod = isu ? 4 : 2;
-  for (int pr = 0; pr < nprod; ++pr) {
-    const C* src = (const C*)a.z[isu ? pr : 4 + pr] + (size_t)pl * L * a.zs;
-    if (tid < Q * TI) {
-      const int t = tid % TI, q = tid / TI, i = i0 + t;
-      C v[P];
-      sfor<0, P>([&](auto pc) {
-        constexpr int p = decltype(pc)::value;
-        v[p] = i < Mx ? src[(size_t)(Q * p + q) * a.zs + i] : mk<C>(0, 0);
-      });
-      dft<false, P>(v);
-      X[q * TI + t] = v[0];
-      sfor<1, P>([&](auto kc) {
-        constexpr int kk = decltype(kc)::value;
-        X[(kk * Q + q) * TI + t] = mul_tw<false>(v[kk], twg[q * kk]);
-      });
-    }
-    __syncthreads();
-    if (tid < P * TI) {
-      C u[Q];
-      sfor<0, Q>([&](auto qc) { u[decltype(qc)::value] = X[(k1 * Q + decltype(qc)::value) * TI + t2]; });
-      dft<false, Q>(u);
-      if (i2 < Mx) {
-        const T kx = (T)a.k1x[i2];
-        sfor<0, Q>([&](auto c) {
-          constexpr int k2 = decltype(c)::value;
-          if constexpr (!(P * k2 >= H && P * k2 + P <= L - H)) {
-            const int sl = k1 + P * k2;
-            if (sl < H || sl >= L - H) {
-              const int k = sl < H ? sl : sl - L + N2;
-              const C Xs = mk<C>(u[k2].x * sc, u[k2].y * sc);
-              const bool zero = (sl == L - H);  // der_x/der_z zero band (FOU3D.f90:329-333)
-              const T kz = (T)a.k1z[k];
-              const size_t o = (size_t)pl * ncol + (size_t)k * Mx + i2;
-              if (isu) {
-                // Nu1 = ikx uu + ikz uw ; Nu3 = ikx uw + ikz ww ; keep vv
-                if (pr == 0) acc1[k2] = imk(kx, Xs);
-                if (pr == 1) {
-                  acc1[k2] = vadd(acc1[k2], imk(kz, Xs));
-                  acc2[k2] = imk(kx, Xs);
-                }
-                if (pr == 2) acc2[k2] = vadd(acc2[k2], imk(kz, Xs));
-                if (pr == 3) {
-                  fvv[o] = Xs;
-                  nxz1[o] = zero ? mk<C>(0, 0) : acc1[k2];
-                  nxz3[o] = zero ? mk<C>(0, 0) : acc2[k2];
-                }
-              } else {
-                // Nu2 = ikx uv + ikz vw ; keep uv, vw
-                if (pr == 0) {
-                  acc1[k2] = imk(kx, Xs);
-                  fuv[o] = Xs;
-                }
-                if (pr == 1) {
-                  fvw[o] = Xs;
-                  if (inter) nxz2[o] = zero ? mk<C>(0, 0) : vadd(acc1[k2], imk(kz, Xs));
-                }
-              }
-            }
-          }
-        });
-      }
-    }
-    __syncthreads();
+  C acc[Q];
+  if (isu) {
+    zfwd_product<T, L, P, Q, TI, true, 0>(a, twg, X, pl, inter, acc);
+    zfwd_product<T, L, P, Q, TI, true, 1>(a, twg, X, pl, inter, acc);
+    zfwd_product<T, L, P, Q, TI, true, 2>(a, twg, X, pl, inter, acc);
+    zfwd_product<T, L, P, Q, TI, true, 3>(a, twg, X, pl, inter, acc);
+  } else {
+    zfwd_product<T, L, P, Q, TI, false, 0>(a, twg, X, pl, inter, acc);
+    zfwd_product<T, L, P, Q, TI, false, 1>(a, twg, X, pl, inter, acc);
   }
 }
 
@@ -881,14 +885,14 @@ void zfwdp_t(Solver& s, const ZFwdJob* jobs, int njobs) {
 // of the x-pass.  The z-inverse takes Q | N2/2 and the z-forward P | N2/2
 // where the length allows, so whole groups of the empty band drop out.
 #define CFD_ZINV4(X) X(48, 6, 8, 16) X(64, 8, 8, 16) X(192, 12, 16, 16) X(384, 24, 16, 8)
-#define CFD_ZFWD4(X) X(48, 8, 6, 16) X(64, 8, 8, 16) X(192, 16, 12, 16)
+#define CFD_ZFWD4_F64(X) X(48, 8, 6, 16) X(64, 8, 8, 16) X(192, 16, 12, 16) X(384, 24, 16, 8)
+#define CFD_ZFWD4_F32(X) CFD_ZFWD4_F64(X)
 #ifndef CFD_XPASS4_F64
-#define CFD_XPASS4_F64(X) X(48, 8, 6, 8) X(64, 8, 8, 8) X(72, 12, 6, 4) X(384, 16, 24, 4)
+#define CFD_XPASS4_F64(X) X(48, 8, 6, 8) X(64, 8, 8, 8) X(72, 12, 6, 4) X(384, 16, 24, 4) X(768, 32, 24, 2)
 #endif
 #ifndef CFD_XPASS4_F32
-#define CFD_XPASS4_F32(X) X(48, 8, 6, 8) X(64, 8, 8, 8) X(72, 12, 6, 4) X(384, 16, 24, 4)
+#define CFD_XPASS4_F32(X) X(48, 8, 6, 8) X(64, 8, 8, 8) X(72, 12, 6, 4) X(384, 16, 24, 4) X(768, 32, 24, 2)
 #endif
-#define CFD_XPASS4(X) CFD_XPASS4_F64(X)
 
 bool four_step_z(const Solver& s) {
   return (s.mixed || !s.fft_stockham) && s.N2 == 2 * (s.ngz / 3) && s.pz.L == s.ngz;
@@ -919,15 +923,25 @@ bool try_zinv4(Solver& s, const ZInvJob* jobs, int njobs, bool nl) {
   return zinv4_t<double>(s, jobs, njobs, s.d_twz, nl ? s.zs : s.Mx, nl);
 }
 
-template <typename T>
-bool zfwd4_t(Solver& s, const ZNLArgs& a, const cplx_t<T>* tw) {
-  switch (s.pz.L) {
-#define C(L, P, Q, TI)                                                                                   \
-  case L:                                                                                                \
-    hipLaunchKernelGGL((k_zfwd4<T, L, P, Q, TI>), dim3((s.Mx + TI - 1) / TI, s.nyu + s.nyv),            \
-                       dim3(TI * cmax(P, Q)), 0, s.stream, a, tw);                                      \
+#define CFD_ZF_CASE(T, L, P, Q, TI)                                                                \
+  case L:                                                                                       \
+    hipLaunchKernelGGL((k_zfwd4<T, L, P, Q, TI>), dim3((s.Mx + TI - 1) / TI, s.nyu + s.nyv),   \
+                       dim3(TI * cmax(P, Q)), 0, s.stream, a, tw);                             \
     return true;
-    CFD_ZFWD4(C)
+
+bool zfwd4_t(Solver& s, const ZNLArgs& a, const double2* tw) {
+  switch (s.pz.L) {
+#define C(L, P, Q, TI) CFD_ZF_CASE(double, L, P, Q, TI)
+    CFD_ZFWD4_F64(C)
+#undef C
+    default: return false;
+  }
+}
+
+bool zfwd4_t(Solver& s, const ZNLArgs& a, const float2* tw) {
+  switch (s.pz.L) {
+#define C(L, P, Q, TI) CFD_ZF_CASE(float, L, P, Q, TI)
+    CFD_ZFWD4_F32(C)
 #undef C
     default: return false;
   }
@@ -935,7 +949,7 @@ bool zfwd4_t(Solver& s, const ZNLArgs& a, const cplx_t<T>* tw) {
 
 bool try_zfwd4(Solver& s, const ZNLArgs& a) {
   if (!four_step_z(s)) return false;
-  return s.mixed ? zfwd4_t<float>(s, a, s.d_twzf) : zfwd4_t<double>(s, a, s.d_twz);
+  return s.mixed ? zfwd4_t(s, a, s.d_twzf) : zfwd4_t(s, a, s.d_twz);
 }
 
 #define CFD_XP_CASE(T, L, P, Q, R)                                                                  \
@@ -973,7 +987,7 @@ bool try_xpass4(Solver& s, const XNLArgs& a) {
 
 }  // namespace
 
-bool fft4_z_supported(int L) {
+bool fft4_z_supported(int L, bool f32) {
   switch (L) {
 #define C(L, P, Q, TI) case L:
     CFD_ZINV4(C)
@@ -981,19 +995,37 @@ bool fft4_z_supported(int L) {
     break;
     default: return false;
   }
+  if (f32) {
+    switch (L) {
+#define C(L, P, Q, TI) case L:
+      CFD_ZFWD4_F32(C)
+#undef C
+      return true;
+      default: return false;
+    }
+  }
   switch (L) {
 #define C(L, P, Q, TI) case L:
-    CFD_ZFWD4(C)
+    CFD_ZFWD4_F64(C)
 #undef C
     return true;
     default: return false;
   }
 }
 
-bool fft4_x_supported(int L) {
+bool fft4_x_supported(int L, bool f32) {
+  if (f32) {
+    switch (L) {
+#define C(L, P, Q, R) case L:
+      CFD_XPASS4_F32(C)
+#undef C
+      return true;
+      default: return false;
+    }
+  }
   switch (L) {
 #define C(L, P, Q, R) case L:
-    CFD_XPASS4(C)
+    CFD_XPASS4_F64(C)
 #undef C
     return true;
     default: return false;

@@ -166,6 +166,8 @@ int cfd_record_out(cfd_handle h, const char* dirout, int iter);
 int cfd_read_in(cfd_handle h, const char* dirin, const char* fnameimb, int flag_init, int* iter0);
 
 int cfd_sync(cfd_handle h);
+/* device memory held by the handle, bytes (no counterpart in the reference) */
+int cfd_device_bytes(cfd_handle h, double* bytes);
 
 /* Kernel-class timing with HIP events on the handle's stream.  classes:
    0 z-inverse FFT, 1 x-pass FFT+products, 2 z-forward FFT, 3 spectral

@@ -21,6 +21,12 @@ WIDE = Params(ngalx=384, ngalz=192, ny=24, dsty=3, ygrid=1, ytanh_beta=2.0, geom
               ntilez=8, extras24=0, flag_ctpress=0, Re=6887.0, alp=1.0, bet=2.0, posth=0.12, r=1.2378,
               c_i=12.0, c_k=12.0, stag=0.5, CFL=0.5)
 
+# the lateral grid and post array of the 768 x 384 x 384 workload on a short
+# channel (32 x 16 posts per wall)
+WIDE768 = Params(ngalx=768, ngalz=384, ny=16, dsty=2, ygrid=1, ytanh_beta=2.0, geometry_type=3, ntilex=32,
+                 ntilez=16, extras24=0, flag_ctpress=0, Re=10440.0, alp=1.0, bet=2.0, posth=0.1, r=1.1,
+                 c_i=12.0, c_k=12.0, stag=0.5, CFL=0.5)
+
 SMOOTH = Params(ngalx=36, ngalz=24, ny=25, dsty=0, ygrid=0, geometry_type=0, ntilex=2, ntilez=2, Re=2000.0,
                 alp=1.0, bet=2.0, posth=0.0, CFL=0.5)
 

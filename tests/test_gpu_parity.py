@@ -14,7 +14,7 @@ import pytest
 from cfd_amd import Channel
 from oracle.oracle import Oracle
 
-from .helpers import EDGE, SHIPPED, SMALLPOSTS, SMOOTH, WIDE, random_field, rel_l2, max_rel
+from .helpers import EDGE, SHIPPED, SMALLPOSTS, SMOOTH, WIDE, WIDE768, random_field, rel_l2, max_rel
 
 pytestmark = pytest.mark.gpu
 
@@ -242,3 +242,29 @@ def test_four_step_fft_equals_stockham_at_workload_size():
     sa, sb = a.scalars(), b.scalars()
     for k in ("t", "dt", "mpgx"):
         assert abs(sa[k] - sb[k]) <= 1e-13 * abs(sb[k]), k
+
+
+def test_substep_768x384_against_oracle():
+    """One RK3 step on the 768 x 384 lateral grid (x-pass four-step at 768
+    points, Stockham z-forward at 384) against the oracle."""
+    ch, orc = _pair(WIDE768, seed=90)
+    for kRK in (1, 2, 3):
+        ch.substep(kRK)
+        orc.substep(kRK)
+    _cmp(ch, orc, ("u1", "u2", "u3", "p"), tol=1e-11)
+
+
+def test_four_step_fft_equals_stockham_768x384():
+    a, _ = _pair(WIDE768, seed=91)
+    os.environ["CFD_AMD_FFT"] = "stockham"
+    try:
+        b, _ = _pair(WIDE768, seed=91)
+    finally:
+        os.environ.pop("CFD_AMD_FFT", None)
+    for kRK in (1, 2, 3):
+        a.substep(kRK)
+        b.substep(kRK)
+    # random full-spectrum fields: p comes from a divergence (cancellation), so
+    # its round-off bound is one decade looser than the velocities'
+    for n, tol in (("u1", 1e-13), ("u2", 1e-13), ("u3", 1e-13), ("p", 1e-12)):
+        assert rel_l2(a.get_field(n), b.get_field(n)) <= tol, (n, rel_l2(a.get_field(n), b.get_field(n)))
