@@ -62,6 +62,7 @@ SIGNATURES = {
                             + [_D, _I]),
     "cfd_record_out": (ctypes.c_int, [_H, ctypes.c_char_p, ctypes.c_int]),
     "cfd_band_table": (ctypes.c_int, [ctypes.POINTER(CParams), _I]),
+    "cfd_four_to_phys": (ctypes.c_int, [_H, ctypes.c_int, _D]),
     "cfd_read_in": (ctypes.c_int, [_H, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, _I]),
 }
 

@@ -189,6 +189,14 @@ class Channel:
         self._call("cfd_read_in", os.fsencode(dirin), os.fsencode(fnameimb), int(flag_init), ctypes.byref(it))
         return it.value
 
+    def four_to_phys(self, name):
+        """Physical values of a field, (planes, Ngalz, Ngalx): four_to_phys's
+        zero padding (without the kz = -N2/2 mode) and unscaled inverse
+        transforms."""
+        out = np.zeros((self._ny[GRID[name]], self.params.ngalz, self.params.ngalx))
+        self._call("cfd_four_to_phys", FIELDS.index(name), out.ctypes.data_as(_D))
+        return out
+
     # -- timing ----------------------------------------------------------------
     def profile(self, enable=True):
         self._call("cfd_profile", 1 if enable else 0)

@@ -930,3 +930,69 @@ extern "C" int cfd_device_bytes(cfd_handle h, double* bytes) {
   *bytes = s->alloc_bytes;
   return CFD_OK;
 }
+
+// ---------------------------------------------------------------------------
+// physical fields (four_to_phys.f90:61-120): every plane of a field zero-padded
+// from N to Ngal modes without the kz = -N2/2 mode (resize, four_to_phys.f90:
+// 328-348), inverse z then x transforms, unscaled
+// ---------------------------------------------------------------------------
+extern "C" int cfd_four_to_phys(cfd_handle h, int which, double* phys) {
+  cfd_solver* s = static_cast<cfd_solver*>(h);
+  if (!s || !phys || which < 0 || which >= CFD_NFIELDS) return fail(s, CFD_EINVAL, "bad field argument");
+  const int ny = ny_of(*s, which);
+  const size_t zpl = (size_t)s->ngz * s->Mx, ppl = (size_t)s->ngz * s->ngx;
+  HIPCHK(s, hipSetDevice(s->dev));
+  double2* z = nullptr;
+  double* p = nullptr;
+  ZInvJob* dz = nullptr;
+  XJob* dx = nullptr;
+  auto cleanup = [&]() {
+    hipFree(z);
+    hipFree(p);
+    hipFree(dz);
+    hipFree(dx);
+  };
+  std::vector<ZInvJob> zj(ny);
+  for (int q = 0; q < ny; ++q) {
+    ZInvJob j{};
+    j.src = s->F[which];
+    j.dst = nullptr;  // set below
+    j.pa = q;
+    j.pd = q;
+    j.mode = 0;
+    j.flags = 1;
+    zj[q] = j;
+  }
+  // x transforms pair planes (q, q+1); a last odd plane pairs with itself
+  const int nx = (ny + 1) / 2;
+  std::vector<XJob> xj(nx);
+  if (hipMalloc(&z, zpl * ny * sizeof(double2)) != hipSuccess || hipMalloc(&p, ppl * ny * sizeof(double)) != hipSuccess ||
+      hipMalloc(&dz, zj.size() * sizeof(ZInvJob)) != hipSuccess || hipMalloc(&dx, xj.size() * sizeof(XJob)) != hipSuccess) {
+    cleanup();
+    return fail(s, CFD_ENOMEM, "four_to_phys: device buffers");
+  }
+  for (auto& j : zj) j.dst = z;
+  for (int m = 0; m < nx; ++m) {
+    const int q = 2 * m;
+    const bool pair = q + 1 < ny;
+    XJob j{};
+    j.za = z;
+    j.zb = pair ? z + zpl : z;
+    j.pa = p;
+    j.pb = pair ? p + ppl : p;
+    j.plane = q;
+    xj[m] = j;
+  }
+  if (hipMemcpy(dz, zj.data(), zj.size() * sizeof(ZInvJob), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dx, xj.data(), xj.size() * sizeof(XJob), hipMemcpyHostToDevice) != hipSuccess) {
+    cleanup();
+    return fail(s, CFD_EHIP, "four_to_phys: job upload");
+  }
+  launch_zinv(*s, dz, ny, false);
+  launch_xpass_inv2(*s, dx, nx);
+  hipError_t e = hipStreamSynchronize(s->stream);
+  if (e == hipSuccess) e = hipMemcpy(phys, p, ppl * ny * sizeof(double), hipMemcpyDeviceToHost);
+  cleanup();
+  if (e != hipSuccess) return fail(s, CFD_EHIP, std::string("four_to_phys: ") + hipGetErrorString(e));
+  return CFD_OK;
+}

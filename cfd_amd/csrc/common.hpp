@@ -40,7 +40,8 @@ struct ZInvJob {
   const double2* src;
   void* dst;  // double2, or float2 for the nonlinear planes of the mixed path
   int pa, pb, pc, pd;
-  int mode, pad;
+  int mode;
+  int flags;  // bit 0: leave out the kz = -N2/2 mode (as four_to_phys's resize does)
   double wa, wb, g;
 };
 

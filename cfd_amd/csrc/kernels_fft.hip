@@ -79,7 +79,9 @@ __global__ void __launch_bounds__(kThreads) k_zinv(const ZInvJob* __restrict__ j
     if (i >= Mx) continue;
     const size_t off = (size_t)k * Mx + i;
     double2 v;
-    if (J.mode == 0) {
+    if ((J.flags & 1) && k == N2 / 2) {
+      v = make_double2(0.0, 0.0);
+    } else if (J.mode == 0) {
       v = J.src[(size_t)J.pa * ncol + off];
     } else {
       const double2 a = J.src[(size_t)J.pa * ncol + off], b = J.src[(size_t)J.pb * ncol + off];
@@ -478,7 +480,8 @@ __global__ void __launch_bounds__(TI * cmax(P, Q)) k_zinv4(const ZInvJob* __rest
         v[p] = mk<C>(0, 0);  // the whole group lies in the empty band
       } else {
         const int n = Q * p + q;
-        const int k = n < H ? n : (n >= L - H ? n - L + N2 : -1);
+        int k = n < H ? n : (n >= L - H ? n - L + N2 : -1);
+        if ((J.flags & 1) && k == H) k = -1;
         v[p] = (k >= 0 && i < Mx) ? from_d2<C>(zinv_load(J, (size_t)k * Mx + i, ncol)) : mk<C>(0, 0);
       }
     });

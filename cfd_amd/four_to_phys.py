@@ -1,0 +1,166 @@
+"""Post-processing: spectral field files -> physical field files
+(four_to_phys.f90:1-326), the transforms on the GPU (cfd_four_to_phys).
+
+    python -m cfd_amd.four_to_phys [input_FtP.txt]
+
+reads, as the reference program does, input_FtP.txt (nx, nz, ny as
+(7X,I10); t as (7X,D10.1); the input directory as (8X,A30); the output
+directory as (7X,A30)), opens <dir>/<f>_<nx>x<nz>x<ny>_t<int(10 t)>.dat for f in
+u1, u2, u3, p and writes <dirout>/<f>_phys_<same>.dat: the header record
+copied, the band table with Ngal = 3/2 N in bands 1-3, the grid record copied,
+then one record per plane j, id, Ngalx+2, Ngalz, y(j), buff(Ngalx+2, Ngalz)
+holding the physical values (x fastest, two trailing zeros per row).  The
+plane sequence is the reference's, band by band: u/w and v files repeat the
+plane at each of the two band interfaces (the u1 and u3 repeats carry id 1,
+the u2 repeats the id of the plane), the p file has every plane once.
+"""
+import os
+import struct
+import sys
+
+import numpy as np
+
+from . import fieldfile
+from .channel import Channel
+from .params import Params
+
+FIELDS = (("u1", 1, "u"), ("u2", 2, "v"), ("u3", 3, "u"), ("p", 4, "p"))
+
+
+def _d10_1(field):
+    """A (D10.1) real: without a decimal point the digits carry one implied
+    decimal, as Fortran formatted input reads them."""
+    f = field.strip().lower().replace("d", "e")
+    if "." in f:
+        return float(f)
+    mant, _, ex = f.partition("e")
+    return int(mant) * 10.0 ** (int(ex) if ex else 0) / 10.0
+
+
+def read_input(path):
+    lines = open(path).read().splitlines()
+    nx = int(lines[0][7:17])
+    nz = int(lines[1][7:17])
+    ny = int(lines[2][7:17])
+    t = _d10_1(lines[3][7:17])
+    dirname = lines[4][8:38].strip()
+    dirout = lines[5][7:37].strip()
+    return nx, nz, ny, t, dirname, dirout
+
+
+def _raw_records(path, n):
+    """The first n record payloads of a Fortran sequential unformatted file."""
+    out = []
+    with open(path, "rb") as fh:
+        for _ in range(n):
+            (m,) = struct.unpack("<i", fh.read(4))
+            out.append(fh.read(m))
+            fh.read(4)
+    return out
+
+
+def _plane_ys(path, nplanes):
+    """y(j) of the first nplanes plane records (the records after the three
+    header records), skipping the plane payloads."""
+    ys = []
+    with open(path, "rb") as fh:
+        for r in range(3 + nplanes):
+            (m,) = struct.unpack("<i", fh.read(4))
+            if r < 3:
+                fh.seek(m + 4, os.SEEK_CUR)
+                continue
+            head = fh.read(24)
+            ys.append(struct.unpack_from("<d", head, 16)[0])
+            fh.seek(m - 24 + 4, os.SEEK_CUR)
+    return ys
+
+
+def _rec(fh, payload):
+    fh.write(struct.pack("<i", len(payload)))
+    fh.write(payload)
+    fh.write(struct.pack("<i", len(payload)))
+
+
+def plane_sequence(grid, N):
+    """(index into the field's planes, record j, record id or None = the
+    plane's own id) in the order four_to_phys writes them; N is the 20-int
+    band table N(1:4, 0:4).
+
+    The reference reads the planes as a stream, band by band, and writes each
+    interface record from the last plane it read.  That is the plane of the
+    interface only when the band limits of the grid are in order; otherwise
+    its loops read past the last plane of the file.  Such a table is refused
+    here."""
+    n = lambda r, b: N[4 * b + r - 1]  # noqa: E731  N(r, b)
+    for r in (3, 4):
+        lims = [n(r, b) for b in range(4)]
+        if lims != sorted(lims):
+            raise ValueError("band limits N(%d, 0:3) = %s are not in order: the plane stream of four_to_phys "
+                             "does not fit this grid" % (r, lims))
+    if grid == "p":
+        j0 = n(4, 0) + 1
+        return [(j - j0, j, None) for j in range(n(4, 0) + 1, n(4, 3) + 1)]
+    r = 4 if grid == "u" else 3
+    j0 = n(r, 0)
+    seq = []
+    for b in (1, 2, 3):
+        if b == 1:
+            seq.append((0, j0, None))
+        else:
+            ji = n(r, b - 1)
+            seq.append((ji - j0, ji, 1 if grid == "u" else None))
+        seq += [(j - j0, j, None) for j in range(n(r, b - 1) + 1, n(r, b) + 1)]
+    seq.append((n(r, 3) + 1 - j0, n(r, 3) + 1, None))
+    return seq
+
+
+def write_phys(path, header_rec, N, grid_rec, grid, field_id, ys, phys):
+    """Write one physical-field file.  header_rec, grid_rec: raw payloads of
+    records 1 and 3 of the spectral file; ys: y(j) of each plane record of
+    the spectral file; phys: (planes, Ngalz, Ngalx)."""
+    nplanes, ngz, ngx = phys.shape
+    ngal = list(N)
+    for b in (1, 2, 3):
+        ngal[4 * b + 0] = N[4 * b + 0] * 3 // 2
+        ngal[4 * b + 1] = N[4 * b + 1] * 3 // 2
+    buf = np.zeros((nplanes, ngz, ngx + 2))
+    buf[:, :, :ngx] = phys
+    with open(path, "wb") as fh:
+        _rec(fh, header_rec)
+        _rec(fh, struct.pack("<20i", *ngal))
+        _rec(fh, grid_rec)
+        for q, j, rid in plane_sequence(grid, N):
+            _rec(fh, struct.pack("<4id", j, field_id if rid is None else rid, ngx + 2, ngz, ys[q]) +
+                 buf[q].astype("<f8").tobytes())
+
+
+def convert(nx, nz, ny, t, dirname, dirout):
+    tail = "%04dx%04dx%04d_t%05d.dat" % (nx, nz, ny, int(10.0 * t))
+    ch = None
+    for name, fid, grid in FIELDS:
+        src = os.path.join(dirname, "%s_%s" % (name, tail))
+        head = _raw_records(src, 3)
+        N = list(struct.unpack("<20i", head[1][:80]))
+        plane_sequence(grid, N)  # refuses a band table the plane stream does not fit
+        if ch is None:
+            p = Params(ngalx=N[4] * 3 // 2, ngalz=N[5] * 3 // 2, ny=ny, dsty=-N[2], geometry_type=0, posth=0.0,
+                       ygrid=0)
+            ch = Channel(p)
+        ny_g = {"u": ch.nyu, "v": ch.nyv, "p": ch.nyp}[grid]
+        h, planes, mm = fieldfile.read(src, ny_g, ch.N2, ch.Mx, ch.params.ngalz)
+        ch.set_field(name, planes.reshape(ny_g, ch.Ncol).T)
+        phys = ch.four_to_phys(name)
+        ys = _plane_ys(src, ny_g)
+        out = os.path.join(dirout, "%s_phys_%s" % (name, tail))
+        write_phys(out, head[0], N, head[2], grid, fid, ys, phys)
+        print("Writing", out)
+    ch.close()
+
+
+def main(argv):
+    path = argv[1] if len(argv) > 1 else "input_FtP.txt"
+    convert(*read_input(path))
+
+
+if __name__ == "__main__":
+    main(sys.argv)

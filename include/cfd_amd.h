@@ -145,6 +145,13 @@ int cfd_field_file_write(const char* path, const cfd_file_header* hd, int id, do
 int cfd_field_file_read(const char* path, cfd_file_header* hd, int ny_g, int nx, int nz, int ngz, double* planes,
                         int* mismatch);
 
+/* four_to_phys (four_to_phys.f90:61-120, resize 328-348): the physical values
+ * of every plane of a field, phys[plane][z][x] (Ngalz x Ngalx per plane): the
+ * N1 x N2 modes zero-padded to Ngalx x Ngalz without the kz = -N2/2 mode,
+ * inverse z then x transforms, unscaled (cft_buff.f / rft_buff.f inverse
+ * conventions). */
+int cfd_four_to_phys(cfd_handle h, int which, double* phys);
+
 /* The band table N(1:4, 0:4) that record_out writes (start.f90:89-111;
  * y_grid_canopy.f90:250-275): x/z mode counts per band and the band limits
  * in y, the bottom band reaching h_ny above the lower tips (0.25 for smooth
