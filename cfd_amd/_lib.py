@@ -63,6 +63,10 @@ SIGNATURES = {
     "cfd_record_out": (ctypes.c_int, [_H, ctypes.c_char_p, ctypes.c_int]),
     "cfd_band_table": (ctypes.c_int, [ctypes.POINTER(CParams), _I]),
     "cfd_four_to_phys": (ctypes.c_int, [_H, ctypes.c_int, _D]),
+    "cfd_stats": (ctypes.c_int, [_H]),
+    "cfd_write_stats": (ctypes.c_int, [_H, ctypes.c_char_p, ctypes.c_int]),
+    "cfd_stats_request": (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]),
+    "cfd_stats_dims": (ctypes.c_int, [_H, ctypes.POINTER(ctypes.c_int)]),
     "cfd_read_in": (ctypes.c_int, [_H, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, _I]),
 }
 

@@ -197,6 +197,34 @@ class Channel:
         self._call("cfd_four_to_phys", FIELDS.index(name), out.ctypes.data_as(_D))
         return out
 
+    # -- statistics and spectra -------------------------------------------------
+    def stats(self):
+        """Add one sample of the spectra and statistics of the current state
+        (spectra.f90, stats.f90)."""
+        self._call("cfd_stats")
+
+    def write_stats(self, dirout, iter=0):
+        """write_spect + write_stats with the current t and mpgx, then zero
+        the accumulators; returns the stats file path."""
+        self._call("cfd_write_stats", os.fsencode(dirout), int(iter))
+        return dirout + "stats_" + self._stats_tag(self.scalars()["t"])
+
+    def stats_request(self, sample=True, write_dir=None, iter=0):
+        """Sample and/or write at the reference's places inside the next
+        substep with kRK = 1 (the nonlinear term's flagst / flagwr)."""
+        flags = (1 if sample else 0) | (2 if write_dir is not None else 0)
+        self._call("cfd_stats_request", flags, os.fsencode(write_dir) if write_dir is not None else None, int(iter))
+
+    def stats_dims(self):
+        """dnx, dnz, wall-band plane counts of the u, v, p grids, istat."""
+        d = (ctypes.c_int * 6)()
+        self._call("cfd_stats_dims", d)
+        d = list(d)
+        return dict(dnx=d[0], dnz=d[1], nyc={"u": d[2], "v": d[3], "p": d[4]}, istat=d[5])
+
+    def _stats_tag(self, t):
+        return "%04dx%04dx%04d_%05d.dat" % (self.N1, self.N2, self.params.ny, int(10.0 * t))
+
     # -- timing ----------------------------------------------------------------
     def profile(self, enable=True):
         self._call("cfd_profile", 1 if enable else 0)

@@ -1,11 +1,13 @@
 // api.cpp -- C ABI of the rough-channel substep (include/cfd_amd.h): setup,
 // device state, layout conversion at the boundary and operator sequencing.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "common.hpp"
@@ -22,6 +24,28 @@ struct cfd_solver : public Solver {
   // per-field offsets into the IB job tables
   int ibz_off[4] = {0, 0, 0, 0}, ibx_off[4] = {0, 0, 0, 0};
   std::vector<void*> allocs;
+  // the interpolating nonlinear z-inverse jobs, per plane: u2 on u planes,
+  // u3 on v planes (reused by the statistics)
+  std::vector<ZInvJob> nl_u2i, nl_u3i;
+  // statistics and spectra (stats.f90, spectra.f90), set up on first use
+  struct Stats {
+    bool ready = false;
+    int N[20] = {};
+    int dnx = 0, dnz = 0, nyc[3] = {0, 0, 0}, istat = 0, chunk = 0;
+    int bot[3][2] = {}, top[3][2] = {}, jref_bot[3] = {}, jref_top[3] = {};
+    double *prof[13] = {}, *cb[13] = {}, *ct[13] = {}, *sp[5] = {};
+    double2* Zc[3] = {};
+    double* Pc[3] = {};
+    double2* wxs = nullptr;
+    ZInvJob *d_zj[3] = {}, *d_ju2 = nullptr;
+    XJob* d_xj[3] = {};
+    StatSumTask* d_sum[3] = {};
+    StatCondTask* d_cond[3] = {};
+    int nsum[3] = {0, 0, 0}, ncond[3] = {0, 0, 0};
+    // sample / write requested for the next substep with kRK = 1
+    int pending = 0, pending_iter = 0;
+    std::string pending_dir;
+  } st;
 };
 
 namespace {
@@ -187,6 +211,13 @@ extern "C" int cfd_create(const cfd_params* pp, cfd_handle* out) {
   std::string err;
   if (!build_ygrid(p, s->yg, err) || !build_iblists(p, s->ib, err)) {
     s->err = err;
+    return bail(CFD_EINVAL);
+  }
+  // a folded grid has no solution to converge to: refuse it (the reference
+  // runs on it)
+  if (const int jf = ygrid_fold(s->yg); jf != INT_MIN) {
+    s->err = "the y grid folds back at face j = " + std::to_string(jf) +
+             ": yu(j) < yv(j) < yu(j+1) fails (canopy depth posth does not fit dsty, or the map is not monotonic)";
     return bail(CFD_EINVAL);
   }
   s->nyu = s->yg.nyu;
@@ -361,6 +392,10 @@ extern "C" int cfd_create(const cfd_params* pp, cfd_handle* out) {
     }
     s->n_nljobs = (int)J.size();
     if ((rc = upload(s, &s->d_nljobs, J))) return bail(rc);
+    for (const ZInvJob& z : J) {
+      if (z.dst == s->Z[1]) s->nl_u2i.push_back(z);
+      if (z.dst == s->Z[5]) s->nl_u3i.push_back(z);
+    }
   }
 
   // IB lists and transform jobs
@@ -662,6 +697,293 @@ int launched(cfd_solver* s) {
   return CFD_OK;
 }
 
+// ---------------------------------------------------------------------------
+// statistics and spectra (stats.f90, spectra.f90)
+//
+// One sample adds, for the current fields and pressure, the spectra of the
+// midband columns (here every column and every plane) and the plane and
+// tile-conditional statistics of the physical planes.  The physical planes
+// come from the solver's own FP64 z- and x-inverse kernels (the nonlinear
+// term's transforms: the same mode placement, the kz = -N2/2 line kept),
+// streamed in chunks of kStatChunk planes so the scratch stays small at any
+// grid size.  Wall bands: the plane sets of the reference's partition with
+// walls (limPL_incw, start.f90:1759-1866); the conditional sums of the two
+// walls are kept apart and added when written, as write_stats adds them.
+// ---------------------------------------------------------------------------
+constexpr int kStatChunk = 16;  // planes per chunk (even: x-inverse jobs pair planes)
+enum { SQ_U, SQ_U2, SQ_V, SQ_V2, SQ_W, SQ_W2, SQ_P, SQ_P2, SQ_UV, SQ_UW, SQ_VW, SQ_WX, SQ_WX2, SQ_N };
+const int kStatGrid[SQ_N] = {0, 0, 1, 1, 0, 0, 2, 2, 0, 0, 1, 1, 1};
+const int kSpecGrid[5] = {0, 1, 0, 0, 2};  // spU spV spW spUV spP
+
+int ny_grid(const Solver& s, int g) { return g == 0 ? s.nyu : g == 1 ? s.nyv : s.nyp; }
+int jlo_grid(const Solver& s, int g) { return g == 0 ? s.jlo_u : g == 1 ? s.jlo_v : s.jlo_p; }
+
+int stats_setup(cfd_solver* s) {
+  auto& S = s->st;
+  if (S.ready) return CFD_OK;
+  const cfd_params& p = s->P;
+  if (p.ntilex < 1 || p.ntilez < 1 || s->ngx / p.ntilex < 1 || s->ngz / p.ntilez < 1)
+    return fail(s, CFD_EINVAL, "statistics: ntilex and ntilez must give tiles of at least one point");
+  band_table(p, s->yg, S.N);
+  auto NB = [&](int r, int b) { return S.N[4 * b + r - 1]; };
+  for (int r = 3; r <= 4; ++r)
+    if (!(NB(r, 0) <= NB(r, 1) && NB(r, 1) <= NB(r, 2) && NB(r, 2) <= NB(r, 3)))
+      return fail(s, CFD_EINVAL,
+                  "statistics: the band limits N(r,0:3) are not in order (the wall bands overlap), so the "
+                  "reference's plane partition has no wall bands to condition on");
+  S.dnx = s->ngx / p.ntilex;  // start.f90:456-457
+  S.dnz = s->ngz / p.ntilez;
+  S.bot[0][0] = NB(4, 0), S.bot[0][1] = NB(4, 1), S.top[0][0] = NB(4, 2) + 1, S.top[0][1] = NB(4, 3) + 1;
+  S.bot[1][0] = NB(3, 0), S.bot[1][1] = NB(3, 1), S.top[1][0] = NB(3, 2) + 1, S.top[1][1] = NB(3, 3) + 1;
+  S.bot[2][0] = NB(4, 0) + 1, S.bot[2][1] = NB(4, 1) - 1, S.top[2][0] = NB(4, 2) + 2, S.top[2][1] = NB(4, 3);
+  // Ny(grid, 0) and Ny(grid, nband) + 1 (start.f90:928-933)
+  S.jref_bot[0] = NB(4, 0), S.jref_top[0] = NB(4, 3) + 1;
+  S.jref_bot[1] = NB(3, 0), S.jref_top[1] = NB(3, 3) + 1;
+  S.jref_bot[2] = NB(4, 0) + 1, S.jref_top[2] = NB(4, 3);
+  for (int g = 0; g < 3; ++g) {
+    S.nyc[g] = S.bot[g][1] - S.bot[g][0] + 1;  // 0:Ny(grid,1)-Ny(grid,0)
+    const int jlo = jlo_grid(*s, g), jhi = jlo + ny_grid(*s, g) - 1;
+    if (S.nyc[g] < 1 || S.bot[g][0] < jlo || S.top[g][1] > jhi || S.jref_top[g] - S.top[g][0] >= S.nyc[g] ||
+        S.jref_top[g] - S.top[g][1] < 0)
+      return fail(s, CFD_EINVAL, "statistics: wall band ranges do not fit the grid");
+  }
+  int rc;
+  const size_t cell = (size_t)S.dnx * S.dnz;
+  for (int q = 0; q < SQ_N; ++q) {
+    const int g = kStatGrid[q];
+    if ((rc = dalloc(s, &S.prof[q], ny_grid(*s, g))) || (rc = dalloc(s, &S.cb[q], cell * S.nyc[g])) ||
+        (rc = dalloc(s, &S.ct[q], cell * S.nyc[g])))
+      return rc;
+  }
+  for (int q = 0; q < 5; ++q)
+    if ((rc = dalloc(s, &S.sp[q], (size_t)s->Ncol * ny_grid(*s, kSpecGrid[q])))) return rc;
+  // chunk buffers
+  S.chunk = kStatChunk;
+  const size_t zpl = (size_t)s->ngz * s->Mx, ppl = (size_t)s->ngz * s->ngx;
+  for (int f = 0; f < 3; ++f)
+    if ((rc = dalloc(s, &S.Zc[f], zpl * S.chunk)) || (rc = dalloc(s, &S.Pc[f], ppl * S.chunk))) return rc;
+  if ((rc = dalloc(s, &S.wxs, (size_t)s->Ncol * S.chunk))) return rc;
+  if ((rc = upload(s, &S.d_ju2, s->nl_u2i))) return rc;
+  // z-inverse jobs, chunk by chunk (nf per plane), and x-inverse jobs
+  // pairing planes t, t+1 of one field (a last odd plane pairs with itself)
+  for (int g = 0; g < 3; ++g) {
+    const int ny = ny_grid(*s, g), nf = g == 2 ? 1 : 3;
+    std::vector<ZInvJob> zj;
+    std::vector<XJob> xj;
+    for (int q0 = 0; q0 < ny; q0 += S.chunk) {
+      const int len = std::min(S.chunk, ny - q0);
+      for (int t = 0; t < len; ++t) {
+        const int q = q0 + t;
+        for (int f = 0; f < nf; ++f) {
+          ZInvJob z{};
+          z.pa = q;
+          z.pd = t;
+          z.mode = 0;
+          if (g == 0) {
+            if (f == 1) z = s->nl_u2i[q];  // u2 interpolated to the u plane
+            else z.src = s->F[f == 0 ? CFD_U1 : CFD_U3];
+          } else if (g == 1) {
+            if (f == 0) z.src = s->F[CFD_U2];
+            else if (f == 1) z = s->nl_u3i[q];  // u3 interpolated to the v plane
+            else {
+              z.src = S.wxs;  // omega_x of the chunk
+              z.pa = t;
+            }
+          } else {
+            z.src = s->F[CFD_P];
+          }
+          z.pd = t;
+          z.flags = 0;
+          z.dst = S.Zc[f];
+          zj.push_back(z);
+        }
+      }
+      for (int f = 0; f < nf; ++f)
+        for (int t = 0; t < len; t += 2) {
+          const bool pair = t + 1 < len;
+          XJob x{};
+          x.za = S.Zc[f];
+          x.zb = pair ? S.Zc[f] + zpl : S.Zc[f];
+          x.pa = S.Pc[f];
+          x.pb = pair ? S.Pc[f] + ppl : S.Pc[f];
+          x.plane = t;
+          xj.push_back(x);
+        }
+    }
+    if ((rc = upload(s, &S.d_zj[g], zj)) || (rc = upload(s, &S.d_xj[g], xj))) return rc;
+  }
+  // reduction tasks per grid (stats.f90:16-108)
+  double** P = S.Pc;
+  const int gx = s->ngx, gz = s->ngz;
+  std::vector<StatSumTask> su = {{P[0], nullptr, S.prof[SQ_U], S.prof[SQ_U2], gx, gz},
+                                 {P[2], nullptr, S.prof[SQ_W], S.prof[SQ_W2], gx, gz},
+                                 {P[0], P[1], S.prof[SQ_UV], nullptr, gx, gz},
+                                 {P[0], P[2], S.prof[SQ_UW], nullptr, gx, gz}};
+  // the omega_x sums run over i = 1..N1, k = 1..N2 of the physical plane (stats.f90:56-62)
+  std::vector<StatSumTask> sv = {{P[0], nullptr, S.prof[SQ_V], S.prof[SQ_V2], gx, gz},
+                                 {P[0], P[1], S.prof[SQ_VW], nullptr, gx, gz},
+                                 {P[2], nullptr, S.prof[SQ_WX], S.prof[SQ_WX2], s->N1, s->N2}};
+  std::vector<StatSumTask> sp = {{P[0], nullptr, S.prof[SQ_P], S.prof[SQ_P2], gx, gz}};
+  auto ct = [&](int g, const double* a, const double* b, int q1, int q2, double sa, double sb) {
+    StatCondTask c{};
+    c.a = a;
+    c.b = b;
+    c.mb = S.cb[q1];
+    c.mt = S.ct[q1];
+    c.m2b = q2 >= 0 ? S.cb[q2] : nullptr;
+    c.m2t = q2 >= 0 ? S.ct[q2] : nullptr;
+    c.sa = sa;
+    c.sb = sb;
+    c.bot0 = S.bot[g][0];
+    c.bot1 = S.bot[g][1];
+    c.top0 = S.top[g][0];
+    c.top1 = S.top[g][1];
+    c.jref_bot = S.jref_bot[g];
+    c.jref_top = S.jref_top[g];
+    return c;
+  };
+  // condup signs (stats.f90:98-105)
+  std::vector<StatCondTask> cu = {ct(0, P[0], nullptr, SQ_U, SQ_U2, 1.0, 1.0), ct(0, P[2], nullptr, SQ_W, SQ_W2, 1.0, 1.0),
+                                  ct(0, P[0], P[1], SQ_UV, -1, 1.0, -1.0), ct(0, P[0], P[2], SQ_UW, -1, 1.0, 1.0)};
+  std::vector<StatCondTask> cv = {ct(1, P[0], nullptr, SQ_V, SQ_V2, -1.0, 1.0),
+                                  ct(1, P[2], nullptr, SQ_WX, SQ_WX2, 1.0, 1.0),
+                                  ct(1, P[0], P[1], SQ_VW, -1, -1.0, 1.0)};
+  std::vector<StatCondTask> cp = {ct(2, P[0], nullptr, SQ_P, SQ_P2, 1.0, 1.0)};
+  const std::vector<StatSumTask>* sums[3] = {&su, &sv, &sp};
+  const std::vector<StatCondTask>* conds[3] = {&cu, &cv, &cp};
+  for (int g = 0; g < 3; ++g) {
+    if ((rc = upload(s, &S.d_sum[g], *sums[g])) || (rc = upload(s, &S.d_cond[g], *conds[g]))) return rc;
+    S.nsum[g] = (int)sums[g]->size();
+    S.ncond[g] = (int)conds[g]->size();
+  }
+  S.ready = true;
+  return CFD_OK;
+}
+
+// one sample of spectra and statistics, queued on the solver's stream
+int stats_sample(cfd_solver* s) {
+  if (int rc = stats_setup(s)) return rc;
+  auto& S = s->st;
+  StatsSpecArgs A{};
+  A.u1 = s->F[CFD_U1];
+  A.u2 = s->F[CFD_U2];
+  A.u3 = s->F[CFD_U3];
+  A.p = s->F[CFD_P];
+  A.ju2 = S.d_ju2;
+  A.spU = S.sp[0];
+  A.spV = S.sp[1];
+  A.spW = S.sp[2];
+  A.spUV = S.sp[3];
+  A.spP = S.sp[4];
+  A.Ncol = s->Ncol;
+  A.nyu = s->nyu;
+  A.nyv = s->nyv;
+  A.nyp = s->nyp;
+  launch_spectra(*s, A);
+  for (int g = 0; g < 3; ++g) {
+    const int ny = ny_grid(*s, g), nf = g == 2 ? 1 : 3;
+    for (int q0 = 0; q0 < ny; q0 += S.chunk) {
+      const int len = std::min(S.chunk, ny - q0);
+      if (g == 1) {
+        StatsWxArgs W{s->F[CFD_U2], s->F[CFD_U3], s->d_dthdyv, s->d_k1z, S.wxs, q0, s->Ncol, s->Mx, s->N2};
+        launch_omega_x(*s, W, len);
+      }
+      launch_zinv(*s, S.d_zj[g] + (size_t)q0 * nf, len * nf, false);
+      launch_xpass_inv2(*s, S.d_xj[g] + (size_t)nf * (q0 / 2), nf * ((len + 1) / 2));
+      launch_stat_sums(*s, S.d_sum[g], S.nsum[g], len, q0);
+      launch_stat_cond(*s, S.d_cond[g], S.ncond[g], len, S.dnx, S.dnz, jlo_grid(*s, g) + q0);
+    }
+  }
+  S.istat += 1;
+  return launched(s);
+}
+
+// write_spect + write_stats (spectra.f90:62-182, stats.f90:255-442), then
+// zero the accumulators.  The header carries t_hdr and mpgx_hdr; the file
+// names end in _<N1>x<N2>x<ny>_<int(10 t_name)>.dat, the stats file directly
+// after dirout ('stats_', no separator, stats.f90:357) and the spectra after
+// dirout/ ('/spU_' ..., spectra.f90:99-108).
+int stats_write(cfd_solver* s, const char* dirout, int iter, double t_hdr, double mpgx_hdr, double t_name) {
+  if (int rc = stats_setup(s)) return rc;
+  auto& S = s->st;
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  const size_t cell = (size_t)S.dnx * S.dnz;
+  std::vector<double> prof[SQ_N], cond[SQ_N];
+  for (int q = 0; q < SQ_N; ++q) {
+    const int g = kStatGrid[q];
+    prof[q].resize(ny_grid(*s, g));
+    HIPCHK(s, hipMemcpy(prof[q].data(), S.prof[q], prof[q].size() * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<double> b(cell * S.nyc[g]), t(cell * S.nyc[g]);
+    HIPCHK(s, hipMemcpy(b.data(), S.cb[q], b.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(s, hipMemcpy(t.data(), S.ct[q], t.size() * sizeof(double), hipMemcpyDeviceToHost));
+    cond[q].resize(b.size());
+    for (size_t e = 0; e < b.size(); ++e) cond[q][e] = b[e] + t[e];  // xC = xC + buffC (recvXC)
+  }
+  cfd_file_header hd{};
+  hd.t = t_hdr;
+  hd.Re = s->P.Re;
+  hd.alp = s->P.alp;
+  hd.bet = s->P.bet;
+  hd.mpgx = mpgx_hdr;
+  hd.nband = 3;
+  hd.iter = iter;
+  std::memcpy(hd.N, S.N, sizeof hd.N);
+  int Ngal[20];
+  std::memcpy(Ngal, S.N, sizeof Ngal);
+  for (int b = 1; b <= 3; ++b) {  // Ngal(1:2, band) = the physical sizes (start.f90:92-100)
+    Ngal[4 * b] = s->ngx;
+    Ngal[4 * b + 1] = s->ngz;
+  }
+  char tag[64];
+  snprintf(tag, sizeof tag, "%04dx%04dx%04d_%05d.dat", s->N1, s->N2, s->ny, (int)(10.0 * t_name));
+  // spectra: fold -kz onto kz (k > N2/2 -> N2 + 2 - k, so kz = -N2/2 lands in
+  // the last row), double every kx > 0 (recvwrspec, spectra.f90:142-170)
+  const char* spname[5] = {"spU", "spV", "spW", "spUV", "spP"};
+  const int mx = s->Mx, nzh = s->N2 / 2 + 1;
+  for (int q = 0; q < 5; ++q) {
+    const int ny = ny_grid(*s, kSpecGrid[q]);
+    std::vector<double> raw((size_t)s->Ncol * ny), buf((size_t)mx * nzh * ny, 0.0);
+    HIPCHK(s, hipMemcpy(raw.data(), S.sp[q], raw.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int pass = 0; pass < 2; ++pass)  // assigned from kz >= 0, then the folded -kz added
+      for (int k = 0; k < s->N2; ++k) {
+        const bool neg = k >= s->N2 / 2;
+        if (neg != (pass == 1)) continue;
+        const int kk = neg ? s->N2 - k : k;  // 0-based row of buffSP(., 1:N2/2+1, .)
+        for (int jq = 0; jq < ny; ++jq)
+          for (int i = 0; i < mx; ++i) {
+            double& d = buf[((size_t)jq * nzh + kk) * mx + i];
+            const double v = raw[(size_t)jq * s->Ncol + (size_t)k * mx + i];
+            d = pass == 0 ? v : d + v;
+          }
+      }
+    for (size_t e = 0; e < buf.size(); ++e)
+      if (e % mx != 0) buf[e] = 2.0 * buf[e];
+    const std::string path = std::string(dirout) + "/" + spname[q] + "_" + tag;
+    const std::string e = write_spectrum_file(path.c_str(), hd, s->yg, S.istat, buf);
+    if (!e.empty()) return fail(s, CFD_EINVAL, "write_spect: " + e);
+  }
+  const std::string path = std::string(dirout) + "stats_" + tag;
+  const std::string e = write_stats_file(path.c_str(), hd, Ngal, s->yg, S.istat, prof, cond);
+  if (!e.empty()) return fail(s, CFD_EINVAL, "write_stats: " + e);
+  // zero (stats.f90:396-426, spectra.f90:124-128)
+  for (int q = 0; q < SQ_N; ++q) {
+    const int g = kStatGrid[q];
+    HIPCHK(s, hipMemsetAsync(S.prof[q], 0, ny_grid(*s, g) * sizeof(double), s->stream));
+    HIPCHK(s, hipMemsetAsync(S.cb[q], 0, cell * S.nyc[g] * sizeof(double), s->stream));
+    HIPCHK(s, hipMemsetAsync(S.ct[q], 0, cell * S.nyc[g] * sizeof(double), s->stream));
+  }
+  for (int q = 0; q < 5; ++q)
+    HIPCHK(s, hipMemsetAsync(S.sp[q], 0, (size_t)s->Ncol * ny_grid(*s, kSpecGrid[q]) * sizeof(double), s->stream));
+  S.istat = 0;
+  return CFD_OK;
+}
+
+int read_scalars(cfd_solver* s, DevScalars& d) {
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  HIPCHK(s, hipMemcpy(&d, s->d_sc, sizeof d, hipMemcpyDeviceToHost));
+  return CFD_OK;
+}
+
 void op_nonlinear(cfd_solver* s, int kRK) {
   if (kRK == 1) launch_reset_max(*s);
   { Timed t(s, KC_ZINV); launch_zinv(*s, s->d_nljobs, s->n_nljobs, true); }
@@ -696,12 +1018,35 @@ void op_solve(cfd_solver* s, int f, int kRK) {
 // Helmholtz system, the (0,0) flow-rate correction, and one sweep per column
 // for solveP + v_corr.  Same arithmetic as the operator-by-operator path;
 // du holds r only on the IB planes afterwards.
-void op_substep(cfd_solver* s, int kRK) {
+//
+// A statistics request (cfd_stats_request) is served by the next substep
+// with kRK = 1 where the reference's nonlinear serves flagst / flagwr
+// (FOU3D.f90:56-69, 107-110, 177-226): the sample is of the fields at the
+// start of the substep, and the files are written after the time step
+// update, with the new t and the mpgx of the start of the substep in the
+// header and the name taken from the t of the start.
+int op_substep(cfd_solver* s, int kRK) {
+  const int req = kRK == 1 ? s->st.pending : 0;
+  DevScalars d0{};
+  if (req) {
+    s->st.pending = 0;
+    if (req & 2)
+      if (int rc = read_scalars(s, d0)) return rc;
+    if (req & 1)
+      if (int rc = stats_sample(s)) return rc;
+  }
   if (kRK == 1) launch_reset_max(*s);
   { Timed t(s, KC_ZINV); launch_zinv(*s, s->d_nljobs, s->n_nljobs, true); }
   { Timed t(s, KC_XPASS); launch_xpass_nl(*s, kRK == 1); }
   { Timed t(s, KC_ZFWD); launch_zfwd_nl(*s); }
   if (kRK == 1) launch_cfl(*s);
+  // a failed write (say, a missing directory) still completes the substep
+  int wrc = CFD_OK;
+  if (req & 2) {
+    DevScalars d1{};
+    wrc = read_scalars(s, d1);
+    if (wrc == CFD_OK) wrc = stats_write(s, s->st.pending_dir.c_str(), s->st.pending_iter, d1.t, d0.mpgx, d0.t);
+  }
   {
     Timed t(s, KC_IB);
     launch_ib_rhs(*s, kRK);
@@ -713,6 +1058,7 @@ void op_substep(cfd_solver* s, int kRK) {
     launch_meanflow_lds(*s, kRK);
   }
   { Timed t(s, KC_PRESS); launch_pressure(*s, kRK); }
+  return wrc;
 }
 
 }  // namespace
@@ -764,7 +1110,7 @@ extern "C" int cfd_vcorr(cfd_handle h, int kRK) {
 extern "C" int cfd_substep(cfd_handle h, int kRK) {
   cfd_solver* s = static_cast<cfd_solver*>(h);
   if (int rc = check(s, kRK)) return rc;
-  op_substep(s, kRK);
+  if (int rc = op_substep(s, kRK)) return rc;
   return launched(s);
 }
 extern "C" int cfd_step(cfd_handle h, int nsteps) {
@@ -772,7 +1118,8 @@ extern "C" int cfd_step(cfd_handle h, int nsteps) {
   if (int rc = check(s, 1)) return rc;
   if (nsteps < 0) return fail(s, CFD_EINVAL, "nsteps must be >= 0");
   for (int it = 0; it < nsteps; ++it) {
-    for (int kRK = 1; kRK <= 3; ++kRK) op_substep(s, kRK);
+    for (int kRK = 1; kRK <= 3; ++kRK)
+      if (int rc = op_substep(s, kRK)) return rc;
     if (s->P.flag_ctpress == 1) launch_flowrate(*s);
     if (int rc = launched(s)) return rc;
   }
@@ -994,5 +1341,51 @@ extern "C" int cfd_four_to_phys(cfd_handle h, int which, double* phys) {
   if (e == hipSuccess) e = hipMemcpy(phys, p, ppl * ny * sizeof(double), hipMemcpyDeviceToHost);
   cleanup();
   if (e != hipSuccess) return fail(s, CFD_EHIP, std::string("four_to_phys: ") + hipGetErrorString(e));
+  return CFD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// statistics and spectra
+// ---------------------------------------------------------------------------
+extern "C" int cfd_stats(cfd_handle h) {
+  cfd_solver* s = static_cast<cfd_solver*>(h);
+  if (int rc = check(s, 1)) return rc;
+  return stats_sample(s);
+}
+
+extern "C" int cfd_write_stats(cfd_handle h, const char* dirout, int iter) {
+  cfd_solver* s = static_cast<cfd_solver*>(h);
+  if (int rc = check(s, 1)) return rc;
+  if (!dirout) return fail(s, CFD_EINVAL, "null dirout");
+  DevScalars d{};
+  if (int rc = read_scalars(s, d)) return rc;
+  return stats_write(s, dirout, iter, d.t, d.mpgx, d.t);
+}
+
+extern "C" int cfd_stats_request(cfd_handle h, int flags, const char* dirout, int iter) {
+  cfd_solver* s = static_cast<cfd_solver*>(h);
+  if (int rc = check(s, 1)) return rc;
+  if (flags < 0 || flags > 3) return fail(s, CFD_EINVAL, "flags must be 0..3 (bit 0 sample, bit 1 write)");
+  if ((flags & 2) && !dirout) return fail(s, CFD_EINVAL, "null dirout");
+  if (flags)
+    if (int rc = stats_setup(s)) return rc;
+  s->st.pending = flags;
+  s->st.pending_dir = (flags & 2) ? dirout : "";
+  s->st.pending_iter = iter;
+  return CFD_OK;
+}
+
+extern "C" int cfd_stats_dims(cfd_handle h, int* d) {
+  cfd_solver* s = static_cast<cfd_solver*>(h);
+  if (int rc = check(s, 1)) return rc;
+  if (!d) return fail(s, CFD_EINVAL, "null argument");
+  if (int rc = stats_setup(s)) return rc;
+  const auto& S = s->st;
+  d[0] = S.dnx;
+  d[1] = S.dnz;
+  d[2] = S.nyc[0];
+  d[3] = S.nyc[1];
+  d[4] = S.nyc[2];
+  d[5] = S.istat;
   return CFD_OK;
 }

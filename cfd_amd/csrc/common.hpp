@@ -181,6 +181,38 @@ void launch_solve_p(Solver& s, int kRK);
 void launch_vcorr(Solver& s, int kRK);
 void launch_divergence(Solver& s);
 void launch_flowrate(Solver& s);
+// statistics (kernels_stats.hip)
+struct StatsSpecArgs {
+  const double2 *u1, *u2, *u3, *p;
+  const ZInvJob* ju2;  // per u plane: u2 interpolated to it (the nonlinear z-inverse job)
+  double *spU, *spV, *spW, *spUV, *spP;  // sp[q * Ncol + col]
+  int Ncol, nyu, nyv, nyp;
+};
+struct StatsWxArgs {
+  const double2 *u2, *u3;
+  const double *dthdyv, *k1z;
+  double2* wx;  // chunk buffer, plane q - q0
+  int q0, Ncol, Mx, N2;
+};
+struct StatSumTask {
+  const double* a;  // physical chunk planes (rows of ngx)
+  const double* b;  // second factor or null
+  double *m, *m2;   // profile accumulators (m2 may be null)
+  int nx, nz;       // summed extent
+};
+struct StatCondTask {
+  const double *a, *b;
+  double *mb, *m2b, *mt, *m2t;  // bottom / top conditional sums (dnx, dnz, j1)
+  double sa, sb;                // top-band signs
+  int bot0, bot1, top0, top1;   // wall band plane ranges
+  int jref_bot, jref_top;       // j1 = j - jref_bot, jref_top - j
+};
+void launch_spectra(Solver& s, const StatsSpecArgs& A);
+void launch_omega_x(Solver& s, const StatsWxArgs& A, int nplanes);
+// plane sums of chunk planes 0..nplanes-1 into profile entries j0.. ; tile
+// sums of the chunk planes whose plane j (jfirst + t) lies in a wall band
+void launch_stat_sums(Solver& s, const StatSumTask* tasks, int ntasks, int nplanes, int j0);
+void launch_stat_cond(Solver& s, const StatCondTask* tasks, int ntasks, int nplanes, int dnx, int dnz, int jfirst);
 // launchers (kernels_fused.hip)
 void launch_ib_rhs(Solver& s, int kRK);
 void launch_rhs_solve(Solver& s, int kRK);

@@ -62,15 +62,7 @@ T get(const std::vector<unsigned char>& b, size_t off) {
   return v;
 }
 
-}  // namespace
-
-std::string write_field_file(const char* path, const cfd_file_header& hd, int id, double dthetavi, const double* y,
-                             const double* dthdy, int ny_arr, int jy0, int ny_g, int j0, int nx, int nz,
-                             const double* planes) {
-  if (j0 < jy0 || j0 + ny_g > jy0 + ny_arr) return "plane range outside the grid array";
-  File F;
-  F.f = fopen(path, "wb");
-  if (!F.f) return std::string("cannot open ") + path + " for writing";
+Rec header_rec(const cfd_file_header& hd) {
   Rec r1;
   r1.put(hd.t);
   r1.put(hd.Re);
@@ -81,8 +73,33 @@ std::string write_field_file(const char* path, const cfd_file_header& hd, int id
   r1.put((int32_t)hd.iter);
   const int32_t zeros[88] = {};
   r1.put(zeros, 88);
-  Rec r2;
-  for (int q = 0; q < 20; ++q) r2.put((int32_t)hd.N[q]);
+  return r1;
+}
+
+Rec int20_rec(const int* v) {
+  Rec r;
+  for (int q = 0; q < 20; ++q) r.put((int32_t)v[q]);
+  return r;
+}
+
+Rec grid_rec(const std::vector<double>& y, double dtheta, const std::vector<double>& dthdy) {
+  Rec r;
+  r.put(y.data(), y.size());
+  r.put(dtheta);
+  r.put(dthdy.data(), dthdy.size());
+  return r;
+}
+
+}  // namespace
+
+std::string write_field_file(const char* path, const cfd_file_header& hd, int id, double dthetavi, const double* y,
+                             const double* dthdy, int ny_arr, int jy0, int ny_g, int j0, int nx, int nz,
+                             const double* planes) {
+  if (j0 < jy0 || j0 + ny_g > jy0 + ny_arr) return "plane range outside the grid array";
+  File F;
+  F.f = fopen(path, "wb");
+  if (!F.f) return std::string("cannot open ") + path + " for writing";
+  const Rec r1 = header_rec(hd), r2 = int20_rec(hd.N);
   Rec r3;
   r3.put(y, (size_t)ny_arr);
   r3.put(dthetavi);
@@ -145,6 +162,44 @@ std::string read_field_file(const char* path, cfd_file_header& hd, int ny_g, int
     for (int kf = nzf - kh + 1; kf <= nzf; ++kf) copy(kf, kf + ngz - nzf);
   }
   return "";
+}
+
+std::string write_stats_file(const char* path, const cfd_file_header& hd, const int Ngal[20], const YGrid& g,
+                             int istat, const std::vector<double> prof[13], const std::vector<double> cond[13]) {
+  File F;
+  F.f = fopen(path, "wb");
+  if (!F.f) return std::string("cannot open ") + path + " for writing";
+  Rec ri;
+  ri.put((int32_t)istat);
+  // dthetai = 1/dtheta = 1 (start.f90:681-682)
+  bool ok = write_rec(F.f, header_rec(hd)) && write_rec(F.f, int20_rec(Ngal)) &&
+            write_rec(F.f, grid_rec(g.yu, 1.0, g.dthdyu)) && write_rec(F.f, grid_rec(g.yv, 1.0, g.dthdyv)) &&
+            write_rec(F.f, ri);
+  for (int q = 0; q < 13 && ok; ++q) {
+    Rec r;
+    r.put(prof[q].data(), prof[q].size());
+    ok = write_rec(F.f, r);
+  }
+  for (int q = 0; q < 13 && ok; ++q) {
+    Rec r;
+    r.put(cond[q].data(), cond[q].size());
+    ok = write_rec(F.f, r);
+  }
+  return ok ? "" : std::string("write error on ") + path;
+}
+
+std::string write_spectrum_file(const char* path, const cfd_file_header& hd, const YGrid& g, int istat,
+                                const std::vector<double>& sp) {
+  File F;
+  F.f = fopen(path, "wb");
+  if (!F.f) return std::string("cannot open ") + path + " for writing";
+  Rec ri, rs;
+  ri.put((int32_t)istat);
+  rs.put(sp.data(), sp.size());
+  const bool ok = write_rec(F.f, header_rec(hd)) && write_rec(F.f, int20_rec(hd.N)) &&
+                  write_rec(F.f, grid_rec(g.yu, 1.0, g.dthdyu)) && write_rec(F.f, grid_rec(g.yv, 1.0, g.dthdyv)) &&
+                  write_rec(F.f, ri) && write_rec(F.f, rs);
+  return ok ? "" : std::string("write error on ") + path;
 }
 
 void band_table(const cfd_params& p, const YGrid& g, int N[20]) {

@@ -1,6 +1,7 @@
 // io.hpp -- field files in the reference layout (host code, io.cpp).
 #pragma once
 #include <string>
+#include <vector>
 
 #include "cfd_amd.h"
 #include "setup.hpp"
@@ -13,6 +14,14 @@ std::string write_field_file(const char* path, const cfd_file_header& hd, int id
                              const double* planes);
 std::string read_field_file(const char* path, cfd_file_header& hd, int ny_g, int nx, int nz, int ngz, double* planes,
                             int& mismatch);
+// write_stats (stats.f90:255-442): header, Ngal(4,0:4), yu/dthetai/dthdyu,
+// yv/dthetai/dthdyv, istat, the 13 profiles, the 13 conditional arrays
+std::string write_stats_file(const char* path, const cfd_file_header& hd, const int Ngal[20], const YGrid& g,
+                             int istat, const std::vector<double> prof[13], const std::vector<double> cond[13]);
+// recvwrspec (spectra.f90:132-182): header, N, the two grid records, istat,
+// buffSP(0:N1/2, 1:N2/2+1, planes)
+std::string write_spectrum_file(const char* path, const cfd_file_header& hd, const YGrid& g, int istat,
+                                const std::vector<double>& sp);
 // N(1:4, 0:4) of the reference for this configuration (uniform resolution)
 void band_table(const cfd_params& p, const YGrid& g, int N[20]);
 

@@ -14,7 +14,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <climits>
 #include <cstdio>
+#include <string>
 
 namespace cfd {
 namespace {
@@ -189,6 +191,12 @@ bool build_ygrid(const cfd_params& p, YGrid& g, std::string& err) {
     c2(g.d2u, k - g.jlo_u, 1.0 / (hu * hu), -2.0 / (hu * hu), 0.0);
   }
   return true;
+}
+
+int ygrid_fold(const YGrid& g) {
+  for (int j = g.jlo_v; j < g.jlo_v + g.nyv; ++j)
+    if (!(g.YU(j) < g.YV(j) && g.YV(j) < g.YU(j + 1))) return j;
+  return INT_MIN;
 }
 
 // ---------------------------------------------------------------------------

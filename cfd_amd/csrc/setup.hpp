@@ -28,6 +28,11 @@ struct IBPoint {
 };
 
 bool build_ygrid(const cfd_params& p, YGrid& g, std::string& err);
+// the first face j that is not strictly between its centres (yu(j) < yv(j) <
+// yu(j+1)), INT_MIN for a grid that increases everywhere.  The reference's
+// maps fold for some inputs (a canopy depth posth far from dsty times the tip
+// spacing, posth = 0 with dsty > 0, steep polynomial maps)
+int ygrid_fold(const YGrid& g);
 // lists[grid][wall]: grid 0 = u/w centres, 1 = v faces; wall 0 bottom, 1 top.
 bool build_iblists(const cfd_params& p, std::vector<IBPoint> lists[2][2], std::string& err);
 

@@ -143,8 +143,12 @@ def convert(nx, nz, ny, t, dirname, dirout):
         N = list(struct.unpack("<20i", head[1][:80]))
         plane_sequence(grid, N)  # refuses a band table the plane stream does not fit
         if ch is None:
-            p = Params(ngalx=N[4] * 3 // 2, ngalz=N[5] * 3 // 2, ny=ny, dsty=-N[2], geometry_type=0, posth=0.0,
-                       ygrid=0)
+            # the transforms need the plane counts and the x/z sizes only: a
+            # grid without canopy points and ny + 2 dsty planes has the same
+            # u, v and p plane counts, whatever the file's y grid was
+            dsty = -N[2]
+            p = Params(ngalx=N[4] * 3 // 2, ngalz=N[5] * 3 // 2, ny=ny + 2 * dsty, dsty=0, geometry_type=0,
+                       posth=0.0, ygrid=0)
             ch = Channel(p)
         ny_g = {"u": ch.nyu, "v": ch.nyv, "p": ch.nyp}[grid]
         h, planes, mm = fieldfile.read(src, ny_g, ch.N2, ch.Mx, ch.params.ngalz)

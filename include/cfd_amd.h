@@ -176,6 +176,41 @@ int cfd_sync(cfd_handle h);
 /* device memory held by the handle, bytes (no counterpart in the reference) */
 int cfd_device_bytes(cfd_handle h, double* bytes);
 
+/* Statistics and spectra (stats.f90, spectra.f90).
+ *
+ * cfd_stats adds one sample of the current state (the reference's
+ * nonlinear with flagst, FOU3D.f90:107-110 and 185-221): the spectra
+ * sp(j, column) += |u|^2 of u1, u2, u3, p and Re(u1 conj u2_itp) over every
+ * plane and column (spectra.f90:17-59); the plane sums of the physical u1,
+ * u1^2, u2, u2^2, u3, u3^2, p, p^2, u1 u2_itp, u1 u3, u2 u3_itp and
+ * omega_x, omega_x^2 (the omega_x sums over i <= N1, k <= N2 of the plane,
+ * stats.f90:56-62); and the tile-conditional sums of the same over the
+ * planes of the two wall bands (conddown / condup, stats.f90:110-253),
+ * tiles of dnx = Ngalx/ntilex by dnz = Ngalz/ntilez points.
+ *
+ * cfd_write_stats writes dirout/spU_<tag> ... dirout/spP_<tag> (recvwrspec,
+ * spectra.f90:132-182) and dirout + "stats_<tag>" (write_stats,
+ * stats.f90:355-392; no separator, as the reference concatenates them),
+ * tag = <N1>x<N2>x<ny>_<int(10 t)>.dat (%04d, %05d), with the current t and
+ * mpgx in the headers, then zeroes every accumulator and istat.
+ *
+ * cfd_stats_request(h, flags, dirout, iter): bit 0 sample, bit 1 write; the
+ * next cfd_substep / cfd_step substep with kRK = 1 serves it where the
+ * reference's nonlinear does: the sample of the fields at the start of the
+ * substep, the files after the time-step update, with the new t and the
+ * start-of-substep mpgx in the headers and the start-of-substep t in the
+ * names.
+ *
+ * cfd_stats_dims: d[6] = dnx, dnz, wall-band plane counts of the u, v and p
+ * grids (the third extent of the conditional records), istat.
+ *
+ * All four return CFD_EINVAL when ntilex or ntilez is below 1 or the band
+ * table's limits are not in order (no wall bands to condition on). */
+int cfd_stats(cfd_handle h);
+int cfd_write_stats(cfd_handle h, const char* dirout, int iter);
+int cfd_stats_request(cfd_handle h, int flags, const char* dirout, int iter);
+int cfd_stats_dims(cfd_handle h, int* d);
+
 /* Kernel-class timing with HIP events on the handle's stream.  classes:
    0 z-inverse FFT, 1 x-pass FFT+products, 2 z-forward FFT, 3 spectral
    elementwise, 4 y-solves (u,v,w), 5 pressure solve + correction, 6 IB.

@@ -180,6 +180,7 @@ struct or_state {
   double* ibw[2][2];
   fplan pz, px2; /* Ngz complex; Ngx/2 complex for the real transform */
   cplx* twx;     /* exp(-2 pi i k/Ngx), k=0..Ngx/2 */
+  struct or_stats* st; /* statistics accumulators, created by or_stats_init */
 };
 
 static int g_threads = 0;
@@ -662,6 +663,8 @@ or_state* or_create(const or_params* p, char* err, int errlen) {
   return s;
 }
 
+static void or_stats_free(or_state* s);
+
 void or_destroy(or_state* s) {
   if (!s) return;
   free(s->yu); free(s->yv); free(s->dthdyu); free(s->dthdyv); free(s->dyu2i); free(s->dyv2i);
@@ -673,6 +676,7 @@ void or_destroy(or_state* s) {
   if (s->pz.tw) fplan_free(&s->pz);
   if (s->px2.tw) fplan_free(&s->px2);
   free(s->twx);
+  or_stats_free(s);
   free(s);
 }
 
@@ -1274,4 +1278,309 @@ void or_rft(int n, double* c, int isa, int m, int iopt) {
   for (int v = 0; v < m; ++v) rft_line(&P, tw, c + (size_t)v * isa, iopt, w0, w1);
   free(w0); free(w1); free(tw);
   fplan_free(&P);
+}
+
+/* ------------------------------------------------------------------------ */
+/* statistics and spectra (stats.f90:1-253, spectra.f90:1-59), accumulated  */
+/* as the reference does at kRK = 1 of a sampled step (FOU3D.f90:107-110,   */
+/* 185-221).  One spectral resolution: the spectra's midband columns are    */
+/* all columns and its y ranges (jlim(.,.,midband), start.f90:1538-1557)     */
+/* all planes of each grid.  The plane sets of the wall bands are those of  */
+/* the reference's plane partition with walls (limPL_incw,                   */
+/* start.f90:1759-1866): u planes N(4,0)..N(4,1) and N(4,2)+1..N(4,3)+1,     */
+/* v planes N(3,0)..N(3,1) and N(3,2)+1..N(3,3)+1, p planes                  */
+/* N(4,0)+1..N(4,1)-1 and N(4,2)+2..N(4,3).                                  */
+/* ------------------------------------------------------------------------ */
+enum { ST_U, ST_U2, ST_V, ST_V2, ST_W, ST_W2, ST_P, ST_P2, ST_UV, ST_UW, ST_VW, ST_WX, ST_WX2, ST_N };
+static const int st_grid[ST_N] = {0, 0, 1, 1, 0, 0, 2, 2, 0, 0, 1, 1, 1};
+
+struct or_stats {
+  int N[20];
+  int dnx, dnz, nyc[3], istat;
+  int bot[3][2], top[3][2], jref_bot[3], jref_top[3]; /* band plane ranges; j1 = j - jref_bot, jref_top - j */
+  double* prof[ST_N];                    /* Um(j) ... over the planes of the grid */
+  double* cb[ST_N];                      /* conddown sums (dnx, dnz, 0:nyc-1), i fastest */
+  double* ct[ST_N];                      /* condup sums, same layout */
+  double* sp[5];                         /* spU spV spW spUV spP: sp(j, column), y fastest */
+};
+
+static void or_stats_free(or_state* s) {
+  struct or_stats* t = s->st;
+  if (!t) return;
+  for (int q = 0; q < ST_N; ++q) { free(t->prof[q]); free(t->cb[q]); free(t->ct[q]); }
+  for (int q = 0; q < 5; ++q) free(t->sp[q]);
+  free(t);
+  s->st = NULL;
+}
+
+static int ny_grid(const or_state* s, int g) { return g == 0 ? s->nyu : g == 1 ? s->nyv : s->nyp; }
+static int jlo_grid(const or_state* s, int g) { return g == 0 ? s->jlo_u : g == 1 ? s->jlo_v : s->jlo_p; }
+
+/* N20: the band table N(1:4, 0:4) (start.f90:89-111, y_grid_canopy.f90:250-275) */
+int or_stats_init(or_state* s, const int* N20) {
+  or_stats_free(s);
+  if (s->P.ntilex <= 0 || s->P.ntilez <= 0) return 1;
+  struct or_stats* t = (struct or_stats*)calloc(1, sizeof *t);
+  memcpy(t->N, N20, sizeof t->N);
+#define NB(r, b) (t->N[4 * (b) + (r) - 1])
+  /* wall bands exist only for band limits in order (the plane partition of
+     proc_lims_planes, start.f90:1759-1866, counts N(r,b) - N(r,b-1) planes) */
+  for (int r = 3; r <= 4; ++r)
+    if (!(NB(r, 0) <= NB(r, 1) && NB(r, 1) <= NB(r, 2) && NB(r, 2) <= NB(r, 3))) {
+      free(t);
+      return 2;
+    }
+  t->dnx = s->ngx / s->P.ntilex; /* start.f90:456-457 */
+  t->dnz = s->ngz / s->P.ntilez;
+  t->bot[0][0] = NB(4, 0); t->bot[0][1] = NB(4, 1); t->top[0][0] = NB(4, 2) + 1; t->top[0][1] = NB(4, 3) + 1;
+  t->bot[1][0] = NB(3, 0); t->bot[1][1] = NB(3, 1); t->top[1][0] = NB(3, 2) + 1; t->top[1][1] = NB(3, 3) + 1;
+  t->bot[2][0] = NB(4, 0) + 1; t->bot[2][1] = NB(4, 1) - 1; t->top[2][0] = NB(4, 2) + 2; t->top[2][1] = NB(4, 3);
+  /* Ny(grid, 0) and Ny(grid, nband) + 1 (start.f90:928-933; stats.f90:137, 186) */
+  t->jref_bot[0] = NB(4, 0); t->jref_top[0] = NB(4, 3) + 1;
+  t->jref_bot[1] = NB(3, 0); t->jref_top[1] = NB(3, 3) + 1;
+  t->jref_bot[2] = NB(4, 0) + 1; t->jref_top[2] = NB(4, 3);
+#undef NB
+  for (int g = 0; g < 3; ++g) {
+    t->nyc[g] = t->bot[g][1] - t->bot[g][0] + 1; /* 0:Ny(grid,1)-Ny(grid,0) (start.f90:2723-2728) */
+    if (t->nyc[g] < 1 || t->jref_top[g] - t->top[g][0] >= t->nyc[g] || t->jref_top[g] - t->top[g][1] < 0) {
+      free(t);
+      return 2;
+    }
+  }
+  const size_t cell = (size_t)t->dnx * t->dnz;
+  for (int q = 0; q < ST_N; ++q) {
+    const int g = st_grid[q];
+    t->prof[q] = calloc(ny_grid(s, g), sizeof(double));
+    t->cb[q] = calloc(cell * t->nyc[g], sizeof(double));
+    t->ct[q] = calloc(cell * t->nyc[g], sizeof(double));
+  }
+  const int spg[5] = {0, 1, 0, 0, 2};
+  for (int q = 0; q < 5; ++q) t->sp[q] = calloc((size_t)s->Ncol * ny_grid(s, spg[q]), sizeof(double));
+  s->st = t;
+  return 0;
+}
+
+void or_stats_dims(const or_state* s, int* d) {
+  const struct or_stats* t = s->st;
+  if (!t) { memset(d, 0, sizeof(int) * 6); return; }
+  d[0] = t->dnx; d[1] = t->dnz; d[2] = t->nyc[0]; d[3] = t->nyc[1]; d[4] = t->nyc[2]; d[5] = t->istat;
+}
+
+/* part 0: profile of quantity q (13, write order of write_stats); 1: bottom
+   conditional sums; 2: top conditional sums; 3: spectrum q (0 U, 1 V, 2 W,
+   3 UV, 4 P), sp(j, column) */
+int or_stats_get(const or_state* s, int q, int part, double* out) {
+  const struct or_stats* t = s->st;
+  if (!t) return 1;
+  const size_t cell = (size_t)t->dnx * t->dnz;
+  if (part == 3) {
+    if (q < 0 || q > 4) return 1;
+    const int spg[5] = {0, 1, 0, 0, 2};
+    memcpy(out, t->sp[q], sizeof(double) * s->Ncol * ny_grid(s, spg[q]));
+    return 0;
+  }
+  if (q < 0 || q >= ST_N || part < 0 || part > 2) return 1;
+  const int g = st_grid[q];
+  if (part == 0) memcpy(out, t->prof[q], sizeof(double) * ny_grid(s, g));
+  else memcpy(out, part == 1 ? t->cb[q] : t->ct[q], sizeof(double) * cell * t->nyc[g]);
+  return 0;
+}
+
+void or_stats_reset(or_state* s) {
+  if (!s->st) return;
+  int N[20];
+  memcpy(N, s->st->N, sizeof N);
+  or_stats_init(s, N);
+}
+
+/* plane sums over the Ngalx x Ngalz points, i fastest (stats.f90:16-88) */
+static void sum1(double* m, double* m2, const double* x, int igal, int nx, int nz) {
+  for (int k = 0; k < nz; ++k)
+    for (int i = 0; i < nx; ++i) *m = *m + x[k * igal + i];
+  if (m2)
+    for (int k = 0; k < nz; ++k)
+      for (int i = 0; i < nx; ++i) *m2 = *m2 + x[k * igal + i] * x[k * igal + i];
+}
+static void sum2(double* m, const double* x1, const double* x2, int igal, int nx, int nz) {
+  for (int k = 0; k < nz; ++k)
+    for (int i = 0; i < nx; ++i) *m = *m + x1[k * igal + i] * x2[k * igal + i];
+}
+
+/* conddown1/condup1 (stats.f90:110-140, 170-205): tile sums at distance j1
+   from the wall; sign applies to the first moment */
+static void cond1(const struct or_stats* t, double* ym, double* y2m, const double* x, int igal, int j1, double sgn,
+                  int ntx, int ntz) {
+  const size_t o = (size_t)j1 * t->dnx * t->dnz;
+  for (int k = 0; k < t->dnz; ++k)
+    for (int i = 0; i < t->dnx; ++i)
+      for (int iz = 0; iz < ntz; ++iz)
+        for (int ix = 0; ix < ntx; ++ix) {
+          const double v = x[(k + t->dnz * iz) * igal + i + t->dnx * ix];
+          const size_t c = o + (size_t)k * t->dnx + i;
+          ym[c] = ym[c] + (sgn == 1.0 ? v : v * sgn);
+          y2m[c] = y2m[c] + v * v;
+        }
+}
+/* conddown2/condup2 (stats.f90:142-168, 207-253) */
+static void cond2(const struct or_stats* t, double* ym, const double* x1, const double* x2, int igal, int j1,
+                  double s1, double s2, int top, int ntx, int ntz) {
+  const size_t o = (size_t)j1 * t->dnx * t->dnz;
+  for (int k = 0; k < t->dnz; ++k)
+    for (int i = 0; i < t->dnx; ++i)
+      for (int iz = 0; iz < ntz; ++iz)
+        for (int ix = 0; ix < ntx; ++ix) {
+          const size_t e = (size_t)(k + t->dnz * iz) * igal + i + t->dnx * ix;
+          const size_t c = o + (size_t)k * t->dnx + i;
+          ym[c] = ym[c] + (top ? x1[e] * s1 * x2[e] * s2 : x1[e] * x2[e]);
+        }
+}
+
+/* band of plane j of grid g: 0 none, 1 bottom (j1 in *j1), 2 top */
+static int band_of(const struct or_stats* t, int g, int j, int* j1) {
+  if (j >= t->bot[g][0] && j <= t->bot[g][1]) { *j1 = j - t->jref_bot[g]; return 1; }
+  if (j >= t->top[g][0] && j <= t->top[g][1]) { *j1 = t->jref_top[g] - j; return 2; }
+  return 0;
+}
+
+/* one sample: spectra (spectra.f90:1-59) then stats (stats.f90:1-108) of
+   the current fields and pressure */
+int or_stats(or_state* s) {
+  struct or_stats* t = s->st;
+  if (!t) return 1;
+#ifdef _OPENMP
+  if (g_threads > 0) omp_set_num_threads(g_threads);
+#endif
+  interp_v(s, s->u2i, U2); /* u2_itp (FOU3D.f90:80) */
+  interp_u(s, s->u3i, U3); /* u3_itp (FOU3D.f90:81) */
+  /* buildsp / buildspUV (spectra.f90:17-59) */
+#pragma omp parallel for schedule(static)
+  for (int col = 0; col < s->Ncol; ++col) {
+    for (int j = s->jlo_u; j <= s->jhi_u; ++j) {
+      const size_t e = fu(s, col, j);
+      t->sp[0][e] = t->sp[0][e] + creal(U1[e]) * creal(U1[e]) + cimag(U1[e]) * cimag(U1[e]);
+      t->sp[2][e] = t->sp[2][e] + creal(U3[e]) * creal(U3[e]) + cimag(U3[e]) * cimag(U3[e]);
+      t->sp[3][e] = t->sp[3][e] + creal(U1[e]) * creal(s->u2i[e]) + cimag(U1[e]) * cimag(s->u2i[e]);
+    }
+    for (int j = s->jlo_v; j <= s->jhi_v; ++j) {
+      const size_t e = fv(s, col, j);
+      t->sp[1][e] = t->sp[1][e] + creal(U2[e]) * creal(U2[e]) + cimag(U2[e]) * cimag(U2[e]);
+    }
+    for (int j = s->jlo_p; j <= s->jhi_p; ++j) {
+      const size_t e = fp(s, col, j);
+      t->sp[4][e] = t->sp[4][e] + creal(PP[e]) * creal(PP[e]) + cimag(PP[e]) * cimag(PP[e]);
+    }
+  }
+  const int igal = s->ngx + 2, ngz = s->ngz, ngx = s->ngx;
+  const size_t psz = (size_t)igal * ngz;
+  const int ntx = s->P.ntilex, ntz = s->P.ntilez;
+  /* u planes (ops_in_planes four_to_phys_u, FOU3D.f90:1556-1558) */
+#pragma omp parallel
+  {
+    double* B = (double*)malloc(sizeof(double) * psz * 3);
+    scratch w = {malloc(sizeof(cplx) * (ngx + ngz + 8)), malloc(sizeof(cplx) * (ngx + ngz + 8))};
+#pragma omp for schedule(dynamic, 1)
+    for (int jj = 0; jj < s->nyu; ++jj) {
+      const int j = s->jlo_u + jj;
+      double *Pu = B, *Pv = B + psz, *Pw = B + 2 * psz;
+      to_plane(s, U1, s->nyu, s->jlo_u, j, Pu);
+      to_plane(s, s->u2i, s->nyu, s->jlo_u, j, Pv);
+      to_plane(s, U3, s->nyu, s->jlo_u, j, Pw);
+      four_to_phys(s, Pu, 1, &w);
+      four_to_phys(s, Pv, 1, &w);
+      four_to_phys(s, Pw, 1, &w);
+      sum1(&t->prof[ST_U][jj], &t->prof[ST_U2][jj], Pu, igal, ngx, ngz);
+      sum1(&t->prof[ST_W][jj], &t->prof[ST_W2][jj], Pw, igal, ngx, ngz);
+      sum2(&t->prof[ST_UV][jj], Pu, Pv, igal, ngx, ngz);
+      sum2(&t->prof[ST_UW][jj], Pu, Pw, igal, ngx, ngz);
+      int j1;
+      const int b = band_of(t, 0, j, &j1);
+      if (b) {
+        double** A = b == 1 ? t->cb : t->ct;
+        const int top = b == 2;
+        cond1(t, A[ST_U], A[ST_U2], Pu, igal, j1, 1.0, ntx, ntz);
+        cond1(t, A[ST_W], A[ST_W2], Pw, igal, j1, 1.0, ntx, ntz);
+        cond2(t, A[ST_UV], Pu, Pv, igal, j1, 1.0, -1.0, top, ntx, ntz);
+        cond2(t, A[ST_UW], Pu, Pw, igal, j1, 1.0, 1.0, top, ntx, ntz);
+      }
+    }
+    free(B); free(w.w0); free(w.w1);
+  }
+  /* v planes: u2, u3_itp and omega_x (FOU3D.f90:187-208) */
+#pragma omp parallel
+  {
+    double* B = (double*)malloc(sizeof(double) * psz * 4);
+    scratch w = {malloc(sizeof(cplx) * (ngx + ngz + 8)), malloc(sizeof(cplx) * (ngx + ngz + 8))};
+    cplx* d = (cplx*)malloc(sizeof(cplx) * s->Ncol);
+#pragma omp for schedule(dynamic, 1)
+    for (int jj = 0; jj < s->nyv; ++jj) {
+      const int j = s->jlo_v + jj;
+      double *Pv = B, *Pw = B + psz, *Wx = B + 2 * psz, *D = B + 3 * psz;
+      to_plane(s, U2, s->nyv, s->jlo_v, j, Pv);
+      /* der_z of the u2 plane (FOU3D.f90:373-413): modes 0..N1/2 of the
+         slots below N2/2 and above Ngz-N2/2, zero elsewhere (the slot of
+         kz = -N2/2 included) */
+      memset(Wx, 0, sizeof(double) * psz);
+      for (int sl = 0; sl < ngz; ++sl) {
+        int k;
+        if (sl < s->N2 / 2) k = sl;
+        else if (sl >= ngz - s->N2 / 2 + 1) k = sl - ngz + s->N2;
+        else continue;
+        for (int i = 0; i < s->Mx; ++i) {
+          const cplx v = imul(s->k1z[k], pl(s, Pv, i, sl));
+          Wx[sl * igal + 2 * i] = creal(v);
+          Wx[sl * igal + 2 * i + 1] = cimag(v);
+        }
+      }
+      /* der_yv_h_wx (FOU3D.f90:495-514) */
+      for (int col = 0; col < s->Ncol; ++col)
+        d[col] = (U3[fu(s, col, j + 1)] - U3[fu(s, col, j)]) * DTHV(s, j) * s->ddthetavi;
+      /* modes_to_planes of the columns into a zeroed plane; only the u2
+         placement wrote other slots, and those were zero */
+      memset(D, 0, sizeof(double) * psz);
+      for (int col = 0; col < s->Ncol; ++col) {
+        const int i = col % s->Mx, k = col / s->Mx, sl = slot_of(s, k);
+        D[sl * igal + 2 * i] = creal(d[col]);
+        D[sl * igal + 2 * i + 1] = cimag(d[col]);
+      }
+      for (size_t e = 0; e < psz; ++e) Wx[e] = -Wx[e] + D[e];
+      four_to_phys(s, Wx, 0, &w);
+      to_plane(s, s->u3i, s->nyv, s->jlo_v, j, Pw);
+      four_to_phys(s, Pv, 1, &w);
+      four_to_phys(s, Pw, 1, &w);
+      sum1(&t->prof[ST_V][jj], &t->prof[ST_V2][jj], Pv, igal, ngx, ngz);
+      /* wx sums run over i = 1..N1, k = 1..N2 of the physical plane (stats.f90:56-62) */
+      sum1(&t->prof[ST_WX][jj], &t->prof[ST_WX2][jj], Wx, igal, s->N1, s->N2);
+      sum2(&t->prof[ST_VW][jj], Pv, Pw, igal, ngx, ngz);
+      int j1;
+      const int b = band_of(t, 1, j, &j1);
+      if (b) {
+        double** A = b == 1 ? t->cb : t->ct;
+        const int top = b == 2;
+        cond1(t, A[ST_V], A[ST_V2], Pv, igal, j1, top ? -1.0 : 1.0, ntx, ntz);
+        cond1(t, A[ST_WX], A[ST_WX2], Wx, igal, j1, 1.0, ntx, ntz);
+        cond2(t, A[ST_VW], Pv, Pw, igal, j1, -1.0, 1.0, top, ntx, ntz);
+      }
+    }
+    free(B); free(d); free(w.w0); free(w.w1);
+  }
+  /* p planes (FOU3D.f90:210-215, four_to_phys_du) */
+#pragma omp parallel
+  {
+    double* Pp = (double*)malloc(sizeof(double) * psz);
+    scratch w = {malloc(sizeof(cplx) * (ngx + ngz + 8)), malloc(sizeof(cplx) * (ngx + ngz + 8))};
+#pragma omp for schedule(dynamic, 1)
+    for (int jj = 0; jj < s->nyp; ++jj) {
+      const int j = s->jlo_p + jj;
+      to_plane(s, PP, s->nyp, s->jlo_p, j, Pp);
+      four_to_phys(s, Pp, 0, &w);
+      sum1(&t->prof[ST_P][jj], &t->prof[ST_P2][jj], Pp, igal, ngx, ngz);
+      int j1;
+      const int b = band_of(t, 2, j, &j1);
+      if (b) cond1(t, b == 1 ? t->cb[ST_P] : t->ct[ST_P], b == 1 ? t->cb[ST_P2] : t->ct[ST_P2], Pp, igal, j1, 1.0,
+                   ntx, ntz);
+    }
+    free(Pp); free(w.w0); free(w.w1);
+  }
+  t->istat += 1;
+  return 0;
 }

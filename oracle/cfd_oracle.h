@@ -80,6 +80,20 @@ void or_substep(or_state* s, int kRK);
 void or_step(or_state* s, int nsteps);
 void or_set_threads(int n);
 
+/* statistics and spectra (stats.f90, spectra.f90): or_stats_init with the
+   band table N(1:4,0:4) (20 ints) allocates zeroed accumulators (returns 1
+   for ntilex/ntilez <= 0, 2 for band ranges that do not fit); or_stats adds
+   one sample of the current fields; or_stats_dims gives dnx, dnz, the wall
+   band plane counts of the u, v, p grids and istat; or_stats_get copies one
+   accumulator (part 0 profile, 1 bottom / 2 top conditional sums
+   (dnx, dnz, j1), 3 spectrum sp(j, column) of U V W UV P); or_stats_reset
+   zeroes everything. */
+int  or_stats_init(or_state* s, const int* N20);
+int  or_stats(or_state* s);
+void or_stats_dims(const or_state* s, int* d);
+int  or_stats_get(const or_state* s, int q, int part, double* out);
+void or_stats_reset(or_state* s);
+
 /* FFT known-answer hooks with the reference calling conventions
    (cft_buff.f:1-65, rft_buff.f:24-86) */
 void or_cft(int n, double* c, int ise, int isa, int m, int iopt);

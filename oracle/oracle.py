@@ -47,7 +47,8 @@ def load():
         for name in ("or_destroy", "or_dims", "or_get_ygrid", "or_get_metrics", "or_get_iblist", "or_get_dg",
                      "or_set_field", "or_get_field", "or_set_scalars", "or_get_scalars", "or_rhs0",
                      "or_nonlinear", "or_solve_u", "or_solve_v", "or_meanflow_ctU", "or_solve_p", "or_vcorr",
-                     "or_divergence", "or_substep", "or_step", "or_flowrate"):
+                     "or_divergence", "or_substep", "or_step", "or_flowrate", "or_stats_init", "or_stats",
+                     "or_stats_dims", "or_stats_get", "or_stats_reset"):
             getattr(lib, name).argtypes = None
         lib.or_flowrate.restype = ctypes.c_double
         _lib = lib
@@ -180,6 +181,50 @@ class Oracle:
 
     def step(self, n=1):
         self.lib.or_step(self.h, ctypes.c_int(int(n)))
+
+    # -- statistics (stats.f90, spectra.f90) -----------------------------------
+    STATS = ("Um", "U2m", "Vm", "V2m", "Wm", "W2m", "Pm", "P2m", "UVm", "UWm", "VWm", "wxm", "wx2m")
+    SPECTRA = ("spU", "spV", "spW", "spUV", "spP")
+    _STAT_GRID = "uuvvuuppuuvvv"
+
+    def stats_init(self, N20):
+        rc = self.lib.or_stats_init(self.h, (ctypes.c_int * 20)(*[int(v) for v in N20]))
+        if rc:
+            raise ValueError("oracle stats: band table or tiles do not fit (%d)" % rc)
+
+    def stats(self):
+        if self.lib.or_stats(self.h):
+            raise RuntimeError("oracle stats: not initialised")
+
+    def stats_dims(self):
+        d = (ctypes.c_int * 6)()
+        self.lib.or_stats_dims(self.h, d)
+        d = list(d)
+        return dict(dnx=d[0], dnz=d[1], nyc={"u": d[2], "v": d[3], "p": d[4]}, istat=d[5])
+
+    def stats_get(self):
+        """Accumulators: profiles {name: (ny,)}, conditional sums {name:
+        (bottom, top)} as (nyc, dnz, dnx) arrays, spectra {name: (Ncol, ny)}."""
+        d = self.stats_dims()
+        out = {"istat": d["istat"], "prof": {}, "cond": {}, "spec": {}}
+        for q, name in enumerate(self.STATS):
+            g = self._STAT_GRID[q]
+            a = np.zeros(self._ny[g])
+            self.lib.or_stats_get(self.h, ctypes.c_int(q), ctypes.c_int(0), a.ctypes.data_as(_D))
+            out["prof"][name] = a
+            cb = np.zeros((d["nyc"][g], d["dnz"], d["dnx"]))
+            ct = np.zeros_like(cb)
+            self.lib.or_stats_get(self.h, ctypes.c_int(q), ctypes.c_int(1), cb.ctypes.data_as(_D))
+            self.lib.or_stats_get(self.h, ctypes.c_int(q), ctypes.c_int(2), ct.ctypes.data_as(_D))
+            out["cond"][name + "C"] = (cb, ct)
+        for q, (name, g) in enumerate(zip(self.SPECTRA, "uvuup")):
+            a = np.zeros((self.Ncol, self._ny[g]))
+            self.lib.or_stats_get(self.h, ctypes.c_int(q), ctypes.c_int(3), a.ctypes.data_as(_D))
+            out["spec"][name] = a
+        return out
+
+    def stats_reset(self):
+        self.lib.or_stats_reset(self.h)
 
 
 # ---------------------------------------------------------------------------

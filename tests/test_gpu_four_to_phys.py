@@ -75,8 +75,8 @@ def test_four_to_phys_matches_numpy(case):
 
 def test_four_to_phys_tool_end_to_end(tmp_path):
     """record_out -> `python -m cfd_amd.four_to_phys input_FtP.txt`."""
-    # tanh grid with dsty = 3 and a band table in order (h = 0.25)
-    p = dataclasses.replace(SMALLPOSTS, geometry_type=0, posth=0.0)
+    # tanh grid with dsty = 3 and a band table in order (smooth walls: h = 0.25)
+    p = dataclasses.replace(SMALLPOSTS, geometry_type=0)
     print("case four_to_phys tool smallposts-smooth", flush=True)
     ch = _channel(p)
     try:

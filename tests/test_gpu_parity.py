@@ -52,6 +52,15 @@ def _cmp(ch, orc, names, tol=TOL_OP, interior=None):
         assert e <= tol, "%s rel-L2 %.3e > %.1e (max rel %.3e)" % (n, e, tol, max_rel(a, b))
 
 
+def test_folded_y_grid_is_refused():
+    """posth = 0 with dsty = 3 folds the canopy extension back on itself
+    (yu(j) < yv(j) < yu(j+1) fails); the solver refuses the grid."""
+    import dataclasses
+    from cfd_amd._lib import CfdError
+    with pytest.raises(CfdError, match="folds"):
+        Channel(dataclasses.replace(SMALLPOSTS, geometry_type=0, posth=0.0))
+
+
 @pytest.mark.parametrize("case", list(CASES))
 def test_iblists_match_oracle(case):
     p = CASES[case]

@@ -1,0 +1,203 @@
+"""Pins of the oracle's statistics and spectra (stats.f90, spectra.f90).
+
+* Parseval: the plane sums of squares and products equal Ngx Ngz times the
+  kx-doubled sums of the spectra of the same sample; the plane means equal
+  Ngx Ngz Re u(0,0).  A wrong transform, a missed plane or a spectra term
+  with the wrong factor fails one of these.
+* conditional sums against numpy: the physical planes by numpy's inverse
+  FFTs, cut into ntilex x ntilez tiles and summed, at the wall distance j1
+  of each wall band (top band with the sign of v flipped).
+* omega_x against its closed form: u3 linear in y on one mode and u2 on
+  another, so omega_x = c (yu(j+1) - yu(j)) dthdyv(j) - i kz a mode by mode;
+  a kz = -N2/2 line in u2 (dropped by der_z) and in u3 (kept by the y
+  derivative); the wx sums over the reference's i <= N1, k <= N2 corner.
+* accumulation over samples and reset.
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from cfd_amd import fieldfile
+from oracle.oracle import Oracle
+
+from .helpers import SMALLPOSTS, random_field
+
+# tanh grid, dsty = 3, 6 x 4 tiles of 12 x 12 points, bands in order
+P = dataclasses.replace(SMALLPOSTS, geometry_type=0)
+
+
+def _slots(p):
+    k = np.arange(p.N2)
+    return np.where(k < p.N2 // 2, k, k - p.N2 + p.ngalz)
+
+
+def _phys(p, f):
+    """(Ncol, ny) spectral -> (ny, Ngz, Ngx) physical, modes at the
+    solver's z slots (kz = -N2/2 at slot Ngz - N2/2), unscaled inverse."""
+    planes = f.T.reshape(-1, p.N2, p.Mx)
+    C = np.zeros((planes.shape[0], p.ngalz, p.ngalx // 2 + 1), dtype=complex)
+    C[:, _slots(p), :p.Mx] = planes
+    return np.fft.irfft(np.fft.ifft(C, axis=1) * p.ngalz, n=p.ngalx, axis=2) * p.ngalx
+
+
+def _oracle(seed=0, fields=None):
+    o = Oracle(P)
+    rng = np.random.default_rng(seed)
+    f = fields or {n: random_field(P, n, rng) for n in ("u1", "u2", "u3", "p")}
+    for n, a in f.items():
+        o.set_field(n, a)
+    o.stats_init(fieldfile.band_table(P))
+    return o, f
+
+
+def _kxw(p):
+    return np.where(np.arange(p.Ncol) % p.Mx == 0, 1.0, 2.0)
+
+
+def _interp_faces(o, u):
+    """u on the u planes -> v faces, linear in y"""
+    g = o.ygrid()
+    yu, yv = g["yu"], g["yv"]
+    a = (yv - yu[:-1]) / (yu[1:] - yu[:-1])
+    return u[:, 1:] * a + u[:, :-1] * (1.0 - a)
+
+
+def test_parseval_and_means():
+    o, f = _oracle(1)
+    o.stats()
+    st = o.stats_get()
+    n, w = P.ngalx * P.ngalz, _kxw(P)
+    pr, sp = st["prof"], st["spec"]
+    for m, m2, s, fld in (("Um", "U2m", "spU", "u1"), ("Vm", "V2m", "spV", "u2"), ("Wm", "W2m", "spW", "u3"),
+                          ("Pm", "P2m", "spP", "p")):
+        assert np.allclose(sp[s], np.abs(f[fld]) ** 2, rtol=1e-15, atol=0)
+        assert np.abs(pr[m2] - n * (w @ sp[s])).max() <= 1e-12 * pr[m2].max(), m2
+        scale = np.sqrt(n * pr[m2])
+        assert np.abs(pr[m] - n * f[fld][0].real).max() <= 1e-12 * scale.max(), m
+    # products: u1 u2_itp (spectrum spUV), u1 u3, u2 u3_itp
+    assert np.abs(pr["UVm"] - n * (w @ sp["spUV"])).max() <= 1e-12 * np.sqrt(pr["U2m"] * pr["U2m"]).max()
+    uw = n * (w @ (f["u1"] * np.conj(f["u3"])).real)
+    assert np.abs(pr["UWm"] - uw).max() <= 1e-12 * np.sqrt(pr["U2m"] * pr["W2m"]).max()
+    u3i = _interp_faces(o, f["u3"])
+    vw = n * (w @ (f["u2"] * np.conj(u3i)).real)
+    assert np.abs(pr["VWm"] - vw).max() <= 1e-12 * np.sqrt(pr["V2m"] * pr["W2m"][:-1]).max()
+    assert st["istat"] == 1
+
+
+def test_conditional_sums_against_numpy_tiles():
+    o, f = _oracle(2)
+    o.stats()
+    st = o.stats_get()
+    N = fieldfile.band_table(P)
+    nb = lambda r, b: N[4 * b + r - 1]  # noqa: E731
+    ntx, ntz = P.ntilex, P.ntilez
+    dnx, dnz = P.ngalx // ntx, P.ngalz // ntz
+
+    def tiles(plane):
+        return plane.reshape(ntz, dnz, ntx, dnx).sum(axis=(0, 2))
+
+    u = _phys(P, f["u1"])
+    v = _phys(P, f["u2"])
+    pp = _phys(P, f["p"])
+    # u planes: bottom j = N(4,0)..N(4,1), top j = N(4,2)+1..N(4,3)+1
+    jlo = -P.dsty
+    cbU, ctU = st["cond"]["UmC"]
+    cbU2, _ = st["cond"]["U2mC"]
+    for j in range(nb(4, 0), nb(4, 1) + 1):
+        j1 = j - nb(4, 0)
+        assert np.allclose(cbU[j1], tiles(u[j - jlo]), rtol=0, atol=1e-12 * np.abs(u).max() * ntx * ntz)
+        assert np.allclose(cbU2[j1], tiles(u[j - jlo] ** 2), rtol=1e-12, atol=0)
+        assert abs(cbU[j1].sum() - st["prof"]["Um"][j - jlo]) <= 1e-11 * np.abs(u[j - jlo]).sum()
+    for j in range(nb(4, 2) + 1, nb(4, 3) + 2):
+        j1 = nb(4, 3) + 1 - j
+        assert np.allclose(ctU[j1], tiles(u[j - jlo]), rtol=0, atol=1e-12 * np.abs(u).max() * ntx * ntz)
+    # v planes: the top band with v's sign flipped
+    cbV, ctV = st["cond"]["VmC"]
+    for j in range(nb(3, 2) + 1, nb(3, 3) + 2):
+        j1 = nb(3, 3) + 1 - j
+        assert np.allclose(ctV[j1], -tiles(v[j - jlo]), rtol=0, atol=1e-12 * np.abs(v).max() * ntx * ntz)
+    for j in range(nb(3, 0), nb(3, 1) + 1):
+        assert np.allclose(cbV[j - nb(3, 0)], tiles(v[j - jlo]), rtol=0, atol=1e-12 * np.abs(v).max() * ntx * ntz)
+    # p planes: bottom N(4,0)+1..N(4,1)-1, top N(4,2)+2..N(4,3)
+    cbP, ctP = st["cond"]["PmC"]
+    jlop = -P.dsty + 1
+    for j in range(nb(4, 0) + 1, nb(4, 1)):
+        assert np.allclose(cbP[j - nb(4, 0) - 1], tiles(pp[j - jlop]), rtol=0, atol=1e-12 * np.abs(pp).max() * 24)
+    for j in range(nb(4, 2) + 2, nb(4, 3) + 1):
+        assert np.allclose(ctP[nb(4, 3) - j], tiles(pp[j - jlop]), rtol=0, atol=1e-12 * np.abs(pp).max() * 24)
+    # planes outside the wall bands add nothing: every j1 row is covered once per wall
+    assert cbU.shape[0] == nb(4, 1) - nb(4, 0) + 1
+
+
+def test_omega_x_closed_form():
+    o0 = Oracle(P)
+    g = o0.ygrid()
+    o0.close()
+    yu, dthdyv = g["yu"], g["dthdyv"]
+    nyu, nyv = len(yu), len(dthdyv)
+    Ncol = P.Ncol
+    col = lambda i, k: k * P.Mx + i  # noqa: E731
+    f = {n: np.zeros((Ncol, {"u1": nyu, "u2": nyv, "u3": nyu, "p": nyu - 2}[n]), dtype=complex)
+         for n in ("u1", "u2", "u3", "p")}
+    c, a, odd_u2, odd_u3 = 0.7 - 0.2j, 0.3 + 0.5j, 1.1 - 0.4j, -0.6 + 0.9j
+    i0, k0, i1, k1 = 2, 3, 1, P.N2 - 2  # k1: kz = -2
+    f["u3"][col(i0, k0)] = c * yu
+    f["u3"][col(3, P.N2 // 2)] = odd_u3 * yu  # kz = -N2/2, kept by the y derivative
+    f["u2"][col(i1, k1)] = a
+    f["u2"][col(2, P.N2 // 2)] = odd_u2  # dropped by der_z
+    o, _ = _oracle(fields=f)
+    o.stats()
+    st = o.stats_get()
+    # omega_x spectra per v plane
+    dy = (yu[1:] - yu[:-1]) * dthdyv
+    W = np.zeros((Ncol, nyv), dtype=complex)
+    W[col(i0, k0)] = c * dy
+    W[col(3, P.N2 // 2)] = odd_u3 * dy
+    kz1 = P.bet * (k1 - P.N2)
+    W[col(i1, k1)] = -1j * kz1 * a
+    wx = _phys(P, W)
+    corner = wx[:, :P.N2, :P.N1]
+    assert np.allclose(st["prof"]["wxm"], corner.sum(axis=(1, 2)), rtol=0, atol=1e-12 * np.abs(wx).max() * wx[0].size)
+    assert np.allclose(st["prof"]["wx2m"], (corner ** 2).sum(axis=(1, 2)), rtol=1e-12, atol=0)
+    # the conditional omega_x sums cover the whole plane
+    cb, ct = st["cond"]["wxmC"]
+    assert np.allclose(cb.sum(axis=(1, 2)), wx[:cb.shape[0]].sum(axis=(1, 2)), rtol=0,
+                       atol=1e-11 * np.abs(wx).max() * wx[0].size)
+
+
+def test_samples_accumulate_and_reset():
+    o, f = _oracle(3)
+    o.stats()
+    a = o.stats_get()
+    rng = np.random.default_rng(4)
+    g = {n: random_field(P, n, rng) for n in ("u1", "u2", "u3", "p")}
+    o2, _ = _oracle(fields=g)
+    o2.stats()
+    b = o2.stats_get()
+    for n, v in g.items():
+        o.set_field(n, v)
+    o.stats()
+    ab = o.stats_get()
+    assert ab["istat"] == 2
+    # the second sample's terms are added to the first sample's running sums
+    # one by one, so the result differs from a + b by the rounding of a sum
+    for k in a["prof"]:
+        scale = np.abs(a["prof"][k]).max() + np.abs(b["prof"][k]).max()
+        assert np.abs(ab["prof"][k] - (a["prof"][k] + b["prof"][k])).max() <= 1e-12 * scale, k
+    # spectra: sp = (sp + Re^2) + Im^2, term by term (spectra.f90:29-33)
+    for k, fld in (("spU", "u1"), ("spV", "u2"), ("spW", "u3"), ("spP", "p")):
+        assert (ab["spec"][k] == (a["spec"][k] + g[fld].real ** 2) + g[fld].imag ** 2).all(), k
+    assert np.allclose(ab["spec"]["spUV"], a["spec"]["spUV"] + b["spec"]["spUV"], rtol=1e-14,
+                       atol=1e-15 * np.abs(b["spec"]["spUV"]).max())
+    o.stats_reset()
+    z = o.stats_get()
+    assert z["istat"] == 0 and all((v == 0).all() for v in z["prof"].values())
+
+
+def test_unordered_band_table_is_refused():
+    o = Oracle(SMALLPOSTS)  # tall posts on a short grid: N(4,1) > N(4,2)
+    N = fieldfile.band_table(SMALLPOSTS)
+    assert N[7] > N[11]
+    with pytest.raises(ValueError):
+        o.stats_init(N)
