@@ -710,7 +710,7 @@ int launched(cfd_solver* s) {
 // walls (limPL_incw, start.f90:1759-1866); the conditional sums of the two
 // walls are kept apart and added when written, as write_stats adds them.
 // ---------------------------------------------------------------------------
-constexpr int kStatChunk = 16;  // planes per chunk (even: x-inverse jobs pair planes)
+constexpr int kStatChunk = 64;  // planes per chunk (even: x-inverse jobs pair planes)
 enum { SQ_U, SQ_U2, SQ_V, SQ_V2, SQ_W, SQ_W2, SQ_P, SQ_P2, SQ_UV, SQ_UW, SQ_VW, SQ_WX, SQ_WX2, SQ_N };
 const int kStatGrid[SQ_N] = {0, 0, 1, 1, 0, 0, 2, 2, 0, 0, 1, 1, 1};
 const int kSpecGrid[5] = {0, 1, 0, 0, 2};  // spU spV spW spUV spP
@@ -984,13 +984,47 @@ int read_scalars(cfd_solver* s, DevScalars& d) {
   return CFD_OK;
 }
 
-void op_nonlinear(cfd_solver* s, int kRK) {
+// A statistics request (cfd_stats_request) is served by the next nonlinear
+// term with kRK = 1, inside cfd_nonlinear or cfd_substep, where the
+// reference's nonlinear serves flagst / flagwr (FOU3D.f90:56-69, 107-110,
+// 177-226): the sample is of the fields at the start of the substep, and the
+// files are written after the time-step update, with the new t and the mpgx
+// of the start of the substep in the header and the name taken from the t of
+// the start.
+struct StatsReq {
+  int flags = 0;
+  DevScalars d0{};
+};
+
+int stats_req_begin(cfd_solver* s, int kRK, StatsReq& r) {
+  r.flags = kRK == 1 ? s->st.pending : 0;
+  if (!r.flags) return CFD_OK;
+  s->st.pending = 0;
+  if (r.flags & 2)
+    if (int rc = read_scalars(s, r.d0)) return rc;
+  if (r.flags & 1) return stats_sample(s);
+  return CFD_OK;
+}
+
+// after the time-step update; a failed write leaves the step to complete
+int stats_req_end(cfd_solver* s, const StatsReq& r) {
+  if (!(r.flags & 2)) return CFD_OK;
+  DevScalars d1{};
+  if (int rc = read_scalars(s, d1)) return rc;
+  return stats_write(s, s->st.pending_dir.c_str(), s->st.pending_iter, d1.t, r.d0.mpgx, r.d0.t);
+}
+
+int op_nonlinear(cfd_solver* s, int kRK) {
+  StatsReq req;
+  if (int rc = stats_req_begin(s, kRK, req)) return rc;
   if (kRK == 1) launch_reset_max(*s);
   { Timed t(s, KC_ZINV); launch_zinv(*s, s->d_nljobs, s->n_nljobs, true); }
   { Timed t(s, KC_XPASS); launch_xpass_nl(*s, kRK == 1); }
   { Timed t(s, KC_ZFWD); launch_zfwd_nl(*s); }
   if (kRK == 1) launch_cfl(*s);
+  const int wrc = stats_req_end(s, req);
   { Timed t(s, KC_ELEM); launch_nlcombine(*s, kRK); }
+  return wrc;
 }
 
 // IB correction of the predictor of fields [f0, f1) (0 u1, 1 u2, 2 u3)
@@ -1018,35 +1052,17 @@ void op_solve(cfd_solver* s, int f, int kRK) {
 // Helmholtz system, the (0,0) flow-rate correction, and one sweep per column
 // for solveP + v_corr.  Same arithmetic as the operator-by-operator path;
 // du holds r only on the IB planes afterwards.
-//
-// A statistics request (cfd_stats_request) is served by the next substep
-// with kRK = 1 where the reference's nonlinear serves flagst / flagwr
-// (FOU3D.f90:56-69, 107-110, 177-226): the sample is of the fields at the
-// start of the substep, and the files are written after the time step
-// update, with the new t and the mpgx of the start of the substep in the
-// header and the name taken from the t of the start.
+// A pending statistics request is served as in op_nonlinear.
 int op_substep(cfd_solver* s, int kRK) {
-  const int req = kRK == 1 ? s->st.pending : 0;
-  DevScalars d0{};
-  if (req) {
-    s->st.pending = 0;
-    if (req & 2)
-      if (int rc = read_scalars(s, d0)) return rc;
-    if (req & 1)
-      if (int rc = stats_sample(s)) return rc;
-  }
+  StatsReq req;
+  if (int rc = stats_req_begin(s, kRK, req)) return rc;
   if (kRK == 1) launch_reset_max(*s);
   { Timed t(s, KC_ZINV); launch_zinv(*s, s->d_nljobs, s->n_nljobs, true); }
   { Timed t(s, KC_XPASS); launch_xpass_nl(*s, kRK == 1); }
   { Timed t(s, KC_ZFWD); launch_zfwd_nl(*s); }
   if (kRK == 1) launch_cfl(*s);
   // a failed write (say, a missing directory) still completes the substep
-  int wrc = CFD_OK;
-  if (req & 2) {
-    DevScalars d1{};
-    wrc = read_scalars(s, d1);
-    if (wrc == CFD_OK) wrc = stats_write(s, s->st.pending_dir.c_str(), s->st.pending_iter, d1.t, d0.mpgx, d0.t);
-  }
+  const int wrc = stats_req_end(s, req);
   {
     Timed t(s, KC_IB);
     launch_ib_rhs(*s, kRK);
@@ -1072,7 +1088,7 @@ extern "C" int cfd_rhs0(cfd_handle h, int kRK) {
 extern "C" int cfd_nonlinear(cfd_handle h, int kRK) {
   cfd_solver* s = static_cast<cfd_solver*>(h);
   if (int rc = check(s, kRK)) return rc;
-  op_nonlinear(s, kRK);
+  if (int rc = op_nonlinear(s, kRK)) return rc;
   return launched(s);
 }
 extern "C" int cfd_solve_u(cfd_handle h, int comp, int kRK) {

@@ -128,6 +128,33 @@ module cfd_amd_mod
       integer(c_int), value :: flag_init
       integer(c_int) :: iter0
     end function
+    ! statistics and spectra (stats.f90, spectra.f90): one sample of the
+    ! current state; write the stats_ / sp*_ files and zero the sums; serve
+    ! flags (bit 0 sample, bit 1 write) in the next nonlinear term with
+    ! kRK = 1 (FOU3D.f90:56-69, 107-110, 177-226); d(6) = dnx, dnz, wall-band
+    ! planes of the u, v, p grids, istat
+    integer(c_int) function cfd_stats(h) bind(C, name="cfd_stats")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function cfd_write_stats(h, dirout, iter) bind(C, name="cfd_write_stats")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: h
+      character(kind=c_char) :: dirout(*)
+      integer(c_int), value :: iter
+    end function
+    integer(c_int) function cfd_stats_request(h, flags, dirout, iter) bind(C, name="cfd_stats_request")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: h
+      integer(c_int), value :: flags
+      character(kind=c_char) :: dirout(*)
+      integer(c_int), value :: iter
+    end function
+    integer(c_int) function cfd_stats_dims(h, d) bind(C, name="cfd_stats_dims")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int) :: d(6)
+    end function
   end interface
 
 contains

@@ -8,15 +8,21 @@
 ! the parabolic profile of flag_init = 3 (start.f90:2109-2249), calls the hot
 ! path through the C ABI in the reference order, writes the field files every
 ! nwrite steps at the start of a step (record_out, FOU3D.f90:58-70, 130-142)
-! and once more at the end (finalize, littleharsh.f90:353-390).
+! and once more at the end (finalize, littleharsh.f90:353-390), samples the
+! statistics and spectra every nstat = min(20, nwrite) steps (start.f90:372;
+! FOU3D.f90:56-61, 107-110, 185-221) and writes them with each record
+! (FOU3D.f90:223-226).  The reference counts the sampling interval from
+! iter0, which it also resets when it samples; a counter of its own, iterst,
+! does that here so iter0 keeps the start of the run for maxiter.
 program littleharsh_gpu
   use iso_c_binding
   use cfd_amd_mod
   implicit none
   type(cfd_params) :: p
   type(c_ptr) :: h
-  integer(c_int) :: rc, d(16), kRK, iter0
+  integer(c_int) :: rc, d(16), kRK, iter0, dst(6), sflags
   integer :: i, Ngalx2, Ngalz2, flag_init, nwrite, iter, iwrite, maxiter, npeakx, npeakz, dstx, dstz
+  integer :: nstat, iterst
   integer :: physlim_bot, physlim_top, nyu, nyv, nyp, ncol, jlo, nn, j
   real(8) :: Kib, maxerr, maxt, maxA, hny, Lfracx, Lfracz, sc(8), Qx
   character(len=90) :: fnameimb, dirin, dirout
@@ -186,12 +192,30 @@ program littleharsh_gpu
   write(*,'(a,3i6,a,i8,a,es12.5)') ' grid', p%ngalx, p%ny, p%ngalz, '  columns', ncol, '  QxT', Qx
   iter = iter0
   iwrite = iter
+  iterst = iter0
+  nstat = min(20, nwrite)
+  ! the conditional statistics need wall bands (band limits in order)
+  rc = cfd_stats_dims(h, dst)
+  if (rc /= 0) then
+    write(*,*) 'statistics off: the band table has no wall bands for this grid'
+    nstat = huge(1)
+  end if
   rc = cfd_get_scalars(h, sc);         call cfd_check(rc, h, 'get scalars')
   do while (sc(CFD_T) < maxt .and. iter - iter0 < maxiter)
+    sflags = 0
+    if (iter - iterst >= nstat) then
+      sflags = 1
+      iterst = iter
+    end if
     if (iter >= iwrite) then
       rc = cfd_record_out(h, trim(dirout)//c_null_char, iter)
       call cfd_check(rc, h, 'record_out')
       iwrite = iwrite + nwrite
+      if (nstat /= huge(1)) sflags = sflags + 2
+    end if
+    if (sflags /= 0) then
+      rc = cfd_stats_request(h, sflags, trim(dirout)//c_null_char, iter)
+      call cfd_check(rc, h, 'stats request')
     end if
     do kRK = 1, 3
       rc = cfd_rhs0(h, kRK);           call cfd_check(rc, h, 'RHS0')

@@ -195,8 +195,8 @@ int cfd_device_bytes(cfd_handle h, double* bytes);
  * mpgx in the headers, then zeroes every accumulator and istat.
  *
  * cfd_stats_request(h, flags, dirout, iter): bit 0 sample, bit 1 write; the
- * next cfd_substep / cfd_step substep with kRK = 1 serves it where the
- * reference's nonlinear does: the sample of the fields at the start of the
+ * next nonlinear term with kRK = 1 (in cfd_nonlinear, cfd_substep or
+ * cfd_step) serves it where the reference's nonlinear does: the sample of the fields at the start of the
  * substep, the files after the time-step update, with the new t and the
  * start-of-substep mpgx in the headers and the start-of-substep t in the
  * names.
