@@ -28,6 +28,7 @@ class CFileHeader(ctypes.Structure):
 
 # symbol -> (restype, argtypes)
 _D = ctypes.POINTER(ctypes.c_double)
+_F = ctypes.POINTER(ctypes.c_float)
 _I = ctypes.POINTER(ctypes.c_int)
 _H = ctypes.c_void_p
 SIGNATURES = {
@@ -63,6 +64,24 @@ SIGNATURES = {
     "cfd_record_out": (ctypes.c_int, [_H, ctypes.c_char_p, ctypes.c_int]),
     "cfd_band_table": (ctypes.c_int, [ctypes.POINTER(CParams), _I]),
     "cfd_four_to_phys": (ctypes.c_int, [_H, ctypes.c_int, _D]),
+    "cfd_cnn_nparams": (ctypes.c_int, []),
+    "cfd_cnn_create": (ctypes.c_int, [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_void_p)]),
+    "cfd_cnn_destroy": (ctypes.c_int, [_H]),
+    "cfd_cnn_last_error": (ctypes.c_char_p, [_H]),
+    "cfd_cnn_set_params": (ctypes.c_int, [_H, _F]),
+    "cfd_cnn_get_params": (ctypes.c_int, [_H, _F]),
+    "cfd_cnn_get_grads": (ctypes.c_int, [_H, _F]),
+    "cfd_cnn_set_input": (ctypes.c_int, [_H, _F]),
+    "cfd_cnn_set_target": (ctypes.c_int, [_H, _F]),
+    "cfd_cnn_forward": (ctypes.c_int, [_H]),
+    "cfd_cnn_get_output": (ctypes.c_int, [_H, _F]),
+    "cfd_cnn_backward": (ctypes.c_int, [_H, _D]),
+    "cfd_cnn_adam": (ctypes.c_int, [_H] + [ctypes.c_float] * 4),
+    "cfd_cnn_dataset": (ctypes.c_int, [_H, ctypes.c_int]),
+    "cfd_cnn_train_slot": (ctypes.c_int, [_H, ctypes.c_int] + [ctypes.c_float] * 4 + [_D]),
+    "cfd_cnn_eval_slot": (ctypes.c_int, [_H, ctypes.c_int, _D]),
+    "cfd_cnn_load_slot": (ctypes.c_int, [_H, ctypes.c_int]),
+    "cfd_cnn_device_bytes": (ctypes.c_int, [_H, _D]),
     "cfd_stats": (ctypes.c_int, [_H]),
     "cfd_write_stats": (ctypes.c_int, [_H, ctypes.c_char_p, ctypes.c_int]),
     "cfd_stats_request": (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]),

@@ -1,0 +1,137 @@
+"""The closure network (cfd_cnn_* of the C ABI): a 3D CNN over a channel
+volume, BF16 MFMA kernels on the device.  Host arrays are float32,
+point-major: input (ny, nz, nx, 4) = u, v, w, mask; target and output
+(ny, nz, nx, 3)."""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+_F = ctypes.POINTER(ctypes.c_float)
+CIN = (4, 32, 32, 32)
+COUT = (32, 32, 32, 3)
+
+
+def nparams():
+    return _lib.load().cfd_cnn_nparams()
+
+
+def split(flat):
+    """flat parameters -> [(W (27, cin, cout), b (cout,))] per layer"""
+    out, o = [], 0
+    for ci, co in zip(CIN, COUT):
+        W = flat[o:o + 27 * ci * co].reshape(27, ci, co)
+        o += 27 * ci * co
+        b = flat[o:o + co]
+        o += co
+        out.append((W, b))
+    assert o == len(flat)
+    return out
+
+
+def join(layers):
+    return np.concatenate([np.concatenate([W.ravel(), b.ravel()]) for W, b in layers]).astype(np.float32)
+
+
+def init_params(seed=0, last_scale=0.1):
+    """He-normal weights for the GELU layers, a small last layer, zero biases."""
+    rng = np.random.default_rng(seed)
+    layers = []
+    for l, (ci, co) in enumerate(zip(CIN, COUT)):
+        std = np.sqrt(2.0 / (27 * ci)) * (last_scale if l == len(CIN) - 1 else 1.0)
+        layers.append((rng.standard_normal((27, ci, co)) * std, np.zeros(co)))
+    return join(layers)
+
+
+class CNN:
+    def __init__(self, ny, nz, nx, device=0):
+        self._lib = _lib.load()
+        h = ctypes.c_void_p()
+        rc = self._lib.cfd_cnn_create(device, ny, nz, nx, ctypes.byref(h))
+        if rc:
+            raise _lib.CfdError("cfd_cnn_create: %s" % self._lib.cfd_cnn_last_error(None).decode())
+        self._h = h
+        self.shape = (ny, nz, nx)
+        self.npts = ny * nz * nx
+
+    def _call(self, name, *args):
+        rc = getattr(self._lib, name)(self._h, *args)
+        if rc:
+            raise _lib.CfdError("%s: %s" % (name, self._lib.cfd_cnn_last_error(self._h).decode()))
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            self._lib.cfd_cnn_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.size == nparams()
+        self._call("cfd_cnn_set_params", w.ctypes.data_as(_F))
+
+    def params(self):
+        w = np.zeros(nparams(), np.float32)
+        self._call("cfd_cnn_get_params", w.ctypes.data_as(_F))
+        return w
+
+    def grads(self):
+        g = np.zeros(nparams(), np.float32)
+        self._call("cfd_cnn_get_grads", g.ctypes.data_as(_F))
+        return g
+
+    def set_input(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        assert x.shape == self.shape + (4,)
+        self._call("cfd_cnn_set_input", x.ctypes.data_as(_F))
+
+    def set_target(self, t):
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        assert t.shape == self.shape + (3,)
+        self._call("cfd_cnn_set_target", t.ctypes.data_as(_F))
+
+    def forward(self):
+        self._call("cfd_cnn_forward")
+        y = np.zeros(self.shape + (3,), np.float32)
+        self._call("cfd_cnn_get_output", y.ctypes.data_as(_F))
+        return y
+
+    def backward(self):
+        loss = ctypes.c_double()
+        self._call("cfd_cnn_backward", ctypes.byref(loss))
+        return loss.value
+
+    def adam(self, lr, b1=0.9, b2=0.999, eps=1e-8):
+        self._call("cfd_cnn_adam", ctypes.c_float(lr), ctypes.c_float(b1), ctypes.c_float(b2), ctypes.c_float(eps))
+
+    def dataset(self, nslots):
+        self._call("cfd_cnn_dataset", int(nslots))
+
+    def train_slot(self, slot, lr, b1=0.9, b2=0.999, eps=1e-8):
+        loss = ctypes.c_double()
+        self._call("cfd_cnn_train_slot", int(slot), ctypes.c_float(lr), ctypes.c_float(b1), ctypes.c_float(b2),
+                   ctypes.c_float(eps), ctypes.byref(loss))
+        return loss.value
+
+    def eval_slot(self, slot):
+        loss = ctypes.c_double()
+        self._call("cfd_cnn_eval_slot", int(slot), ctypes.byref(loss))
+        return loss.value
+
+    def load_slot(self, slot):
+        self._call("cfd_cnn_load_slot", int(slot))
+
+    def device_bytes(self):
+        v = ctypes.c_double()
+        self._call("cfd_cnn_device_bytes", ctypes.byref(v))
+        return v.value

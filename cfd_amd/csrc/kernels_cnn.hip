@@ -1,0 +1,387 @@
+// kernels_cnn.hip -- the learned nonlinear-term closure: a 3D CNN over the
+// coarse channel volume, trained and run with BF16 MFMA kernels.
+//
+// Volume: v[y][z][x][c], channels last, periodic in x and z, zero outside
+// the y range (the planes of the u grid).  Network: four 3x3x3 convolutions,
+// 4 -> 32 -> 32 -> 32 -> 3 channels, GELU after the first three; BF16 weights
+// and activations, FP32 accumulation, FP32 master weights (Adam).
+//
+// Every convolution is an implicit GEMM on v_mfma_f32_32x32x16_bf16: a wave
+// computes 32 output points (one x-run of a z row) by 32 output channels,
+// K = 27 taps x input channels in steps of 16.  Lane layout of that MFMA on
+// gfx950 (checked on the device): A[m][k] in lane m % 32, elements
+// k = 8 (l / 32) + i; B[k][n] in lane n % 32, elements k = 8 (l / 32) + i;
+// C[m][n] in lane n % 32, element v at m = 8 (v / 4) + 4 (l / 32) + v % 4.
+// A block (4 waves, 4 z rows of one plane) stages its input halo
+// (3 planes x 6 rows x 34 points) and the layer's weights in LDS; the LDS
+// point stride is padded (48 B for 16 channels, 80 B for 32) so the 16-byte
+// reads of one MFMA operand hit distinct banks.
+#include <hip/hip_bf16.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "common.hpp"
+
+namespace cfd {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int kC = 32;      // hidden channels
+constexpr int kTX = 32;     // points per wave (x)
+constexpr int kTZ = 4;      // z rows per block (one per wave)
+constexpr int kHX = kTX + 2, kHZ = kTZ + 2, kHY = 3;
+constexpr int kHalo = kHX * kHZ * kHY;
+
+__device__ __forceinline__ int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+
+__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_d(float z) {
+  // d/dz of z Phi(z) = Phi(z) + z phi(z)
+  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+}
+
+// LDS channel stride (elements) of a point for CIN channels
+template <int CIN>
+struct Lds {
+  static constexpr int CP = CIN == 16 ? 24 : 40;
+};
+
+// stage the halo of block (x0, z0, y) into LDS: hal[(hy * kHZ + hz) * kHX + hx][c]
+// from a global volume of CG channels (CG <= CIN; the rest zero)
+template <int CIN, int CG>
+__device__ __forceinline__ void stage_halo(__bf16* hal, const __bf16* __restrict__ in, int x0, int z0, int y, int NY,
+                                           int NZ, int NX) {
+  constexpr int CP = Lds<CIN>::CP;
+  if constexpr (CG == CIN) {
+    // 8-channel (16 B) chunks
+    constexpr int NCH = CIN / 8;
+    for (int w = threadIdx.x; w < kHalo * NCH; w += blockDim.x) {
+      const int pnt = w / NCH, ch = w - pnt * NCH;
+      const int hx = pnt % kHX, hz = (pnt / kHX) % kHZ, hy = pnt / (kHX * kHZ);
+      const int yy = y + hy - 1;
+      bf16x8 v;
+      if (yy < 0 || yy >= NY) {
+        for (int i = 0; i < 8; ++i) v[i] = (__bf16)0.0f;
+      } else {
+        const int xx = wrap(x0 + hx - 1, NX), zz = wrap(z0 + hz - 1, NZ);
+        v = *(const bf16x8*)(in + (((size_t)yy * NZ + zz) * NX + xx) * CG + 8 * ch);
+      }
+      *(bf16x8*)(hal + (size_t)pnt * CP + 8 * ch) = v;
+    }
+  } else {
+    // CG (4) channels per point, padded with zeros to CIN
+    for (int pnt = threadIdx.x; pnt < kHalo; pnt += blockDim.x) {
+      const int hx = pnt % kHX, hz = (pnt / kHX) % kHZ, hy = pnt / (kHX * kHZ);
+      const int yy = y + hy - 1;
+      bf16x8 v0, v1;
+      for (int i = 0; i < 8; ++i) {
+        v0[i] = (__bf16)0.0f;
+        v1[i] = (__bf16)0.0f;
+      }
+      if (yy >= 0 && yy < NY) {
+        const int xx = wrap(x0 + hx - 1, NX), zz = wrap(z0 + hz - 1, NZ);
+        const __bf16* s = in + (((size_t)yy * NZ + zz) * NX + xx) * CG;
+        for (int c = 0; c < CG; ++c) v0[c] = s[c];
+      }
+      *(bf16x8*)(hal + (size_t)pnt * CP) = v0;
+      *(bf16x8*)(hal + (size_t)pnt * CP + 8) = v1;
+    }
+  }
+}
+
+}  // namespace
+
+// Convolution of one layer.  MODE 0: z = conv + b -> zout (BF16), a = gelu(z)
+// -> aout (BF16).  MODE 1: y = conv + b -> yout (FP32, channels < nout).
+// MODE 2 (backward data): taps flipped, weights W'[t][ci][co] given as the
+// operand layout, d = conv * gelu'(zprev) -> aout (BF16); no bias.
+// Weights w: [27][32 (n)][CIN (k)] BF16.  Grid (NX/32, NZ/4, NY), 256 threads.
+template <int CIN, int CG, int MODE>
+__global__ void __launch_bounds__(256) k_conv(const __bf16* __restrict__ in, const __bf16* __restrict__ w,
+                                              const float* __restrict__ bias, const __bf16* __restrict__ zprev,
+                                              __bf16* __restrict__ zout, __bf16* __restrict__ aout,
+                                              float* __restrict__ yout, int nout, int NY, int NZ, int NX) {
+  constexpr int CP = Lds<CIN>::CP;
+  extern __shared__ __align__(16) unsigned char smem[];
+  __bf16* hal = (__bf16*)smem;                 // kHalo * CP
+  __bf16* wl = hal + (size_t)kHalo * CP;       // 27 * 32 * CP
+  const int x0 = blockIdx.x * kTX, z0 = blockIdx.y * kTZ, y = blockIdx.z;
+  stage_halo<CIN, CG>(hal, in, x0, z0, y, NY, NZ, NX);
+  for (int e = threadIdx.x; e < 27 * kC * (CIN / 8); e += blockDim.x) {
+    const int ch = e % (CIN / 8), row = e / (CIN / 8);  // row = t * 32 + n
+    *(bf16x8*)(wl + (size_t)row * CP + 8 * ch) = *(const bf16x8*)(w + (size_t)row * CIN + 8 * ch);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int m = lane & 31, kh = lane >> 5;
+  f32x16 acc = {};
+  for (int t = 0; t < 27; ++t) {
+    int dy = t / 9 - 1, dz = (t / 3) % 3 - 1, dx = t % 3 - 1;
+    if (MODE == 2) {
+      dy = -dy;
+      dz = -dz;
+      dx = -dx;
+    }
+    const __bf16* arow = hal + (size_t)(((1 + dy) * kHZ + (1 + wv + dz)) * kHX + (1 + m + dx)) * CP;
+    const __bf16* brow = wl + (size_t)(t * kC + m) * CP;
+#pragma unroll
+    for (int s = 0; s < CIN / 16; ++s) {
+      const bf16x8 a = *(const bf16x8*)(arow + 16 * s + 8 * kh);
+      const bf16x8 b = *(const bf16x8*)(brow + 16 * s + 8 * kh);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
+  }
+  // epilogue: lane holds channel n = m of points 8 (v/4) + 4 kh + v % 4
+  const int n = m;
+  const int z = z0 + wv;
+  const size_t rowp = ((size_t)y * NZ + z) * NX + x0;
+#pragma unroll
+  for (int v = 0; v < 16; ++v) {
+    const int pm = 8 * (v / 4) + 4 * kh + (v & 3);
+    const size_t p = rowp + pm;
+    if (MODE == 0) {
+      const float zz = acc[v] + bias[n];
+      zout[p * kC + n] = (__bf16)zz;
+      aout[p * kC + n] = (__bf16)gelu(zz);
+    } else if (MODE == 1) {
+      if (n < nout) yout[p * nout + n] = acc[v] + bias[n];
+    } else {
+      aout[p * kC + n] = (__bf16)(acc[v] * gelu_d((float)zprev[p * kC + n]));
+    }
+  }
+}
+
+// weight gradients: dW[t][ci][co] = sum_p x[p + off(t)][ci] dz[p][co] and
+// db[co] = sum_p dz[p][co], per block into partial[blk][27 * 32 * 32 + 32]
+// (ci padded to 32).  Wave w accumulates the taps t = w, w + 4, ... over the
+// 4 z rows of the block tile and the blocks of its grid-stride loop.
+// Grid: nblk blocks of 256 threads looping over the (NX/32, NZ/4, NY) tiles.
+template <int CIN, int CG>
+__global__ void __launch_bounds__(256) k_conv_dw(const __bf16* __restrict__ in, const __bf16* __restrict__ dz,
+                                                 float* __restrict__ partial, int NY, int NZ, int NX) {
+  constexpr int CP = Lds<CIN>::CP;
+  constexpr int DP = 40;  // LDS stride of a dz point
+  extern __shared__ __align__(16) unsigned char smem[];
+  __bf16* hal = (__bf16*)smem;
+  __bf16* dzl = hal + (size_t)kHalo * CP;  // kTZ * kTX * DP
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int m = lane & 31, kh = lane >> 5;
+  constexpr int NT = 7;  // taps per wave (27 = 4 * 7 - 1)
+  f32x16 acc[NT];
+  for (int q = 0; q < NT; ++q) acc[q] = f32x16{};
+  float dbs = 0.0f;
+  const int tx = NX / kTX, tz = NZ / kTZ, ntiles = tx * tz * NY;
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int bx = tile % tx, bz = (tile / tx) % tz, y = tile / (tx * tz);
+    const int x0 = bx * kTX, z0 = bz * kTZ;
+    __syncthreads();
+    stage_halo<CIN, CG>(hal, in, x0, z0, y, NY, NZ, NX);
+    for (int e = threadIdx.x; e < kTZ * kTX * (kC / 8); e += blockDim.x) {
+      const int ch = e % (kC / 8), pnt = e / (kC / 8);
+      const int zr = pnt / kTX, xr = pnt % kTX;
+      *(bf16x8*)(dzl + (size_t)pnt * DP + 8 * ch) =
+          *(const bf16x8*)(dz + ((((size_t)y * NZ + z0 + zr) * NX + x0 + xr) * kC + 8 * ch));
+    }
+    __syncthreads();
+    for (int zr = 0; zr < kTZ; ++zr) {
+      // bias gradient: lanes 0..31 of wave 0 own a channel each
+      if (wv == 0 && kh == 0)
+        for (int xr = 0; xr < kTX; ++xr) dbs += (float)dzl[(size_t)(zr * kTX + xr) * DP + m];
+      for (int s = 0; s < 2; ++s) {  // 16 points per MFMA
+        // B[k = point][n = co]: points 16 s + 8 kh + i, channel m
+        bf16x8 b;
+        for (int i = 0; i < 8; ++i) b[i] = dzl[(size_t)(zr * kTX + 16 * s + 8 * kh + i) * DP + m];
+#pragma unroll
+        for (int q = 0; q < NT; ++q) {
+          const int t = wv + 4 * q;
+          if (t < 27) {
+            const int dy = t / 9 - 1, dz2 = (t / 3) % 3 - 1, dx = t % 3 - 1;
+            // A[m = ci][k = point]: x[point + off][ci], ci = m (zero above CIN)
+            bf16x8 a;
+            const __bf16* base =
+                hal + (size_t)(((1 + dy) * kHZ + (1 + zr + dz2)) * kHX + (1 + 16 * s + 8 * kh + dx)) * CP;
+            for (int i = 0; i < 8; ++i) a[i] = m < CIN ? base[(size_t)i * CP + m] : (__bf16)0.0f;
+            acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[q], 0, 0, 0);
+          }
+        }
+      }
+    }
+  }
+  float* out = partial + (size_t)blockIdx.x * (27 * kC * kC + kC);
+#pragma unroll
+  for (int q = 0; q < NT; ++q) {
+    const int t = wv + 4 * q;
+    if (t >= 27) continue;
+    for (int v = 0; v < 16; ++v) {
+      const int ci = 8 * (v / 4) + 4 * kh + (v & 3);
+      out[((size_t)t * kC + ci) * kC + m] = acc[q][v];
+    }
+  }
+  if (wv == 0 && kh == 0) out[27 * kC * kC + m] = dbs;
+}
+
+// sum of the per-block partials: g[e] = sum_b partial[b][e]
+__global__ void k_reduce_partials(const float* __restrict__ partial, int nblk, int n, float* __restrict__ g) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  float s = 0.0f;
+  for (int b = 0; b < nblk; ++b) s += partial[(size_t)b * n + e];
+  g[e] = s;
+}
+
+// loss and output gradient: L = sum over points and the nout channels of
+// (y - t)^2 / (npts nout); dy = 2 (y - t) / (npts nout) into a 32-channel
+// BF16 volume (zero in the other channels); lsum[blockIdx] = block sum
+__global__ void __launch_bounds__(256) k_loss(const float* __restrict__ y, const __bf16* __restrict__ tgt, int tc,
+                                              int nout, size_t npts, __bf16* __restrict__ dy,
+                                              float* __restrict__ lsum) {
+  const float sc = 1.0f / (float)((double)npts * nout);
+  float s = 0.0f;
+  for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < npts; p += (size_t)gridDim.x * blockDim.x) {
+    bf16x8 g0, g1, g2, g3;
+    for (int i = 0; i < 8; ++i) {
+      g0[i] = (__bf16)0.0f;
+      g1[i] = (__bf16)0.0f;
+      g2[i] = (__bf16)0.0f;
+      g3[i] = (__bf16)0.0f;
+    }
+    for (int c = 0; c < nout; ++c) {
+      const float d = y[p * nout + c] - (float)tgt[p * tc + c];
+      s += d * d;
+      g0[c] = (__bf16)(2.0f * d * sc);
+    }
+    bf16x8* o = (bf16x8*)(dy + p * kC);
+    o[0] = g0;
+    o[1] = g1;
+    o[2] = g2;
+    o[3] = g3;
+  }
+  __shared__ float red[256];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) lsum[blockIdx.x] = red[0] * sc;
+}
+
+// Adam on the FP32 master parameters (bias-corrected, Kingma & Ba 2015)
+__global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m1,
+                       float* __restrict__ m2, int n, float lr, float b1, float b2, float eps, float c1, float c2) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const float gg = g[e];
+  const float a = b1 * m1[e] + (1.0f - b1) * gg;
+  const float b = b2 * m2[e] + (1.0f - b2) * gg * gg;
+  m1[e] = a;
+  m2[e] = b;
+  w[e] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
+}
+
+// BF16 operand layouts of one layer from the FP32 master [t][ci 32][co 32]:
+// forward wf[t][co][ci (cin)], backward-data wb[t][ci][co]
+__global__ void k_wlayouts(const float* __restrict__ w, int cin, __bf16* __restrict__ wf, __bf16* __restrict__ wb) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= 27 * kC * kC) return;
+  const int co = e % kC, ci = (e / kC) % kC, t = e / (kC * kC);
+  const float v = w[e];
+  if (ci < cin) wf[((size_t)t * kC + co) * cin + ci] = (__bf16)v;
+  wb[((size_t)t * kC + ci) * kC + co] = (__bf16)v;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+namespace {
+
+template <int CIN>
+constexpr size_t conv_lds() {
+  return ((size_t)kHalo * Lds<CIN>::CP + (size_t)27 * kC * Lds<CIN>::CP) * sizeof(__bf16);
+}
+template <int CIN>
+constexpr size_t dw_lds() {
+  return ((size_t)kHalo * Lds<CIN>::CP + (size_t)kTZ * kTX * 40) * sizeof(__bf16);
+}
+
+// raise the dynamic-LDS limit of a kernel once (kernels above 64 KB)
+template <typename K>
+void lds_attr(K kern, size_t bytes) {
+  static std::vector<const void*> done;
+  const void* p = (const void*)kern;
+  for (const void* q : done)
+    if (q == p) return;
+  (void)hipFuncSetAttribute(p, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  done.push_back(p);
+}
+
+}  // namespace
+
+bool cnn_dims_ok(int NY, int NZ, int NX) { return NY >= 1 && NZ % kTZ == 0 && NX % kTX == 0 && NZ > 0 && NX > 0; }
+
+void launch_conv_fwd(hipStream_t st, int layer, const __bf16* in, const __bf16* wf, const float* bias, __bf16* zout,
+                     __bf16* aout, float* yout, int nout, int NY, int NZ, int NX) {
+  const dim3 grid(NX / kTX, NZ / kTZ, NY);
+  if (layer == 0) {
+    auto k = k_conv<16, 4, 0>;
+    lds_attr(k, conv_lds<16>());
+    hipLaunchKernelGGL(k, grid, dim3(256), conv_lds<16>(), st, in, wf, bias, nullptr, zout, aout, nullptr, 0, NY, NZ,
+                       NX);
+  } else if (yout) {
+    auto k = k_conv<32, 32, 1>;
+    lds_attr(k, conv_lds<32>());
+    hipLaunchKernelGGL(k, grid, dim3(256), conv_lds<32>(), st, in, wf, bias, nullptr, nullptr, nullptr, yout, nout,
+                       NY, NZ, NX);
+  } else {
+    auto k = k_conv<32, 32, 0>;
+    lds_attr(k, conv_lds<32>());
+    hipLaunchKernelGGL(k, grid, dim3(256), conv_lds<32>(), st, in, wf, bias, nullptr, zout, aout, nullptr, 0, NY, NZ,
+                       NX);
+  }
+}
+
+void launch_conv_bwd_data(hipStream_t st, const __bf16* dz, const __bf16* wb, const __bf16* zprev, __bf16* dprev,
+                          int NY, int NZ, int NX) {
+  const dim3 grid(NX / kTX, NZ / kTZ, NY);
+  auto k = k_conv<32, 32, 2>;
+  lds_attr(k, conv_lds<32>());
+  hipLaunchKernelGGL(k, grid, dim3(256), conv_lds<32>(), st, dz, wb, nullptr, zprev, nullptr, dprev, nullptr, 0, NY,
+                     NZ, NX);
+}
+
+void launch_conv_dw(hipStream_t st, int layer, const __bf16* in, const __bf16* dz, float* partial, int nblk,
+                    float* grad, int NY, int NZ, int NX) {
+  if (layer == 0) {
+    auto k = k_conv_dw<16, 4>;
+    lds_attr(k, dw_lds<16>());
+    hipLaunchKernelGGL(k, dim3(nblk), dim3(256), dw_lds<16>(), st, in, dz, partial, NY, NZ, NX);
+  } else {
+    auto k = k_conv_dw<32, 32>;
+    lds_attr(k, dw_lds<32>());
+    hipLaunchKernelGGL(k, dim3(nblk), dim3(256), dw_lds<32>(), st, in, dz, partial, NY, NZ, NX);
+  }
+  const int n = 27 * kC * kC + kC;
+  hipLaunchKernelGGL(k_reduce_partials, dim3((n + 255) / 256), dim3(256), 0, st, partial, nblk, n, grad);
+}
+
+void launch_loss(hipStream_t st, const float* y, const __bf16* tgt, int tc, int nout, size_t npts, __bf16* dy,
+                 float* lsum, int nblk) {
+  hipLaunchKernelGGL(k_loss, dim3(nblk), dim3(256), 0, st, y, tgt, tc, nout, npts, dy, lsum);
+}
+
+void launch_adam(hipStream_t st, float* w, const float* g, float* m1, float* m2, int n, float lr, float b1, float b2,
+                 float eps, int step) {
+  const float c1 = 1.0f - std::pow(b1, (float)step), c2 = 1.0f - std::pow(b2, (float)step);
+  hipLaunchKernelGGL(k_adam, dim3((n + 255) / 256), dim3(256), 0, st, w, g, m1, m2, n, lr, b1, b2, eps, c1, c2);
+}
+
+void launch_wlayouts(hipStream_t st, const float* w, int cin, __bf16* wf, __bf16* wb) {
+  hipLaunchKernelGGL(k_wlayouts, dim3((27 * kC * kC + 255) / 256), dim3(256), 0, st, w, cin, wf, wb);
+}
+
+}  // namespace cfd

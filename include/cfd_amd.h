@@ -218,6 +218,42 @@ int cfd_stats_dims(cfd_handle h, int* d);
 int cfd_profile(cfd_handle h, int enable);
 int cfd_profile_read(cfd_handle h, double* ms, int* cnt);
 
+
+/* ------------------------------------------------------------------------
+ * Learned closure network (no counterpart in the reference: the learned
+ * coarse-grid correction of the nonlinear term).  A 3D CNN over a volume of
+ * ny planes x nz x nx points (nx a multiple of 32, nz of 4), periodic in x
+ * and z, zero beyond the planes: four 3x3x3 convolutions 4 -> 32 -> 32 -> 32
+ * -> 3 channels with GELU after the first three, BF16 weights and
+ * activations, FP32 accumulation and FP32 master weights trained by Adam.
+ * Parameters in cfd_cnn_nparams() floats: per layer W[t][ci][co] (t = 9 (dy+1)
+ * + 3 (dz+1) + (dx+1)) then b[co].  Volumes on the host are point-major,
+ * point = (y nz + z) nx + x: input [p][4] (u, v, w, mask), target and
+ * output [p][3].  The dataset holds nslots (input, target) pairs on the
+ * device, filled by cfd_closure_snapshot.
+ * ------------------------------------------------------------------------ */
+typedef struct cfd_cnn_s* cfd_cnn;
+int cfd_cnn_nparams(void);
+int cfd_cnn_create(int device, int ny, int nz, int nx, cfd_cnn* out);
+int cfd_cnn_destroy(cfd_cnn c);
+const char* cfd_cnn_last_error(cfd_cnn c);
+int cfd_cnn_set_params(cfd_cnn c, const float* w);
+int cfd_cnn_get_params(cfd_cnn c, float* w);
+int cfd_cnn_get_grads(cfd_cnn c, float* g);
+int cfd_cnn_set_input(cfd_cnn c, const float* x);
+int cfd_cnn_set_target(cfd_cnn c, const float* t);
+int cfd_cnn_forward(cfd_cnn c);
+int cfd_cnn_get_output(cfd_cnn c, float* y);
+/* gradients of the mean squared error of the last forward pass against the
+ * target; loss may be null */
+int cfd_cnn_backward(cfd_cnn c, double* loss);
+int cfd_cnn_adam(cfd_cnn c, float lr, float b1, float b2, float eps);
+int cfd_cnn_dataset(cfd_cnn c, int nslots);
+int cfd_cnn_train_slot(cfd_cnn c, int slot, float lr, float b1, float b2, float eps, double* loss);
+int cfd_cnn_eval_slot(cfd_cnn c, int slot, double* loss);
+int cfd_cnn_load_slot(cfd_cnn c, int slot);
+int cfd_cnn_device_bytes(cfd_cnn c, double* bytes);
+
 #ifdef __cplusplus
 }
 #endif
