@@ -225,6 +225,19 @@ class Channel:
     def _stats_tag(self, t):
         return "%04dx%04dx%04d_%05d.dat" % (self.N1, self.N2, self.params.ny, int(10.0 * t))
 
+    # -- learned closure ---------------------------------------------------------
+    def closure_attach(self, net):
+        """Apply the network's correction in every nonlinear term (None detaches)."""
+        self._call("cfd_closure_attach", net.handle if net is not None else None)
+
+    def closure_phys(self, which):
+        """Physical planes of the last closure sample / application: 0..2 the
+        velocities, 3..5 the corrections; (planes, Ngalz, Ngalx)."""
+        ny = self.nyv if which % 3 == 1 else self.nyu
+        out = np.zeros((ny, self.params.ngalz, self.params.ngalx))
+        self._call("cfd_closure_phys", int(which), out.ctypes.data_as(_D))
+        return out
+
     # -- timing ----------------------------------------------------------------
     def profile(self, enable=True):
         self._call("cfd_profile", 1 if enable else 0)

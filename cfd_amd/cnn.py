@@ -131,6 +131,15 @@ class CNN:
     def load_slot(self, slot):
         self._call("cfd_cnn_load_slot", int(slot))
 
+    def set_scales(self, s6):
+        s = np.ascontiguousarray(s6, dtype=np.float32)
+        self._call("cfd_cnn_set_scales", s.ctypes.data_as(_F))
+
+    def scales(self):
+        s = np.zeros(6, np.float32)
+        self._call("cfd_cnn_get_scales", s.ctypes.data_as(_F))
+        return s
+
     def device_bytes(self):
         v = ctypes.c_double()
         self._call("cfd_cnn_device_bytes", ctypes.byref(v))

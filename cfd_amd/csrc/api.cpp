@@ -10,6 +10,7 @@
 #include <string>
 #include <vector>
 
+#include "cnn.hpp"
 #include "common.hpp"
 #include "io.hpp"
 
@@ -46,6 +47,21 @@ struct cfd_solver : public Solver {
     int pending = 0, pending_iter = 0;
     std::string pending_dir;
   } st;
+  // learned closure: buffers between the solver and the network volume
+  struct Closure {
+    bool ready = false;
+    cfd_cnn_s* cnn = nullptr;  // attached network (applied in every nonlinear term)
+    double2* Z = nullptr;      // one field of half-transformed planes
+    double* Pu[3] = {};        // physical u1 (u planes), u2 (v planes), u3 (u planes)
+    double* Pc[3] = {};        // physical corrections of the three components
+    double2* Cs[3] = {};       // spectral corrections
+    unsigned char* mask = nullptr;  // 1 at the forcing points of the u grid
+    ZInvJob* d_zj[6] = {};     // z-inverse jobs: u1, u2, u3 (fields), C1, C2, C3
+    XJob *d_xi[6] = {}, *d_xf[3] = {};
+    ZFwdJob* d_zf[3] = {};
+    int nzj[6] = {}, nxi[6] = {}, nxf[3] = {};
+    double* acc = nullptr;     // 6 sums of squares of a sample
+  } cl;
 };
 
 namespace {
@@ -984,6 +1000,127 @@ int read_scalars(cfd_solver* s, DevScalars& d) {
   return CFD_OK;
 }
 
+// ---------------------------------------------------------------------------
+// learned closure: the coarse solver's side (the network is cnn.cpp)
+//
+// The network sees the volume of the u planes (nyu x Ngz x Ngx): channel 0
+// u1 and channel 2 u3 of u plane y, channel 1 u2 of v plane y (zero on the
+// last plane), channel 3 the IB mask (the forcing points of the u grid); it
+// returns the three components of a correction of the nonlinear term on the
+// same planes (components 1 and 3 on the interior u planes, 2 on the interior
+// v planes), which the solver transforms to the spectral modes and adds to
+// the x/z part of the nonlinear term before the y-derivative assembly.
+// ---------------------------------------------------------------------------
+int closure_setup(cfd_solver* s) {
+  auto& C = s->cl;
+  if (C.ready) return CFD_OK;
+  if (s->mixed) return fail(s, CFD_EINVAL, "closure: the learned closure runs on the FP64 path (precision 0)");
+  if (!cnn_dims_ok(s->nyu, s->ngz, s->ngx))
+    return fail(s, CFD_EINVAL, "closure: Ngalx must be a multiple of 32 and Ngalz of 4");
+  const size_t zpl = (size_t)s->ngz * s->Mx, ppl = (size_t)s->ngz * s->ngx;
+  const int nyf[3] = {s->nyu, s->nyv, s->nyu};
+  int rc;
+  if ((rc = dalloc(s, &C.Z, zpl * s->nyu)) || (rc = dalloc(s, &C.mask, ppl * s->nyu)) || (rc = dalloc(s, &C.acc, 6)))
+    return rc;
+  for (int f = 0; f < 3; ++f)
+    if ((rc = dalloc(s, &C.Pu[f], ppl * nyf[f])) || (rc = dalloc(s, &C.Pc[f], ppl * nyf[f])) ||
+        (rc = dalloc(s, &C.Cs[f], (size_t)s->Ncol * nyf[f])))
+      return rc;
+  // mask: the forcing points of both walls of the u grid
+  {
+    std::vector<unsigned char> m(ppl * s->nyu, 0);
+    for (int w = 0; w < 2; ++w)
+      for (const IBPoint& e : s->ib[0][w]) {
+        const int q = e.j - s->jlo_u;
+        if (q >= 0 && q < s->nyu && e.i >= 1 && e.i <= s->ngx && e.k >= 1 && e.k <= s->ngz)
+          m[(size_t)q * ppl + (size_t)(e.k - 1) * s->ngx + (e.i - 1)] = 1;
+      }
+    HIPCHK(s, hipMemcpy(C.mask, m.data(), m.size(), hipMemcpyHostToDevice));
+  }
+  // jobs: z-inverse of a field (mode 0, all planes) into Z; x-inverse pairs
+  // of planes of Z into a physical buffer; x-forward of a physical buffer
+  // into Z; z-forward of Z into a spectral buffer
+  for (int f = 0; f < 6; ++f) {
+    const int g = f % 3, ny = nyf[g];
+    const double2* src = f < 3 ? s->F[g == 0 ? CFD_U1 : g == 1 ? CFD_U2 : CFD_U3] : C.Cs[g];
+    double* dst = f < 3 ? C.Pu[g] : C.Pc[g];
+    std::vector<ZInvJob> zj(ny);
+    for (int q = 0; q < ny; ++q) {
+      ZInvJob z{};
+      z.src = src;
+      z.dst = C.Z;
+      z.pa = q;
+      z.pd = q;
+      zj[q] = z;
+    }
+    std::vector<XJob> xj;
+    for (int q = 0; q < ny; q += 2) {
+      const bool pair = q + 1 < ny;
+      XJob x{};
+      x.za = C.Z;
+      x.zb = pair ? C.Z + zpl : C.Z;
+      x.pa = dst;
+      x.pb = pair ? dst + ppl : dst;
+      x.plane = q;
+      xj.push_back(x);
+    }
+    if ((rc = upload(s, &C.d_zj[f], zj)) || (rc = upload(s, &C.d_xi[f], xj))) return rc;
+    C.nzj[f] = (int)zj.size();
+    C.nxi[f] = (int)xj.size();
+  }
+  for (int g = 0; g < 3; ++g) {
+    std::vector<XJob> xf;
+    std::vector<ZFwdJob> zf;
+    for (int q = 0; q < nyf[g]; ++q) {
+      XJob x{};
+      x.pa = C.Pc[g];
+      x.zo = C.Z;
+      x.plane = q;
+      xf.push_back(x);
+      ZFwdJob z{};
+      z.src = C.Z;
+      z.dst = C.Cs[g];
+      z.ps = q;
+      z.pd = q;
+      zf.push_back(z);
+    }
+    if ((rc = upload(s, &C.d_xf[g], xf)) || (rc = upload(s, &C.d_zf[g], zf))) return rc;
+    C.nxf[g] = (int)xf.size();
+  }
+  C.ready = true;
+  return CFD_OK;
+}
+
+// physical planes of job set f (0..2 velocity, 3..5 corrections)
+void closure_to_phys(cfd_solver* s, int f) {
+  auto& C = s->cl;
+  launch_zinv(*s, C.d_zj[f], C.nzj[f], false);
+  launch_xpass_inv2(*s, C.d_xi[f], C.nxi[f]);
+}
+
+// the network's correction of the current state into the x/z part of the
+// nonlinear term (nxz1..3, interior planes)
+int closure_apply(cfd_solver* s) {
+  auto& C = s->cl;
+  cfd_cnn_s* c = C.cnn;
+  for (int f = 0; f < 3; ++f) closure_to_phys(s, f);
+  launch_pack(s->stream, C.Pu[0], C.Pu[1], C.Pu[2], C.mask, s->nyu, s->nyv, s->ngz, s->ngx, c->scale, 1, c->x);
+  hipStream_t keep = c->stream;
+  c->stream = s->stream;  // the network runs on the solver's stream here
+  const int rc = cnn_forward_dev(c, c->x);
+  c->stream = keep;
+  if (rc) return fail(s, rc, std::string("closure: ") + c->err);
+  launch_unpack(s->stream, c->y, s->nyu, s->nyv, s->ngz, s->ngx, c->scale + 3, C.Pc[0], C.Pc[1], C.Pc[2]);
+  void* nxz[3] = {s->nxz1, s->nxz2, s->nxz3};
+  const int nyf[3] = {s->nyu, s->nyv, s->nyu};
+  for (int g = 0; g < 3; ++g) {
+    launch_xpass_fwd2(*s, C.d_xf[g], C.nxf[g]);
+    launch_zfwd_plain(*s, C.d_zf[g], C.nxf[g]);
+    launch_add_planes(s->stream, (double2*)nxz[g], C.Cs[g], s->Ncol, 1, nyf[g] - 1);
+  }
+  return CFD_OK;
+}
+
 // A statistics request (cfd_stats_request) is served by the next nonlinear
 // term with kRK = 1, inside cfd_nonlinear or cfd_substep, where the
 // reference's nonlinear serves flagst / flagwr (FOU3D.f90:56-69, 107-110,
@@ -1023,6 +1160,8 @@ int op_nonlinear(cfd_solver* s, int kRK) {
   { Timed t(s, KC_ZFWD); launch_zfwd_nl(*s); }
   if (kRK == 1) launch_cfl(*s);
   const int wrc = stats_req_end(s, req);
+  if (s->cl.cnn)
+    if (int rc = closure_apply(s)) return rc;
   { Timed t(s, KC_ELEM); launch_nlcombine(*s, kRK); }
   return wrc;
 }
@@ -1063,6 +1202,8 @@ int op_substep(cfd_solver* s, int kRK) {
   if (kRK == 1) launch_cfl(*s);
   // a failed write (say, a missing directory) still completes the substep
   const int wrc = stats_req_end(s, req);
+  if (s->cl.cnn)
+    if (int rc = closure_apply(s)) return rc;
   {
     Timed t(s, KC_IB);
     launch_ib_rhs(*s, kRK);
@@ -1403,5 +1544,100 @@ extern "C" int cfd_stats_dims(cfd_handle h, int* d) {
   d[3] = S.nyc[1];
   d[4] = S.nyc[2];
   d[5] = S.istat;
+  return CFD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// learned closure
+// ---------------------------------------------------------------------------
+extern "C" int cfd_closure_attach(cfd_handle h, cfd_cnn c) {
+  cfd_solver* s = static_cast<cfd_solver*>(h);
+  if (int rc = check(s, 1)) return rc;
+  if (!c) {
+    s->cl.cnn = nullptr;
+    return CFD_OK;
+  }
+  if (c->NY != s->nyu || c->NZ != s->ngz || c->NX != s->ngx)
+    return fail(s, CFD_EINVAL, "closure: the network volume must be nyu x Ngalz x Ngalx of this solver");
+  for (int k = 0; k < 6; ++k)
+    if (!(c->scale[k] > 0.0f)) return fail(s, CFD_EINVAL, "closure: the network has no normalisation scales");
+  if (int rc = closure_setup(s)) return rc;
+  s->cl.cnn = c;
+  return CFD_OK;
+}
+
+// one training sample: the box-filtered state of the fine solver on the
+// coarse grid (input) and the correction filt(N(u)) - N_coarse(filt(u))
+// (target), into dataset slot `slot`.  The coarse solver's fields are
+// overwritten; rms[6] returns the root mean square of u1, u2, u3 and of the
+// three corrections; the first sample of a network without scales sets them.
+extern "C" int cfd_closure_snapshot(cfd_handle hf, cfd_handle hc, cfd_cnn c, int slot, double* rms) {
+  cfd_solver* f = static_cast<cfd_solver*>(hf);
+  cfd_solver* s = static_cast<cfd_solver*>(hc);
+  if (int rc = check(f, 2)) return rc;
+  if (int rc = check(s, 2)) return rc;
+  if (!c || slot < 0 || slot >= c->nslots) return fail(s, CFD_EINVAL, "closure: bad network or slot");
+  if (f->ny != s->ny || f->dsty != s->dsty || f->P.ygrid != s->P.ygrid || s->N1 > f->N1 || s->N2 > f->N2)
+    return fail(s, CFD_EINVAL, "closure: the coarse solver must share the y grid and have fewer x/z modes");
+  if (c->NY != s->nyu || c->NZ != s->ngz || c->NX != s->ngx)
+    return fail(s, CFD_EINVAL, "closure: the network volume must be nyu x Ngalz x Ngalx of the coarse solver");
+  if (f->mixed) return fail(s, CFD_EINVAL, "closure: samples come from the FP64 path");
+  if (int rc = closure_setup(s)) return rc;
+  auto& C = s->cl;
+  cfd_cnn_s* keep = C.cnn;
+  C.cnn = nullptr;  // the coarse nonlinear term below is the plain one
+  // fine nonlinear term (kRK = 2: no time-step update), then on the coarse side
+  if (int rc = op_nonlinear(f, 2)) return rc;
+  HIPCHK(s, hipStreamSynchronize(f->stream));
+  const int nyf[3] = {s->nyu, s->nyv, s->nyu};
+  const int uf[3] = {CFD_U1, CFD_U2, CFD_U3}, nf[3] = {CFD_NU1, CFD_NU2, CFD_NU3};
+  for (int g = 0; g < 3; ++g)
+    launch_filter(s->stream, f->F[uf[g]], nullptr, s->F[uf[g]], f->Mx, f->N2, s->Mx, s->N2, s->ngx, s->ngz, 1.0, 0.0,
+                  nyf[g]);
+  launch_filter(s->stream, f->F[CFD_P], nullptr, s->F[CFD_P], f->Mx, f->N2, s->Mx, s->N2, s->ngx, s->ngz, 1.0, 0.0,
+                s->nyp);
+  const int rcn = op_nonlinear(s, 2);
+  C.cnn = keep;
+  if (rcn) return rcn;
+  for (int g = 0; g < 3; ++g) {
+    launch_filter(s->stream, f->F[nf[g]], s->F[nf[g]], C.Cs[g], f->Mx, f->N2, s->Mx, s->N2, s->ngx, s->ngz, 1.0, -1.0,
+                  nyf[g]);
+    // the nonlinear term is defined on the interior planes only
+    HIPCHK(s, hipMemsetAsync(C.Cs[g], 0, (size_t)s->Ncol * sizeof(double2), s->stream));
+    HIPCHK(s, hipMemsetAsync(C.Cs[g] + (size_t)(nyf[g] - 1) * s->Ncol, 0, (size_t)s->Ncol * sizeof(double2),
+                             s->stream));
+  }
+  for (int k = 0; k < 6; ++k) closure_to_phys(s, k);
+  // root mean squares
+  const size_t ppl = (size_t)s->ngz * s->ngx;
+  HIPCHK(s, hipMemsetAsync(C.acc, 0, 6 * sizeof(double), s->stream));
+  for (int k = 0; k < 6; ++k) launch_sumsq(s->stream, k < 3 ? C.Pu[k] : C.Pc[k - 3], ppl * nyf[k % 3], C.acc + k);
+  double h[6];
+  HIPCHK(s, hipMemcpyAsync(h, C.acc, sizeof h, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  for (int k = 0; k < 6; ++k) h[k] = std::sqrt(h[k] / (double)(ppl * nyf[k % 3]));
+  if (rms)
+    for (int k = 0; k < 6; ++k) rms[k] = h[k];
+  if (!(c->scale[0] > 0.0f))
+    for (int k = 0; k < 6; ++k) c->scale[k] = h[k] > 0.0 ? (float)h[k] : 1.0f;
+  __bf16* xs = c->dx + c->npts * 4 * (size_t)slot;
+  __bf16* ts = c->dt + c->npts * 4 * (size_t)slot;
+  launch_pack(s->stream, C.Pu[0], C.Pu[1], C.Pu[2], C.mask, s->nyu, s->nyv, s->ngz, s->ngx, c->scale, 1, xs);
+  launch_pack(s->stream, C.Pc[0], C.Pc[1], C.Pc[2], nullptr, s->nyu, s->nyv, s->ngz, s->ngx, c->scale + 3, 0, ts);
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  return launched(s);
+}
+
+// physical values of the last sample's or the last application's buffers:
+// which 0..2 the velocities u1, u2, u3, 3..5 the corrections (planes of the
+// component's grid x Ngalz x Ngalx)
+extern "C" int cfd_closure_phys(cfd_handle h, int which, double* out) {
+  cfd_solver* s = static_cast<cfd_solver*>(h);
+  if (int rc = check(s, 1)) return rc;
+  if (!out || which < 0 || which > 5 || !s->cl.ready) return fail(s, CFD_EINVAL, "closure: nothing to read");
+  const int g = which % 3, ny = g == 1 ? s->nyv : s->nyu;
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  HIPCHK(s, hipMemcpy(out, which < 3 ? s->cl.Pu[g] : s->cl.Pc[g], (size_t)ny * s->ngz * s->ngx * sizeof(double),
+                      hipMemcpyDeviceToHost));
   return CFD_OK;
 }

@@ -300,3 +300,16 @@ extern "C" int cfd_cnn_device_bytes(cfd_cnn c, double* bytes) {
   *bytes = c->bytes;
   return CFD_OK;
 }
+
+extern "C" int cfd_cnn_set_scales(cfd_cnn c, const float* s6) {
+  if (!c || !s6) return cfail(c, CFD_EINVAL, "null argument");
+  for (int k = 0; k < 6; ++k)
+    if (!(s6[k] > 0.0f)) return cfail(c, CFD_EINVAL, "scales must be positive");
+  for (int k = 0; k < 6; ++k) c->scale[k] = s6[k];
+  return CFD_OK;
+}
+extern "C" int cfd_cnn_get_scales(cfd_cnn c, float* s6) {
+  if (!c || !s6) return cfail(c, CFD_EINVAL, "null argument");
+  for (int k = 0; k < 6; ++k) s6[k] = c->scale[k];
+  return CFD_OK;
+}

@@ -33,6 +33,9 @@ struct cfd_cnn_s {
   float* y = nullptr;                                              // output [p][3]
   __bf16 *dx = nullptr, *dt = nullptr;                             // dataset slots
   int nslots = 0, adam_step = 0;
+  // normalisation of the closure data: inputs u1, u2, u3 divided by
+  // scale[0..2], targets (corrections) by scale[3..5]; 0 = not set yet
+  float scale[6] = {0, 0, 0, 0, 0, 0};
   double bytes = 0.0;
   std::vector<void*> allocs;
   std::string err;
@@ -60,4 +63,16 @@ void launch_adam(hipStream_t st, float* w, const float* g, float* m1, float* m2,
                  float eps, int step);
 void launch_wlayouts(hipStream_t st, const float* w, int cin, __bf16* wf, __bf16* wb);
 
+}  // namespace cfd
+
+namespace cfd {
+// closure data movement (kernels_cnn.hip)
+void launch_filter(hipStream_t st, const double2* fine, const double2* other, double2* out, int Mxf, int N2f, int Mxc,
+                   int N2c, int ngxc, int ngzc, double a, double b, int nplanes);
+void launch_pack(hipStream_t st, const double* p1, const double* p2, const double* p3, const unsigned char* mask,
+                 int nyu, int nyv, int ngz, int ngx, const float* s, int withmask, __bf16* out);
+void launch_unpack(hipStream_t st, const float* y, int nyu, int nyv, int ngz, int ngx, const float* s, double* p1,
+                   double* p2, double* p3);
+void launch_add_planes(hipStream_t st, double2* a, const double2* c, int ncol, int q0, int q1);
+void launch_sumsq(hipStream_t st, const double* v, size_t n, double* acc);
 }  // namespace cfd

@@ -385,3 +385,127 @@ void launch_wlayouts(hipStream_t st, const float* w, int cin, __bf16* wf, __bf16
 }
 
 }  // namespace cfd
+
+// ---------------------------------------------------------------------------
+// closure data movement between the solver and the network volume
+// ---------------------------------------------------------------------------
+namespace cfd {
+
+namespace {
+
+__device__ __forceinline__ double sinc(double a) { return a == 0.0 ? 1.0 : sin(a) / a; }
+
+// out[q][k Mxc + i] = a * Hx(i) Hz(kz) fine[q][kf Mxf + i] + b * other[q][...]
+// (box filter of one coarse cell width, spectral form, on the coarse modes)
+__global__ void k_filter(const double2* __restrict__ fine, const double2* __restrict__ other, double2* __restrict__ out,
+                         int Mxf, int N2f, int Mxc, int N2c, int ngxc, int ngzc, double a, double b, int nplanes) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = blockIdx.y;
+  const int ncol = Mxc * N2c;
+  if (col >= ncol || q >= nplanes) return;
+  const int k = col / Mxc, i = col - k * Mxc;
+  const int kz = k < N2c / 2 ? k : k - N2c;
+  const int kf = kz >= 0 ? kz : N2f + kz;
+  const double pi = 3.14159265358979323846;
+  const double h = sinc(pi * i / ngxc) * sinc(pi * (kz < 0 ? -kz : kz) / ngzc);
+  const double2 f = fine[(size_t)q * Mxf * N2f + (size_t)kf * Mxf + i];
+  double2 v = make_double2(a * h * f.x, a * h * f.y);
+  if (other) {
+    const double2 o = other[(size_t)q * ncol + col];
+    v = make_double2(v.x + b * o.x, v.y + b * o.y);
+  }
+  out[(size_t)q * ncol + col] = v;
+}
+
+// volume point (y, z, x) <- physical planes: u1, u3 on u plane y, u2 on v
+// plane y (zero on the last plane), the mask; scaled by 1 / s
+__global__ void k_pack(const double* __restrict__ p1, const double* __restrict__ p2, const double* __restrict__ p3,
+                       const unsigned char* __restrict__ mask, int nyu, int nyv, int ngz, int ngx, float s1, float s2,
+                       float s3, int withmask, __bf16* __restrict__ out) {
+  const size_t pl = (size_t)ngz * ngx;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= pl * nyu) return;
+  const int y = (int)(e / pl);
+  const float a = (float)p1[e] / s1;
+  const float b = y < nyv ? (float)p2[e] / s2 : 0.0f;
+  const float c = (float)p3[e] / s3;
+  const float m = withmask && mask ? (float)mask[e] : 0.0f;
+  __bf16* o = out + e * 4;
+  o[0] = (__bf16)a;
+  o[1] = (__bf16)b;
+  o[2] = (__bf16)c;
+  o[3] = (__bf16)m;
+}
+
+// physical corrections <- network output y[p][3] * s on the interior planes
+// (u planes 1..nyu-2 for components 1 and 3, v planes 1..nyv-2 for 2)
+__global__ void k_unpack(const float* __restrict__ y, int nyu, int nyv, int ngz, int ngx, float s1, float s2, float s3,
+                         double* __restrict__ p1, double* __restrict__ p2, double* __restrict__ p3) {
+  const size_t pl = (size_t)ngz * ngx;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= pl * nyu) return;
+  const int q = (int)(e / pl);
+  const bool iu = q >= 1 && q <= nyu - 2, iv = q >= 1 && q <= nyv - 2;
+  p1[e] = iu ? (double)(y[e * 3 + 0] * s1) : 0.0;
+  if (q < nyv) p2[e] = iv ? (double)(y[e * 3 + 1] * s2) : 0.0;
+  p3[e] = iu ? (double)(y[e * 3 + 2] * s3) : 0.0;
+}
+
+// a[q] += c[q] on planes q0 .. q1 - 1 of Ncol complex values
+__global__ void k_add_planes(double2* __restrict__ a, const double2* __restrict__ c, int ncol, int q0, int q1) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t n = (size_t)(q1 - q0) * ncol;
+  if (e >= n) return;
+  const size_t o = (size_t)q0 * ncol + e;
+  a[o] = make_double2(a[o].x + c[o].x, a[o].y + c[o].y);
+}
+
+// sum of squares of n values into *acc (one atomic per block)
+__global__ void k_sumsq(const double* __restrict__ v, size_t n, double* __restrict__ acc) {
+  double s = 0.0;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+    s += v[e] * v[e];
+  __shared__ double red[256];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(acc, red[0]);
+}
+
+}  // namespace
+
+void launch_filter(hipStream_t st, const double2* fine, const double2* other, double2* out, int Mxf, int N2f, int Mxc,
+                   int N2c, int ngxc, int ngzc, double a, double b, int nplanes) {
+  if (nplanes <= 0) return;
+  hipLaunchKernelGGL(k_filter, dim3((Mxc * N2c + 255) / 256, nplanes), dim3(256), 0, st, fine, other, out, Mxf, N2f,
+                     Mxc, N2c, ngxc, ngzc, a, b, nplanes);
+}
+
+void launch_pack(hipStream_t st, const double* p1, const double* p2, const double* p3, const unsigned char* mask,
+                 int nyu, int nyv, int ngz, int ngx, const float* s, int withmask, __bf16* out) {
+  const size_t n = (size_t)nyu * ngz * ngx;
+  hipLaunchKernelGGL(k_pack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p1, p2, p3, mask, nyu, nyv, ngz, ngx,
+                     s[0], s[1], s[2], withmask, out);
+}
+
+void launch_unpack(hipStream_t st, const float* y, int nyu, int nyv, int ngz, int ngx, const float* s, double* p1,
+                   double* p2, double* p3) {
+  const size_t n = (size_t)nyu * ngz * ngx;
+  hipLaunchKernelGGL(k_unpack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, y, nyu, nyv, ngz, ngx, s[0], s[1],
+                     s[2], p1, p2, p3);
+}
+
+void launch_add_planes(hipStream_t st, double2* a, const double2* c, int ncol, int q0, int q1) {
+  if (q1 <= q0) return;
+  const size_t n = (size_t)(q1 - q0) * ncol;
+  hipLaunchKernelGGL(k_add_planes, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, c, ncol, q0, q1);
+}
+
+void launch_sumsq(hipStream_t st, const double* v, size_t n, double* acc) {
+  hipLaunchKernelGGL(k_sumsq, dim3(512), dim3(256), 0, st, v, n, acc);
+}
+
+}  // namespace cfd

@@ -34,6 +34,16 @@ WORKLOADS = {
                       r=_radius(16.0, 395.0, 384), c_i=12.0, c_k=12.0, stag=0.5, extras24=0, CFL=0.5,
                       flag_ctpress=0),
         steps=200, re_tau=395.0),
+    # the 384 x 192 x 192 case on a 2x coarser x/z grid (the learned
+    # closure's grid): posts of the same diameter (r = 0.62 cells), centres a
+    # quarter cell off the grid points so that no forcing point sits on a
+    # centre
+    "rt395_coarse_192x192x96": dict(
+        params=Params(ngalx=192, ngalz=96, ny=192, dsty=8, ygrid=1, ytanh_beta=2.0, Re=round(_rb(395.0)),
+                      alp=1.0, bet=2.0, posth=16.0 / 395.0, ntilex=16, ntilez=8,
+                      r=_radius(16.0, 395.0, 192), c_i=6.25, c_k=6.25, stag=0.5, extras24=0, CFL=0.5,
+                      flag_ctpress=0),
+        steps=5000, re_tau=395.0),
     # 768 x 384 x 384, Re_tau 550, 32 x 16 posts, D+ = 16, k+ = 20
     "rt550_768x384x384": dict(
         params=Params(ngalx=768, ngalz=384, ny=384, dsty=16, ygrid=1, ytanh_beta=2.0, Re=round(_rb(550.0)),

@@ -253,6 +253,34 @@ int cfd_cnn_train_slot(cfd_cnn c, int slot, float lr, float b1, float b2, float 
 int cfd_cnn_eval_slot(cfd_cnn c, int slot, double* loss);
 int cfd_cnn_load_slot(cfd_cnn c, int slot);
 int cfd_cnn_device_bytes(cfd_cnn c, double* bytes);
+/* normalisation of the closure data: s6 = u1, u2, u3 input scales, then the
+ * three correction scales (all > 0; set by the first cfd_closure_snapshot
+ * when unset) */
+int cfd_cnn_set_scales(cfd_cnn c, const float* s6);
+int cfd_cnn_get_scales(cfd_cnn c, float* s6);
+
+/* The closure on the solver side (no counterpart in the reference).
+ *
+ * cfd_closure_snapshot(fine, coarse, c, slot, rms): one training sample
+ * into dataset slot `slot`: the fine state box-filtered onto the coarse
+ * modes (u_c = Hx(kx) Hz(kz) u_f, H(k) = sinc(k dx_c / 2), dx_c the coarse
+ * spacing) is the input, filt(N(u_f)) - N_c(u_c) the target (N the nonlinear
+ * term of each solver, FOU3D.f90:41-259), both as physical planes of the
+ * coarse grid scaled by the network's scales.  The solvers share the y grid;
+ * the coarse solver's fields are overwritten.  rms[6] (may be null) returns
+ * the root mean squares of u1, u2, u3 and the three targets.
+ *
+ * cfd_closure_attach(h, c): from now on every nonlinear term of h (kRK
+ * 1..3, cfd_nonlinear and cfd_substep) adds the network's correction of the
+ * current state; c = NULL detaches.  The FP64 path only (precision 0); the
+ * network volume must be nyu x Ngalz x Ngalx of h.
+ *
+ * cfd_closure_phys(h, which, out): the physical planes of the last sample
+ * or application: which 0..2 the velocities u1, u2, u3, 3..5 the
+ * corrections (planes of the component's grid x Ngalz x Ngalx). */
+int cfd_closure_snapshot(cfd_handle fine, cfd_handle coarse, cfd_cnn c, int slot, double* rms);
+int cfd_closure_attach(cfd_handle h, cfd_cnn c);
+int cfd_closure_phys(cfd_handle h, int which, double* out);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,208 @@
+"""The learned closure on the solver side (cfd_closure_*), on a small pair of
+grids (128 x 33 x 64 fine, 64 x 33 x 32 coarse, tanh grid, dsty 3):
+
+* a sample: the input planes equal numpy's inverse transforms of the
+  box-filtered fine velocity, the target planes those of
+  filt(N_fine(u)) - N_coarse(filt(u)) computed from two more solvers on the
+  host's filter; the RMS values and the scales they set;
+* inference: with the network attached, the nonlinear term changes by the
+  forward transform of the network's correction (numpy rfft/fft of the
+  applied physical planes, the coarse modes), and that correction is the
+  network's own output on the same BF16 input;
+* training on samples of a short fine run lowers the held-out loss of a new
+  sample below half that of predicting no correction.
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from cfd_amd import Channel, Params, closure, cnn
+from oracle import cnn_ref
+
+from .helpers import random_field
+
+pytestmark = pytest.mark.gpu
+
+FINE = Params(ngalx=128, ngalz=64, ny=33, dsty=3, ygrid=1, ytanh_beta=2.0, geometry_type=0, posth=0.06,
+              ntilex=4, ntilez=2, Re=2800.0, alp=1.0, bet=2.0, CFL=0.5, flag_ctpress=0)
+COARSE = dataclasses.replace(FINE, ngalx=64, ngalz=32)
+VEL = ("u1", "u2", "u3")
+NL = ("Nu1", "Nu2", "Nu3")
+
+
+def _fields(p, seed):
+    rng = np.random.default_rng(seed)
+    f = {}
+    for n in ("u1", "u2", "u3", "p"):
+        a = random_field(p, n, rng, amp=0.3)
+        if n == "u1":
+            a[0, :] += 1.0
+        if n == "u2":
+            a[:, 0] = 0.0
+            a[:, -1] = 0.0
+        f[n] = a
+    return f
+
+
+def _load(ch, f):
+    for n, a in f.items():
+        ch.set_field(n, a)
+    ch.set_scalars(t=0.0, dt=2e-3, dti=500.0, mpgx=-1e-3, QxT=2.0)
+
+
+def _slots(p):
+    N2 = 2 * (p.ngalz // 3)
+    k = np.arange(N2)
+    return np.where(k < N2 // 2, k, k - N2 + p.ngalz)
+
+
+def _phys(p, f):
+    N1, N2 = 2 * (p.ngalx // 3), 2 * (p.ngalz // 3)
+    Mx = N1 // 2 + 1
+    planes = f.T.reshape(-1, N2, Mx)
+    C = np.zeros((planes.shape[0], p.ngalz, p.ngalx // 2 + 1), dtype=complex)
+    C[:, _slots(p), :Mx] = planes
+    return np.fft.irfft(np.fft.ifft(C, axis=1) * p.ngalz, n=p.ngalx, axis=2) * p.ngalx
+
+
+def _spec(p, P):
+    """physical planes -> the solver's spectral modes (forward, 1/(Ngx Ngz))"""
+    N1, N2 = 2 * (p.ngalx // 3), 2 * (p.ngalz // 3)
+    Mx = N1 // 2 + 1
+    F = np.fft.fft(np.fft.rfft(P, axis=2) / p.ngalx, axis=1) / p.ngalz
+    return F[:, _slots(p), :Mx].reshape(P.shape[0], -1).T
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def _reference_sample(f):
+    """(filtered velocity, target) on the coarse modes, from two solvers"""
+    fine = Channel(FINE)
+    _load(fine, f)
+    fine.nonlinear(2)
+    nf = {n: fine.get_field(n) for n in NL}
+    fine.close()
+    uc = closure.filtered_start(COARSE, f, FINE)
+    co = Channel(COARSE)
+    _load(co, uc)
+    co.nonlinear(2)
+    nc = {n: co.get_field(n) for n in NL}
+    co.close()
+    filt = closure.filtered_start(COARSE, nf, FINE)
+    tgt = {n: filt[n] - nc[n] for n in NL}
+    for n in NL:
+        tgt[n][:, 0] = 0.0
+        tgt[n][:, -1] = 0.0
+    return uc, tgt
+
+
+def test_snapshot_matches_host_pipeline():
+    print("case closure snapshot", FINE.ngalx, COARSE.ngalx, flush=True)
+    f = _fields(FINE, 11)
+    uc, tgt = _reference_sample(f)
+    fine, co = Channel(FINE), Channel(COARSE)
+    net = cnn.CNN(co.nyu, COARSE.ngalz, COARSE.ngalx)
+    try:
+        _load(fine, f)
+        net.dataset(2)
+        rms = closure.snapshot(fine, co, net, 0)
+        for k, n in enumerate(VEL):
+            got, ref = co.closure_phys(k), _phys(COARSE, uc[n])
+            assert _rel(got, ref) <= 1e-12, (n, _rel(got, ref))
+            assert abs(rms[k] - np.sqrt(np.mean(ref ** 2))) <= 1e-12 * rms[k]
+        for k, n in enumerate(NL):
+            got, ref = co.closure_phys(3 + k), _phys(COARSE, tgt[n])
+            assert _rel(got, ref) <= 1e-10, (n, _rel(got, ref))
+            assert abs(rms[3 + k] - np.sqrt(np.mean(ref ** 2))) <= 1e-10 * rms[3 + k]
+        assert np.allclose(net.scales(), rms.astype(np.float32))
+    finally:
+        net.close()
+        fine.close()
+        co.close()
+
+
+def test_attached_network_corrects_the_nonlinear_term():
+    print("case closure inference", COARSE.ngalx, flush=True)
+    f = closure.filtered_start(COARSE, _fields(FINE, 12), FINE)
+    a, b = Channel(COARSE), Channel(COARSE)
+    net = cnn.CNN(a.nyu, COARSE.ngalz, COARSE.ngalx)
+    try:
+        w = cnn.init_params(seed=2, last_scale=1.0)
+        net.set_params(w)
+        s = np.array([1.0, 0.3, 0.3, 0.05, 0.05, 0.05], np.float32)
+        net.set_scales(s)
+        _load(a, f)
+        _load(b, f)
+        a.nonlinear(2)
+        b.closure_attach(net)
+        b.nonlinear(2)
+        b.closure_attach(None)
+        # the applied correction is the network's output on the same input
+        u = [b.closure_phys(k) for k in range(3)]
+        ny, nz, nx = b.nyu, COARSE.ngalz, COARSE.ngalx
+        x = np.zeros((ny, nz, nx, 4), np.float32)
+        x[..., 0] = u[0].astype(np.float32) / s[0]
+        x[:ny - 1, ..., 1] = u[1].astype(np.float32) / s[1]
+        x[..., 2] = u[2].astype(np.float32) / s[2]
+        net.set_input(x)
+        y = net.forward()
+        corr = [b.closure_phys(3 + k) for k in range(3)]
+        exp0 = np.zeros_like(corr[0])
+        exp0[1:-1] = y[1:-1, ..., 0] * s[3]
+        exp1 = np.zeros_like(corr[1])
+        exp1[1:-1] = y[1:ny - 2, ..., 1] * s[4]
+        assert _rel(corr[0], exp0) <= 1e-6 and _rel(corr[1], exp1) <= 1e-6
+        # the nonlinear term moved by the forward transform of the correction
+        for k, n in enumerate(NL):
+            d = b.get_field(n) - a.get_field(n)
+            e = _spec(COARSE, corr[k])
+            assert _rel(d[:, 1:-1], e[:, 1:-1]) <= 1e-10, (n, _rel(d[:, 1:-1], e[:, 1:-1]))
+    finally:
+        net.close()
+        a.close()
+        b.close()
+
+
+def test_training_beats_no_correction_on_a_new_sample():
+    print("case closure training", FINE.ngalx, COARSE.ngalx, flush=True)
+    fine, co = Channel(FINE), Channel(COARSE)
+    net = cnn.CNN(co.nyu, COARSE.ngalz, COARSE.ngalx)
+    zero = cnn.CNN(co.nyu, COARSE.ngalz, COARSE.ngalx)
+    try:
+        from cfd_amd import ic
+        closure.start_fields(fine, FINE, seed=0, amp=0.3)
+        net.set_params(cnn.init_params(seed=1))
+        zero.set_params(np.zeros(cnn.nparams(), np.float32))
+        net.dataset(17)
+        rng = np.random.default_rng(0)
+        for k in range(48):
+            fine.step(1)
+            closure.snapshot(fine, co, net, k % 16)
+            for _ in range(4):
+                net.train_slot(int(rng.integers(0, min(k + 1, 16))), 2e-3)
+        fine.step(3)
+        closure.snapshot(fine, co, net, 16)
+        zero.set_scales(net.scales())
+        trained = net.eval_slot(16)
+        zero.dataset(1)
+        # the no-correction loss: mean square of the normalised target
+        net2 = cnn.CNN(co.nyu, COARSE.ngalz, COARSE.ngalx)
+        try:
+            net2.set_params(np.zeros(cnn.nparams(), np.float32))
+            net2.set_scales(net.scales())
+            net2.dataset(1)
+            closure.snapshot(fine, co, net2, 0)
+            base = net2.eval_slot(0)
+        finally:
+            net2.close()
+        print("held-out loss %.4g, no correction %.4g" % (trained, base), flush=True)
+        assert trained < 0.5 * base, (trained, base)
+        del ic
+    finally:
+        net.close()
+        zero.close()
+        fine.close()
+        co.close()
