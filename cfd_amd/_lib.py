@@ -85,7 +85,7 @@ SIGNATURES = {
     "cfd_cnn_set_scales": (ctypes.c_int, [_H, _F]),
     "cfd_cnn_get_scales": (ctypes.c_int, [_H, _F]),
     "cfd_closure_snapshot": (ctypes.c_int, [_H, _H, _H, ctypes.c_int, _D]),
-    "cfd_closure_attach": (ctypes.c_int, [_H, _H]),
+    "cfd_closure_attach": (ctypes.c_int, [_H, _H, ctypes.c_int]),
     "cfd_closure_phys": (ctypes.c_int, [_H, ctypes.c_int, _D]),
     "cfd_stats": (ctypes.c_int, [_H]),
     "cfd_write_stats": (ctypes.c_int, [_H, ctypes.c_char_p, ctypes.c_int]),

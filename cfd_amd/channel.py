@@ -226,9 +226,10 @@ class Channel:
         return "%04dx%04dx%04d_%05d.dat" % (self.N1, self.N2, self.params.ny, int(10.0 * t))
 
     # -- learned closure ---------------------------------------------------------
-    def closure_attach(self, net):
-        """Apply the network's correction in every nonlinear term (None detaches)."""
-        self._call("cfd_closure_attach", net.handle if net is not None else None)
+    def closure_attach(self, net, every=1):
+        """Add the network's correction in every nonlinear term, evaluated at
+        kRK = 1 of every `every`-th step and reused in between (None detaches)."""
+        self._call("cfd_closure_attach", net.handle if net is not None else None, int(every))
 
     def closure_phys(self, which):
         """Physical planes of the last closure sample / application: 0..2 the

@@ -61,6 +61,9 @@ struct cfd_solver : public Solver {
     ZFwdJob* d_zf[3] = {};
     int nzj[6] = {}, nxi[6] = {}, nxf[3] = {};
     double* acc = nullptr;     // 6 sums of squares of a sample
+    int every = 1;             // the correction is recomputed at kRK = 1 of every `every`-th step
+    int count = 0;             // steps since attaching
+    bool have = false;         // Cs holds a correction
   } cl;
 };
 
@@ -1098,11 +1101,23 @@ void closure_to_phys(cfd_solver* s, int f) {
   launch_xpass_inv2(*s, C.d_xi[f], C.nxi[f]);
 }
 
-// the network's correction of the current state into the x/z part of the
-// nonlinear term (nxz1..3, interior planes)
-int closure_apply(cfd_solver* s) {
+// the network's correction in the x/z part of the nonlinear term (nxz1..3,
+// interior planes): recomputed from the current state at kRK = 1 of every
+// `every`-th step (and when none is held), otherwise the held one
+int closure_apply(cfd_solver* s, int kRK) {
   auto& C = s->cl;
   cfd_cnn_s* c = C.cnn;
+  bool fresh = !C.have;
+  if (kRK == 1) {
+    fresh = fresh || C.count % C.every == 0;
+    C.count += 1;
+  }
+  void* nxz[3] = {s->nxz1, s->nxz2, s->nxz3};
+  const int nyf[3] = {s->nyu, s->nyv, s->nyu};
+  if (!fresh) {
+    for (int g = 0; g < 3; ++g) launch_add_planes(s->stream, (double2*)nxz[g], C.Cs[g], s->Ncol, 1, nyf[g] - 1);
+    return CFD_OK;
+  }
   for (int f = 0; f < 3; ++f) closure_to_phys(s, f);
   launch_pack(s->stream, C.Pu[0], C.Pu[1], C.Pu[2], C.mask, s->nyu, s->nyv, s->ngz, s->ngx, c->scale, 1, c->x);
   hipStream_t keep = c->stream;
@@ -1111,8 +1126,7 @@ int closure_apply(cfd_solver* s) {
   c->stream = keep;
   if (rc) return fail(s, rc, std::string("closure: ") + c->err);
   launch_unpack(s->stream, c->y, s->nyu, s->nyv, s->ngz, s->ngx, c->scale + 3, C.Pc[0], C.Pc[1], C.Pc[2]);
-  void* nxz[3] = {s->nxz1, s->nxz2, s->nxz3};
-  const int nyf[3] = {s->nyu, s->nyv, s->nyu};
+  C.have = true;
   for (int g = 0; g < 3; ++g) {
     launch_xpass_fwd2(*s, C.d_xf[g], C.nxf[g]);
     launch_zfwd_plain(*s, C.d_zf[g], C.nxf[g]);
@@ -1161,7 +1175,7 @@ int op_nonlinear(cfd_solver* s, int kRK) {
   if (kRK == 1) launch_cfl(*s);
   const int wrc = stats_req_end(s, req);
   if (s->cl.cnn)
-    if (int rc = closure_apply(s)) return rc;
+    if (int rc = closure_apply(s, kRK)) return rc;
   { Timed t(s, KC_ELEM); launch_nlcombine(*s, kRK); }
   return wrc;
 }
@@ -1203,7 +1217,7 @@ int op_substep(cfd_solver* s, int kRK) {
   // a failed write (say, a missing directory) still completes the substep
   const int wrc = stats_req_end(s, req);
   if (s->cl.cnn)
-    if (int rc = closure_apply(s)) return rc;
+    if (int rc = closure_apply(s, kRK)) return rc;
   {
     Timed t(s, KC_IB);
     launch_ib_rhs(*s, kRK);
@@ -1550,19 +1564,24 @@ extern "C" int cfd_stats_dims(cfd_handle h, int* d) {
 // ---------------------------------------------------------------------------
 // learned closure
 // ---------------------------------------------------------------------------
-extern "C" int cfd_closure_attach(cfd_handle h, cfd_cnn c) {
+extern "C" int cfd_closure_attach(cfd_handle h, cfd_cnn c, int every) {
   cfd_solver* s = static_cast<cfd_solver*>(h);
   if (int rc = check(s, 1)) return rc;
   if (!c) {
     s->cl.cnn = nullptr;
+    s->cl.have = false;
     return CFD_OK;
   }
+  if (every < 1) return fail(s, CFD_EINVAL, "closure: every must be >= 1");
   if (c->NY != s->nyu || c->NZ != s->ngz || c->NX != s->ngx)
     return fail(s, CFD_EINVAL, "closure: the network volume must be nyu x Ngalz x Ngalx of this solver");
   for (int k = 0; k < 6; ++k)
     if (!(c->scale[k] > 0.0f)) return fail(s, CFD_EINVAL, "closure: the network has no normalisation scales");
   if (int rc = closure_setup(s)) return rc;
   s->cl.cnn = c;
+  s->cl.every = every;
+  s->cl.count = 0;
+  s->cl.have = false;
   return CFD_OK;
 }
 

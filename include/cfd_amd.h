@@ -270,16 +270,18 @@ int cfd_cnn_get_scales(cfd_cnn c, float* s6);
  * the coarse solver's fields are overwritten.  rms[6] (may be null) returns
  * the root mean squares of u1, u2, u3 and the three targets.
  *
- * cfd_closure_attach(h, c): from now on every nonlinear term of h (kRK
- * 1..3, cfd_nonlinear and cfd_substep) adds the network's correction of the
- * current state; c = NULL detaches.  The FP64 path only (precision 0); the
- * network volume must be nyu x Ngalz x Ngalx of h.
+ * cfd_closure_attach(h, c, every): from now on every nonlinear term of h
+ * (cfd_nonlinear, cfd_substep, cfd_step) adds the network's correction: the
+ * network evaluates the state in the nonlinear term of kRK = 1 of every
+ * `every`-th step (and in the first one after attaching) and the other
+ * substeps reuse that correction; c = NULL detaches.  The FP64 path only
+ * (precision 0); the network volume must be nyu x Ngalz x Ngalx of h.
  *
  * cfd_closure_phys(h, which, out): the physical planes of the last sample
  * or application: which 0..2 the velocities u1, u2, u3, 3..5 the
  * corrections (planes of the component's grid x Ngalz x Ngalx). */
 int cfd_closure_snapshot(cfd_handle fine, cfd_handle coarse, cfd_cnn c, int slot, double* rms);
-int cfd_closure_attach(cfd_handle h, cfd_cnn c);
+int cfd_closure_attach(cfd_handle h, cfd_cnn c, int every);
 int cfd_closure_phys(cfd_handle h, int which, double* out);
 
 #ifdef __cplusplus

@@ -27,6 +27,12 @@ WIDE768 = Params(ngalx=768, ngalz=384, ny=16, dsty=2, ygrid=1, ytanh_beta=2.0, g
                  ntilez=16, extras24=0, flag_ctpress=0, Re=10440.0, alp=1.0, bet=2.0, posth=0.1, r=1.1,
                  c_i=12.0, c_k=12.0, stag=0.5, CFL=0.5)
 
+# the lateral grid and posts of the 192 x 192 x 96 closure grid on a short
+# channel (four-step lengths z 96, x 192; posts of r = 0.62 cells)
+COARSE = Params(ngalx=192, ngalz=96, ny=24, dsty=3, ygrid=1, ytanh_beta=2.0, geometry_type=3, ntilex=16,
+                ntilez=8, extras24=0, flag_ctpress=0, Re=6887.0, alp=1.0, bet=2.0, posth=0.12, r=0.619,
+                c_i=6.25, c_k=6.25, stag=0.5, CFL=0.5)
+
 SMOOTH = Params(ngalx=36, ngalz=24, ny=25, dsty=0, ygrid=0, geometry_type=0, ntilex=2, ntilez=2, Re=2000.0,
                 alp=1.0, bet=2.0, posth=0.0, CFL=0.5)
 

@@ -160,6 +160,25 @@ def test_attached_network_corrects_the_nonlinear_term():
             d = b.get_field(n) - a.get_field(n)
             e = _spec(COARSE, corr[k])
             assert _rel(d[:, 1:-1], e[:, 1:-1]) <= 1e-10, (n, _rel(d[:, 1:-1], e[:, 1:-1]))
+        # attached with every = 2: the network evaluates at kRK = 1 of steps
+        # 0, 2, ... and the other substeps reuse the held correction, so the
+        # kRK = 2 term moves by that same correction
+        _load(a, f)
+        _load(b, f)
+        b.closure_attach(net, every=2)
+        b.nonlinear(1)
+        held = [b.closure_phys(3 + k) for k in range(3)]
+        _load(b, f)
+        b.set_field("u1", b.get_field("u1") * 1.01)  # a different state: no new evaluation
+        a.set_field("u1", a.get_field("u1") * 1.01)
+        a.nonlinear(2)
+        b.nonlinear(2)
+        b.closure_attach(None)
+        for k, n in enumerate(NL):
+            assert (b.closure_phys(3 + k) == held[k]).all()
+            d = b.get_field(n) - a.get_field(n)
+            e = _spec(COARSE, held[k])
+            assert _rel(d[:, 1:-1], e[:, 1:-1]) <= 1e-10, (n, _rel(d[:, 1:-1], e[:, 1:-1]))
     finally:
         net.close()
         a.close()

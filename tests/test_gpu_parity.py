@@ -14,11 +14,12 @@ import pytest
 from cfd_amd import Channel
 from oracle.oracle import Oracle
 
-from .helpers import EDGE, SHIPPED, SMALLPOSTS, SMOOTH, WIDE, WIDE768, random_field, rel_l2, max_rel
+from .helpers import COARSE, EDGE, SHIPPED, SMALLPOSTS, SMOOTH, WIDE, WIDE768, random_field, rel_l2, max_rel
 
 pytestmark = pytest.mark.gpu
 
-CASES = {"shipped": SHIPPED, "smallposts": SMALLPOSTS, "smooth": SMOOTH, "wide": WIDE, "edge": EDGE}
+CASES = {"shipped": SHIPPED, "smallposts": SMALLPOSTS, "smooth": SMOOTH, "wide": WIDE, "edge": EDGE,
+         "coarse": COARSE}
 STATE = ("u1", "u2", "u3", "p", "Nu1", "Nu2", "Nu3", "du1", "du2", "du3")
 TOL_OP = 1e-11
 
@@ -218,12 +219,14 @@ def _channel(p, fft=None):
         os.environ.pop("CFD_AMD_FFT", None)
 
 
-def test_four_step_fft_equals_stockham_at_workload_size():
+@pytest.mark.parametrize("workload", ["rt395_384x192x192", "rt395_coarse_192x192x96"])
+def test_four_step_fft_equals_stockham_at_workload_size(workload):
     """The four-step register FFT kernels and the LDS Stockham kernels give
-    the same three substeps on the full 384 x 192 x 192 workload."""
+    the same three substeps on the full 384 x 192 x 192 workload and on the
+    192 x 192 x 96 closure grid."""
     from cfd_amd import ic, workloads
-    p, _ = workloads.get("rt395_384x192x192")
-    print("case rt395_384x192x192 four-step vs stockham", flush=True)
+    p, _ = workloads.get(workload)
+    print("case %s four-step vs stockham" % workload, flush=True)
     a = _channel(p)
     b = _channel(p, "stockham")
     f = ic.initial_fields(p, a.ygrid(), seed=3, amp=0.1)
