@@ -1019,7 +1019,7 @@ int closure_setup(cfd_solver* s) {
   if (C.ready) return CFD_OK;
   if (s->mixed) return fail(s, CFD_EINVAL, "closure: the learned closure runs on the FP64 path (precision 0)");
   if (!cnn_dims_ok(s->nyu, s->ngz, s->ngx))
-    return fail(s, CFD_EINVAL, "closure: Ngalx must be a multiple of 32 and Ngalz of 4");
+    return fail(s, CFD_EINVAL, "closure: Ngalx must be a multiple of 32 and Ngalz of 8");
   const size_t zpl = (size_t)s->ngz * s->Mx, ppl = (size_t)s->ngz * s->ngx;
   const int nyf[3] = {s->nyu, s->nyv, s->nyu};
   int rc;

@@ -112,7 +112,7 @@ extern "C" int cfd_cnn_create(int device, int ny, int nz, int nx, cfd_cnn* out) 
   if (!out) return cfail(nullptr, CFD_EINVAL, "null argument");
   *out = nullptr;
   if (!cnn_dims_ok(ny, nz, nx))
-    return cfail(nullptr, CFD_EINVAL, "volume: nx must be a multiple of 32 and nz of 4");
+    return cfail(nullptr, CFD_EINVAL, "volume: nx must be a multiple of 32 and nz of 8");
   cfd_cnn_s* c = new cfd_cnn_s();
   auto bail = [&](int rc) {
     g_cnn_err = c->err.empty() ? g_cnn_err : c->err;

@@ -96,62 +96,143 @@ __device__ __forceinline__ void stage_halo(__bf16* hal, const __bf16* __restrict
 
 }  // namespace
 
+// Tiles of the forward / backward-data convolution: 8 z rows per block, two
+// per wave, so every weight fragment feeds two independent accumulators.
+constexpr int kTZF = 8;
+constexpr int kHZF = kTZF + 2;
+constexpr int kYB = 8;  // planes walked by one block
+
+// One plane of a block's halo (kHZF rows x kHX points), loaded into
+// registers by all threads and stored to an LDS ring slot later, so the
+// global loads of the next plane are in flight while the MFMAs of the
+// current plane run.  Slot layout [hz * kHX + hx][c], point stride CP.
+template <int CIN, int CG>
+struct PlaneLoad {
+  static constexpr int CP = Lds<CIN>::CP;
+  static constexpr int NPL = kHX * kHZF;
+  static constexpr int NCH = CG == CIN ? CIN / 8 : 1;
+  static constexpr int NITEM = NPL * NCH;
+  static constexpr int PER = (NITEM + 255) / 256;
+  bf16x8 r[PER];
+
+  __device__ __forceinline__ void load(const __bf16* __restrict__ in, int x0, int z0, int yy, int NY, int NZ,
+                                       int NX) {
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int w = threadIdx.x + 256 * k;
+      for (int i = 0; i < 8; ++i) r[k][i] = (__bf16)0.0f;
+      if (w >= NITEM || yy < 0 || yy >= NY) continue;
+      const int pnt = w / NCH, ch = w - pnt * NCH;
+      const int hx = pnt % kHX, hz = pnt / kHX;
+      const int xx = wrap(x0 + hx - 1, NX), zz = wrap(z0 + hz - 1, NZ);
+      const __bf16* src = in + (((size_t)yy * NZ + zz) * NX + xx) * CG;
+      if constexpr (CG == CIN) {
+        r[k] = *(const bf16x8*)(src + 8 * ch);
+      } else {
+        for (int c = 0; c < CG; ++c) r[k][c] = src[c];
+      }
+    }
+  }
+  __device__ __forceinline__ void store(__bf16* slot) const {
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int w = threadIdx.x + 256 * k;
+      if (w >= NITEM) continue;
+      const int pnt = w / NCH, ch = w - pnt * NCH;
+      *(bf16x8*)(slot + (size_t)pnt * CP + 8 * ch) = r[k];
+      if constexpr (CG != CIN) {
+        bf16x8 z;
+        for (int i = 0; i < 8; ++i) z[i] = (__bf16)0.0f;
+        for (int c = 8; c < CIN; c += 8) *(bf16x8*)(slot + (size_t)pnt * CP + c) = z;
+      }
+    }
+  }
+};
+
 // Convolution of one layer.  MODE 0: z = conv + b -> zout (BF16), a = gelu(z)
 // -> aout (BF16).  MODE 1: y = conv + b -> yout (FP32, channels < nout).
 // MODE 2 (backward data): taps flipped, weights W'[t][ci][co] given as the
 // operand layout, d = conv * gelu'(zprev) -> aout (BF16); no bias.
-// Weights w: [27][32 (n)][CIN (k)] BF16.  Grid (NX/32, NZ/4, NY), 256 threads.
+// Weights w: [27][32 (n)][CIN (k)] BF16, staged in LDS once per block.  A
+// block walks kYB planes of one (x, 8 z rows) tile with its halo in a ring
+// of three plane slots; the plane two ahead is loaded into registers during
+// the MFMAs of the current one.  Grid (NX/32, NZ/8, ceil(NY/kYB)), 256
+// threads.
 template <int CIN, int CG, int MODE>
 __global__ void __launch_bounds__(256) k_conv(const __bf16* __restrict__ in, const __bf16* __restrict__ w,
                                               const float* __restrict__ bias, const __bf16* __restrict__ zprev,
                                               __bf16* __restrict__ zout, __bf16* __restrict__ aout,
                                               float* __restrict__ yout, int nout, int NY, int NZ, int NX) {
   constexpr int CP = Lds<CIN>::CP;
+  constexpr int SLOT = kHX * kHZF * CP;
   extern __shared__ __align__(16) unsigned char smem[];
-  __bf16* hal = (__bf16*)smem;                 // kHalo * CP
-  __bf16* wl = hal + (size_t)kHalo * CP;       // 27 * 32 * CP
-  const int x0 = blockIdx.x * kTX, z0 = blockIdx.y * kTZ, y = blockIdx.z;
-  stage_halo<CIN, CG>(hal, in, x0, z0, y, NY, NZ, NX);
+  __bf16* ring = (__bf16*)smem;                // 3 * SLOT
+  __bf16* wl = ring + (size_t)3 * SLOT;        // 27 * 32 * CP
+  const int x0 = blockIdx.x * kTX, z0 = blockIdx.y * kTZF;
+  const int y0 = blockIdx.z * kYB, y1 = min(y0 + kYB, NY);
   for (int e = threadIdx.x; e < 27 * kC * (CIN / 8); e += blockDim.x) {
     const int ch = e % (CIN / 8), row = e / (CIN / 8);  // row = t * 32 + n
     *(bf16x8*)(wl + (size_t)row * CP + 8 * ch) = *(const bf16x8*)(w + (size_t)row * CIN + 8 * ch);
   }
+  PlaneLoad<CIN, CG> pl;
+  for (int d = -1; d <= 1; ++d) {
+    pl.load(in, x0, z0, y0 + d, NY, NZ, NX);
+    pl.store(ring + (size_t)((y0 + d + 3) % 3) * SLOT);
+  }
   __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int m = lane & 31, kh = lane >> 5;
-  f32x16 acc = {};
-  for (int t = 0; t < 27; ++t) {
-    int dy = t / 9 - 1, dz = (t / 3) % 3 - 1, dx = t % 3 - 1;
-    if (MODE == 2) {
-      dy = -dy;
-      dz = -dz;
-      dx = -dx;
-    }
-    const __bf16* arow = hal + (size_t)(((1 + dy) * kHZ + (1 + wv + dz)) * kHX + (1 + m + dx)) * CP;
-    const __bf16* brow = wl + (size_t)(t * kC + m) * CP;
-#pragma unroll
-    for (int s = 0; s < CIN / 16; ++s) {
-      const bf16x8 a = *(const bf16x8*)(arow + 16 * s + 8 * kh);
-      const bf16x8 b = *(const bf16x8*)(brow + 16 * s + 8 * kh);
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
-    }
-  }
-  // epilogue: lane holds channel n = m of points 8 (v/4) + 4 kh + v % 4
   const int n = m;
-  const int z = z0 + wv;
-  const size_t rowp = ((size_t)y * NZ + z) * NX + x0;
+  const float bn = MODE == 2 ? 0.0f : bias[n];
+  for (int y = y0; y < y1; ++y) {
+    const bool more = y + 1 < y1;
+    if (more) pl.load(in, x0, z0, y + 2, NY, NZ, NX);
+    f32x16 acc0 = {}, acc1 = {};  // z rows 2 wv and 2 wv + 1 of the tile
 #pragma unroll
-  for (int v = 0; v < 16; ++v) {
-    const int pm = 8 * (v / 4) + 4 * kh + (v & 3);
-    const size_t p = rowp + pm;
-    if (MODE == 0) {
-      const float zz = acc[v] + bias[n];
-      zout[p * kC + n] = (__bf16)zz;
-      aout[p * kC + n] = (__bf16)gelu(zz);
-    } else if (MODE == 1) {
-      if (n < nout) yout[p * nout + n] = acc[v] + bias[n];
-    } else {
-      aout[p * kC + n] = (__bf16)(acc[v] * gelu_d((float)zprev[p * kC + n]));
+    for (int t = 0; t < 27; ++t) {
+      int dy = t / 9 - 1, dz = (t / 3) % 3 - 1, dx = t % 3 - 1;
+      if (MODE == 2) {
+        dy = -dy;
+        dz = -dz;
+        dx = -dx;
+      }
+      const __bf16* arow = ring + (size_t)((y + dy + 3) % 3) * SLOT +
+                           (size_t)((1 + 2 * wv + dz) * kHX + (1 + m + dx)) * CP;
+      const __bf16* brow = wl + (size_t)(t * kC + m) * CP;
+#pragma unroll
+      for (int s = 0; s < CIN / 16; ++s) {
+        const bf16x8 b = *(const bf16x8*)(brow + 16 * s + 8 * kh);
+        const bf16x8 a0 = *(const bf16x8*)(arow + 16 * s + 8 * kh);
+        const bf16x8 a1 = *(const bf16x8*)(arow + (size_t)kHX * CP + 16 * s + 8 * kh);
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b, acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b, acc1, 0, 0, 0);
+      }
+    }
+    // epilogue: lane holds channel n = m of points 8 (v/4) + 4 kh + v % 4
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const int z = z0 + 2 * wv + r;
+      const size_t rowp = ((size_t)y * NZ + z) * NX + x0;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int pm = 8 * (v / 4) + 4 * kh + (v & 3);
+        const size_t p = rowp + pm;
+        const float acc = r == 0 ? acc0[v] : acc1[v];
+        if (MODE == 0) {
+          const float zz = acc + bn;
+          zout[p * kC + n] = (__bf16)zz;
+          aout[p * kC + n] = (__bf16)gelu(zz);
+        } else if (MODE == 1) {
+          if (n < nout) yout[p * nout + n] = acc + bn;
+        } else {
+          aout[p * kC + n] = (__bf16)(acc * gelu_d((float)zprev[p * kC + n]));
+        }
+      }
+    }
+    if (more) {
+      __syncthreads();  // every wave is done with plane y - 1
+      pl.store(ring + (size_t)((y + 2) % 3) * SLOT);
+      __syncthreads();
     }
   }
 }
@@ -225,13 +306,15 @@ __global__ void __launch_bounds__(256) k_conv_dw(const __bf16* __restrict__ in, 
   if (wv == 0 && kh == 0) out[27 * kC * kC + m] = dbs;
 }
 
-// sum of the per-block partials: g[e] = sum_b partial[b][e]
+// sum of the per-block partials: g[e] = sum_b partial[b][e]; block y sums
+// the partials b = y, y + gridDim.y, ... and adds its sum atomically (the
+// order of the additions, and so the last bits of g, vary between runs)
 __global__ void k_reduce_partials(const float* __restrict__ partial, int nblk, int n, float* __restrict__ g) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   float s = 0.0f;
-  for (int b = 0; b < nblk; ++b) s += partial[(size_t)b * n + e];
-  g[e] = s;
+  for (int b = blockIdx.y; b < nblk; b += gridDim.y) s += partial[(size_t)b * n + e];
+  atomicAdd(g + e, s);
 }
 
 // loss and output gradient: L = sum over points and the nout channels of
@@ -302,7 +385,7 @@ namespace {
 
 template <int CIN>
 constexpr size_t conv_lds() {
-  return ((size_t)kHalo * Lds<CIN>::CP + (size_t)27 * kC * Lds<CIN>::CP) * sizeof(__bf16);
+  return ((size_t)3 * kHX * kHZF * Lds<CIN>::CP + (size_t)27 * kC * Lds<CIN>::CP) * sizeof(__bf16);
 }
 template <int CIN>
 constexpr size_t dw_lds() {
@@ -322,11 +405,11 @@ void lds_attr(K kern, size_t bytes) {
 
 }  // namespace
 
-bool cnn_dims_ok(int NY, int NZ, int NX) { return NY >= 1 && NZ % kTZ == 0 && NX % kTX == 0 && NZ > 0 && NX > 0; }
+bool cnn_dims_ok(int NY, int NZ, int NX) { return NY >= 1 && NZ % kTZF == 0 && NX % kTX == 0 && NZ > 0 && NX > 0; }
 
 void launch_conv_fwd(hipStream_t st, int layer, const __bf16* in, const __bf16* wf, const float* bias, __bf16* zout,
                      __bf16* aout, float* yout, int nout, int NY, int NZ, int NX) {
-  const dim3 grid(NX / kTX, NZ / kTZ, NY);
+  const dim3 grid(NX / kTX, NZ / kTZF, (NY + kYB - 1) / kYB);
   if (layer == 0) {
     auto k = k_conv<16, 4, 0>;
     lds_attr(k, conv_lds<16>());
@@ -347,7 +430,7 @@ void launch_conv_fwd(hipStream_t st, int layer, const __bf16* in, const __bf16* 
 
 void launch_conv_bwd_data(hipStream_t st, const __bf16* dz, const __bf16* wb, const __bf16* zprev, __bf16* dprev,
                           int NY, int NZ, int NX) {
-  const dim3 grid(NX / kTX, NZ / kTZ, NY);
+  const dim3 grid(NX / kTX, NZ / kTZF, (NY + kYB - 1) / kYB);
   auto k = k_conv<32, 32, 2>;
   lds_attr(k, conv_lds<32>());
   hipLaunchKernelGGL(k, grid, dim3(256), conv_lds<32>(), st, dz, wb, nullptr, zprev, nullptr, dprev, nullptr, 0, NY,
@@ -366,7 +449,8 @@ void launch_conv_dw(hipStream_t st, int layer, const __bf16* in, const __bf16* d
     hipLaunchKernelGGL(k, dim3(nblk), dim3(256), dw_lds<32>(), st, in, dz, partial, NY, NZ, NX);
   }
   const int n = 27 * kC * kC + kC;
-  hipLaunchKernelGGL(k_reduce_partials, dim3((n + 255) / 256), dim3(256), 0, st, partial, nblk, n, grad);
+  (void)hipMemsetAsync(grad, 0, (size_t)n * sizeof(float), st);
+  hipLaunchKernelGGL(k_reduce_partials, dim3((n + 255) / 256, 16), dim3(256), 0, st, partial, nblk, n, grad);
 }
 
 void launch_loss(hipStream_t st, const float* y, const __bf16* tgt, int tc, int nout, size_t npts, __bf16* dy,

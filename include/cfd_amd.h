@@ -222,7 +222,7 @@ int cfd_profile_read(cfd_handle h, double* ms, int* cnt);
 /* ------------------------------------------------------------------------
  * Learned closure network (no counterpart in the reference: the learned
  * coarse-grid correction of the nonlinear term).  A 3D CNN over a volume of
- * ny planes x nz x nx points (nx a multiple of 32, nz of 4), periodic in x
+ * ny planes x nz x nx points (nx a multiple of 32, nz of 8), periodic in x
  * and z, zero beyond the planes: four 3x3x3 convolutions 4 -> 32 -> 32 -> 32
  * -> 3 channels with GELU after the first three, BF16 weights and
  * activations, FP32 accumulation and FP32 master weights trained by Adam.
