@@ -4,13 +4,18 @@ The network (cfd_amd.cnn, BF16 MFMA kernels) sees the coarse velocity and
 the IB mask and predicts filt(N(u)) - N_coarse(filt(u)), the part of the
 box-filtered fine nonlinear term that the coarse grid misses.  Samples come
 from a fine FP64 run (cfd_closure_snapshot); the trained network is attached
-to the coarse solver, which then adds the correction in every nonlinear term.
+to the coarse solver, which then adds the correction in its nonlinear terms.
 
-    python -m cfd_amd.closure [--snapshots 2000] [--steps 5000] [--out DIR]
+    python -m cfd_amd.closure [--samples 2000] [--every 5] [--out DIR]
 
-runs the whole experiment at Re_tau 395: fine DNS 384 x 192 x 192 as the
-data source and the judge, coarse 192 x 192 x 96 with and without the
-closure, and writes closure_weights.npz and closure_report.json.
+runs the experiment at Re_tau 395: the fine DNS on 384 x 192 x 192 produces
+the samples (one every `every` steps, trained on online from a device ring
+buffer) and is the judge; the coarse solver on 192 x 192 x 96 then runs from
+the box-filtered start field over the same simulated time without and with
+the closure.  Writes closure_weights.npz and closure_report.json: a-priori
+loss on held-out samples against predicting no correction, drag (the mean
+pressure gradient at constant flow rate) and mean-profile errors against the
+fine run, and simulated time per wall-clock second of each run.
 """
 import argparse
 import ctypes
@@ -37,15 +42,16 @@ def start_fields(ch, params, seed=0, amp=0.1):
     """Seeded laminar + perturbation field, projected, loaded into ch."""
     g = ch.ygrid()
     f = ic.project(ch, ic.initial_fields(params, g, seed=seed, amp=amp))
+    q0 = ic.flowrate(f["u1"], g, params.ny)
     for k in ("u1", "u2", "u3", "p"):
         ch.set_field(k, f[k])
-    ch.set_scalars(t=0.0, dt=0.0, mpgx=0.0, QxT=ic.flowrate(f["u1"], g, params.ny))
-    return f
+    ch.set_scalars(t=0.0, dt=0.0, mpgx=0.0, QxT=q0)
+    return f, q0
 
 
 def filtered_start(coarse_params, fine_fields, fine_params):
-    """The fine start field box-filtered onto the coarse modes (host side,
-    the same filter as the device's)."""
+    """Fine spectral fields box-filtered onto the coarse modes (host side,
+    the same filter as the device's: H(k) = sinc(k dx_c / 2))."""
     N1f, N2f = 2 * (fine_params.ngalx // 3), 2 * (fine_params.ngalz // 3)
     N1c, N2c = 2 * (coarse_params.ngalx // 3), 2 * (coarse_params.ngalz // 3)
     Mxf, Mxc = N1f // 2 + 1, N1c // 2 + 1
@@ -62,59 +68,179 @@ def filtered_start(coarse_params, fine_fields, fine_params):
     return out
 
 
-def drag(ch):
-    """Mean pressure gradient (the drag per unit volume at constant flow rate)."""
-    return -ch.scalars()["mpgx"]
+def run_until(ch, t_end, max_steps, drag_cap):
+    """Step ch until t >= t_end, or until the drag leaves [-drag_cap,
+    drag_cap] or stops being finite (a blown-up run); returns (t, drag) per
+    step and the wall time."""
+    hist = []
+    ch.sync()
+    w0 = time.perf_counter()
+    for _ in range(max_steps):
+        ch.step(1)
+        s = ch.scalars()
+        d = -s["mpgx"]
+        if not (np.isfinite(d) and abs(d) <= drag_cap):
+            break
+        hist.append((s["t"], d))
+        if s["t"] >= t_end:
+            break
+    ch.sync()
+    return np.array(hist).reshape(-1, 2), time.perf_counter() - w0
+
+
+def compare(hist, ref, t0, t1):
+    """Drag of hist against ref (both (t, drag) arrays) over t0 <= t <= t1:
+    relative error of the mean and RMS of the pointwise difference."""
+    m = (hist[:, 0] >= t0) & (hist[:, 0] <= t1)
+    mr = (ref[:, 0] >= t0) & (ref[:, 0] <= t1)
+    if not m.any():
+        return {"window": [t0, t1], "mean_drag": None, "drag_error": None, "drag_rms_error": None}
+    a, b = hist[m, 1].mean(), ref[mr, 1].mean()
+    rb = np.interp(hist[m, 0], ref[:, 0], ref[:, 1])
+    return {"window": [float(t0), float(t1)], "mean_drag": float(a), "mean_drag_ref": float(b),
+            "drag_error": float(abs(a - b) / abs(b)),
+            "drag_rms_error": float(np.sqrt(np.mean((hist[m, 1] - rb) ** 2)) / abs(b))}
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--snapshots", type=int, default=2000)
-    ap.add_argument("--every", type=int, default=2, help="fine steps between samples")
+    ap.add_argument("--samples", type=int, default=2000)
+    ap.add_argument("--every", type=int, default=5, help="fine steps between samples")
     ap.add_argument("--slots", type=int, default=256, help="device ring buffer of samples")
-    ap.add_argument("--train-per-sample", type=int, default=1)
+    ap.add_argument("--train-per-sample", type=int, default=2)
     ap.add_argument("--lr", type=float, default=1e-3)
-    ap.add_argument("--steps", type=int, default=5000, help="steps of the a-posteriori runs")
+    ap.add_argument("--held-out", type=int, default=8)
+    ap.add_argument("--closure-every", type=int, default=2, help="coarse steps between network evaluations")
+    ap.add_argument("--max-coarse-steps", type=int, default=20000)
+    ap.add_argument("--coarse-start", type=float, default=0.0,
+                    help="fraction of the fine run's time at which the coarse runs start from its filtered state")
     ap.add_argument("--out", default=".")
     ap.add_argument("--seed", type=int, default=0)
     a = ap.parse_args(argv)
     pf, _ = workloads.get("rt395_384x192x192")
     pc, _ = workloads.get("rt395_coarse_192x192x96")
-    fine, coarse = Channel(pf), Channel(pc)
-    net = cnn.CNN(coarse.nyu, pc.ngalz, pc.ngalx)
+    fine, scratch = Channel(pf), Channel(pc)
+    net = cnn.CNN(scratch.nyu, pc.ngalz, pc.ngalx)
     net.set_params(cnn.init_params(seed=a.seed))
-    net.dataset(a.slots)
-    f0 = start_fields(fine, pf, seed=0)
+    net.dataset(a.slots + a.held_out)
     rng = np.random.default_rng(a.seed)
-    report = {"snapshots": a.snapshots, "every": a.every, "slots": a.slots}
-    t0 = time.perf_counter()
-    losses, held = [], []
+    report = {"samples": a.samples, "every": a.every, "slots": a.slots, "train_per_sample": a.train_per_sample,
+              "closure_every": a.closure_every}
+    # pass 1: the fine run alone, probing the sample magnitudes along the
+    # trajectory; the normalisation takes the largest RMS of each quantity
+    f0, q0 = start_fields(fine, pf, seed=0)
+    probe = cnn.CNN(scratch.nyu, pc.ngalz, pc.ngalx)
+    probe.dataset(1)
+    nsteps = a.samples * a.every
+    rmax = np.zeros(6)
+    for k in range(nsteps):
+        fine.step(1)
+        if (k + 1) % max(nsteps // 24, 1) == 0:
+            rmax = np.maximum(rmax, snapshot(fine, scratch, probe, 0))
+    probe.close()
+    report["scales_max_rms"] = rmax.tolist()
+    net.set_scales(rmax.astype(np.float32))
+    print("scales", rmax, flush=True)
+    # pass 2: the same fine run from the same start: samples, online
+    # training, and the judge's drag history
+    f0, q0 = start_fields(fine, pf, seed=0)
+    k_start = int(a.coarse_start * a.samples)
+    fs, ts = (f0, 0.0), None
+    hist_f = []
+    losses = []
     filled = 0
-    for k in range(a.snapshots):
-        fine.step(a.every)
+    w0 = time.perf_counter()
+    wfine = 0.0
+    for k in range(a.samples):
+        t1 = time.perf_counter()
+        for _ in range(a.every):
+            fine.step(1)
+            s = fine.scalars()
+            hist_f.append((s["t"], -s["mpgx"]))
+        wfine += time.perf_counter() - t1
+        if k + 1 == k_start:
+            fs = ({n: fine.get_field(n) for n in ("u1", "u2", "u3", "p")}, fine.scalars()["t"])
         slot = k % a.slots
-        rms = snapshot(fine, coarse, net, slot)
+        rms = snapshot(fine, scratch, net, slot)
         filled = min(filled + 1, a.slots)
         if k == 0:
             report["rms_first"] = rms.tolist()
-            report["scales"] = net.scales().tolist()
-        # held-out check before training on the new sample
-        if k >= 10 and k % 50 == 0:
-            held.append((k, net.eval_slot(slot)))
-            print("sample %5d  t %.4f  held-out loss %.4f  train loss %.4f" % (
-                k, fine.scalars()["t"], held[-1][1], np.mean(losses[-50:]) if losses else float("nan")), flush=True)
         for _ in range(a.train_per_sample):
             s = slot if rng.random() < 0.5 else int(rng.integers(0, filled))
             losses.append(net.train_slot(s, a.lr))
-    report["train_seconds"] = time.perf_counter() - t0
-    report["held_out"] = held
-    report["train_loss_last100"] = float(np.mean(losses[-100:]))
+        if k % 100 == 0:
+            print("sample %5d  t %.3f  drag %.5f  train loss %.4f  (%.0f s)" % (
+                k, hist_f[-1][0], hist_f[-1][1], np.mean(losses[-100:]), time.perf_counter() - w0), flush=True)
+    report["train_seconds"] = time.perf_counter() - w0 - wfine
+    report["train_loss_last200"] = float(np.mean(losses[-200:]))
+    # held-out samples further along the fine run
+    zero = cnn.CNN(scratch.nyu, pc.ngalz, pc.ngalx)
+    zero.set_params(np.zeros(cnn.nparams(), np.float32))
+    zero.set_scales(net.scales())
+    zero.dataset(a.held_out)
+    held = []
+    for k in range(a.held_out):
+        for _ in range(a.every):
+            fine.step(1)
+            s = fine.scalars()
+            hist_f.append((s["t"], -s["mpgx"]))
+        snapshot(fine, scratch, net, a.slots + k)
+        snapshot(fine, scratch, zero, k)
+        held.append((net.eval_slot(a.slots + k), zero.eval_slot(k)))
+    held = np.array(held)
+    report["held_out_loss"] = float(held[:, 0].mean())
+    report["held_out_loss_no_correction"] = float(held[:, 1].mean())
+    report["held_out_gain"] = float(held[:, 1].mean() / held[:, 0].mean())
+    print("held-out loss %.4f vs %.4f without correction" % (held[:, 0].mean(), held[:, 1].mean()), flush=True)
+    hist_f = np.array(hist_f)
+    t_end = float(hist_f[-1, 0])
+    # simulated time per second of the fine steps of the sampling loop alone
+    report["fine"] = {"steps": len(hist_f), "t_end": t_end, "wall_s": wfine,
+                      "sim_time_per_s": float(hist_f[a.samples * a.every - 1, 0] / max(wfine, 1e-9))}
     w = net.params()
     np.savez(os.path.join(a.out, "closure_weights.npz"), params=w, scales=net.scales())
+    zero.close()
+    scratch.close()
+    # the coarse runs from the filtered start field, without and with the
+    # closure, over the fine run's simulated time (or until they blow up)
+    fc = filtered_start(pc, fs[0], pf)
+    t_start = fs[1]
+    report["coarse_start_t"] = t_start
+    cap = 50.0 * float(np.abs(hist_f[:, 1]).max())
+    runs, prof = {}, {}
+    for label, attach in (("coarse", False), ("coarse_closure", True)):
+        ch = Channel(pc)
+        g = ch.ygrid()
+        for kname in ("u1", "u2", "u3", "p"):
+            ch.set_field(kname, fc[kname])
+        ch.set_scalars(t=t_start, dt=0.0, mpgx=-float(np.interp(t_start, hist_f[:, 0], hist_f[:, 1])) if t_start else 0.0,
+                       QxT=ic.flowrate(fc["u1"], g, pc.ny))
+        if attach:
+            ch.closure_attach(net, every=a.closure_every)
+        hist, wall = run_until(ch, t_end, a.max_coarse_steps, cap)
+        t_last = float(hist[-1, 0]) if len(hist) else 0.0
+        runs[label] = {"hist": hist, "steps": len(hist), "wall_s": wall, "t_reached": t_last,
+                       "completed": bool(t_last >= t_end),
+                       "sim_time_per_s": float((t_last - t_start) / wall) if wall > 0 else 0.0}
+        runs[label]["speedup_vs_fine"] = runs[label]["sim_time_per_s"] / report["fine"]["sim_time_per_s"]
+        prof[label] = ch.get_field("u1")[0].real if runs[label]["completed"] else None
+        ch.closure_attach(None)
+        ch.close()
+    t1 = min(r["t_reached"] for r in runs.values())
+    for label, r in runs.items():
+        hist = r.pop("hist")
+        r["drag"] = compare(hist, hist_f, t_start + 0.25 * (t1 - t_start), t1)
+        r["drag_own_horizon"] = compare(hist, hist_f, t_start + 0.25 * (r["t_reached"] - t_start), r["t_reached"])
+        if prof[label] is not None:
+            uf = fine.get_field("u1")[0].real
+            r["mean_profile_error"] = float(np.linalg.norm(prof[label] - uf) / np.linalg.norm(uf))
+        print(label, json.dumps(r), flush=True)
+    report.update(runs)
     fine.close()
-    coarse.close()
-    print(json.dumps(report), flush=True)
     net.close()
+    with open(os.path.join(a.out, "closure_report.json"), "w") as fh:
+        json.dump(report, fh, indent=1)
+    print(json.dumps(report), flush=True)
     return report
 
 
