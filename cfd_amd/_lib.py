@@ -87,6 +87,17 @@ SIGNATURES = {
     "cfd_closure_snapshot": (ctypes.c_int, [_H, _H, _H, ctypes.c_int, _D]),
     "cfd_closure_attach": (ctypes.c_int, [_H, _H, ctypes.c_int]),
     "cfd_closure_phys": (ctypes.c_int, [_H, ctypes.c_int, _D]),
+    "cfd_create_tuned": (ctypes.c_int, [ctypes.POINTER(CParams), ctypes.POINTER(ctypes.c_int),
+                                        ctypes.POINTER(ctypes.c_void_p)]),
+    "cfd_get_tuning": (ctypes.c_int, [_H, ctypes.POINTER(ctypes.c_int)]),
+    "cfd_set_tuning": (ctypes.c_int, [_H, ctypes.POINTER(ctypes.c_int)]),
+    "cfd_ensemble_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
+                                           ctypes.POINTER(ctypes.c_void_p)]),
+    "cfd_ensemble_destroy": (ctypes.c_int, [_H]),
+    "cfd_ensemble_last_error": (ctypes.c_char_p, [_H]),
+    "cfd_ensemble_substep": (ctypes.c_int, [_H, ctypes.c_int]),
+    "cfd_ensemble_step": (ctypes.c_int, [_H, ctypes.c_int]),
+    "cfd_ensemble_sync": (ctypes.c_int, [_H]),
     "cfd_stats": (ctypes.c_int, [_H]),
     "cfd_write_stats": (ctypes.c_int, [_H, ctypes.c_char_p, ctypes.c_int]),
     "cfd_stats_request": (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]),

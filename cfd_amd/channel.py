@@ -56,15 +56,33 @@ def build_iblist(params, grid, wall):
     return l9, w3
 
 
+TUNING_KEYS = ("fft", "seg_rows", "solve_threads")
+TUNING_DEFAULT = dict(fft=0, seg_rows=13, solve_threads=256)
+
+
+def _tuning_c(t):
+    bad = set(t) - set(TUNING_KEYS)
+    if bad:
+        raise ValueError("unknown tuning keys %s" % sorted(bad))
+    d = dict(TUNING_DEFAULT, **t)
+    return (ctypes.c_int * 3)(*(int(d[k]) for k in TUNING_KEYS))
+
+
 class Channel:
     """One rough channel resident on one GPU."""
 
-    def __init__(self, params=None, **kw):
+    def __init__(self, params=None, tuning=None, **kw):
+        """tuning: None (defaults) or a dict with keys fft (0 four-step, 1
+        Stockham), seg_rows (5, 9, 13, 21) and solve_threads (64, 128, 256);
+        missing keys take the defaults 0, 13, 256."""
         self.params = (params or Params()).replace(**kw) if kw else (params or Params())
         self._lib = _lib.load()
         self._c = self.params.to_c()
         h = ctypes.c_void_p()
-        rc = self._lib.cfd_create(ctypes.byref(self._c), ctypes.byref(h))
+        if tuning is None:
+            rc = self._lib.cfd_create(ctypes.byref(self._c), ctypes.byref(h))
+        else:
+            rc = self._lib.cfd_create_tuned(ctypes.byref(self._c), _tuning_c(tuning), ctypes.byref(h))
         _lib.check(rc, None)
         self._h = h
         d = (ctypes.c_int * 16)()
@@ -240,6 +258,15 @@ class Channel:
         return out
 
     # -- timing ----------------------------------------------------------------
+    def tuning(self):
+        t = (ctypes.c_int * 3)()
+        self._call("cfd_get_tuning", t)
+        return dict(zip(TUNING_KEYS, list(t)))
+
+    def set_tuning(self, **t):
+        """Change seg_rows / solve_threads (the FFT engine is fixed)."""
+        self._call("cfd_set_tuning", _tuning_c(dict(self.tuning(), **t)))
+
     def profile(self, enable=True):
         self._call("cfd_profile", 1 if enable else 0)
 

@@ -188,7 +188,49 @@ struct Timed {
 // ---------------------------------------------------------------------------
 // create / destroy
 // ---------------------------------------------------------------------------
-extern "C" int cfd_create(const cfd_params* pp, cfd_handle* out) {
+namespace {
+int tuning_check(const cfd_tuning& t, std::string& err) {
+  if (t.fft != 0 && t.fft != 1) err = "tuning: fft must be 0 (four-step where it applies) or 1 (Stockham)";
+  else if (!seg_rows_supported(t.seg_rows)) err = "tuning: seg_rows must be 5, 9, 13 or 21";
+  else if (t.solve_threads != 64 && t.solve_threads != 128 && t.solve_threads != 256)
+    err = "tuning: solve_threads must be 64, 128 or 256";
+  else return CFD_OK;
+  return CFD_EINVAL;
+}
+int create_impl(const cfd_params* pp, const cfd_tuning* tp, cfd_handle* out);
+}  // namespace
+
+extern "C" int cfd_create(const cfd_params* pp, cfd_handle* out) { return create_impl(pp, nullptr, out); }
+
+extern "C" int cfd_create_tuned(const cfd_params* pp, const cfd_tuning* t, cfd_handle* out) {
+  if (!t) return fail(nullptr, CFD_EINVAL, "null tuning");
+  std::string err;
+  if (tuning_check(*t, err)) return fail(nullptr, CFD_EINVAL, err);
+  return create_impl(pp, t, out);
+}
+
+extern "C" int cfd_get_tuning(cfd_handle h, cfd_tuning* t) {
+  cfd_solver* s = static_cast<cfd_solver*>(h);
+  if (!s || !t) return fail(s, CFD_EINVAL, "null argument");
+  t->fft = s->fft_stockham ? 1 : 0;
+  t->seg_rows = s->seg_rows;
+  t->solve_threads = s->solve_threads;
+  return CFD_OK;
+}
+
+extern "C" int cfd_set_tuning(cfd_handle h, const cfd_tuning* t) {
+  cfd_solver* s = static_cast<cfd_solver*>(h);
+  if (!s || !t) return fail(s, CFD_EINVAL, "null argument");
+  std::string err;
+  if (tuning_check(*t, err)) return fail(s, CFD_EINVAL, err);
+  if ((t->fft == 1) != s->fft_stockham) return fail(s, CFD_EINVAL, "tuning: the FFT engine is fixed at creation");
+  s->seg_rows = t->seg_rows;
+  s->solve_threads = t->solve_threads;
+  return CFD_OK;
+}
+
+namespace {
+int create_impl(const cfd_params* pp, const cfd_tuning* tp, cfd_handle* out) {
   if (!pp || !out) return fail(nullptr, CFD_EINVAL, "null argument");
   *out = nullptr;
   const cfd_params& p = *pp;
@@ -284,11 +326,19 @@ extern "C" int cfd_create(const cfd_params* pp, cfd_handle* out) {
     s->err = "Ngalx and Ngalz must factor into 2, 3 and 5";
     return bail(CFD_EINVAL);
   }
-  {
+  if (tp) {
+    s->fft_stockham = tp->fft == 1;
+    s->seg_rows = tp->seg_rows;
+    s->solve_threads = tp->solve_threads;
+  } else {
     const char* e = getenv("CFD_AMD_FFT");
     s->fft_stockham = e && std::string(e) == "stockham";
   }
   s->mixed = p.precision == 1;
+  if (s->mixed && tp && tp->fft == 1) {
+    s->err = "the mixed path runs the four-step FFT kernels only";
+    return bail(CFD_EINVAL);
+  }
   if (s->mixed) s->fft_stockham = false;
   int rc;
   if ((rc = upload(s, &s->d_twz, twiddles(s->ngz))) || (rc = upload(s, &s->d_twx, twiddles(s->ngx))) ||
@@ -538,6 +588,7 @@ extern "C" int cfd_create(const cfd_params* pp, cfd_handle* out) {
   *out = s;
   return CFD_OK;
 }
+}  // namespace
 
 extern "C" int cfd_build_ygrid(const cfd_params* p, double* yu, double* yv, double* du, double* dv, double* d2u,
                                double* d2v) {
@@ -1229,6 +1280,7 @@ int op_substep(cfd_solver* s, int kRK) {
     launch_meanflow_lds(*s, kRK);
   }
   { Timed t(s, KC_PRESS); launch_pressure(*s, kRK); }
+  ++s->nsub;
   return wrc;
 }
 
@@ -1658,5 +1710,141 @@ extern "C" int cfd_closure_phys(cfd_handle h, int which, double* out) {
   HIPCHK(s, hipStreamSynchronize(s->stream));
   HIPCHK(s, hipMemcpy(out, which < 3 ? s->cl.Pu[g] : s->cl.Pc[g], (size_t)ny * s->ngz * s->ngx * sizeof(double),
                       hipMemcpyDeviceToHost));
+  return CFD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// ensembles: independent solvers stepped by one HIP graph per RK substep
+// ---------------------------------------------------------------------------
+struct cfd_ensemble_s {
+  int dev = 0;
+  std::vector<cfd_solver*> m;
+  hipStream_t root = nullptr;
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> join;
+  hipGraph_t g[3] = {};
+  hipGraphExec_t ge[3] = {};
+  std::string err;
+};
+
+namespace {
+thread_local std::string g_ens_err;
+
+void ens_free(cfd_ensemble_s* e) {
+  for (int k = 0; k < 3; ++k) {
+    if (e->ge[k]) hipGraphExecDestroy(e->ge[k]);
+    if (e->g[k]) hipGraphDestroy(e->g[k]);
+  }
+  for (hipEvent_t v : e->join)
+    if (v) hipEventDestroy(v);
+  if (e->fork) hipEventDestroy(e->fork);
+  if (e->root) hipStreamDestroy(e->root);
+  delete e;
+}
+
+int ens_fail(cfd_ensemble_s* e, int rc, const std::string& msg) {
+  g_ens_err = msg;
+  if (e) e->err = msg;
+  return rc;
+}
+}  // namespace
+
+// Capture: the root stream records an event that every member's stream
+// waits on; each member's substep kRK (and, after kRK = 3 of a member with a
+// prescribed pressure gradient, its flow-rate integral, as cfd_step does)
+// is launched on its own stream; the root waits on an event of each member.
+// The graph holds kernel nodes only.  Members must have run at least one
+// substep eagerly (so every kernel's launch attributes are set), have
+// profiling off, no closure attached and no statistics request pending.
+extern "C" int cfd_ensemble_create(cfd_handle* hs, int n, cfd_ensemble* out) {
+  if (!hs || n < 1 || !out) return ens_fail(nullptr, CFD_EINVAL, "ensemble: bad arguments");
+  *out = nullptr;
+  cfd_ensemble_s* e = new cfd_ensemble_s();
+  auto bail = [&](int rc, const std::string& msg) {
+    g_ens_err = msg;
+    ens_free(e);
+    return rc;
+  };
+  for (int i = 0; i < n; ++i) {
+    cfd_solver* s = static_cast<cfd_solver*>(hs[i]);
+    if (!s) return bail(CFD_EINVAL, "ensemble: null member");
+    if (i && s->dev != e->dev) return bail(CFD_EINVAL, "ensemble: members on different devices");
+    if (s->prof || s->cl.cnn || s->st.pending)
+      return bail(CFD_EINVAL, "ensemble: a member has profiling, a closure or a statistics request");
+    if (!s->nsub) return bail(CFD_EINVAL, "ensemble: every member must run one substep before capture");
+    for (cfd_solver* q : e->m)
+      if (q == s) return bail(CFD_EINVAL, "ensemble: a member appears twice");
+    if (!i) e->dev = s->dev;
+    e->m.push_back(s);
+  }
+  if (hipSetDevice(e->dev) != hipSuccess || hipStreamCreateWithFlags(&e->root, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) != hipSuccess)
+    return bail(CFD_EHIP, "ensemble: stream or event");
+  e->join.assign(n, nullptr);
+  for (int i = 0; i < n; ++i)
+    if (hipEventCreateWithFlags(&e->join[i], hipEventDisableTiming) != hipSuccess)
+      return bail(CFD_EHIP, "ensemble: event");
+  // nothing of the members may be in flight while capturing
+  for (cfd_solver* s : e->m)
+    if (hipStreamSynchronize(s->stream) != hipSuccess) return bail(CFD_EHIP, "ensemble: member stream");
+  for (int k = 0; k < 3; ++k) {
+    const int kRK = k + 1;
+    if (hipStreamBeginCapture(e->root, hipStreamCaptureModeThreadLocal) != hipSuccess)
+      return bail(CFD_EHIP, "ensemble: begin capture");
+    hipError_t err = hipEventRecord(e->fork, e->root);
+    int rc = CFD_OK;
+    for (int i = 0; i < n && err == hipSuccess && rc == CFD_OK; ++i) {
+      cfd_solver* s = e->m[i];
+      err = hipStreamWaitEvent(s->stream, e->fork, 0);
+      if (err != hipSuccess) break;
+      rc = op_substep(s, kRK);
+      if (kRK == 3 && s->P.flag_ctpress == 1) launch_flowrate(*s);
+      if (err == hipSuccess) err = hipEventRecord(e->join[i], s->stream);
+      if (err == hipSuccess) err = hipStreamWaitEvent(e->root, e->join[i], 0);
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(e->root, &g);
+    e->g[k] = g;
+    if (err != hipSuccess || ec != hipSuccess || rc != CFD_OK || !g)
+      return bail(CFD_EHIP, std::string("ensemble: capture of substep ") + std::to_string(kRK) + ": " +
+                                hipGetErrorString(err != hipSuccess ? err : ec));
+    if (hipGraphInstantiate(&e->ge[k], g, nullptr, nullptr, 0) != hipSuccess)
+      return bail(CFD_EHIP, "ensemble: graph instantiation");
+  }
+  *out = e;
+  return CFD_OK;
+}
+
+extern "C" int cfd_ensemble_destroy(cfd_ensemble e) {
+  if (!e) return CFD_OK;
+  hipSetDevice(e->dev);
+  if (e->root) hipStreamSynchronize(e->root);
+  ens_free(e);
+  return CFD_OK;
+}
+
+extern "C" const char* cfd_ensemble_last_error(cfd_ensemble e) { return e ? e->err.c_str() : g_ens_err.c_str(); }
+
+extern "C" int cfd_ensemble_substep(cfd_ensemble e, int kRK) {
+  if (!e || kRK < 1 || kRK > 3) return ens_fail(e, CFD_EINVAL, "ensemble: kRK must be 1, 2 or 3");
+  hipSetDevice(e->dev);
+  const hipError_t err = hipGraphLaunch(e->ge[kRK - 1], e->root);
+  if (err != hipSuccess) return ens_fail(e, CFD_EHIP, std::string("ensemble: graph launch: ") + hipGetErrorString(err));
+  return CFD_OK;
+}
+
+extern "C" int cfd_ensemble_step(cfd_ensemble e, int nsteps) {
+  if (!e || nsteps < 0) return ens_fail(e, CFD_EINVAL, "ensemble: bad arguments");
+  for (int it = 0; it < nsteps; ++it)
+    for (int kRK = 1; kRK <= 3; ++kRK)
+      if (int rc = cfd_ensemble_substep(e, kRK)) return rc;
+  return CFD_OK;
+}
+
+extern "C" int cfd_ensemble_sync(cfd_ensemble e) {
+  if (!e) return ens_fail(e, CFD_EINVAL, "null ensemble");
+  hipSetDevice(e->dev);
+  const hipError_t err = hipStreamSynchronize(e->root);
+  if (err != hipSuccess) return ens_fail(e, CFD_EHIP, std::string("ensemble: ") + hipGetErrorString(err));
   return CFD_OK;
 }

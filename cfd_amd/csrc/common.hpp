@@ -126,6 +126,11 @@ struct Solver {
   // FFT
   FftPlan pz{}, px{};
   bool fft_stockham = false;  // force the LDS Stockham kernels (CFD_AMD_FFT=stockham)
+  // launch shapes of the segmented y solves (k_rhs_seg, k_press_seg): rows
+  // per segment (5, 9, 13 or 21) and the cap on threads per block (64..256)
+  int seg_rows = 13;
+  int solve_threads = 256;
+  long long nsub = 0;  // substeps launched (captured ones included)
   double2 *d_twz = nullptr, *d_twx = nullptr;
   float2 *d_twzf = nullptr, *d_twxf = nullptr;  // the same tables rounded to FP32
 
@@ -181,6 +186,7 @@ void launch_solve_p(Solver& s, int kRK);
 void launch_vcorr(Solver& s, int kRK);
 void launch_divergence(Solver& s);
 void launch_flowrate(Solver& s);
+bool seg_rows_supported(int rows);
 // statistics (kernels_stats.hip)
 struct StatsSpecArgs {
   const double2 *u1, *u2, *u3, *p;

@@ -872,21 +872,42 @@ void launch_ib_rhs(Solver& s, int kRK) {
                        (const int*)s.d_ibq[1], (const int*)s.d_ibq[2], n0, n1, n2);
 }
 
-constexpr int kSeg = 13;  // rows per segment of the segmented solve
+// The segmented solves are instantiated for these rows per segment; the
+// solver's seg_rows picks one, solve_threads caps the block.
+template <int SEG>
+void rhs_seg_launch(Solver& s, const FusedArgs& a, int sy) {
+  int cb = 1;
+  while (cb * 2 * sy <= s.solve_threads) cb *= 2;
+  const size_t lds = sizeof(double) * 15 * (size_t)cb * sy;
+  const dim3 grid((s.Ncol + cb - 1) / cb, 3), block(cb, sy);
+  if (s.mixed)
+    hipLaunchKernelGGL((k_rhs_seg<SEG, float2>), grid, block, lds, s.stream, a);
+  else
+    hipLaunchKernelGGL((k_rhs_seg<SEG, double2>), grid, block, lds, s.stream, a);
+}
+
+template <int SEG>
+void press_seg_launch(Solver& s, const PressArgs& a, int sy) {
+  int cb = 1;
+  while (cb * 2 * sy <= s.solve_threads) cb *= 2;
+  const size_t lds = sizeof(double) * 15 * (size_t)cb * sy;
+  hipLaunchKernelGGL(k_press_seg<SEG>, dim3((s.Ncol + cb - 1) / cb), dim3(cb, sy), lds, s.stream, a);
+}
+
+bool seg_rows_supported(int r) { return r == 5 || r == 9 || r == 13 || r == 21; }
 
 void launch_rhs_solve(Solver& s, int kRK) {
   FusedArgs a;
   fused_args(s, kRK, &a);
-  const int sy = (s.nyu + kSeg + 1) / (kSeg + 1);  // segments of the longest column
+  const int R = s.seg_rows;
+  const int sy = (s.nyu + R + 1) / (R + 1);  // segments of the longest column (S of k_rhs_seg)
   if (sy <= 256) {
-    int cb = 1;
-    while (cb * 2 * sy <= 256) cb *= 2;
-    const size_t lds = sizeof(double) * 15 * (size_t)cb * sy;
-    const dim3 grid((s.Ncol + cb - 1) / cb, 3), block(cb, sy);
-    if (s.mixed)
-      hipLaunchKernelGGL((k_rhs_seg<kSeg, float2>), grid, block, lds, s.stream, a);
-    else
-      hipLaunchKernelGGL((k_rhs_seg<kSeg, double2>), grid, block, lds, s.stream, a);
+    switch (R) {
+      case 5: rhs_seg_launch<5>(s, a, sy); break;
+      case 9: rhs_seg_launch<9>(s, a, sy); break;
+      case 21: rhs_seg_launch<21>(s, a, sy); break;
+      default: rhs_seg_launch<13>(s, a, sy); break;
+    }
   } else if (s.mixed) {
     hipLaunchKernelGGL(k_rhs_solve<float2>, dim3((s.Ncol + 63) / 64, 3), dim3(64), 0, s.stream, a);
   } else {
@@ -911,12 +932,15 @@ void launch_pressure(Solver& s, int kRK) {
   a.Mx = s.Mx;
   a.Ncol = s.Ncol;
   a.nyp = s.nyp;
-  const int sy = (s.nyp + kSeg + 1) / (kSeg + 1);
+  const int R = s.seg_rows;
+  const int sy = (s.nyp + R + 1) / (R + 1);  // S of k_press_seg
   if (sy <= 256) {
-    int cb = 1;
-    while (cb * 2 * sy <= 256) cb *= 2;
-    const size_t lds = sizeof(double) * 15 * (size_t)cb * sy;
-    hipLaunchKernelGGL(k_press_seg<kSeg>, dim3((s.Ncol + cb - 1) / cb), dim3(cb, sy), lds, s.stream, a);
+    switch (R) {
+      case 5: press_seg_launch<5>(s, a, sy); break;
+      case 9: press_seg_launch<9>(s, a, sy); break;
+      case 21: press_seg_launch<21>(s, a, sy); break;
+      default: press_seg_launch<13>(s, a, sy); break;
+    }
   } else {
     hipLaunchKernelGGL(k_pressure, dim3((s.Ncol + 63) / 64), dim3(64), 0, s.stream, a);
   }

@@ -72,6 +72,21 @@ enum { CFD_T = 0, CFD_DT, CFD_DTI, CFD_MPGX, CFD_DGX, CFD_QXT, CFD_QX, CFD_DTC, 
    boundary_circ_rough (IB lists), gridweighting, LU_buildP. */
 int cfd_create(const cfd_params* p, cfd_handle* out);
 
+/* Launch shapes (no counterpart in the reference, whose CPU loops have
+   none).  fft: 0 = the four-step register FFT kernels where the lengths
+   have them, 1 = the LDS Stockham kernels (FP64 path only); fixed at
+   creation.  seg_rows: rows per segment of the segmented y solves, 5, 9, 13
+   or 21; solve_threads: threads per block of those solves, 64, 128 or 256.
+   Every choice gives the same solution to rounding.  cfd_create uses fft 0
+   (1 when the environment sets CFD_AMD_FFT=stockham), seg_rows 13 and
+   solve_threads 256. */
+typedef struct cfd_tuning {
+  int fft, seg_rows, solve_threads;
+} cfd_tuning;
+int cfd_create_tuned(const cfd_params* p, const cfd_tuning* t, cfd_handle* out);
+int cfd_get_tuning(cfd_handle h, cfd_tuning* t);
+int cfd_set_tuning(cfd_handle h, const cfd_tuning* t); /* fft must not change */
+
 /* Host-only setup products (no GPU needed): the y grid and IB lists that
    cfd_create builds, for a caller that keeps its own copy or checks them.
    Array sizes: nyu = ny+2*dsty+1, nyv = ny+2*dsty; see cfd_get_ygrid and
@@ -283,6 +298,25 @@ int cfd_cnn_get_scales(cfd_cnn c, float* s6);
 int cfd_closure_snapshot(cfd_handle fine, cfd_handle coarse, cfd_cnn c, int slot, double* rms);
 int cfd_closure_attach(cfd_handle h, cfd_cnn c, int every);
 int cfd_closure_phys(cfd_handle h, int which, double* out);
+
+/* Ensembles (no counterpart in the reference: many independent channels,
+ * e.g. of uneven grids, stepped together).  cfd_ensemble_create captures,
+ * for each kRK, one HIP graph that runs the substep of every member, each on
+ * its own stream (after kRK = 3, the flow-rate integral of members with
+ * flag_ctpress = 1, as cfd_step does); cfd_ensemble_substep / _step launch
+ * the graphs on the ensemble's stream.  Members keep their state and
+ * handles: synchronise the ensemble (cfd_ensemble_sync) before reading or
+ * driving a member directly.  Every member must have run one substep
+ * eagerly before (launch attributes are set on first use), with profiling
+ * off, no closure attached and no statistics request pending; the graphs
+ * hold kernel nodes only. */
+typedef struct cfd_ensemble_s* cfd_ensemble;
+int cfd_ensemble_create(cfd_handle* members, int n, cfd_ensemble* out);
+int cfd_ensemble_destroy(cfd_ensemble e);
+const char* cfd_ensemble_last_error(cfd_ensemble e);
+int cfd_ensemble_substep(cfd_ensemble e, int kRK);
+int cfd_ensemble_step(cfd_ensemble e, int nsteps);
+int cfd_ensemble_sync(cfd_ensemble e);
 
 #ifdef __cplusplus
 }
