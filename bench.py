@@ -10,7 +10,10 @@ before timing; the timed region runs on device-resident data only.
 `value` is the FP64 path.  The same run times the mixed path (precision 1:
 FP32 transforms, products and nonlinear scratch; FP64 fields, solves,
 projection and flow-rate correction) over the same substeps from the same
-field and reports its throughput and its error against the FP64 run.
+field and reports its throughput and its error against the FP64 run; the
+learned path (the 192 x 192 x 96 coarse grid with the trained BF16 closure
+network, simulated time per second against the FP64 run's); and the
+ensemble of 16 uneven channels stepped by captured HIP graphs.
 
 Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
 """
@@ -121,6 +124,73 @@ def copy_calibration():
                       "copy_GBps": 3 * 2 * n * 8 / (t1 - t0) / 1e9}))
 
 
+def learned_leg(args, fine_value, fine_dt):
+    """The learned path: the 192 x 192 x 96 coarse grid with the trained
+    closure network (cfd_amd/data/closure_rt395.npz) evaluated every
+    ``every`` steps, timed like the FP64 leg from a seeded start; simulated
+    time per second against the fine FP64 run's (substeps/s x dt)."""
+    from cfd_amd import Channel, closure, cnn, workloads
+
+    pc, _ = workloads.get("rt395_coarse_192x192x96")
+    w = np.load(os.path.join(ROOT, "cfd_amd", "data", "closure_rt395.npz"))
+    out = {"workload": "rt395_coarse_192x192x96", "grid": [pc.ngalx, pc.ny, pc.ngalz],
+           "weights": "cfd_amd/data/closure_rt395.npz", "dtype": "bf16 network (fp32 accumulation), f64 solver"}
+    net = None
+    for every in (0, 1, 2, 4):
+        ch = Channel(pc)
+        closure.start_fields(ch, pc, seed=0)
+        if every:
+            if net is None:
+                net = cnn.CNN(ch.nyu, pc.ngalz, pc.ngalx)
+                net.set_params(w["params"])
+                net.set_scales(w["scales"])
+            ch.closure_attach(net, every=every)
+        for s in range(args.warmup):
+            ch.substep(s % 3 + 1)
+        ch.sync()
+        t0 = time.perf_counter()
+        for s in range(args.steps):
+            ch.substep((args.warmup + s) % 3 + 1)
+        ch.sync()
+        t1 = time.perf_counter()
+        v = args.steps / (t1 - t0)
+        dt = ch.scalars()["dt"]
+        key = "closure_every_%d" % every if every else "no_closure"
+        out[key] = {"substeps_per_s": v, "dt": dt, "sim_time_per_s_vs_fine_fp64": (v * dt) / (fine_value * fine_dt)}
+        if every:
+            ch.closure_attach(None)
+        ch.close()
+    if net is not None:
+        net.close()
+    rep = os.path.join(ROOT, "profiles", "closure_rt395_384x192x192_to_192x192x96.json")
+    if os.path.exists(rep):
+        with open(rep) as fh:
+            r = json.load(fh)
+        out["accuracy_from_report"] = {
+            "report": "profiles/closure_rt395_384x192x192_to_192x192x96.json",
+            "held_out_gain": r.get("held_out_gain"),
+            "coarse": {k: r["coarse"].get(k) for k in ("completed", "t_reached", "drag")},
+            "coarse_closure": {k: r["coarse_closure"].get(k) for k in ("completed", "t_reached", "drag")}}
+    return out
+
+
+def ensemble_leg(fine_value, fine_points):
+    """16 uneven channels, tuned, 50 RK3 steps (the first eager, 49 by the
+    captured graphs)."""
+    from cfd_amd import ensemble
+
+    rep = ensemble.run(16, 1, 50, autotune=True, log=None)
+    return {"members": 16, "seed": 1, "steps": 50,
+            "member_substeps_per_s": rep["member_substeps_per_s"],
+            "ensemble_substeps_per_s": rep["ensemble_substeps_per_s"],
+            "point_substeps_per_s": rep["point_substeps_per_s"],
+            "point_substeps_per_s_single_channel": fine_value * fine_points,
+            "points": rep["points"], "tuning_seconds": rep["tuning_seconds"],
+            "all_finite": all(m["finite"] for m in rep["members"]),
+            "max_div_over_max_u": max(m["max_div_over_max_u"] for m in rep["members"]),
+            "tunings": [m["tuning"] for m in rep["members"]]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,6 +199,8 @@ def main():
     ap.add_argument("--workload", default="rt395_384x192x192")
     ap.add_argument("--cpu-substeps", type=int, default=3, help="oracle substeps for the CPU baseline (0: skip)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-closure leg")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the 16-channel ensemble leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy 3 times and print its bandwidth (the reference "
                          "kernel for scaling HBM byte counters)")
@@ -174,6 +246,9 @@ def main():
         }
         chm.close()
 
+    learned = None if args.no_learned else learned_leg(args, value, sc["dt"])
+    ens = None if args.no_ensemble else ensemble_leg(value, params.ngalx * params.ny * params.ngalz)
+
     # CPU baseline: the FP64 oracle on the host cores, a bounded sample
     cpu = None
     if args.cpu_substeps > 0:
@@ -213,6 +288,8 @@ def main():
         "kernel_ms_per_substep": kms,
         "device_bytes": st64["device_bytes"],
         "mixed": mixed,
+        "learned": learned,
+        "ensemble": ens,
         "cpu_baseline": cpu,
         "sim": {"t": sc["t"], "dt": sc["dt"], "mpgx": sc["mpgx"], "finite": finite},
     }

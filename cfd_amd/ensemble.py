@@ -224,3 +224,85 @@ class Ensemble:
 
     def __exit__(self, *a):
         self.close()
+
+
+def run(n=16, seed=1, steps=50, autotune=True, log=print):
+    """The ensemble run: members drawn, each tuned (or left at the default
+    shapes), started, its first step eager (the capture needs one substep run
+    on every member), the other ``steps - 1`` steps by the captured graphs.
+    Returns a report: members, tunings, throughput, final state per member."""
+    import time
+    from . import diagnostics
+    ms = members(n, seed)
+    rows = []
+    t0 = time.perf_counter()
+    for i, m in enumerate(ms):
+        if autotune:
+            best, table = tune(m)
+            dflt = [x for t, x in table if t == dict(fft=0, seg_rows=13, solve_threads=256)][0]
+            bms = min(x for _, x in table)
+        else:
+            best, dflt, bms = None, None, None
+        rows.append(dict(tuning=best, ms_default=dflt, ms_tuned=bms))
+        if log and autotune:
+            log("member %2d %dx%dx%d tuned %s: %.4f ms/substep (default shapes %.4f)"
+                % (i, m.params.ngalx, m.params.ny, m.params.ngalz, best, bms, dflt))
+    t_tune = time.perf_counter() - t0
+    chs = []
+    try:
+        for m, r in zip(ms, rows):
+            ch = Channel(m.params, tuning=r["tuning"])
+            start(ch, m)
+            chs.append(ch)
+        for ch in chs:
+            ch.step(1)
+        for ch in chs:
+            ch.sync()
+        with Ensemble(chs) as ens:
+            t1 = time.perf_counter()
+            ens.step(steps - 1)
+            ens.sync()
+            T = time.perf_counter() - t1
+        out = []
+        for m, ch, r in zip(ms, chs, rows):
+            p = m.params
+            u1 = ch.get_field("u1")
+            sc = ch.scalars()
+            div = np.abs(ch.get_field("div")).max()
+            bw, tw = diagnostics.wall_shear(u1, ch.ygrid(), p.Re)
+            out.append(dict(grid=[p.ngalx, p.ny, p.ngalz], dsty=p.dsty, re_tau=m.re_tau, Re_bulk=p.Re,
+                            posts_per_wall=[p.ntilex, p.ntilez], dplus=m.dplus, dplus_sim=m.dplus_sim,
+                            kplus=m.kplus, r_cells=p.r, Lz=2 * math.pi / p.bet, seed=m.seed, **r,
+                            t=sc["t"], dt=sc["dt"], mpgx=sc["mpgx"], wall_shear=[bw, tw],
+                            max_div_over_max_u=float(div / np.abs(u1).max()),
+                            finite=bool(np.isfinite(u1).all())))
+    finally:
+        for ch in chs:
+            ch.close()
+    pts = sum(m.params.ngalx * m.params.ny * m.params.ngalz for m in ms)
+    sub = n * 3 * (steps - 1)
+    return dict(members=out, steps=steps, graph_steps=steps - 1, seconds=T, tuning_seconds=t_tune,
+                member_substeps_per_s=sub / T, ensemble_substeps_per_s=3 * (steps - 1) / T,
+                point_substeps_per_s=pts * 3 * (steps - 1) / T, points=pts, autotune=autotune)
+
+
+def main(argv=None):
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description="16 uneven rough channels stepped by captured HIP graphs")
+    ap.add_argument("--members", type=int, default=16)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args(argv)
+    rep = run(a.members, a.seed, a.steps, autotune=not a.no_tune)
+    s = json.dumps(rep, indent=1)
+    if a.out:
+        with open(a.out, "w") as fh:
+            fh.write(s + "\n")
+    print(json.dumps({k: v for k, v in rep.items() if k != "members"}))
+
+
+if __name__ == "__main__":
+    main()
