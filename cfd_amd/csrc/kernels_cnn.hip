@@ -12,10 +12,11 @@
 // gfx950 (checked on the device): A[m][k] in lane m % 32, elements
 // k = 8 (l / 32) + i; B[k][n] in lane n % 32, elements k = 8 (l / 32) + i;
 // C[m][n] in lane n % 32, element v at m = 8 (v / 4) + 4 (l / 32) + v % 4.
-// A block (4 waves, 4 z rows of one plane) stages its input halo
-// (3 planes x 6 rows x 34 points) and the layer's weights in LDS; the LDS
-// point stride is padded (48 B for 16 channels, 80 B for 32) so the 16-byte
-// reads of one MFMA operand hit distinct banks.
+// The forward and backward-data blocks (4 waves, 8 z rows, two per wave)
+// keep the layer's weights and a ring of three halo planes (10 rows x 34
+// points) in LDS; the weight-gradient blocks (4 z rows) stage a 3-plane halo
+// of 6 rows.  The LDS point stride is padded (48 B for 16 channels, 80 B for
+// 32) so the 16-byte reads of one MFMA operand hit distinct banks.
 #include <hip/hip_bf16.h>
 
 #include <algorithm>

@@ -275,6 +275,7 @@ def run(n=16, seed=1, steps=50, autotune=True, log=print):
                             kplus=m.kplus, r_cells=p.r, Lz=2 * math.pi / p.bet, seed=m.seed, **r,
                             t=sc["t"], dt=sc["dt"], mpgx=sc["mpgx"], wall_shear=[bw, tw],
                             max_div_over_max_u=float(div / np.abs(u1).max()),
+                            device_bytes=ch.device_bytes(),
                             finite=bool(np.isfinite(u1).all())))
     finally:
         for ch in chs:
@@ -283,7 +284,8 @@ def run(n=16, seed=1, steps=50, autotune=True, log=print):
     sub = n * 3 * (steps - 1)
     return dict(members=out, steps=steps, graph_steps=steps - 1, seconds=T, tuning_seconds=t_tune,
                 member_substeps_per_s=sub / T, ensemble_substeps_per_s=3 * (steps - 1) / T,
-                point_substeps_per_s=pts * 3 * (steps - 1) / T, points=pts, autotune=autotune)
+                point_substeps_per_s=pts * 3 * (steps - 1) / T, points=pts, autotune=autotune,
+                device_bytes=sum(m["device_bytes"] for m in out))
 
 
 def main(argv=None):
