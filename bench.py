@@ -53,12 +53,30 @@ def class_bytes(ch, zb):
     }
 
 
-def timed(params, f, q0, args, nprof):
+def tuned(params, f, q0, args):
+    """Launch shapes from the autotuner (cfd_amd.ensemble.tune_params) on a
+    scratch solver started from the same field; (tuning, summary)."""
+    if args.no_tune:
+        return None, None
+    from cfd_amd import ensemble
+
+    def load(ch):
+        for k in ("u1", "u2", "u3", "p"):
+            ch.set_field(k, f[k])
+        ch.set_scalars(t=0.0, dt=0.0, mpgx=0.0, QxT=q0)
+
+    best, table = ensemble.tune_params(params, load)
+    dflt = [ms for t, ms in table if t == dict(fft=0, seg_rows=13, solve_threads=256)][0]
+    return best, {"tuning": best, "kernel_ms_per_substep_tuned": min(ms for _, ms in table),
+                  "kernel_ms_per_substep_default_shapes": dflt, "candidates": len(table)}
+
+
+def timed(params, f, q0, args, nprof, tuning=None):
     """Create a Channel, load the field, run warmup + timed + profiled
     substeps; returns (substeps/s, ms per substep, per-class profile, final state)."""
     from cfd_amd import Channel
 
-    ch = Channel(params)
+    ch = Channel(params, tuning=tuning)
     for k in ("u1", "u2", "u3", "p"):
         ch.set_field(k, f[k])
     ch.set_scalars(t=0.0, dt=0.0, mpgx=0.0, QxT=q0)
@@ -199,6 +217,7 @@ def main():
     ap.add_argument("--workload", default="rt395_384x192x192")
     ap.add_argument("--cpu-substeps", type=int, default=3, help="oracle substeps for the CPU baseline (0: skip)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--no-tune", action="store_true", help="default launch shapes instead of the autotuner's")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-closure leg")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 16-channel ensemble leg")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -224,7 +243,8 @@ def main():
     q0 = ic.flowrate(f["u1"], g, params.ny)
     nprof = 6
 
-    ch, value, ms_step, prof, st64 = timed(params, f, q0, args, nprof)
+    tun64, tsum64 = tuned(params, f, q0, args)
+    ch, value, ms_step, prof, st64 = timed(params, f, q0, args, nprof, tun64)
     rl = roofline(ch, prof, 16, args.workload, "")
     sc = st64["scalars"]
     finite = bool(np.isfinite(st64["u1"]).all())
@@ -234,12 +254,14 @@ def main():
     mixed = None
     if not args.no_mixed:
         pm = dataclasses.replace(params, precision=1)
-        chm, vm, msm, profm, stm = timed(pm, f, q0, args, nprof)
+        tunm, tsumm = tuned(pm, f, q0, args)
+        chm, vm, msm, profm, stm = timed(pm, f, q0, args, nprof, tunm)
         mixed = {
             "value": vm, "unit": "substeps/s", "ms_per_step": msm, "speedup_vs_fp64": vm / value,
             "dtype": "f32 transforms/products/nonlinear scratch; f64 fields, solves, projection, flow rate",
             "roofline": roofline(chm, profm, 8, args.workload, "_mixed"),
             "kernel_ms_per_substep": {c: ms / nprof for c, (ms, cnt) in profm.items()},
+            "autotune": tsumm,
             "device_bytes": stm["device_bytes"],
             "err_vs_fp64": dict(diagnostics.compare(stm, st64, g, params.Re),
                                 substeps=args.warmup + args.steps + nprof),
@@ -286,6 +308,7 @@ def main():
                    "precision": "fp64", "parallelism": "single-gpu"},
         "roofline": rl,
         "kernel_ms_per_substep": kms,
+        "autotune": tsum64,
         "device_bytes": st64["device_bytes"],
         "mixed": mixed,
         "learned": learned,

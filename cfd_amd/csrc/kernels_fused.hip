@@ -772,8 +772,15 @@ __global__ void __launch_bounds__(256) k_press_seg(PressArgs a) {
 }
 
 // (0,0) flow-rate correction staged through LDS: the column, metrics and
-// scratch live in LDS; one thread runs the sequential recurrences and sums
-// in the reference order (flowrate_corr, littleharsh.f90:452-484).
+// scratch live in LDS; one thread runs the sequential recurrences in the
+// reference order (flowrate_corr, littleharsh.f90:452-484) while a thread of
+// the second wave forms the flow-rate sum of the old profile.  The
+// recurrences walk the column in chunks of kMfChunk rows: the chunk's
+// coefficients are read from LDS into registers together (their loads
+// overlap), the chunk is eliminated in registers, and only the carried
+// values link one chunk to the next.
+constexpr int kMfChunk = 8;
+
 __global__ void __launch_bounds__(256) k_meanflow_lds(double2* __restrict__ u1, size_t Ncol, int jlo_u, int nn,
                                                       const double* __restrict__ d2u, const double* __restrict__ yv,
                                                       int jlo_v, DevScalars* sc, double bRK, double dRK, double Re,
@@ -799,20 +806,89 @@ __global__ void __launch_bounds__(256) k_meanflow_lds(double2* __restrict__ u1, 
     }
   }
   __syncthreads();
-  __shared__ double gsh;
-  if (threadIdx.x == 0) {
-    a2[0] = 1.0 / a2[0];
-    for (int j = 1; j < n; ++j) {
-      a3[j - 1] = a3[j - 1] * a2[j - 1];
-      a2[j] = 1.0 / (a2[j] - a1[j] * a3[j - 1]);
-    }
-    x[0] = x[0] * a2[0];
-    for (int j = 1; j < n; ++j) x[j] = (x[j] - a1[j] * x[j - 1]) * a2[j];
-    for (int j = n - 2; j >= 0; --j) x[j] = x[j] - a3[j] * x[j + 1];
-    double Qx1 = 0.0, Qx0 = 0.0;
-    for (int j = 1; j <= nn + 1; ++j) Qx1 = Qx1 + dy[j] * x[j];
+  __shared__ double gsh, q0sh;
+  if (threadIdx.x == 64) {
+    double Qx0 = 0.0;
     for (int j = 1; j <= nn + 1; ++j) Qx0 = Qx0 + dy[j] * ur[j];
-    const double g = (sc->QxT - Qx0) / Qx1;
+    q0sh = Qx0;
+  }
+  if (threadIdx.x == 0) {
+    constexpr int C = kMfChunk;
+    // factorisation: a2 <- 1 / pivot, a3 <- a3 / pivot of the row above
+    double p2 = 1.0 / a2[0], p3 = a3[0];  // row j-1 carried in registers
+    a2[0] = p2;
+    int j = 1;
+    for (; j + C <= n; j += C) {
+      double c1[C], c2[C], c3[C];
+#pragma unroll
+      for (int q = 0; q < C; ++q) {
+        c1[q] = a1[j + q];
+        c2[q] = a2[j + q];
+        c3[q] = a3[j + q];
+      }
+#pragma unroll
+      for (int q = 0; q < C; ++q) {
+        const double s3 = p3 * p2;
+        a3[j + q - 1] = s3;
+        p2 = 1.0 / (c2[q] - c1[q] * s3);
+        a2[j + q] = p2;
+        p3 = c3[q];
+      }
+    }
+    for (; j < n; ++j) {
+      const double c1 = a1[j], c2 = a2[j], c3 = a3[j];
+      const double s3 = p3 * p2;
+      a3[j - 1] = s3;
+      p2 = 1.0 / (c2 - c1 * s3);
+      a2[j] = p2;
+      p3 = c3;
+    }
+    // forward substitution
+    double xp = x[0] * a2[0];
+    x[0] = xp;
+    for (j = 1; j + C <= n; j += C) {
+      double c1[C], c2[C], cx[C];
+#pragma unroll
+      for (int q = 0; q < C; ++q) {
+        c1[q] = a1[j + q];
+        c2[q] = a2[j + q];
+        cx[q] = x[j + q];
+      }
+#pragma unroll
+      for (int q = 0; q < C; ++q) {
+        xp = (cx[q] - c1[q] * xp) * c2[q];
+        x[j + q] = xp;
+      }
+    }
+    for (; j < n; ++j) {
+      xp = (x[j] - a1[j] * xp) * a2[j];
+      x[j] = xp;
+    }
+    // back substitution
+    for (j = n - 2; j - C + 1 >= 0; j -= C) {
+      double c3[C], cx[C];
+#pragma unroll
+      for (int q = 0; q < C; ++q) {
+        c3[q] = a3[j - q];
+        cx[q] = x[j - q];
+      }
+#pragma unroll
+      for (int q = 0; q < C; ++q) {
+        xp = cx[q] - c3[q] * xp;
+        x[j - q] = xp;
+      }
+    }
+    for (; j >= 0; --j) {
+      xp = x[j] - a3[j] * xp;
+      x[j] = xp;
+    }
+    double Qx1 = 0.0;
+    for (int j = 1; j <= nn + 1; ++j) Qx1 = Qx1 + dy[j] * x[j];
+    gsh = Qx1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double g = (sc->QxT - q0sh) / gsh;
     sc->mpgx = sc->mpgx + g;
     sc->dgx = g;
     gsh = g;

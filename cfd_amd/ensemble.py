@@ -156,11 +156,18 @@ def tune(member, nsteps=2, repeats=2):
     per FFT engine) started from the member's initial field; kernel time per
     substep from the solvers' event timers.  Returns (best tuning, table of
     (tuning, ms per substep))."""
+    return tune_params(member.params, lambda ch: start(ch, member), nsteps, repeats)
+
+
+def tune_params(params, load, nsteps=2, repeats=2):
+    """The autotuner for any solver: ``load(ch)`` puts the starting state
+    into a scratch solver.  The mixed path (precision 1) runs the four-step
+    engine only, so only its column-solve shapes are tuned."""
     table = []
-    for fft in FFT_CHOICES:
-        ch = Channel(member.params, tuning=dict(fft=fft))
+    for fft in (FFT_CHOICES if params.precision == 0 else (0,)):
+        ch = Channel(params, tuning=dict(fft=fft))
         try:
-            start(ch, member)
+            load(ch)
             ch.step(1)  # dt from the CFL condition, first-use set-up
             for sr in SEG_ROWS:
                 for th in SOLVE_THREADS:
