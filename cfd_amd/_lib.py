@@ -91,6 +91,7 @@ SIGNATURES = {
                                         ctypes.POINTER(ctypes.c_void_p)]),
     "cfd_get_tuning": (ctypes.c_int, [_H, ctypes.POINTER(ctypes.c_int)]),
     "cfd_set_tuning": (ctypes.c_int, [_H, ctypes.POINTER(ctypes.c_int)]),
+    "cfd_closure_clip": (ctypes.c_int, [_H, ctypes.c_int]),
     "cfd_ensemble_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
                                            ctypes.POINTER(ctypes.c_void_p)]),
     "cfd_ensemble_destroy": (ctypes.c_int, [_H]),

@@ -244,10 +244,13 @@ class Channel:
         return "%04dx%04dx%04d_%05d.dat" % (self.N1, self.N2, self.params.ny, int(10.0 * t))
 
     # -- learned closure ---------------------------------------------------------
-    def closure_attach(self, net, every=1):
+    def closure_attach(self, net, every=1, clip=False):
         """Add the network's correction in every nonlinear term, evaluated at
-        kRK = 1 of every `every`-th step and reused in between (None detaches)."""
+        kRK = 1 of every `every`-th step and reused in between (None detaches);
+        clip: zero the components that would feed energy into the resolved
+        field (u c < 0)."""
         self._call("cfd_closure_attach", net.handle if net is not None else None, int(every))
+        self._call("cfd_closure_clip", 1 if clip else 0)
 
     def closure_phys(self, which):
         """Physical planes of the last closure sample / application: 0..2 the

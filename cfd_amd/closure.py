@@ -12,7 +12,9 @@ runs the experiment at Re_tau 395: the fine DNS on 384 x 192 x 192 produces
 the samples (one every `every` steps, trained on online from a device ring
 buffer) and is the judge; the coarse solver on 192 x 192 x 96 then runs from
 the box-filtered start field over the same simulated time without and with
-the closure.  Writes closure_weights.npz and closure_report.json: a-priori
+the closure (as predicted, and backscatter-clipped: components that would feed
+energy into the resolved field set to zero).  Writes closure_weights.npz and
+closure_report.json: a-priori
 loss on held-out samples against predicting no correction, drag (the mean
 pressure gradient at constant flow rate) and mean-profile errors against the
 fine run, and simulated time per wall-clock second of each run.
@@ -208,7 +210,7 @@ def main(argv=None):
     report["coarse_start_t"] = t_start
     cap = 50.0 * float(np.abs(hist_f[:, 1]).max())
     runs, prof = {}, {}
-    for label, attach in (("coarse", False), ("coarse_closure", True)):
+    for label, attach in (("coarse", None), ("coarse_closure", "plain"), ("coarse_closure_clip", "clip")):
         ch = Channel(pc)
         g = ch.ygrid()
         for kname in ("u1", "u2", "u3", "p"):
@@ -216,7 +218,7 @@ def main(argv=None):
         ch.set_scalars(t=t_start, dt=0.0, mpgx=-float(np.interp(t_start, hist_f[:, 0], hist_f[:, 1])) if t_start else 0.0,
                        QxT=ic.flowrate(fc["u1"], g, pc.ny))
         if attach:
-            ch.closure_attach(net, every=a.closure_every)
+            ch.closure_attach(net, every=a.closure_every, clip=attach == "clip")
         hist, wall = run_until(ch, t_end, a.max_coarse_steps, cap)
         t_last = float(hist[-1, 0]) if len(hist) else 0.0
         runs[label] = {"hist": hist, "steps": len(hist), "wall_s": wall, "t_reached": t_last,

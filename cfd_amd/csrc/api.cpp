@@ -64,6 +64,7 @@ struct cfd_solver : public Solver {
     int every = 1;             // the correction is recomputed at kRK = 1 of every `every`-th step
     int count = 0;             // steps since attaching
     bool have = false;         // Cs holds a correction
+    bool clip = false;         // backscatter clipping of the correction
   } cl;
 };
 
@@ -1176,7 +1177,8 @@ int closure_apply(cfd_solver* s, int kRK) {
   const int rc = cnn_forward_dev(c, c->x);
   c->stream = keep;
   if (rc) return fail(s, rc, std::string("closure: ") + c->err);
-  launch_unpack(s->stream, c->y, s->nyu, s->nyv, s->ngz, s->ngx, c->scale + 3, C.Pc[0], C.Pc[1], C.Pc[2]);
+  launch_unpack(s->stream, c->y, s->nyu, s->nyv, s->ngz, s->ngx, c->scale + 3, C.Pc[0], C.Pc[1], C.Pc[2],
+                C.clip ? C.Pu[0] : nullptr, C.clip ? C.Pu[1] : nullptr, C.clip ? C.Pu[2] : nullptr);
   C.have = true;
   for (int g = 0; g < 3; ++g) {
     launch_xpass_fwd2(*s, C.d_xf[g], C.nxf[g]);
@@ -1634,6 +1636,13 @@ extern "C" int cfd_closure_attach(cfd_handle h, cfd_cnn c, int every) {
   s->cl.every = every;
   s->cl.count = 0;
   s->cl.have = false;
+  return CFD_OK;
+}
+
+extern "C" int cfd_closure_clip(cfd_handle h, int clip) {
+  cfd_solver* s = static_cast<cfd_solver*>(h);
+  if (!s) return fail(s, CFD_EINVAL, "null handle");
+  s->cl.clip = clip != 0;
   return CFD_OK;
 }
 

@@ -71,8 +71,9 @@ void launch_filter(hipStream_t st, const double2* fine, const double2* other, do
                    int N2c, int ngxc, int ngzc, double a, double b, int nplanes);
 void launch_pack(hipStream_t st, const double* p1, const double* p2, const double* p3, const unsigned char* mask,
                  int nyu, int nyv, int ngz, int ngx, const float* s, int withmask, __bf16* out);
+// u1..u3 null: the corrections as predicted; else backscatter-clipped
 void launch_unpack(hipStream_t st, const float* y, int nyu, int nyv, int ngz, int ngx, const float* s, double* p1,
-                   double* p2, double* p3);
+                   double* p2, double* p3, const double* u1, const double* u2, const double* u3);
 void launch_add_planes(hipStream_t st, double2* a, const double2* c, int ncol, int q0, int q1);
 void launch_sumsq(hipStream_t st, const double* v, size_t n, double* acc);
 }  // namespace cfd

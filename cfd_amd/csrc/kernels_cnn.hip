@@ -524,16 +524,32 @@ __global__ void k_pack(const double* __restrict__ p1, const double* __restrict__
 
 // physical corrections <- network output y[p][3] * s on the interior planes
 // (u planes 1..nyu-2 for components 1 and 3, v planes 1..nyv-2 for 2)
+// With u1..u3 given (the physical velocities of the same planes; all three
+// or none), a
+// component whose correction c would feed energy into the resolved field
+// (u c < 0: the correction enters the momentum equation as -c) is set to
+// zero there, so the correction only removes energy (backscatter clipping).
 __global__ void k_unpack(const float* __restrict__ y, int nyu, int nyv, int ngz, int ngx, float s1, float s2, float s3,
-                         double* __restrict__ p1, double* __restrict__ p2, double* __restrict__ p3) {
+                         double* __restrict__ p1, double* __restrict__ p2, double* __restrict__ p3,
+                         const double* __restrict__ u1, const double* __restrict__ u2, const double* __restrict__ u3) {
   const size_t pl = (size_t)ngz * ngx;
   const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= pl * nyu) return;
   const int q = (int)(e / pl);
   const bool iu = q >= 1 && q <= nyu - 2, iv = q >= 1 && q <= nyv - 2;
-  p1[e] = iu ? (double)(y[e * 3 + 0] * s1) : 0.0;
-  if (q < nyv) p2[e] = iv ? (double)(y[e * 3 + 1] * s2) : 0.0;
-  p3[e] = iu ? (double)(y[e * 3 + 2] * s3) : 0.0;
+  double c1 = iu ? (double)(y[e * 3 + 0] * s1) : 0.0;
+  double c3 = iu ? (double)(y[e * 3 + 2] * s3) : 0.0;
+  if (u1) {
+    if (u1[e] * c1 < 0.0) c1 = 0.0;
+    if (u3[e] * c3 < 0.0) c3 = 0.0;
+  }
+  p1[e] = c1;
+  p3[e] = c3;
+  if (q < nyv) {
+    double c2 = iv ? (double)(y[e * 3 + 1] * s2) : 0.0;
+    if (u1 && u2[e] * c2 < 0.0) c2 = 0.0;
+    p2[e] = c2;
+  }
 }
 
 // a[q] += c[q] on planes q0 .. q1 - 1 of Ncol complex values
@@ -577,10 +593,10 @@ void launch_pack(hipStream_t st, const double* p1, const double* p2, const doubl
 }
 
 void launch_unpack(hipStream_t st, const float* y, int nyu, int nyv, int ngz, int ngx, const float* s, double* p1,
-                   double* p2, double* p3) {
+                   double* p2, double* p3, const double* u1, const double* u2, const double* u3) {
   const size_t n = (size_t)nyu * ngz * ngx;
   hipLaunchKernelGGL(k_unpack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, y, nyu, nyv, ngz, ngx, s[0], s[1],
-                     s[2], p1, p2, p3);
+                     s[2], p1, p2, p3, u1, u2, u3);
 }
 
 void launch_add_planes(hipStream_t st, double2* a, const double2* c, int ncol, int q0, int q1) {

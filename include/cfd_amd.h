@@ -297,6 +297,12 @@ int cfd_cnn_get_scales(cfd_cnn c, float* s6);
  * corrections (planes of the component's grid x Ngalz x Ngalx). */
 int cfd_closure_snapshot(cfd_handle fine, cfd_handle coarse, cfd_cnn c, int slot, double* rms);
 int cfd_closure_attach(cfd_handle h, cfd_cnn c, int every);
+/* backscatter clipping of the attached correction (default off): where a
+ * component's correction c and velocity u have u c < 0 (the correction, which
+ * enters the momentum equation as -c, would feed energy into the resolved
+ * field) that component of the correction is zero; the velocity of a point
+ * is the physical value the network reads there. */
+int cfd_closure_clip(cfd_handle h, int clip);
 int cfd_closure_phys(cfd_handle h, int which, double* out);
 
 /* Ensembles (no counterpart in the reference: many independent channels,

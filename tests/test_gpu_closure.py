@@ -160,6 +160,18 @@ def test_attached_network_corrects_the_nonlinear_term():
             d = b.get_field(n) - a.get_field(n)
             e = _spec(COARSE, corr[k])
             assert _rel(d[:, 1:-1], e[:, 1:-1]) <= 1e-10, (n, _rel(d[:, 1:-1], e[:, 1:-1]))
+        # backscatter clipping, same state: a component with u c < 0 is zero,
+        # every other value is the unclipped correction
+        _load(b, f)
+        b.closure_attach(net, clip=True)
+        b.nonlinear(2)
+        b.closure_attach(None)
+        for k in range(3):
+            assert np.array_equal(b.closure_phys(k), u[k])
+            cc = b.closure_phys(3 + k)
+            keep = u[k] * corr[k] >= 0
+            assert np.array_equal(cc[keep], corr[k][keep]) and (cc[~keep] == 0).all()
+            assert (~keep).any() and keep.any()
         # attached with every = 2: the network evaluates at kRK = 1 of steps
         # 0, 2, ... and the other substeps reuse the held correction, so the
         # kRK = 2 term moves by that same correction
