@@ -50,6 +50,12 @@ def class_bytes(ch, zb):
         # read 4 interior u-plane + 2 v-plane product stacks, write nxz1, nxz3,
         # vv (interior u planes), uv, vw (v planes) and nxz2 (interior v planes)
         "zfwd": 4 * Z(nyu - 2) + 2 * Z(nyv) + 3 * S(nyu - 2, zb) + 2 * S(nyv, zb) + S(nyv - 2, zb),
+        # k_rhs_seg, three components: read u, the old Nu and the x/z part of
+        # the nonlinear term of each, the three y fluxes (uv, vw on v planes,
+        # vv on u planes) and p once; write u and Nu of each
+        "ysolve": 4 * (2 * S(nyu) + S(nyv)) + S(ch.nyp) + zb / 16 * (2 * S(nyu) + S(nyv) + S(nyu) + 2 * S(nyv)),
+        # k_press_seg: read u1, u3, v, p; write psi, p, u1, u3, v and div
+        "press": 2 * (2 * S(nyu) + S(nyv)) + 4 * S(ch.nyp),
     }
 
 
@@ -105,6 +111,17 @@ def timed(params, f, q0, args, nprof, tuning=None):
     state["scalars"] = sc
     state["device_bytes"] = ch.device_bytes()
     return ch, args.steps / dt_wall, 1e3 * dt_wall / args.steps, prof, state
+
+
+def roofline_classes(ch, prof, zb, nprof):
+    """HBM roofline fraction of every kernel class with an algorithmic byte
+    count: bytes per substep / measured class time per substep."""
+    out = {}
+    for c, b in class_bytes(ch, zb).items():
+        ms = prof[c][0] / nprof
+        gbs = b / (ms * 1e-3) / 1e9
+        out[c] = {"algorithmic_bytes": b, "ms": ms, "achieved_GBps": gbs, "frac": gbs / PEAK_HBM_GBS}
+    return out
 
 
 def roofline(ch, prof, zb, workload, label):
@@ -261,6 +278,7 @@ def main():
             "dtype": "f32 transforms/products/nonlinear scratch; f64 fields, solves, projection, flow rate",
             "roofline": roofline(chm, profm, 8, args.workload, "_mixed"),
             "kernel_ms_per_substep": {c: ms / nprof for c, (ms, cnt) in profm.items()},
+            "roofline_per_class": roofline_classes(chm, profm, 8, nprof),
             "autotune": tsumm,
             "device_bytes": stm["device_bytes"],
             "err_vs_fp64": dict(diagnostics.compare(stm, st64, g, params.Re),
@@ -308,6 +326,7 @@ def main():
                    "precision": "fp64", "parallelism": "single-gpu"},
         "roofline": rl,
         "kernel_ms_per_substep": kms,
+        "roofline_per_class": roofline_classes(ch, prof, 16, nprof),
         "autotune": tsum64,
         "device_bytes": st64["device_bytes"],
         "mixed": mixed,

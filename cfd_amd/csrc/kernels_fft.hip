@@ -887,11 +887,17 @@ void zfwdp_t(Solver& s, const ZFwdJob* jobs, int njobs) {
 // four-step specializations: (L, P, Q, TI) of the z kernels, (L, P, Q, R)
 // of the x-pass.  The z-inverse takes Q | N2/2 and the z-forward P | N2/2
 // where the length allows, so whole groups of the empty band drop out.
+#ifndef CFD_ZINV4
 #define CFD_ZINV4(X) X(48, 6, 8, 16) X(64, 8, 8, 16) X(72, 6, 12, 16) X(96, 6, 16, 16) X(128, 8, 16, 16) \
   X(192, 12, 16, 16) X(384, 24, 16, 8)
+#endif
+#ifndef CFD_ZFWD4_F64
 #define CFD_ZFWD4_F64(X) X(48, 8, 6, 16) X(64, 8, 8, 16) X(72, 8, 9, 16) X(96, 8, 12, 16) X(128, 8, 16, 16) \
   X(192, 16, 12, 16) X(384, 24, 16, 8)
+#endif
+#ifndef CFD_ZFWD4_F32
 #define CFD_ZFWD4_F32(X) CFD_ZFWD4_F64(X)
+#endif
 #ifndef CFD_XPASS4_F64
 #define CFD_XPASS4_F64(X) X(48, 8, 6, 8) X(64, 8, 8, 8) X(72, 12, 6, 4) X(96, 8, 12, 8) X(128, 8, 16, 8) \
   X(144, 12, 12, 4) X(192, 16, 12, 4) X(256, 16, 16, 4) X(384, 16, 24, 4) X(768, 32, 24, 2)
