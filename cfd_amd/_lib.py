@@ -92,6 +92,8 @@ SIGNATURES = {
     "cfd_get_tuning": (ctypes.c_int, [_H, ctypes.POINTER(ctypes.c_int)]),
     "cfd_set_tuning": (ctypes.c_int, [_H, ctypes.POINTER(ctypes.c_int)]),
     "cfd_closure_clip": (ctypes.c_int, [_H, ctypes.c_int]),
+    "cfd_meanflow_ctP": (ctypes.c_int, [_H]),
+    "cfd_hist": (ctypes.c_int, [_H, _D]),
     "cfd_ensemble_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
                                            ctypes.POINTER(ctypes.c_void_p)]),
     "cfd_ensemble_destroy": (ctypes.c_int, [_H]),

@@ -261,6 +261,18 @@ class Channel:
         return out
 
     # -- timing ----------------------------------------------------------------
+    HIST = ("t", "dtv", "dtc", "dt", "err", "Qx", "QxT", "mpgx", "dgx", "Umax", "Uslip")
+
+    def hist(self):
+        """The history-file record of the current state (record_out,
+        FOU3D.f90:2504-2534) as a dict; sets the Qx scalar."""
+        v = np.zeros(len(self.HIST))
+        self._call("cfd_hist", _dp(v))
+        return dict(zip(self.HIST, v.tolist()))
+
+    def meanflow_ctP(self):
+        self._call("cfd_meanflow_ctP")
+
     def tuning(self):
         t = (ctypes.c_int * 3)()
         self._call("cfd_get_tuning", t)

@@ -1350,6 +1350,45 @@ extern "C" int cfd_step(cfd_handle h, int nsteps) {
   }
   return CFD_OK;
 }
+extern "C" int cfd_meanflow_ctP(cfd_handle h) {
+  cfd_solver* s = static_cast<cfd_solver*>(h);
+  if (int rc = check(s, 1)) return rc;
+  launch_flowrate(*s);
+  return launched(s);
+}
+
+// The quantities record_out prints and appends to the history file
+// (FOU3D.f90:2504-2534): Uslip, the flow rate (flowrateIm, which also sets
+// Qx), Umax (maxvel, littleharsh.f90:486-502) from the (0,0) column of u1,
+// and err, the largest |div| over the p planes (error, littleharsh.f90:322-349).
+extern "C" int cfd_hist(cfd_handle h, double* v) {
+  cfd_solver* s = static_cast<cfd_solver*>(h);
+  if (!s || !v) return fail(s, CFD_EINVAL, "null argument");
+  HIPCHK(s, hipSetDevice(s->dev));
+  launch_absmax(*s, s->F[CFD_DIV], (size_t)s->nyp * s->Ncol);
+  std::vector<double2> col(s->nyu);
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  HIPCHK(s, hipMemcpy2D(col.data(), sizeof(double2), s->F[CFD_U1], s->Ncol * sizeof(double2), sizeof(double2),
+                        s->nyu, hipMemcpyDeviceToHost));
+  DevScalars d{};
+  HIPCHK(s, hipMemcpy(&d, s->d_sc, sizeof d, hipMemcpyDeviceToHost));
+  const YGrid& g = s->yg;
+  auto U = [&](int j) { return col[j - s->jlo_u].x; };
+  const int nn = s->nn;
+  const double Uslip = (U(1) * (g.YV(0) - g.YU(0)) + U(0) * (g.YU(1) - g.YV(0))) / (g.YU(1) - g.YU(0));
+  double Q = 0.0;
+  for (int j = 1; j <= nn + 1; ++j) Q = Q + (g.YV(j) - g.YV(j - 1)) * U(j);
+  double Umax = 0.0;
+  for (int j = 0; j <= nn + 2; ++j) Umax = std::max(Umax, U(j));
+  double err;
+  std::memcpy(&err, &d.amax, sizeof err);
+  d.Qx = Q;
+  HIPCHK(s, hipMemcpy(&s->d_sc->Qx, &d.Qx, sizeof d.Qx, hipMemcpyHostToDevice));
+  const double out[11] = {d.t, s->dtv, d.dtc, d.dt, err, Q, d.QxT, d.mpgx, d.dgx, Umax, Uslip};
+  std::memcpy(v, out, sizeof out);
+  return CFD_OK;
+}
+
 extern "C" int cfd_sync(cfd_handle h) {
   cfd_solver* s = static_cast<cfd_solver*>(h);
   if (!s) return fail(s, CFD_EINVAL, "null handle");

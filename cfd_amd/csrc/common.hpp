@@ -75,7 +75,7 @@ struct IBDev {
 struct DevScalars {
   double t, dt, dti, mpgx, dgx, QxT, Qx, dtc;
   unsigned long long m1, m2, m3;  // CFL maxima as IEEE bit patterns (>= 0)
-  unsigned long long pad;
+  unsigned long long amax;        // max |div| of the history record, bit pattern (>= 0)
 };
 
 enum KClass { KC_ZINV = 0, KC_XPASS, KC_ZFWD, KC_ELEM, KC_YSOLVE, KC_PRESS, KC_IB, KC_N };
@@ -186,6 +186,7 @@ void launch_solve_p(Solver& s, int kRK);
 void launch_vcorr(Solver& s, int kRK);
 void launch_divergence(Solver& s);
 void launch_flowrate(Solver& s);
+void launch_absmax(Solver& s, const double2* a, size_t n);  // max |a| -> d_sc->amax
 bool seg_rows_supported(int rows);
 // statistics (kernels_stats.hip)
 struct StatsSpecArgs {

@@ -510,6 +510,22 @@ void launch_meanflow(Solver& s, int kRK) {
                      s.d_yv, s.jlo_v, s.d_mf, s.d_sc, s.bRK[kRK - 1], s.dRK[kRK - 1], s.P.Re, s.gw[0], s.gw[1]);
 }
 
+// max |a| over n complex values (the modulus, as Fortran's abs of a complex)
+// into *out, as the bit pattern of a non-negative double; *out starts at 0
+__global__ void k_absmax(const double2* __restrict__ a, size_t n, unsigned long long* out) {
+  double m = 0.0;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+    m = fmax(m, hypot(a[e].x, a[e].y));
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+}
+
+void launch_absmax(Solver& s, const double2* a, size_t n) {
+  unsigned long long* out = &s.d_sc->amax;
+  (void)hipMemsetAsync(out, 0, sizeof *out, s.stream);
+  hipLaunchKernelGGL(k_absmax, dim3(512), dim3(256), 0, s.stream, a, n, out);
+}
+
 void launch_flowrate(Solver& s) {
   hipLaunchKernelGGL(k_flowrate, dim3(1), dim3(1), 0, s.stream, (const double2*)s.F[CFD_U1], (size_t)s.Ncol, s.jlo_u,
                      s.nn, (const double*)s.d_yv, s.jlo_v, s.d_sc);

@@ -113,6 +113,24 @@ module cfd_amd_mod
       import :: c_int, c_ptr
       type(c_ptr), value :: h
     end function
+    ! meanflow_ctP (littleharsh.f90:504-520): flow rate Qx of the (0,0) column
+    integer(c_int) function cfd_meanflow_ctP(h) bind(C, name="cfd_meanflow_ctP")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    ! the history record of record_out (FOU3D.f90:2504-2534): t, dtv, dtc,
+    ! dt, err, Qx, QxT, mpgx, dgx, Umax, Uslip
+    integer(c_int) function cfd_hist(h, v) bind(C, name="cfd_hist")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double) :: v(11)
+    end function
+    ! the band table N(1:4, 0:4) (start.f90:89-111; y_grid_canopy.f90:250-275)
+    integer(c_int) function cfd_band_table(p, N20) bind(C, name="cfd_band_table")
+      import :: c_int, cfd_params
+      type(cfd_params), intent(in) :: p
+      integer(c_int) :: N20(20)
+    end function
     ! record_out (FOU3D.f90:2172-2345): u1, u2, u3, p files in dirout
     integer(c_int) function cfd_record_out(h, dirout, iter) bind(C, name="cfd_record_out")
       import :: c_int, c_ptr, c_char

@@ -11,7 +11,9 @@
 ! and once more at the end (finalize, littleharsh.f90:353-390), samples the
 ! statistics and spectra every nstat = min(20, nwrite) steps (start.f90:372;
 ! FOU3D.f90:56-61, 107-110, 185-221) and writes them with each record
-! (FOU3D.f90:223-226).  The reference counts the sampling interval from
+! (FOU3D.f90:223-226), and appends a record to the history file hist_*.dat
+! with each field record (start.f90:245-259; FOU3D.f90:2504-2534).  The
+! reference counts the sampling interval from
 ! iter0, which it also resets when it samples; a counter of its own, iterst,
 ! does that here so iter0 keeps the start of the run for maxiter.
 program littleharsh_gpu
@@ -23,6 +25,11 @@ program littleharsh_gpu
   integer(c_int) :: rc, d(16), kRK, iter0, dst(6), sflags
   integer :: i, Ngalx2, Ngalz2, flag_init, nwrite, iter, iwrite, maxiter, npeakx, npeakz, dstx, dstz
   integer :: nstat, iterst
+  integer(c_int) :: N20(20)
+  real(8) :: hv(11)
+  character(len=4) :: e1, e2, e3
+  character(len=200) :: hname
+  logical :: hexist
   integer :: physlim_bot, physlim_top, nyu, nyv, nyp, ncol, jlo, nn, j
   real(8) :: Kib, maxerr, maxt, maxA, hny, Lfracx, Lfracz, sc(8), Qx
   character(len=90) :: fnameimb, dirin, dirout
@@ -190,6 +197,22 @@ program littleharsh_gpu
   end if
 
   write(*,'(a,3i6,a,i8,a,es12.5)') ' grid', p%ngalx, p%ny, p%ngalz, '  columns', ncol, '  QxT', Qx
+  ! history file (start.f90:245-259): appended to when it exists, else
+  ! created with the header records; t in the header is the value the
+  ! reference's t holds before any initial condition is read, 0
+  rc = cfd_band_table(p, N20);         call cfd_check(rc, h, 'band table')
+  write(e1,'(i4.4)') d(1)
+  write(e2,'(i4.4)') d(2)
+  write(e3,'(i4.4)') p%ny
+  hname = trim(dirout)//'hist_'//e1//'x'//e2//'x'//e3//'.dat'
+  inquire(file=trim(hname), exist=hexist)
+  if (hexist) then
+    open(30, file=trim(hname), form='unformatted', status='old', position='append', action='write')
+  else
+    open(30, file=trim(hname), form='unformatted', status='new', action='write')
+    write(30) 0d0, p%Re, p%alp, p%bet, p%mpgx, 3
+    write(30) N20
+  end if
   iter = iter0
   iwrite = iter
   iterst = iter0
@@ -210,6 +233,7 @@ program littleharsh_gpu
     if (iter >= iwrite) then
       rc = cfd_record_out(h, trim(dirout)//c_null_char, iter)
       call cfd_check(rc, h, 'record_out')
+      call write_hist()
       iwrite = iwrite + nwrite
       if (nstat /= huge(1)) sflags = sflags + 2
     end if
@@ -229,6 +253,9 @@ program littleharsh_gpu
       rc = cfd_solve_p(h, kRK);        call cfd_check(rc, h, 'solveP')
       rc = cfd_vcorr(h, kRK);          call cfd_check(rc, h, 'v_corr')
     end do
+    if (p%flag_ctpress == 1) then
+      rc = cfd_meanflow_ctP(h);        call cfd_check(rc, h, 'meanflow_ctP')
+    end if
     iter = iter + 1
     rc = cfd_get_scalars(h, sc);       call cfd_check(rc, h, 'get scalars')
     if (mod(iter, max(nwrite, 1)) == 0 .or. iter - iter0 == maxiter) then
@@ -239,7 +266,23 @@ program littleharsh_gpu
   ! finalize: one more record of the final state
   rc = cfd_record_out(h, trim(dirout)//c_null_char, iter)
   call cfd_check(rc, h, 'record_out')
+  call write_hist()
+  close(30)
   rc = cfd_get_field(h, CFD_U1, u1);   call cfd_check(rc, h, 'get u1')
   write(*,'(a,es22.15)') ' u1(0,0) at the centreline ', real(u1(nyu/2 + 1, 1), 8)
   rc = cfd_destroy(h)
+
+contains
+
+  ! the history record that record_out appends (FOU3D.f90:2504-2534)
+  subroutine write_hist()
+    rc = cfd_hist(h, hv);              call cfd_check(rc, h, 'hist')
+    if (p%flag_ctpress == 0) then
+      write(30) p%flag_ctpress, iter, hv(1), hv(2), hv(3), hv(4), hv(5), hv(6), hv(7), hv(8), hv(9), hv(10), hv(11)
+    else
+      write(30) p%flag_ctpress, iter, hv(1), hv(2), hv(3), hv(4), hv(5), hv(6), hv(8), hv(10), hv(11)
+    end if
+    flush(30)
+  end subroutine
+
 end program littleharsh_gpu

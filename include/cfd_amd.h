@@ -126,6 +126,15 @@ int cfd_substep(cfd_handle h, int kRK);          /* body of the RK loop, littleh
    with the RK loop restored), plus meanflow_ctP when flag_ctpress = 1.
    Asynchronous: returns once the work is queued on the handle's stream. */
 int cfd_step(cfd_handle h, int nsteps);
+int cfd_meanflow_ctP(cfd_handle h);              /* meanflow_ctP    littleharsh.f90:504-520 */
+/* The record of the history file (FOU3D.f90:2504-2534), v[11] = t, dtv,
+   dtc, dt, err, Qx, QxT, mpgx, dgx, Umax, Uslip: err = max |div| over the p
+   planes (error, littleharsh.f90:322-349), Qx the tip-to-tip flow rate of
+   the (0,0) column of u1 (flowrateIm; stored as the Qx scalar, as record_out
+   does), Umax its largest real part between the tips (maxvel,
+   littleharsh.f90:486-502), Uslip its value interpolated to the lower tip
+   face yv(0).  Synchronous. */
+int cfd_hist(cfd_handle h, double* v);
 /* ---------------------------------------------------------------------------
  * Field files in the reference layout: Fortran sequential unformatted, 4-byte
  * little-endian record markers (record_out, FOU3D.f90:2172-2345; u_to_buff /
