@@ -79,19 +79,20 @@ __device__ __forceinline__ FieldPtrs field_ptrs(const FusedArgs& a, int f) {
 
 // du and r of row q (interior) of field f for column col, from the loaded
 // values; returns r, stores the new Nu through nnew
-__device__ __forceinline__ double2 row_r(const FusedArgs& a, int f, int q, int col, double k2x, double k2z,
-                                         double kk, double dt, double2 um, double2 u0, double2 up, double2 nold,
-                                         double2 nx, double2 fa, double2 fb, double2 pa, double2 pb,
+// d: the three d2 coefficients of row q, dd: dtheta/dy of row q (d2v and
+// dthdyv for f = 1, d2u and dthdyu otherwise)
+__device__ __forceinline__ double2 row_r(const FusedArgs& a, int f, int col, double k2x, double k2z, double kk,
+                                         double dt, const double* d, double dd, double2 um, double2 u0, double2 up,
+                                         double2 nold, double2 nx, double2 fa, double2 fb, double2 pa, double2 pb,
                                          double2& nnew) {
   struct {
     double2 um, u0, up, nold, nx, fa, fb, pa, pb;
   } v = {um, u0, up, nold, nx, fa, fb, pa, pb};
-  const double* d = (f == 1 ? a.d2v : a.d2u) + 3 * q;
   const double c2 = k2x + k2z + d[1];
   const double2 L = make_double2(d[0] * v.um.x + c2 * v.u0.x + d[2] * v.up.x, d[0] * v.um.y + c2 * v.u0.y + d[2] * v.up.y);
   double2 du;
   if (f == 1) {
-    const double C3 = -a.dRK * a.ddthetavi * a.dthdyv[q];
+    const double C3 = -a.dRK * a.ddthetavi * dd;
     du = make_double2(a.C1 * L.x + a.C2 * v.nold.x + C3 * (v.pa.x - v.pb.x),
                       a.C1 * L.y + a.C2 * v.nold.y + C3 * (v.pa.y - v.pb.y));
   } else {
@@ -99,7 +100,6 @@ __device__ __forceinline__ double2 row_r(const FusedArgs& a, int f, int q, int c
     du = make_double2(a.C1 * L.x + a.C2 * v.nold.x + gp.x, a.C1 * L.y + a.C2 * v.nold.y + gp.y);
     if (f == 0 && col == 0) du.x += -a.dRK * a.sc->mpgx;
   }
-  const double dd = (f == 1 ? a.dthdyv[q] : a.dthdyu[q]);
   nnew = make_double2(v.nx.x + (v.fa.x - v.fb.x) * dd * a.ddthetavi, v.nx.y + (v.fa.y - v.fb.y) * dd * a.ddthetavi);
   du = make_double2(du.x + a.C1g * nnew.x, du.y + a.C1g * nnew.y);
   return make_double2(v.u0.x + dt * du.x, v.u0.y + dt * du.y);
@@ -172,8 +172,8 @@ __global__ void __launch_bounds__(64) k_rhs_solve(FusedArgs a) {
       const int q = q0 + c;
       if (q >= n - 1) break;
       double2 nnew;
-      double2 rq = row_r(a, f, q, col, k2x, k2z, kk, dt, um, u0, UP[c], NO[c], NX[c], FA[c], FB[c], PA[c], PB[c],
-                         nnew);
+      double2 rq = row_r(a, f, col, k2x, k2z, kk, dt, d2 + 3 * q, f == 1 ? a.dthdyv[q] : a.dthdyu[q], um, u0, UP[c],
+                         NO[c], NX[c], FA[c], FB[c], PA[c], PB[c], nnew);
       F.nu[(size_t)q * N + col] = nnew;
       if (ib[q]) rq = RB[c];
       um = u0;
@@ -244,7 +244,8 @@ struct RowSys {
 
 struct RowCtx {
   double k2x, k2z, kk, dt, beta, g0, g1;
-  const double* d2;
+  const double* md;          // LDS: per row the three d2 coefficients and dtheta/dy
+  const unsigned char* ibl;  // LDS: 1 on the IB rows
   int n, col;
   size_t N;
 };
@@ -268,10 +269,10 @@ __device__ __forceinline__ RowSys seg_row(const FusedArgs& a, const FieldPtrs& F
   } else {
     double2 up, nold, nx, fa, fb, pa, pb, nnew;
     row_load<TN>(a, F, f, q, c.N, c.col, up, nold, nx, fa, fb, pa, pb);
-    s.r = row_r(a, f, q, c.col, c.k2x, c.k2z, c.kk, c.dt, um, u0, un, nold, nx, fa, fb, pa, pb, nnew);
+    const double* dq = c.md + 4 * q;
+    s.r = row_r(a, f, c.col, c.k2x, c.k2z, c.kk, c.dt, dq, dq[3], um, u0, un, nold, nx, fa, fb, pa, pb, nnew);
     F.nu[(size_t)q * c.N + c.col] = nnew;
-    if (F.ib[q]) s.r = F.r[(size_t)q * c.N + c.col];
-    const double* dq = c.d2 + 3 * q;
+    if (c.ibl[q]) s.r = F.r[(size_t)q * c.N + c.col];
     s.lo = -c.beta * dq[0];
     s.dg = 1.0 - c.beta * (dq[1] + c.k2x + c.k2z);
     s.up = -c.beta * dq[2];
@@ -301,9 +302,25 @@ __global__ void __launch_bounds__(256) k_rhs_seg(FusedArgs a) {
   double* As = He + NT;
   double* Ds = As + NT;
   double* Cs = Ds + NT;
+  // per-row metadata of field f staged in LDS: the row loop then waits on
+  // the field loads only
+  double* md = Cs + NT;
+  unsigned char* ibl = reinterpret_cast<unsigned char*>(md + 4 * n);
   const bool act = col < a.Ncol && sg < S;
   const size_t N = a.Ncol;
   const FieldPtrs F = field_ptrs(a, f);
+  {
+    const double* d2 = (f == 1) ? a.d2v : a.d2u;
+    const double* dth = (f == 1) ? a.dthdyv : a.dthdyu;
+    for (int q = me; q < n; q += NT) {
+      md[4 * q] = d2[3 * q];
+      md[4 * q + 1] = d2[3 * q + 1];
+      md[4 * q + 2] = d2[3 * q + 2];
+      md[4 * q + 3] = dth[q];
+      ibl[q] = F.ib[q];
+    }
+  }
+  __syncthreads();
   const double2* u = F.u;
   const int b = sg * (SEG + 1);
   double2 y[SEG];
@@ -316,7 +333,8 @@ __global__ void __launch_bounds__(256) k_rhs_seg(FusedArgs a) {
     c.kk = (f == 0) ? a.k1x[i] : a.k1z[k];
     c.dt = a.sc->dt;
     c.beta = a.bRK * c.dt / a.Re;
-    c.d2 = (f == 1) ? a.d2v : a.d2u;
+    c.md = md;
+    c.ibl = ibl;
     c.g0 = (f == 1) ? 0.0 : a.g0;
     c.g1 = (f == 1) ? 0.0 : a.g1;
     c.n = n;
@@ -423,7 +441,8 @@ __global__ void __launch_bounds__(256) k_ib_rhs(FusedArgs a, const int* __restri
   row_load<TN>(a, F, f, q, N, col, up, nold, nx, fa, fb, pa, pb);
   double2 nnew;
   F.r[(size_t)q * N + col] =
-      row_r(a, f, q, col, a.k2x[i], a.k2z[k], kk, a.sc->dt, um, u0, up, nold, nx, fa, fb, pa, pb, nnew);
+      row_r(a, f, col, a.k2x[i], a.k2z[k], kk, a.sc->dt, (f == 1 ? a.d2v : a.d2u) + 3 * q,
+            f == 1 ? a.dthdyv[q] : a.dthdyu[q], um, u0, up, nold, nx, fa, fb, pa, pb, nnew);
 }
 
 // ---------------------------------------------------------------------------
@@ -595,6 +614,21 @@ __global__ void __launch_bounds__(256) k_press_seg(PressArgs a) {
   double* As = He + NT;
   double* Ds = As + NT;
   double* Cs = Ds + NT;
+  // per-row metadata staged in LDS: plo, pdg, pup of p row q, dtheta/dy of
+  // the u rows 0..n+1 and of the v faces 0..n
+  double* lp = Cs + NT;
+  double* ldu = lp + 3 * n;
+  double* ldv = ldu + n + 2;
+  for (int q = me; q < n + 2; q += NT) {
+    if (q < n) {
+      lp[3 * q] = a.plo[q];
+      lp[3 * q + 1] = a.pdg[q];
+      lp[3 * q + 2] = a.pup[q];
+    }
+    ldu[q] = a.dthdyu[q];
+    if (q < n + 1) ldv[q] = a.dthdyv[q];
+  }
+  __syncthreads();
   const bool act = col < a.Ncol && sg < S;
   const size_t N = a.Ncol;
   const int b = sg * (SEG + 1);
@@ -618,13 +652,13 @@ __global__ void __launch_bounds__(256) k_press_seg(PressArgs a) {
       const double2 U = a.u1[(size_t)(q + 1) * N + col], W = a.u3[(size_t)(q + 1) * N + col];
       const double2 V1 = a.u2[(size_t)(q + 1) * N + col];
       const double2 du = imulk(kx, U), dw = imulk(kz, W);
-      const double m = a.ddthetavi * a.dthdyu[q + 1];
+      const double m = a.ddthetavi * ldu[q + 1];
       const double2 dv = make_double2(du.x + dw.x + m * (V1.x - vq.x), du.y + dw.y + m * (V1.y - vq.y));
       vq = V1;
       rhs = make_double2(dtirk * dv.x, dtirk * dv.y);
-      lo = a.plo[q];
-      dg = a.pdg[q] + k2;
-      up = a.pup[q];
+      lo = lp[3 * q];
+      dg = lp[3 * q + 1] + k2;
+      up = lp[3 * q + 2];
       if (col == 0 && q == 0) {
         rhs = make_double2(0.0, 0.0);
         lo = 0.0;
@@ -727,13 +761,13 @@ __global__ void __launch_bounds__(256) k_press_seg(PressArgs a) {
     double2 vf = make_double2(0.0, 0.0);  // bottom wall face
     if (q > 0) {
       const double2 V = a.u2[(size_t)q * N + col];
-      const double cc = dtrk * a.ddthetavi * a.dthdyv[q];
+      const double cc = dtrk * a.ddthetavi * ldv[q];
       vf = make_double2(V.x - cc * (ps.x - psm.x), V.y - cc * (ps.y - psm.y));
     }
     a.u2[(size_t)q * N + col] = vf;
     if (q > b) {
       const double2 du = imulk(kx, unp), dw = imulk(kz, wnp);
-      const double m = a.ddthetavi * a.dthdyu[q];
+      const double m = a.ddthetavi * ldu[q];
       a.div[(size_t)(q - 1) * N + col] =
           make_double2(du.x + dw.x + m * (vf.x - vprev.x), du.y + dw.y + m * (vf.y - vprev.y));
     }
@@ -763,11 +797,11 @@ __global__ void __launch_bounds__(256) k_press_seg(PressArgs a) {
     const int B = me + CB;
     const double2 xr2 = sg + 1 < S - 1 ? Xs[B] : make_double2(0.0, 0.0);
     const double2 ps1 = make_double2(Yb[B].x + Gb[B] * xr2.x + Hb[B] * xr.x, Yb[B].y + Gb[B] * xr2.y + Hb[B] * xr.y);
-    const double cc = dtrk * a.ddthetavi * a.dthdyv[last + 1];
+    const double cc = dtrk * a.ddthetavi * ldv[last + 1];
     vf = make_double2(vnext.x - cc * (ps1.x - psm.x), vnext.y - cc * (ps1.y - psm.y));
   }
   const double2 du = imulk(kx, unp), dw = imulk(kz, wnp);
-  const double m = a.ddthetavi * a.dthdyu[last + 1];
+  const double m = a.ddthetavi * ldu[last + 1];
   a.div[(size_t)last * N + col] = make_double2(du.x + dw.x + m * (vf.x - vprev.x), du.y + dw.y + m * (vf.y - vprev.y));
 }
 
@@ -954,7 +988,9 @@ template <int SEG>
 void rhs_seg_launch(Solver& s, const FusedArgs& a, int sy) {
   int cb = 1;
   while (cb * 2 * sy <= s.solve_threads) cb *= 2;
-  const size_t lds = sizeof(double) * 15 * (size_t)cb * sy;
+  // separator system (15 doubles per thread), then per row 4 doubles of
+  // metadata and the IB flag (nyu rows: the longer of the u and v columns)
+  const size_t lds = sizeof(double) * (15 * (size_t)cb * sy + 4 * (size_t)s.nyu) + (size_t)s.nyu;
   const dim3 grid((s.Ncol + cb - 1) / cb, 3), block(cb, sy);
   if (s.mixed)
     hipLaunchKernelGGL((k_rhs_seg<SEG, float2>), grid, block, lds, s.stream, a);
@@ -966,7 +1002,9 @@ template <int SEG>
 void press_seg_launch(Solver& s, const PressArgs& a, int sy) {
   int cb = 1;
   while (cb * 2 * sy <= s.solve_threads) cb *= 2;
-  const size_t lds = sizeof(double) * 15 * (size_t)cb * sy;
+  // separator system (15 doubles per thread), then 5 nyp + 3 doubles of
+  // per-row metadata
+  const size_t lds = sizeof(double) * (15 * (size_t)cb * sy + 5 * (size_t)s.nyp + 3);
   hipLaunchKernelGGL(k_press_seg<SEG>, dim3((s.Ncol + cb - 1) / cb), dim3(cb, sy), lds, s.stream, a);
 }
 
