@@ -250,27 +250,43 @@ struct RowCtx {
   size_t N;
 };
 
-// row q of the segmented solve; um/u0 hold u(q-1), u(q) and are advanced;
-// interior rows store the new Nu
+// The same row in two halves, so that the loads of row q+1 can be issued
+// before the arithmetic of row q: seg_fetch loads what row q reads from HBM
+// (u(q+1) and, for interior rows, the old Nu, N_xz, the y fluxes and p),
+// seg_build forms the row from them.
+struct RowData {
+  double2 un, nold, nx, fa, fb, pa, pb;
+};
+
 template <typename TN>
-__device__ __forceinline__ RowSys seg_row(const FusedArgs& a, const FieldPtrs& F, int f, const RowCtx& c, int q,
-                                          double2& um, double2& u0) {
+__device__ __forceinline__ RowData seg_fetch(const FusedArgs& a, const FieldPtrs& F, int f, const RowCtx& c, int q) {
+  RowData d;
+  d.un = make_double2(0.0, 0.0);
+  if (q >= c.n) return d;
+  if (q + 1 < c.n) d.un = F.u[(size_t)(q + 1) * c.N + c.col];
+  if (q > 0 && q < c.n - 1) {
+    double2 up;
+    row_load<TN>(a, F, f, q, c.N, c.col, up, d.nold, d.nx, d.fa, d.fb, d.pa, d.pb);
+  }
+  return d;
+}
+
+__device__ __forceinline__ RowSys seg_build(const FusedArgs& a, const FieldPtrs& F, int f, const RowCtx& c, int q,
+                                            const RowData& d, double2& um, double2& u0) {
   RowSys s;
   if (q >= c.n) {
     s.lo = 0.0; s.dg = 1.0; s.up = 0.0; s.r = make_double2(0.0, 0.0);
     return s;
   }
-  double2 un = make_double2(0.0, 0.0);
-  if (q + 1 < c.n) un = F.u[(size_t)(q + 1) * c.N + c.col];
   if (q == 0) {
     s.lo = 0.0; s.dg = 1.0; s.up = c.g0; s.r = make_double2(0.0, 0.0);
   } else if (q == c.n - 1) {
     s.lo = c.g1; s.dg = 1.0; s.up = 0.0; s.r = make_double2(0.0, 0.0);
   } else {
-    double2 up, nold, nx, fa, fb, pa, pb, nnew;
-    row_load<TN>(a, F, f, q, c.N, c.col, up, nold, nx, fa, fb, pa, pb);
+    double2 nnew;
     const double* dq = c.md + 4 * q;
-    s.r = row_r(a, f, c.col, c.k2x, c.k2z, c.kk, c.dt, dq, dq[3], um, u0, un, nold, nx, fa, fb, pa, pb, nnew);
+    s.r = row_r(a, f, c.col, c.k2x, c.k2z, c.kk, c.dt, dq, dq[3], um, u0, d.un, d.nold, d.nx, d.fa, d.fb, d.pa, d.pb,
+                nnew);
     F.nu[(size_t)q * c.N + c.col] = nnew;
     if (c.ibl[q]) s.r = F.r[(size_t)q * c.N + c.col];
     s.lo = -c.beta * dq[0];
@@ -278,7 +294,7 @@ __device__ __forceinline__ RowSys seg_row(const FusedArgs& a, const FieldPtrs& F
     s.up = -c.beta * dq[2];
   }
   um = u0;
-  u0 = un;
+  u0 = d.un;
   return s;
 }
 
@@ -345,9 +361,14 @@ __global__ void __launch_bounds__(256) k_rhs_seg(FusedArgs a) {
     // forward elimination inside the segment
     double2 yp = make_double2(0.0, 0.0);
     double gp = 0.0, hp = 1.0;
+    RowData cur = seg_fetch<TN>(a, F, f, c, b);
 #pragma unroll
     for (int j = 0; j < SEG; ++j) {
-      const RowSys rs = seg_row<TN>(a, F, f, c, b + j, um, u0);
+      // rows b .. b+SEG-1, then the separator b+SEG (fetched for every
+      // segment: past the column it loads nothing)
+      const RowData nxt = seg_fetch<TN>(a, F, f, c, b + j + 1);
+      const RowSys rs = seg_build(a, F, f, c, b + j, cur, um, u0);
+      cur = nxt;
       const double inv = 1.0 / (rs.dg - rs.lo * gp);
       y[j] = make_double2((rs.r.x - rs.lo * yp.x) * inv, (rs.r.y - rs.lo * yp.y) * inv);
       g[j] = rs.up * inv;
@@ -358,7 +379,7 @@ __global__ void __launch_bounds__(256) k_rhs_seg(FusedArgs a) {
     }
     // separator row after the segment
     if (sg < S - 1) {
-      const RowSys rs = seg_row<TN>(a, F, f, c, b + SEG, um, u0);
+      const RowSys rs = seg_build(a, F, f, c, b + SEG, cur, um, u0);
       As[me] = rs.lo;
       Ds[me] = rs.dg;
       Cs[me] = rs.up;
