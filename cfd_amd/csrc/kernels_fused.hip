@@ -664,14 +664,27 @@ __global__ void __launch_bounds__(256) k_press_seg(PressArgs a) {
     const double k2 = a.k2x[i] + a.k2z[k];
     const double dtirk = a.sc->dti / a.apb;
     double2 vq = a.u2[(size_t)b * N + col];  // old v at the face below row b
-    // row q: divergence of the old field as right-hand side
-    auto row = [&](int q, double& lo, double& dg, double& up, double2& rhs) {
+    // row q: divergence of the old field as right-hand side; fetch issues
+    // the loads of a row (u1, u3 above p row q and v of the face above it)
+    // so that they can be in flight during the previous row's arithmetic
+    struct Vel {
+      double2 U, W, V1;
+    };
+    auto fetch = [&](int q) {
+      Vel v;
+      if (q < n) {
+        v.U = a.u1[(size_t)(q + 1) * N + col];
+        v.W = a.u3[(size_t)(q + 1) * N + col];
+        v.V1 = a.u2[(size_t)(q + 1) * N + col];
+      }
+      return v;
+    };
+    auto row = [&](int q, const Vel& v, double& lo, double& dg, double& up, double2& rhs) {
       if (q >= n) {
         lo = 0.0; dg = 1.0; up = 0.0; rhs = make_double2(0.0, 0.0);
         return;
       }
-      const double2 U = a.u1[(size_t)(q + 1) * N + col], W = a.u3[(size_t)(q + 1) * N + col];
-      const double2 V1 = a.u2[(size_t)(q + 1) * N + col];
+      const double2 U = v.U, W = v.W, V1 = v.V1;
       const double2 du = imulk(kx, U), dw = imulk(kz, W);
       const double m = a.ddthetavi * ldu[q + 1];
       const double2 dv = make_double2(du.x + dw.x + m * (V1.x - vq.x), du.y + dw.y + m * (V1.y - vq.y));
@@ -689,11 +702,14 @@ __global__ void __launch_bounds__(256) k_press_seg(PressArgs a) {
     };
     double2 yp = make_double2(0.0, 0.0);
     double gp = 0.0, hp = 1.0;
+    Vel cur = fetch(b);
 #pragma unroll
     for (int j = 0; j < SEG; ++j) {
       double lo, dg, up;
       double2 r;
-      row(b + j, lo, dg, up, r);
+      const Vel nxt = fetch(b + j + 1);
+      row(b + j, cur, lo, dg, up, r);
+      cur = nxt;
       const double inv = 1.0 / (dg - lo * gp);
       y[j] = make_double2((r.x - lo * yp.x) * inv, (r.y - lo * yp.y) * inv);
       g[j] = up * inv;
@@ -705,7 +721,7 @@ __global__ void __launch_bounds__(256) k_press_seg(PressArgs a) {
     if (sg < S - 1) {
       double lo, dg, up;
       double2 r;
-      row(b + SEG, lo, dg, up, r);
+      row(b + SEG, cur, lo, dg, up, r);
       vnext = vq;  // old v of face b+SEG+1
       As[me] = lo;
       Ds[me] = dg;
