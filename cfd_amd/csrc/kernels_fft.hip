@@ -431,17 +431,6 @@ __device__ __forceinline__ C from_d2(double2 v) {
   return mk<C>((real_t<C>)v.x, (real_t<C>)v.y);
 }
 
-__device__ __forceinline__ double2 zinv_load(const ZInvJob& J, size_t off, size_t ncol) {
-  if (J.mode == 0) return J.src[(size_t)J.pa * ncol + off];
-  const double2 a = J.src[(size_t)J.pa * ncol + off], b = J.src[(size_t)J.pb * ncol + off];
-  double2 v = make_double2(J.wa * a.x + J.wb * b.x, J.wa * a.y + J.wb * b.y);
-  if (J.mode == 2) {
-    const double2 c = J.src[(size_t)J.pc * ncol + off];
-    v = make_double2(J.g * (c.x - v.x) + v.x, J.g * (c.y - v.y) + v.y);
-  }
-  return v;
-}
-
 // z-inverse, four-step: TI*max(P,Q) threads; LDS X[k1][q][t].  Destination
 // rows have stride zs elements.  Block -> (job, x-tile): with jper = 0 the
 // grid is (ntiles, njobs); with jper > 0 it is one-dimensional and XCD-aware:
@@ -473,17 +462,66 @@ __global__ void __launch_bounds__(TI * cmax(P, Q)) k_zinv4(const ZInvJob* __rest
   const size_t ncol = (size_t)N2 * Mx;
   if (tid < Q * TI) {
     const int t = tid % TI, q = tid / TI, i = i0 + t;
-    C v[P];
+    // element offsets of the P inputs (-1: zero); all loads of the thread are
+    // issued before any of them is used (the job's mode is uniform)
+    int o[P];
     sfor<0, P>([&](auto pc) {
       constexpr int p = decltype(pc)::value;
       if constexpr (Q * p >= H && Q * p + Q <= L - H) {
-        v[p] = mk<C>(0, 0);  // the whole group lies in the empty band
+        o[p] = -1;  // the whole group lies in the empty band
       } else {
         const int n = Q * p + q;
         int k = n < H ? n : (n >= L - H ? n - L + N2 : -1);
         if ((J.flags & 1) && k == H) k = -1;
-        v[p] = (k >= 0 && i < Mx) ? from_d2<C>(zinv_load(J, (size_t)k * Mx + i, ncol)) : mk<C>(0, 0);
+        o[p] = (k >= 0 && i < Mx) ? k * Mx + i : -1;
       }
+    });
+    // global-address-space loads of 16-byte vectors (the job's pointer,
+    // read from memory, would otherwise be a flat pointer)
+    typedef double d2v __attribute__((ext_vector_type(2)));
+    using G = const __attribute__((address_space(1))) d2v*;
+    auto ld = [&](G src, int off) {
+      if (off < 0) return make_double2(0.0, 0.0);
+      const d2v r = src[off];
+      return make_double2(r.x, r.y);
+    };
+    const G srcA = (G)(J.src + (size_t)J.pa * ncol);
+    double2 x[P];
+    sfor<0, P>([&](auto pc) {
+      constexpr int p = decltype(pc)::value;
+      x[p] = ld(srcA, o[p]);
+    });
+    if (J.mode != 0) {
+      // y interpolation (interp_u / interp_v); mode 2 adds the ghost weight
+      const G srcB = (G)(J.src + (size_t)J.pb * ncol);
+      double2 y[P];
+      sfor<0, P>([&](auto pc) {
+        constexpr int p = decltype(pc)::value;
+        y[p] = ld(srcB, o[p]);
+      });
+      if (J.mode == 2) {
+        const G srcC = (G)(J.src + (size_t)J.pc * ncol);
+        double2 z[P];
+        sfor<0, P>([&](auto pc) {
+          constexpr int p = decltype(pc)::value;
+          z[p] = ld(srcC, o[p]);
+        });
+        sfor<0, P>([&](auto pc) {
+          constexpr int p = decltype(pc)::value;
+          const double2 w = make_double2(J.wa * x[p].x + J.wb * y[p].x, J.wa * x[p].y + J.wb * y[p].y);
+          x[p] = make_double2(J.g * (z[p].x - w.x) + w.x, J.g * (z[p].y - w.y) + w.y);
+        });
+      } else {
+        sfor<0, P>([&](auto pc) {
+          constexpr int p = decltype(pc)::value;
+          x[p] = make_double2(J.wa * x[p].x + J.wb * y[p].x, J.wa * x[p].y + J.wb * y[p].y);
+        });
+      }
+    }
+    C v[P];
+    sfor<0, P>([&](auto pc) {
+      constexpr int p = decltype(pc)::value;
+      v[p] = from_d2<C>(x[p]);
     });
     dft<true, P>(v);
     X[q * TI + t] = v[0];
