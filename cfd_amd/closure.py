@@ -13,7 +13,8 @@ the samples (one every `every` steps, trained on online from a device ring
 buffer) and is the judge; the coarse solver on 192 x 192 x 96 then runs from
 the box-filtered start field over the same simulated time without and with
 the closure (as predicted, and backscatter-clipped: components that would feed
-energy into the resolved field set to zero).  Writes closure_weights.npz and
+energy into the resolved field set to zero), and once more with the closure
+and the full DNS as fallback (run_with_fallback).  Writes closure_weights.npz and
 closure_report.json: a-priori
 loss on held-out samples against predicting no correction, drag (the mean
 pressure gradient at constant flow rate) and mean-profile errors against the
@@ -68,6 +69,73 @@ def filtered_start(coarse_params, fine_fields, fine_params):
         a = f.reshape(N2f, Mxf, ny)[kf][:, :Mxc, :] * h[:, :, None]
         out[name] = a.reshape(N2c * Mxc, ny)
     return out
+
+
+def prolong(fine_params, coarse_fields, coarse_params):
+    """Coarse spectral fields zero-padded onto the fine modes: the column
+    (i, kz) of the coarse grid goes to the column (i, kz) of the fine grid
+    (the mode selection of filtered_start inverted; the box filter is not
+    undone), every other fine mode is zero."""
+    N1f, N2f = 2 * (fine_params.ngalx // 3), 2 * (fine_params.ngalz // 3)
+    N1c, N2c = 2 * (coarse_params.ngalx // 3), 2 * (coarse_params.ngalz // 3)
+    Mxf, Mxc = N1f // 2 + 1, N1c // 2 + 1
+    kz = np.arange(N2c)
+    kz = np.where(kz < N2c // 2, kz, kz - N2c)
+    kf = np.where(kz >= 0, kz, N2f + kz)
+    out = {}
+    for name, f in coarse_fields.items():
+        ny = f.shape[1]
+        a = np.zeros((N2f, Mxf, ny), dtype=complex)
+        a[kf, :Mxc, :] = f.reshape(N2c, Mxc, ny)
+        out[name] = a.reshape(N2f * Mxf, ny)
+    return out
+
+
+def run_with_fallback(coarse, fine, t_end, max_steps, drag_cap, checkpoint_every=50):
+    """The coarse run with the full DNS as its fallback.  The coarse solver
+    (closure attached or not) steps until t >= t_end; every checkpoint_every
+    steps its state is copied to the host.  When its drag leaves
+    [-drag_cap, drag_cap] or stops being finite, the fine solver takes the
+    last checkpoint (zero-padded onto the fine modes, with its t, dt and mean
+    pressure gradient) and runs on to t_end.  Returns the (t, drag, on_fine)
+    history, the t of the switch (None without one) and the wall time."""
+    names = ("u1", "u2", "u3", "p")
+
+    def snap(ch):
+        return {n: ch.get_field(n) for n in names}, ch.scalars()
+
+    hist = []
+    w0 = time.perf_counter()
+    ck = snap(coarse)
+    switch = None
+    for k in range(max_steps):
+        coarse.step(1)
+        s = coarse.scalars()
+        d = -s["mpgx"]
+        if not (np.isfinite(d) and abs(d) <= drag_cap):
+            switch = ck[1]["t"]
+            break
+        hist.append((s["t"], d, 0))
+        if s["t"] >= t_end:
+            break
+        if (k + 1) % checkpoint_every == 0:
+            ck = snap(coarse)
+    if switch is not None:
+        hist = [h for h in hist if h[0] <= switch]
+        f = prolong(fine.params, ck[0], coarse.params)
+        for n in names:
+            fine.set_field(n, f[n])
+        sc = ck[1]
+        fine.set_scalars(t=sc["t"], dt=sc["dt"], dti=sc["dti"], mpgx=sc["mpgx"], QxT=sc["QxT"])
+        for _ in range(max_steps):
+            fine.step(1)
+            s = fine.scalars()
+            hist.append((s["t"], -s["mpgx"], 1))
+            if s["t"] >= t_end:
+                break
+    fine.sync()
+    coarse.sync()
+    return np.array(hist).reshape(-1, 3), switch, time.perf_counter() - w0
 
 
 def run_until(ch, t_end, max_steps, drag_cap):
@@ -238,6 +306,28 @@ def main(argv=None):
             r["mean_profile_error"] = float(np.linalg.norm(prof[label] - uf) / np.linalg.norm(uf))
         print(label, json.dumps(r), flush=True)
     report.update(runs)
+    # the learned path with the full DNS as its fallback: the coarse run with
+    # the closure until its drag leaves the bound, then the fine solver from
+    # its last checkpoint, zero-padded onto the fine modes, to the same t_end
+    ch, fb = Channel(pc), Channel(pf)
+    g = ch.ygrid()
+    for kname in ("u1", "u2", "u3", "p"):
+        ch.set_field(kname, fc[kname])
+    ch.set_scalars(t=t_start, dt=0.0, mpgx=-float(np.interp(t_start, hist_f[:, 0], hist_f[:, 1])) if t_start else 0.0,
+                   QxT=ic.flowrate(fc["u1"], g, pc.ny))
+    ch.closure_attach(net, every=a.closure_every)
+    hist, switch, wall = run_with_fallback(ch, fb, t_end, a.max_coarse_steps, cap)
+    ch.closure_attach(None)
+    r = {"switch_t": switch, "coarse_steps": int((hist[:, 2] == 0).sum()), "fine_steps": int((hist[:, 2] == 1).sum()),
+         "wall_s": wall, "t_reached": float(hist[-1, 0]) if len(hist) else t_start}
+    r["completed"] = bool(r["t_reached"] >= t_end)
+    r["sim_time_per_s"] = float((r["t_reached"] - t_start) / wall) if wall > 0 else 0.0
+    r["speedup_vs_fine"] = r["sim_time_per_s"] / report["fine"]["sim_time_per_s"]
+    r["drag"] = compare(hist[:, :2], hist_f, t_start + 0.25 * (t_end - t_start), t_end)
+    print("coarse_closure_fallback", json.dumps(r), flush=True)
+    report["coarse_closure_fallback"] = r
+    ch.close()
+    fb.close()
     fine.close()
     net.close()
     with open(os.path.join(a.out, "closure_report.json"), "w") as fh:

@@ -10,7 +10,11 @@ grids (128 x 33 x 64 fine, 64 x 33 x 32 coarse, tanh grid, dsty 3):
   applied physical planes, the coarse modes), and that correction is the
   network's own output on the same BF16 input;
 * training on samples of a short fine run lowers the held-out loss of a new
-  sample below half that of predicting no correction.
+  sample below half that of predicting no correction;
+* backscatter clipping zeroes exactly the components with u c < 0;
+* the full-DNS fallback: the fine solver takes over from the last coarse
+  checkpoint, zero-padded onto its modes, exactly as a fine run from that
+  state; without a failure the coarse run is that of run_until.
 """
 import dataclasses
 
@@ -237,3 +241,43 @@ def test_training_beats_no_correction_on_a_new_sample():
         zero.close()
         fine.close()
         co.close()
+
+
+def test_full_dns_fallback():
+    """run_with_fallback: with a drag bound the coarse run cannot meet, the
+    fine solver takes over from the last checkpoint (the start), zero-padded
+    onto its modes, and runs to t_end exactly as a fine solver started from
+    that state does; with a loose bound the coarse run finishes alone, step
+    for step the run of run_until."""
+    print("case closure fallback", FINE.ngalx, COARSE.ngalx, flush=True)
+    from cfd_amd import ic
+    fine, co = Channel(FINE), Channel(COARSE)
+    f0, q0 = closure.start_fields(fine, FINE, seed=0, amp=0.3)
+    fc = closure.filtered_start(COARSE, f0, FINE)
+
+    def load_coarse(ch):
+        for k in ("u1", "u2", "u3", "p"):
+            ch.set_field(k, fc[k])
+        ch.set_scalars(t=0.0, dt=0.0, mpgx=0.0, QxT=ic.flowrate(fc["u1"], ch.ygrid(), COARSE.ny))
+
+    load_coarse(co)
+    hist, switch, _ = closure.run_with_fallback(co, fine, t_end=0.05, max_steps=200, drag_cap=-1.0,
+                                                checkpoint_every=5)
+    assert switch == 0.0 and len(hist) and (hist[:, 2] == 1).all() and hist[-1, 0] >= 0.05
+    # the same fine run from the prolongated start
+    ref = Channel(FINE)
+    fp = closure.prolong(FINE, fc, COARSE)
+    for k in ("u1", "u2", "u3", "p"):
+        ref.set_field(k, fp[k])
+    ref.set_scalars(t=0.0, dt=0.0, dti=0.0, mpgx=0.0, QxT=ic.flowrate(fc["u1"], co.ygrid(), COARSE.ny))
+    ref.step(len(hist))
+    for k in ("u1", "u2", "u3"):
+        assert np.array_equal(ref.get_field(k), fine.get_field(k)), k
+    # loose bound: no switch, the coarse history of run_until
+    load_coarse(co)
+    h2, sw2, _ = closure.run_with_fallback(co, fine, t_end=0.05, max_steps=200, drag_cap=1e9)
+    load_coarse(co)
+    h3, _ = closure.run_until(co, 0.05, 200, 1e9)
+    assert sw2 is None and (h2[:, 2] == 0).all() and np.array_equal(h2[:, :2], h3)
+    for ch in (fine, co, ref):
+        ch.close()
