@@ -205,7 +205,9 @@ def learned_leg(args, fine_value, fine_dt):
             "report": "profiles/closure_rt395_384x192x192_to_192x192x96.json",
             "held_out_gain": r.get("held_out_gain"),
             "coarse": {k: r["coarse"].get(k) for k in ("completed", "t_reached", "drag")},
-            "coarse_closure": {k: r["coarse_closure"].get(k) for k in ("completed", "t_reached", "drag")}}
+            "coarse_closure": {k: r["coarse_closure"].get(k) for k in ("completed", "t_reached", "drag")},
+            "coarse_closure_fallback": {k: r.get("coarse_closure_fallback", {}).get(k)
+                                        for k in ("switch_t", "completed", "t_reached", "drag", "speedup_vs_fine")}}
     return out
 
 
