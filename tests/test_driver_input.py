@@ -41,6 +41,7 @@ def test_reads_example(tmp_path):
 
 @pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran driver not built")
 def test_implied_decimal(tmp_path):
-    # 'alp = 128d0' of the shipped input.in reads as 12.8
+    # 'alp = 128d0' reads as 12.8: format D10.1 on a real without a decimal
+    # point implies one decimal place (start.f90:434, read(40,10))
     v = parse(tmp_path, {"alp": "     128d0", "bet": "     256d0"})
     assert abs(v["reals"][1] - 12.8) < 1e-12 and abs(v["reals"][2] - 25.6) < 1e-12
