@@ -160,10 +160,11 @@ def copy_calibration():
 
 
 def learned_leg(args, fine_value, fine_dt):
-    """The learned path: the 192 x 192 x 96 coarse grid with the trained
-    closure network (cfd_amd/data/closure_rt395.npz) evaluated every
-    ``every`` steps, timed like the FP64 leg from a seeded start; simulated
-    time per second against the fine FP64 run's (substeps/s x dt)."""
+    """The learned path: the 192 x 192 x 96 coarse grid (modal filter alpha
+    36, order 32, as in the closure experiment) with the trained closure
+    network (cfd_amd/data/closure_rt395.npz) evaluated every ``every`` steps,
+    timed like the FP64 leg from a seeded start; simulated time per second
+    against the fine FP64 run's (substeps/s x dt)."""
     from cfd_amd import Channel, closure, cnn, workloads
 
     pc, _ = workloads.get("rt395_coarse_192x192x96")
@@ -174,6 +175,7 @@ def learned_leg(args, fine_value, fine_dt):
     for every in (0, 1, 2, 4):
         ch = Channel(pc)
         closure.start_fields(ch, pc, seed=0)
+        ch.set_mode_filter(36.0, 32, 0.0)  # the coarse runs' modal filter
         if every:
             if net is None:
                 net = cnn.CNN(ch.nyu, pc.ngalz, pc.ngalx)
@@ -204,10 +206,11 @@ def learned_leg(args, fine_value, fine_dt):
         out["accuracy_from_report"] = {
             "report": "profiles/closure_rt395_384x192x192_to_192x192x96.json",
             "held_out_gain": r.get("held_out_gain"),
-            "coarse": {k: r["coarse"].get(k) for k in ("completed", "t_reached", "drag")},
-            "coarse_closure": {k: r["coarse_closure"].get(k) for k in ("completed", "t_reached", "drag")},
+            **{lab: {k: r[lab].get(k) for k in ("completed", "t_reached", "drag")}
+               for lab in r if lab == "coarse" or lab.startswith("coarse_closure_every")},
             "coarse_closure_fallback": {k: r.get("coarse_closure_fallback", {}).get(k)
-                                        for k in ("switch_t", "completed", "t_reached", "drag", "speedup_vs_fine")}}
+                                        for k in ("closure_every", "switch_t", "completed", "t_reached", "drag")},
+            "closure_every": r.get("closure_every"), "coarse_filter": r.get("coarse_filter")}
     return out
 
 

@@ -94,6 +94,7 @@ SIGNATURES = {
     "cfd_closure_clip": (ctypes.c_int, [_H, ctypes.c_int]),
     "cfd_meanflow_ctP": (ctypes.c_int, [_H]),
     "cfd_hist": (ctypes.c_int, [_H, _D]),
+    "cfd_set_mode_filter": (ctypes.c_int, [_H, ctypes.c_double, ctypes.c_int, ctypes.c_double]),
     "cfd_ensemble_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
                                            ctypes.POINTER(ctypes.c_void_p)]),
     "cfd_ensemble_destroy": (ctypes.c_int, [_H]),

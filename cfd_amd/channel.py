@@ -270,6 +270,12 @@ class Channel:
         self._call("cfd_hist", _dp(v))
         return dict(zip(self.HIST, v.tolist()))
 
+    def set_mode_filter(self, alpha, order=8, eta_c=0.5):
+        """Modal filter after every RK step: sigma = exp(-alpha ((eta -
+        eta_c) / (1 - eta_c))^order) for eta = max(|i|/(N1/2), |kz|/(N2/2))
+        above eta_c; alpha = 0 switches it off."""
+        self._call("cfd_set_mode_filter", float(alpha), int(order), float(eta_c))
+
     def meanflow_ctP(self):
         self._call("cfd_meanflow_ctP")
 

@@ -91,13 +91,15 @@ def prolong(fine_params, coarse_fields, coarse_params):
     return out
 
 
-def run_with_fallback(coarse, fine, t_end, max_steps, drag_cap, checkpoint_every=50):
+def run_with_fallback(coarse, fine, t_end, max_steps, drag_cap, checkpoint_every=50, drag_lo=None):
     """The coarse run with the full DNS as its fallback.  The coarse solver
     (closure attached or not) steps until t >= t_end; every checkpoint_every
     steps its state is copied to the host.  When its drag leaves
-    [-drag_cap, drag_cap] or stops being finite, the fine solver takes the
+    [-drag_cap, drag_cap] (and [drag_lo, inf) when drag_lo is given) or
+    stops being finite, the fine solver takes the
     last checkpoint (zero-padded onto the fine modes, with its t, dt and mean
-    pressure gradient) and runs on to t_end.  Returns the (t, drag, on_fine)
+    pressure gradient) and runs on to t_end (or to a drag that is not
+    finite, the last entry then).  Returns the (t, drag, on_fine)
     history, the t of the switch (None without one) and the wall time."""
     names = ("u1", "u2", "u3", "p")
 
@@ -112,7 +114,7 @@ def run_with_fallback(coarse, fine, t_end, max_steps, drag_cap, checkpoint_every
         coarse.step(1)
         s = coarse.scalars()
         d = -s["mpgx"]
-        if not (np.isfinite(d) and abs(d) <= drag_cap):
+        if not (np.isfinite(d) and abs(d) <= drag_cap and (drag_lo is None or d >= drag_lo)):
             switch = ck[1]["t"]
             break
         hist.append((s["t"], d, 0))
@@ -131,7 +133,7 @@ def run_with_fallback(coarse, fine, t_end, max_steps, drag_cap, checkpoint_every
             fine.step(1)
             s = fine.scalars()
             hist.append((s["t"], -s["mpgx"], 1))
-            if s["t"] >= t_end:
+            if s["t"] >= t_end or not np.isfinite(s["mpgx"]):
                 break
     fine.sync()
     coarse.sync()
@@ -180,13 +182,19 @@ def main(argv=None):
     ap.add_argument("--train-per-sample", type=int, default=2)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--held-out", type=int, default=8)
-    ap.add_argument("--closure-every", type=int, default=2, help="coarse steps between network evaluations")
+    ap.add_argument("--closure-every", default="1,2,4",
+                    help="coarse steps between network evaluations, one coarse run per value (comma list)")
     ap.add_argument("--max-coarse-steps", type=int, default=20000)
     ap.add_argument("--coarse-start", type=float, default=0.0,
                     help="fraction of the fine run's time at which the coarse runs start from its filtered state")
+    ap.add_argument("--filter", default="36,32,0",
+                    help="modal filter of the coarse runs, alpha,order,eta_c (alpha 0: none)")
     ap.add_argument("--out", default=".")
     ap.add_argument("--seed", type=int, default=0)
     a = ap.parse_args(argv)
+    fa, fo, fe = a.filter.split(",")
+    filt = (float(fa), int(fo), float(fe))
+    every_list = [int(e) for e in a.closure_every.split(",")]
     pf, _ = workloads.get("rt395_384x192x192")
     pc, _ = workloads.get("rt395_coarse_192x192x96")
     fine, scratch = Channel(pf), Channel(pc)
@@ -195,7 +203,7 @@ def main(argv=None):
     net.dataset(a.slots + a.held_out)
     rng = np.random.default_rng(a.seed)
     report = {"samples": a.samples, "every": a.every, "slots": a.slots, "train_per_sample": a.train_per_sample,
-              "closure_every": a.closure_every}
+              "closure_every": every_list, "coarse_filter": {"alpha": filt[0], "order": filt[1], "eta_c": filt[2]}}
     # pass 1: the fine run alone, probing the sample magnitudes along the
     # trajectory; the normalisation takes the largest RMS of each quantity
     f0, q0 = start_fields(fine, pf, seed=0)
@@ -278,15 +286,18 @@ def main(argv=None):
     report["coarse_start_t"] = t_start
     cap = 50.0 * float(np.abs(hist_f[:, 1]).max())
     runs, prof = {}, {}
-    for label, attach in (("coarse", None), ("coarse_closure", "plain"), ("coarse_closure_clip", "clip")):
+    cases = [("coarse", None, 0)] + [("coarse_closure_every%d" % e, "plain", e) for e in every_list]
+    cases.append(("coarse_closure_clip_every%d" % every_list[0], "clip", every_list[0]))
+    for label, attach, every in cases:
         ch = Channel(pc)
         g = ch.ygrid()
         for kname in ("u1", "u2", "u3", "p"):
             ch.set_field(kname, fc[kname])
         ch.set_scalars(t=t_start, dt=0.0, mpgx=-float(np.interp(t_start, hist_f[:, 0], hist_f[:, 1])) if t_start else 0.0,
                        QxT=ic.flowrate(fc["u1"], g, pc.ny))
+        ch.set_mode_filter(*filt)
         if attach:
-            ch.closure_attach(net, every=a.closure_every, clip=attach == "clip")
+            ch.closure_attach(net, every=every, clip=attach == "clip")
         hist, wall = run_until(ch, t_end, a.max_coarse_steps, cap)
         t_last = float(hist[-1, 0]) if len(hist) else 0.0
         runs[label] = {"hist": hist, "steps": len(hist), "wall_s": wall, "t_reached": t_last,
@@ -309,18 +320,27 @@ def main(argv=None):
     # the learned path with the full DNS as its fallback: the coarse run with
     # the closure until its drag leaves the bound, then the fine solver from
     # its last checkpoint, zero-padded onto the fine modes, to the same t_end
+    # (the network every max(closure_every) steps)
     ch, fb = Channel(pc), Channel(pf)
     g = ch.ygrid()
     for kname in ("u1", "u2", "u3", "p"):
         ch.set_field(kname, fc[kname])
     ch.set_scalars(t=t_start, dt=0.0, mpgx=-float(np.interp(t_start, hist_f[:, 0], hist_f[:, 1])) if t_start else 0.0,
                    QxT=ic.flowrate(fc["u1"], g, pc.ny))
-    ch.closure_attach(net, every=a.closure_every)
-    hist, switch, wall = run_with_fallback(ch, fb, t_end, a.max_coarse_steps, cap)
+    ch.set_mode_filter(*filt)
+    ch.closure_attach(net, every=max(every_list))
+    # the trust band of the coarse drag: at least 0 (a mean pressure gradient
+    # that opposes the constant flow rate drives the flow backwards) and at
+    # most 4 times the largest drag of the fine run before the switch-on
+    pre = hist_f[:, 0] <= max(t_start, float(hist_f[0, 0]))
+    hi = 4.0 * float(np.abs(hist_f[pre, 1]).max())
+    hist, switch, wall = run_with_fallback(ch, fb, t_end, a.max_coarse_steps, hi, drag_lo=0.0)
     ch.closure_attach(None)
-    r = {"switch_t": switch, "coarse_steps": int((hist[:, 2] == 0).sum()), "fine_steps": int((hist[:, 2] == 1).sum()),
+    r = {"closure_every": max(every_list), "drag_band": [0.0, hi], "switch_t": switch, "coarse_steps": int((hist[:, 2] == 0).sum()), "fine_steps": int((hist[:, 2] == 1).sum()),
          "wall_s": wall, "t_reached": float(hist[-1, 0]) if len(hist) else t_start}
-    r["completed"] = bool(r["t_reached"] >= t_end)
+    r["completed"] = bool(r["t_reached"] >= t_end and np.isfinite(hist[:, 1]).all())
+    bad = ~np.isfinite(hist[:, 1])
+    r["first_nonfinite_drag_t"] = float(hist[bad, 0][0]) if bad.any() else None
     r["sim_time_per_s"] = float((r["t_reached"] - t_start) / wall) if wall > 0 else 0.0
     r["speedup_vs_fine"] = r["sim_time_per_s"] / report["fine"]["sim_time_per_s"]
     r["drag"] = compare(hist[:, :2], hist_f, t_start + 0.25 * (t_end - t_start), t_end)

@@ -1183,6 +1183,9 @@ int closure_apply(cfd_solver* s, int kRK) {
   for (int g = 0; g < 3; ++g) {
     launch_xpass_fwd2(*s, C.d_xf[g], C.nxf[g]);
     launch_zfwd_plain(*s, C.d_zf[g], C.nxf[g]);
+    // no net streamwise or spanwise force from the network (u and v planes
+    // share their first index, so v face q - 1 lies below u plane q)
+    if (g != 1) launch_zero_net_force(s->stream, C.Cs[g], s->Ncol, 1, nyf[g] - 1, s->d_yv);
     launch_add_planes(s->stream, (double2*)nxz[g], C.Cs[g], s->Ncol, 1, nyf[g] - 1);
   }
   return CFD_OK;
@@ -1282,6 +1285,7 @@ int op_substep(cfd_solver* s, int kRK) {
     launch_meanflow_lds(*s, kRK);
   }
   { Timed t(s, KC_PRESS); launch_pressure(*s, kRK); }
+  if (kRK == 3 && s->filt) launch_mode_filter(*s);
   ++s->nsub;
   return wrc;
 }
@@ -1330,6 +1334,7 @@ extern "C" int cfd_vcorr(cfd_handle h, int kRK) {
   if (int rc = check(s, kRK)) return rc;
   launch_vcorr(*s, kRK);
   launch_divergence(*s);
+  if (kRK == 3 && s->filt) launch_mode_filter(*s);
   return launched(s);
 }
 extern "C" int cfd_substep(cfd_handle h, int kRK) {
@@ -1386,6 +1391,36 @@ extern "C" int cfd_hist(cfd_handle h, double* v) {
   HIPCHK(s, hipMemcpy(&s->d_sc->Qx, &d.Qx, sizeof d.Qx, hipMemcpyHostToDevice));
   const double out[11] = {d.t, s->dtv, d.dtc, d.dt, err, Q, d.QxT, d.mpgx, d.dgx, Umax, Uslip};
   std::memcpy(v, out, sizeof out);
+  return CFD_OK;
+}
+
+// modal filter (no counterpart in the reference): sigma = 1 for eta <=
+// eta_c, exp(-alpha ((eta - eta_c) / (1 - eta_c))^order) above, with eta =
+// max(|i| / (N1/2), |kz index| / (N2/2)) of the column; alpha = 0 switches
+// it off
+extern "C" int cfd_set_mode_filter(cfd_handle h, double alpha, int order, double eta_c) {
+  cfd_solver* s = static_cast<cfd_solver*>(h);
+  if (!s) return fail(s, CFD_EINVAL, "null handle");
+  if (!(alpha >= 0.0) || order < 1 || !(eta_c >= 0.0 && eta_c < 1.0))
+    return fail(s, CFD_EINVAL, "mode filter: alpha >= 0, order >= 1, 0 <= eta_c < 1");
+  if (alpha == 0.0) {
+    s->filt = false;
+    return CFD_OK;
+  }
+  std::vector<double> sig(s->Ncol);
+  for (int k = 0; k < s->N2; ++k) {
+    const int kzi = k < s->N2 / 2 ? k : s->N2 - k;
+    for (int i = 0; i < s->Mx; ++i) {
+      const double eta = std::max((double)i / (s->N1 / 2), (double)kzi / (s->N2 / 2));
+      sig[(size_t)k * s->Mx + i] = eta > eta_c ? std::exp(-alpha * std::pow((eta - eta_c) / (1.0 - eta_c), order)) : 1.0;
+    }
+  }
+  HIPCHK(s, hipSetDevice(s->dev));
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  if (!s->d_sigma)
+    if (int rc = dalloc(s, &s->d_sigma, sig.size())) return rc;
+  HIPCHK(s, hipMemcpy(s->d_sigma, sig.data(), sig.size() * sizeof(double), hipMemcpyHostToDevice));
+  s->filt = true;
   return CFD_OK;
 }
 

@@ -131,6 +131,10 @@ struct Solver {
   int seg_rows = 13;
   int solve_threads = 256;
   long long nsub = 0;  // substeps launched (captured ones included)
+  // modal filter of the velocity after every RK step (off by default):
+  // sigma per column, applied to every plane of u1, u2, u3
+  double* d_sigma = nullptr;
+  bool filt = false;
   double2 *d_twz = nullptr, *d_twx = nullptr;
   float2 *d_twzf = nullptr, *d_twxf = nullptr;  // the same tables rounded to FP32
 
@@ -186,6 +190,7 @@ void launch_solve_p(Solver& s, int kRK);
 void launch_vcorr(Solver& s, int kRK);
 void launch_divergence(Solver& s);
 void launch_flowrate(Solver& s);
+void launch_mode_filter(Solver& s);  // u_i *= sigma(column) on every plane
 void launch_absmax(Solver& s, const double2* a, size_t n);  // max |a| -> d_sc->amax
 bool seg_rows_supported(int rows);
 // statistics (kernels_stats.hip)

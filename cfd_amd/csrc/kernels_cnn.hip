@@ -561,6 +561,22 @@ __global__ void k_add_planes(double2* __restrict__ a, const double2* __restrict_
   a[o] = make_double2(a[o].x + c[o].x, a[o].y + c[o].y);
 }
 
+// The (0,0) column of a correction on u planes q0 .. q1 - 1 loses its
+// mean over y weighted by the plane widths yv[q] - yv[q - 1] (u plane q lies
+// between v faces q - 1 and q): the correction then exerts no net force on
+// the flow, as the divergence of a stress that vanishes at both walls does.
+// One thread: a column of at most a few hundred planes.
+__global__ void k_zero_net_force(double2* __restrict__ c, int ncol, int q0, int q1, const double* __restrict__ yv) {
+  double F = 0.0, H = 0.0;
+  for (int q = q0; q < q1; ++q) {
+    const double w = yv[q] - yv[q - 1];
+    F += w * c[(size_t)q * ncol].x;
+    H += w;
+  }
+  const double m = F / H;
+  for (int q = q0; q < q1; ++q) c[(size_t)q * ncol].x -= m;
+}
+
 // sum of squares of n values into *acc (one atomic per block)
 __global__ void k_sumsq(const double* __restrict__ v, size_t n, double* __restrict__ acc) {
   double s = 0.0;
@@ -603,6 +619,11 @@ void launch_add_planes(hipStream_t st, double2* a, const double2* c, int ncol, i
   if (q1 <= q0) return;
   const size_t n = (size_t)(q1 - q0) * ncol;
   hipLaunchKernelGGL(k_add_planes, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, c, ncol, q0, q1);
+}
+
+void launch_zero_net_force(hipStream_t st, double2* c, int ncol, int q0, int q1, const double* yv) {
+  if (q1 <= q0) return;
+  hipLaunchKernelGGL(k_zero_net_force, dim3(1), dim3(1), 0, st, c, ncol, q0, q1, yv);
 }
 
 void launch_sumsq(hipStream_t st, const double* v, size_t n, double* acc) {

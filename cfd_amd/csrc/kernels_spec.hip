@@ -526,6 +526,25 @@ void launch_absmax(Solver& s, const double2* a, size_t n) {
   hipLaunchKernelGGL(k_absmax, dim3(512), dim3(256), 0, s.stream, a, n, out);
 }
 
+// u1, u3 (nyu planes) and u2 (nyv planes) times sigma[column]: the same factor
+// for every plane and component of a column, so the discrete divergence of
+// each mode scales with it and the boundary rows keep their form
+__global__ void k_mode_filter(double2* __restrict__ u1, double2* __restrict__ u2, double2* __restrict__ u3,
+                              const double* __restrict__ sigma, int ncol, int nyu, int nyv) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)nyu * ncol) return;
+  const double f = sigma[e % ncol];
+  u1[e] = make_double2(f * u1[e].x, f * u1[e].y);
+  u3[e] = make_double2(f * u3[e].x, f * u3[e].y);
+  if (e < (size_t)nyv * ncol) u2[e] = make_double2(f * u2[e].x, f * u2[e].y);
+}
+
+void launch_mode_filter(Solver& s) {
+  const size_t n = (size_t)s.nyu * s.Ncol;
+  hipLaunchKernelGGL(k_mode_filter, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s.stream, s.F[CFD_U1],
+                     s.F[CFD_U2], s.F[CFD_U3], (const double*)s.d_sigma, s.Ncol, s.nyu, s.nyv);
+}
+
 void launch_flowrate(Solver& s) {
   hipLaunchKernelGGL(k_flowrate, dim3(1), dim3(1), 0, s.stream, (const double2*)s.F[CFD_U1], (size_t)s.Ncol, s.jlo_u,
                      s.nn, (const double*)s.d_yv, s.jlo_v, s.d_sc);

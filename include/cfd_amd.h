@@ -135,6 +135,13 @@ int cfd_meanflow_ctP(cfd_handle h);              /* meanflow_ctP    littleharsh.
    littleharsh.f90:486-502), Uslip its value interpolated to the lower tip
    face yv(0).  Synchronous. */
 int cfd_hist(cfd_handle h, double* v);
+/* Modal filter of the velocity after every RK step (no counterpart in the
+   reference; off by default): every plane of column (i, kz) of u1, u2, u3
+   is multiplied by sigma = exp(-alpha ((eta - eta_c)/(1 - eta_c))^order)
+   where eta = max(|i|/(N1/2), |kz|/(N2/2)) > eta_c, 1 elsewhere.  One factor
+   per column keeps each mode divergence-free.  A dissipation for coarse,
+   under-resolved grids; alpha = 0 switches it off. */
+int cfd_set_mode_filter(cfd_handle h, double alpha, int order, double eta_c);
 /* ---------------------------------------------------------------------------
  * Field files in the reference layout: Fortran sequential unformatted, 4-byte
  * little-endian record markers (record_out, FOU3D.f90:2172-2345; u_to_buff /
@@ -298,8 +305,10 @@ int cfd_cnn_get_scales(cfd_cnn c, float* s6);
  * (cfd_nonlinear, cfd_substep, cfd_step) adds the network's correction: the
  * network evaluates the state in the nonlinear term of kRK = 1 of every
  * `every`-th step (and in the first one after attaching) and the other
- * substeps reuse that correction; c = NULL detaches.  The FP64 path only
- * (precision 0); the network volume must be nyu x Ngalz x Ngalx of h.
+ * substeps reuse that correction; c = NULL detaches.  The (0,0) column of
+ * the u1 and u3 corrections loses its mean over y weighted by the plane
+ * widths, so the correction exerts no net force on the flow.  The FP64 path
+ * only (precision 0); the network volume must be nyu x Ngalz x Ngalx of h.
  *
  * cfd_closure_phys(h, which, out): the physical planes of the last sample
  * or application: which 0..2 the velocities u1, u2, u3, 3..5 the

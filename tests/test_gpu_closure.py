@@ -7,7 +7,8 @@ grids (128 x 33 x 64 fine, 64 x 33 x 32 coarse, tanh grid, dsty 3):
   host's filter; the RMS values and the scales they set;
 * inference: with the network attached, the nonlinear term changes by the
   forward transform of the network's correction (numpy rfft/fft of the
-  applied physical planes, the coarse modes), and that correction is the
+  applied physical planes, the coarse modes; the (0,0) column of u1 and u3
+  less its plane-width weighted y mean), and that correction is the
   network's own output on the same BF16 input;
 * training on samples of a short fine run lowers the held-out loss of a new
   sample below half that of predicting no correction;
@@ -159,10 +160,17 @@ def test_attached_network_corrects_the_nonlinear_term():
         exp1 = np.zeros_like(corr[1])
         exp1[1:-1] = y[1:ny - 2, ..., 1] * s[4]
         assert _rel(corr[0], exp0) <= 1e-6 and _rel(corr[1], exp1) <= 1e-6
-        # the nonlinear term moved by the forward transform of the correction
+        # the nonlinear term moved by the forward transform of the correction,
+        # its (0,0) u1 and u3 columns less their width-weighted y mean (no net
+        # force on the flow)
+        yv = b.ygrid()["yv"]
+        w = yv[1:b.nyu - 1] - yv[:b.nyu - 2]
         for k, n in enumerate(NL):
             d = b.get_field(n) - a.get_field(n)
             e = _spec(COARSE, corr[k])
+            if k != 1:
+                e[0, 1:-1] -= (w * e[0, 1:-1].real).sum() / w.sum()
+                assert abs((w * d[0, 1:-1].real).sum()) <= 1e-10 * np.abs(e[0]).max() * w.sum()
             assert _rel(d[:, 1:-1], e[:, 1:-1]) <= 1e-10, (n, _rel(d[:, 1:-1], e[:, 1:-1]))
         # backscatter clipping, same state: a component with u c < 0 is zero,
         # every other value is the unclipped correction
@@ -194,6 +202,8 @@ def test_attached_network_corrects_the_nonlinear_term():
             assert (b.closure_phys(3 + k) == held[k]).all()
             d = b.get_field(n) - a.get_field(n)
             e = _spec(COARSE, held[k])
+            if k != 1:
+                e[0, 1:-1] -= (w * e[0, 1:-1].real).sum() / w.sum()
             assert _rel(d[:, 1:-1], e[:, 1:-1]) <= 1e-10, (n, _rel(d[:, 1:-1], e[:, 1:-1]))
     finally:
         net.close()
@@ -279,5 +289,55 @@ def test_full_dns_fallback():
     load_coarse(co)
     h3, _ = closure.run_until(co, 0.05, 200, 1e9)
     assert sw2 is None and (h2[:, 2] == 0).all() and np.array_equal(h2[:, :2], h3)
+    # a lower bound above every drag of that run: the switch at the start
+    load_coarse(co)
+    h4, sw4, _ = closure.run_with_fallback(co, fine, t_end=0.01, max_steps=200, drag_cap=1e9,
+                                           drag_lo=h3[:, 1].max() + 1.0)
+    assert sw4 == 0.0 and (h4[:, 2] == 1).all()
     for ch in (fine, co, ref):
+        ch.close()
+
+
+def test_mode_filter_scales_each_mode():
+    """cfd_set_mode_filter: after one RK step the filtered run's velocity is
+    sigma(column) times the unfiltered run's, sigma = exp(-alpha ((eta -
+    eta_c) / (1 - eta_c))^order) above eta_c with eta = max(|i|/(N1/2),
+    |kz|/(N2/2)): bit for bit where sigma = 1, to 1e-12 relative elsewhere
+    (the table's exp/pow round differently from numpy's, by up to a few
+    1e-15 times the exponent); alpha = 0 switches it off; bad arguments are
+    refused."""
+    print("case mode filter", COARSE.ngalx, flush=True)
+    f = closure.filtered_start(COARSE, _fields(FINE, 21), FINE)
+    a, b = Channel(COARSE), Channel(COARSE)
+    for ch in (a, b):
+        _load(ch, f)
+    alpha, order, eta_c = 36.0, 8, 0.3
+    b.set_mode_filter(alpha, order, eta_c)
+    a.step(1)
+    b.step(1)
+    N1, N2, Mx = COARSE.N1, COARSE.N2, COARSE.Mx
+    k = np.arange(N2)
+    kz = np.where(k < N2 // 2, k, N2 - k)
+    eta = np.maximum(np.arange(Mx)[None, :] / (N1 // 2), kz[:, None] / (N2 // 2)).reshape(-1)
+    sig = np.where(eta > eta_c, np.exp(-alpha * ((eta - eta_c) / (1 - eta_c)) ** order), 1.0)
+    assert sig.min() < 1e-10 and (sig == 1.0).sum() > 0
+    for n in VEL:
+        ua, ub = a.get_field(n), b.get_field(n)
+        assert np.array_equal(ub[sig == 1.0], ua[sig == 1.0]), n
+        assert np.all(np.abs(ub - ua * sig[:, None]) <= 1e-12 * np.abs(ua) * sig[:, None]), n
+        assert np.abs(ub[sig < 0.5]).max() < np.abs(ua[sig < 0.5]).max(), n
+    for bad in ((-1.0, 8, 0.3), (1.0, 0, 0.3), (1.0, 8, 1.0)):
+        with pytest.raises(RuntimeError):
+            b.set_mode_filter(*bad)
+    b.set_mode_filter(0.0)
+    a.step(1)
+    b2 = Channel(COARSE)
+    for n in VEL + ("p",):
+        b2.set_field(n, b.get_field(n))
+    b2.set_scalars(**b.scalars())
+    b.step(1)
+    b2.step(1)
+    for n in VEL:
+        assert np.array_equal(b.get_field(n), b2.get_field(n)), n
+    for ch in (a, b, b2):
         ch.close()
