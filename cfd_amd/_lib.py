@@ -63,6 +63,7 @@ SIGNATURES = {
                             + [_D, _I]),
     "cfd_record_out": (ctypes.c_int, [_H, ctypes.c_char_p, ctypes.c_int]),
     "cfd_band_table": (ctypes.c_int, [ctypes.POINTER(CParams), _I]),
+    "cfd_plane_partition": (ctypes.c_int, [ctypes.POINTER(CParams), ctypes.c_int, _I]),
     "cfd_four_to_phys": (ctypes.c_int, [_H, ctypes.c_int, _D]),
     "cfd_cnn_nparams": (ctypes.c_int, []),
     "cfd_cnn_create": (ctypes.c_int, [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_void_p)]),
@@ -95,6 +96,7 @@ SIGNATURES = {
     "cfd_meanflow_ctP": (ctypes.c_int, [_H]),
     "cfd_hist": (ctypes.c_int, [_H, _D]),
     "cfd_set_mode_filter": (ctypes.c_int, [_H, ctypes.c_double, ctypes.c_int, ctypes.c_double]),
+    "cfd_set_compat_lastplane": (ctypes.c_int, [_H, ctypes.c_int]),
     "cfd_ensemble_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
                                            ctypes.POINTER(ctypes.c_void_p)]),
     "cfd_ensemble_destroy": (ctypes.c_int, [_H]),

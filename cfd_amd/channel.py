@@ -276,6 +276,12 @@ class Channel:
         above eta_c; alpha = 0 switches it off."""
         self._call("cfd_set_mode_filter", float(alpha), int(order), float(eta_c))
 
+    def set_compat_lastplane(self, np_ranks):
+        """The reference's as-shipped nonlinear term of a run on np_ranks
+        ranks (every plane of a rank takes the x/z terms of the rank's last
+        plane, FOU3D.f90:1553-1613); 0 restores the per-plane term."""
+        self._call("cfd_set_compat_lastplane", int(np_ranks))
+
     def meanflow_ctP(self):
         self._call("cfd_meanflow_ctP")
 

@@ -1227,7 +1227,11 @@ int op_nonlinear(cfd_solver* s, int kRK) {
   if (kRK == 1) launch_reset_max(*s);
   { Timed t(s, KC_ZINV); launch_zinv(*s, s->d_nljobs, s->n_nljobs, true); }
   { Timed t(s, KC_XPASS); launch_xpass_nl(*s, kRK == 1); }
-  { Timed t(s, KC_ZFWD); launch_zfwd_nl(*s); }
+  {
+    Timed t(s, KC_ZFWD);
+    launch_zfwd_nl(*s);
+    launch_plane_map(*s);
+  }
   if (kRK == 1) launch_cfl(*s);
   const int wrc = stats_req_end(s, req);
   if (s->cl.cnn)
@@ -1268,7 +1272,11 @@ int op_substep(cfd_solver* s, int kRK) {
   if (kRK == 1) launch_reset_max(*s);
   { Timed t(s, KC_ZINV); launch_zinv(*s, s->d_nljobs, s->n_nljobs, true); }
   { Timed t(s, KC_XPASS); launch_xpass_nl(*s, kRK == 1); }
-  { Timed t(s, KC_ZFWD); launch_zfwd_nl(*s); }
+  {
+    Timed t(s, KC_ZFWD);
+    launch_zfwd_nl(*s);
+    launch_plane_map(*s);
+  }
   if (kRK == 1) launch_cfl(*s);
   // a failed write (say, a missing directory) still completes the substep
   const int wrc = stats_req_end(s, req);
@@ -1421,6 +1429,57 @@ extern "C" int cfd_set_mode_filter(cfd_handle h, double alpha, int order, double
     if (int rc = dalloc(s, &s->d_sigma, sig.size())) return rc;
   HIPCHK(s, hipMemcpy(s->d_sigma, sig.data(), sig.size() * sizeof(double), hipMemcpyHostToDevice));
   s->filt = true;
+  return CFD_OK;
+}
+
+// the reference's as-shipped nonlinear term for np ranks (no counterpart as
+// a switch in the reference, which always runs it): in ops_in_planes
+// (FOU3D.f90:1553-1613) the derivative scratch is overwritten plane by plane
+// over a rank's planes with walls (limPL_incw) and then copied to all of its
+// planes without walls (limPL_excw), so every plane takes the x/z terms of
+// the rank's last plane; the top rank's last planes are the top ghost u
+// plane and the top wall v plane.  np = 0 restores the per-plane term.
+extern "C" int cfd_set_compat_lastplane(cfd_handle h, int np) {
+  cfd_solver* s = static_cast<cfd_solver*>(h);
+  if (!s) return fail(s, CFD_EINVAL, "null handle");
+  if (np == 0) {
+    s->compat_np = 0;
+    return CFD_OK;
+  }
+  if (np < 3) return fail(s, CFD_EINVAL, "last-plane replay: the reference's plane partition needs np >= 3");
+  int N[20];
+  band_table(s->P, s->yg, N);
+  std::vector<int> lim(4 * (size_t)np);
+  if (!plane_partition(s->P, N, np, lim.data()))
+    return fail(s, CFD_EINVAL, "last-plane replay: no plane partition for this np");
+  std::vector<int> um(s->nyu), vm(s->nyv);
+  for (int q = 0; q < s->nyu; ++q) um[q] = q;
+  for (int q = 0; q < s->nyv; ++q) vm[q] = q;
+  for (int g = 0; g < 2; ++g) {
+    std::vector<int>& m = g == 0 ? um : vm;
+    const int jlo = g == 0 ? s->jlo_u : s->jlo_v, ny = (int)m.size();
+    for (int r = 0; r < np; ++r) {
+      const int from = lim[(g * 2 + 0) * np + r], to = lim[(g * 2 + 1) * np + r];
+      // limPL_incw: the first rank's range starts one plane lower, the last
+      // rank's ends one plane higher (start.f90:1866-1869)
+      const int last = to + (r == np - 1 ? 1 : 0);
+      if (from > to) continue;
+      if (from - jlo < 1 || to - jlo > ny - 2 || last - jlo > ny - 1)
+        return fail(s, CFD_EINVAL, "last-plane replay: partition outside the grid");
+      for (int j = from; j <= to; ++j) m[j - jlo] = last - jlo;
+    }
+    for (int q = 0; q < ny; ++q)
+      if (m[m[q]] != m[q]) return fail(s, CFD_EINVAL, "last-plane replay: a source plane is also replaced");
+  }
+  HIPCHK(s, hipSetDevice(s->dev));
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  if (!s->d_umap) {
+    if (int rc = dalloc(s, &s->d_umap, um.size())) return rc;
+    if (int rc = dalloc(s, &s->d_vmap, vm.size())) return rc;
+  }
+  HIPCHK(s, hipMemcpy(s->d_umap, um.data(), um.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(s, hipMemcpy(s->d_vmap, vm.data(), vm.size() * sizeof(int), hipMemcpyHostToDevice));
+  s->compat_np = np;
   return CFD_OK;
 }
 

@@ -135,6 +135,10 @@ struct Solver {
   // sigma per column, applied to every plane of u1, u2, u3
   double* d_sigma = nullptr;
   bool filt = false;
+  // the reference's last-plane nonlinear term for np ranks (0: per plane):
+  // plane q of the x/z part takes plane map[q] (u and v grids)
+  int compat_np = 0;
+  int *d_umap = nullptr, *d_vmap = nullptr;
   double2 *d_twz = nullptr, *d_twx = nullptr;
   float2 *d_twzf = nullptr, *d_twxf = nullptr;  // the same tables rounded to FP32
 
@@ -191,6 +195,7 @@ void launch_vcorr(Solver& s, int kRK);
 void launch_divergence(Solver& s);
 void launch_flowrate(Solver& s);
 void launch_mode_filter(Solver& s);  // u_i *= sigma(column) on every plane
+void launch_plane_map(Solver& s);   // last-plane replay of nxz1, nxz3 (u) and nxz2 (v)
 void launch_absmax(Solver& s, const double2* a, size_t n);  // max |a| -> d_sc->amax
 bool seg_rows_supported(int rows);
 // statistics (kernels_stats.hip)

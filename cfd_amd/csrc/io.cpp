@@ -6,6 +6,7 @@
 // summarised in cfd_amd.h.  The plane buffer of u_to_buff (FOU3D.f90:2598-2627)
 // is the spectral plane with x-modes fastest and the z-modes in column order,
 // which is also the device layout of one plane, so planes move as one block.
+#include <cmath>
 #include <algorithm>
 #include <cstdint>
 #include <cstdio>
@@ -218,7 +219,79 @@ void band_table(const cfd_params& p, const YGrid& g, int N[20]) {
     for (int r = 0; r < 4; ++r) N[b * 4 + r] = tab[b][r];
 }
 
+// The reference's plane partition over np ranks (start.f90:1652-1866),
+// restated for one resolution in every band (Ngal(1:2, iband) = Ngalx,
+// Ngalz): ranks per band from its load balance, then contiguous ranges of
+// planes per rank, the ranks with one plane more first.  lim[(g * 2 + e) *
+// np + r] is planelim(grid, e + 1, r) for g 0 the u grid (band limits N(4,
+// .)), 1 the v grid (N(3, .)); walls excluded.  False when np < 3 (the
+// balance then gives the central band no rank and never ends) and when the
+// band limits are not strictly increasing (tall posts on a short grid: a
+// band with no or a negative number of planes, whose ranges overlap their
+// neighbours').
+bool plane_partition(const cfd_params& p, const int N[20], int np, int* lim) {
+  if (np < 3) return false;
+  auto NB = [&](int r, int b) { return N[b * 4 + r - 1]; };
+  for (int r = 3; r <= 4; ++r)
+    if (!(NB(r, 0) < NB(r, 1) && NB(r, 1) < NB(r, 2) && NB(r, 2) < NB(r, 3))) return false;
+  const int bot = 1, mid = 2, top = 3;
+  double load[4] = {0, 0, 0, 0}, il[4] = {0, 0, 0, 0};
+  for (int b = 1; b <= 3; ++b) load[b] = (double)(p.ngalx + 2) * p.ngalz * (NB(4, b) - NB(4, b - 1));
+  auto iceil = [](double x) { return (int)std::ceil(x); };
+  int procs[4] = {0, 0, 0, 0};
+  // ranks of the two fine bands with iplanes planes each, the rest to the
+  // central band, and the per-rank loads
+  auto assign = [&](int iplanes) {
+    int nplanes = NB(4, bot) - NB(4, 0);
+    procs[bot] = iceil(nplanes * 1.0 / iplanes);
+    procs[top] = procs[bot];
+    const int rem = procs[bot] + procs[top];
+    il[bot] = load[bot] / nplanes * iplanes;
+    il[top] = load[top] / nplanes * iplanes;
+    nplanes = NB(4, mid) - NB(4, bot);
+    procs[mid] = np - rem;
+    if (procs[mid] > 0) il[mid] = load[mid] / nplanes * iceil(nplanes * 1.0 / procs[mid]);
+  };
+  auto maxl = [&]() { return std::max(il[1], std::max(il[2], il[3])); };
+  int iplanes = 1;
+  assign(1);
+  while (procs[mid] <= 0) assign(++iplanes);
+  double maxload = 2.0 * maxl();
+  while (maxl() < maxload) {
+    maxload = maxl();
+    assign(++iplanes);
+  }
+  assign(--iplanes);
+  // contiguous ranges per band: u grid over N(4, .), v grid over N(3, .)
+  for (int g = 0; g < 2; ++g) {
+    const int r = g == 0 ? 4 : 3;
+    int first = 0;  // ranks of the bands below
+    for (int b = 1; b <= 3; ++b) {
+      const int nplanes = NB(r, b) - NB(r, b - 1), nr = procs[b];
+      const int ip = nplanes / nr, rem = nplanes - ip * nr;
+      for (int k = 1; k <= nr; ++k) {
+        const int from = k <= rem ? (ip + 1) * (k - 1) + 1 : ip * (k - rem - 1) + (ip + 1) * rem + 1;
+        const int to = k <= rem ? (ip + 1) * k : ip * (k - rem) + (ip + 1) * rem;
+        lim[(g * 2 + 0) * np + first + k - 1] = from + NB(r, b - 1);
+        lim[(g * 2 + 1) * np + first + k - 1] = to + NB(r, b - 1);
+      }
+      first += nr;
+    }
+  }
+  return true;
+}
+
 }  // namespace cfd
+
+extern "C" int cfd_plane_partition(const cfd_params* p, int np, int* lim) {
+  if (!p || !lim || np < 3) return CFD_EINVAL;
+  cfd::YGrid g;
+  std::string err;
+  if (!cfd::build_ygrid(*p, g, err)) return CFD_EINVAL;
+  int N[20];
+  cfd::band_table(*p, g, N);
+  return cfd::plane_partition(*p, N, np, lim) ? CFD_OK : CFD_EINVAL;
+}
 
 extern "C" int cfd_field_file_write(const char* path, const cfd_file_header* hd, int id, double dthetavi,
                                     const double* y, const double* dthdy, int ny_arr, int jy0, int ny_g, int j0,

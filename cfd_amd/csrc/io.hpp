@@ -24,5 +24,6 @@ std::string write_spectrum_file(const char* path, const cfd_file_header& hd, con
                                 const std::vector<double>& sp);
 // N(1:4, 0:4) of the reference for this configuration (uniform resolution)
 void band_table(const cfd_params& p, const YGrid& g, int N[20]);
+bool plane_partition(const cfd_params& p, const int N[20], int np, int* lim);
 
 }  // namespace cfd

@@ -163,7 +163,8 @@ struct XNLArgs {
   const double* dthdyv;
   DevScalars* sc;
   int nyu, nyv, Mx, ngz, cfl;
-  int zs;  // Z row stride (elements)
+  int zs;   // Z row stride (elements)
+  int top;  // also the products of the top ghost u plane (the last-plane replay reads them)
 };
 
 template <int LC>
@@ -178,7 +179,7 @@ __global__ void __launch_bounds__(kThreads) k_xpass_nl(XNLArgs a, FftPlan P, con
   const int z = blockIdx.x, q = blockIdx.y;
   const int ngz = gridDim.x;
   const bool hasv = q < a.nyv;
-  const bool uint = q > 0 && q < a.nyu - 1;
+  const bool uint = q > 0 && (q < a.nyu - 1 || (a.top && q == a.nyu - 1));
   const bool vint = hasv && q > 0 && q < a.nyv - 1;
   load_twiddles(tw, twg, L);
   const size_t ru = ((size_t)q * ngz + z) * Mx;
@@ -248,7 +249,8 @@ struct ZNLArgs {
   void *nxz1, *nxz3, *fvv, *nxz2, *fuv, *fvw;
   const double *k1x, *k1z;
   int nyu, nyv, N2, Mx;
-  int zs;  // Z row stride (elements)
+  int zs;   // Z row stride (elements)
+  int top;  // also the x/z terms of the top ghost u plane and the top wall v plane
 };
 
 constexpr int kMaxOut = 8;  // outputs (k,t) per thread: N2*TI <= kMaxOut*kThreads
@@ -265,8 +267,8 @@ __global__ void __launch_bounds__(kThreads) k_zfwd_nl(ZNLArgs a, FftPlan P, cons
   const bool isu = (int)blockIdx.y < a.nyu;
   const int q = isu ? blockIdx.y : blockIdx.y - a.nyu;
   const int ny = isu ? a.nyu : a.nyv;
-  const bool inter = q > 0 && q < ny - 1;
-  if (isu && !inter) return;  // ghost u planes: no product is needed there
+  const bool inter = q > 0 && (q < ny - 1 || (a.top && q == ny - 1));
+  if (isu && !inter) return;  // ghost u planes: no product is needed there (but see a.top)
   load_twiddles(tw, twg, L);
   const int ncol = N2 * Mx;
   const double sc = 1.0 / L;
@@ -632,8 +634,8 @@ __global__ void __launch_bounds__(TI * cmax(P, Q)) k_zfwd4(ZNLArgs a, const cplx
   const bool isu = (int)blockIdx.y < a.nyu;
   const int pl = isu ? blockIdx.y : blockIdx.y - a.nyu;
   const int ny = isu ? a.nyu : a.nyv;
-  const bool inter = pl > 0 && pl < ny - 1;
-  if (isu && !inter) return;  // ghost u planes: no product is needed there
+  const bool inter = pl > 0 && (pl < ny - 1 || (a.top && pl == ny - 1));
+  if (isu && !inter) return;  // ghost u planes: no product is needed there (but see a.top)
   C acc[Q];
   if (isu) {
     zfwd_product<T, L, P, Q, TI, true, 0>(a, twg, X, pl, inter, acc);
@@ -713,7 +715,7 @@ __global__ void __launch_bounds__(3 * xp_width(P, Q, R)) k_xpass4(XNLArgs a, con
   const int tr = tid / W, lane = tid - tr * W;
   const int q = blockIdx.y, z0 = blockIdx.x * R, ngz = a.ngz;
   const bool hasv = q < a.nyv;
-  const bool uint = q > 0 && q < a.nyu - 1;
+  const bool uint = q > 0 && (q < a.nyu - 1 || (a.top && q == a.nyu - 1));
   const bool vint = hasv && q > 0 && q < a.nyv - 1;
   const C* const* zi = reinterpret_cast<const C* const*>(a.zi);
 
@@ -1108,6 +1110,7 @@ void launch_xpass_nl(Solver& s, int cfl) {
   a.ngz = s.ngz;
   a.cfl = cfl;
   a.zs = s.zs;
+  a.top = s.compat_np > 0;
   if (try_xpass4(s, a)) return;
   switch (s.px.L) {
 #define C(n) case n: xpass_nl_t<n>(s, a); break;
@@ -1133,6 +1136,7 @@ void launch_zfwd_nl(Solver& s) {
   a.N2 = s.N2;
   a.Mx = s.Mx;
   a.zs = s.zs;
+  a.top = s.compat_np > 0;
   if (try_zfwd4(s, a)) return;
   switch (s.pz.L) {
 #define C(n) case n: zfwd_nl_t<n>(s, a); break;

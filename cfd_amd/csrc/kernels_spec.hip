@@ -545,6 +545,40 @@ void launch_mode_filter(Solver& s) {
                      s.F[CFD_U2], s.F[CFD_U3], (const double*)s.d_sigma, s.Ncol, s.nyu, s.nyv);
 }
 
+// The x/z part of the nonlinear term of plane q replaced by that of plane
+// map[q] in the stacks a and b (b may be null) of nq planes: the nonlinear
+// term as ops_in_planes leaves it on each rank (FOU3D.f90:1553-1613, every
+// plane of a rank gets the derivative sums of the rank's last plane).  Every
+// source plane maps to itself, so no plane is read after it is written.
+template <typename C>
+__global__ void k_plane_map(C* __restrict__ a, C* __restrict__ b, const int* __restrict__ map, int nq, int ncol) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)nq * ncol) return;
+  const int q = (int)(e / ncol);
+  const int src = map[q];
+  if (src == q) return;
+  const size_t o = (size_t)src * ncol + (e - (size_t)q * ncol);
+  a[e] = a[o];
+  if (b) b[e] = b[o];
+}
+
+template <typename C>
+void plane_map_t(Solver& s) {
+  const size_t nu = (size_t)s.nyu * s.Ncol, nv = (size_t)s.nyv * s.Ncol;
+  hipLaunchKernelGGL(k_plane_map<C>, dim3((unsigned)((nu + 255) / 256)), dim3(256), 0, s.stream, (C*)s.nxz1,
+                     (C*)s.nxz3, (const int*)s.d_umap, s.nyu, s.Ncol);
+  hipLaunchKernelGGL(k_plane_map<C>, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, s.stream, (C*)s.nxz2,
+                     (C*)nullptr, (const int*)s.d_vmap, s.nyv, s.Ncol);
+}
+
+void launch_plane_map(Solver& s) {
+  if (!s.compat_np) return;
+  if (s.mixed)
+    plane_map_t<float2>(s);
+  else
+    plane_map_t<double2>(s);
+}
+
 void launch_flowrate(Solver& s) {
   hipLaunchKernelGGL(k_flowrate, dim3(1), dim3(1), 0, s.stream, (const double2*)s.F[CFD_U1], (size_t)s.Ncol, s.jlo_u,
                      s.nn, (const double*)s.d_yv, s.jlo_v, s.d_sc);

@@ -59,3 +59,17 @@ def band_table(params):
     if rc != 0:
         raise _lib.CfdError("cfd_band_table failed")
     return list(out)
+
+
+def plane_partition(params, np_ranks):
+    """The reference's plane partition over np_ranks >= 3 ranks
+    (start.f90:1652-1866): {"u": [(first, last), ...], "v": [...]} per rank,
+    planes without walls in the reference's y indices."""
+    c = params.to_c()
+    out = (ctypes.c_int * (4 * np_ranks))()
+    rc = _lib.load().cfd_plane_partition(ctypes.byref(c), int(np_ranks), out)
+    if rc != 0:
+        raise _lib.CfdError("cfd_plane_partition failed")
+    lim = list(out)
+    return {g: [(lim[(k * 2) * np_ranks + r], lim[(k * 2 + 1) * np_ranks + r]) for r in range(np_ranks)]
+            for k, g in enumerate(("u", "v"))}

@@ -125,6 +125,13 @@ module cfd_amd_mod
       type(c_ptr), value :: h
       real(c_double) :: v(11)
     end function
+    ! the as-shipped last-plane nonlinear term of a run on np ranks
+    ! (ops_in_planes, FOU3D.f90:1553-1613); np = 0: per plane
+    integer(c_int) function cfd_set_compat_lastplane(h, np) bind(C, name="cfd_set_compat_lastplane")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: np
+    end function
     ! the band table N(1:4, 0:4) (start.f90:89-111; y_grid_canopy.f90:250-275)
     integer(c_int) function cfd_band_table(p, N20) bind(C, name="cfd_band_table")
       import :: c_int, cfd_params

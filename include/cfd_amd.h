@@ -142,6 +142,21 @@ int cfd_hist(cfd_handle h, double* v);
    per column keeps each mode divergence-free.  A dissipation for coarse,
    under-resolved grids; alpha = 0 switches it off. */
 int cfd_set_mode_filter(cfd_handle h, double alpha, int order, double eta_c);
+/* The reference's as-shipped nonlinear term for a run on np ranks
+   (ops_in_planes, FOU3D.f90:1553-1613): the x/z derivative scratch is
+   overwritten plane by plane over each rank's planes with walls and then
+   copied to all the rank's planes without walls, so every plane of a rank
+   takes the x/z terms of its last plane (the top rank's: the top ghost u
+   plane, the top wall v plane); the y terms stay per plane.  np >= 3 (the
+   reference's partition needs three ranks); np = 0 restores the per-plane
+   term, the default.  Kept across substeps. */
+int cfd_set_compat_lastplane(cfd_handle h, int np);
+/* The reference's plane partition over np >= 3 ranks (start.f90:1652-1866,
+   one resolution in every band): lim[(g * 2 + e) * np + r] = planelim(grid,
+   e + 1, r), the first (e = 0) and last (e = 1) plane of rank r without
+   walls, g = 0 the u grid, 1 the v grid, in the reference's y indices.
+   Host-only. */
+int cfd_plane_partition(const cfd_params* p, int np, int* lim);
 /* ---------------------------------------------------------------------------
  * Field files in the reference layout: Fortran sequential unformatted, 4-byte
  * little-endian record markers (record_out, FOU3D.f90:2172-2345; u_to_buff /

@@ -181,6 +181,10 @@ struct or_state {
   fplan pz, px2; /* Ngz complex; Ngx/2 complex for the real transform */
   cplx* twx;     /* exp(-2 pi i k/Ngx), k=0..Ngx/2 */
   struct or_stats* st; /* statistics accumulators, created by or_stats_init */
+  /* the as-shipped last-plane nonlinear term of np ranks (0: per plane) */
+  int cnp;
+  int* clim;             /* planelim(grid, 1:2, 0:np-1), u then v: [(g*2+e)*np + r] */
+  cplx *xz1, *xz3, *xz2; /* x/z derivative sums of every plane */
 };
 
 static int g_threads = 0;
@@ -677,6 +681,7 @@ void or_destroy(or_state* s) {
   if (s->px2.tw) fplan_free(&s->px2);
   free(s->twx);
   or_stats_free(s);
+  free(s->clim); free(s->xz1); free(s->xz3); free(s->xz2);
   free(s);
 }
 
@@ -848,6 +853,118 @@ static void interp_v(or_state* s, cplx* ui, const cplx* v) {
 
 static double dmax(double a, double b) { return a > b ? a : b; }
 
+/* ------------------------------------------------------------------------ */
+/* the as-shipped nonlinear term of a run on np ranks                        */
+/* ------------------------------------------------------------------------ */
+/* proc_lims_planes for one resolution in every band (start.f90:1652-1866):
+   procs per band from the load balance, then planelim per rank; u grid over
+   N(4,.), v grid over N(3,.).  lim[(g*2+e)*np + r], g 0 u, 1 v, e 0 from,
+   1 to.  1 when the balance cannot run (np < 3), 3 when the band limits are
+   not strictly increasing. */
+#define NBT(r, b) (N20[4 * (b) + (r) - 1])
+int or_plane_partition(const int* N20, int ngalx, int ngalz, int np, int* lim) {
+  if (np < 3) return 1;
+  /* band limits not strictly increasing: an empty or negative band, ranges
+     that overlap */
+  for (int row = 3; row <= 4; ++row)
+    if (!(NBT(row, 0) < NBT(row, 1) && NBT(row, 1) < NBT(row, 2) && NBT(row, 2) < NBT(row, 3))) return 3;
+  enum { bot = 1, mid = 2, top = 3, nband = 3 };
+  double load_band[4] = {0}, i_load[4] = {0}, maxload;
+  int procs[4] = {0}, procs_b[4], nplanes, iplanes, rem;
+  for (int iband = 1; iband <= nband; ++iband)
+    load_band[iband] = (double)(ngalx + 2) * ngalz * (NBT(4, iband) - NBT(4, iband - 1));
+#define BALANCE()                                                                         \
+  do {                                                                                    \
+    rem = 0;                                                                              \
+    nplanes = NBT(4, bot) - NBT(4, 0);                                                    \
+    procs[bot] = (int)ceil(nplanes * 1.0 / iplanes);                                      \
+    procs[top] = procs[bot];                                                              \
+    rem = rem + procs[bot] + procs[top];                                                  \
+    i_load[bot] = load_band[bot] / nplanes * iplanes;                                     \
+    i_load[top] = load_band[top] / nplanes * iplanes;                                     \
+    nplanes = NBT(4, mid) - NBT(4, bot);                                                  \
+    procs[mid] = np - rem;                                                                \
+    if (procs[mid] > 0) i_load[mid] = load_band[mid] / nplanes * ceil(nplanes * 1.0 / procs[mid]); \
+  } while (0)
+#define MAXLOAD() fmax(i_load[1], fmax(i_load[2], i_load[3]))
+  iplanes = 1; /* one rank per fine plane first */
+  BALANCE();
+  while (procs[mid] <= 0) { iplanes = iplanes + 1; BALANCE(); }
+  maxload = 2.0 * MAXLOAD();
+  while (MAXLOAD() < maxload) { iplanes = iplanes + 1; maxload = MAXLOAD(); BALANCE(); }
+  iplanes = iplanes - 1;
+  BALANCE();
+#undef BALANCE
+#undef MAXLOAD
+  for (int g = 0; g < 2; ++g) {
+    const int row = g == 0 ? 4 : 3;
+    for (int b = 0; b <= nband; ++b) procs_b[b] = procs[b];
+    for (int iband = 1; iband <= nband; ++iband) {
+      nplanes = NBT(row, iband) - NBT(row, iband - 1);
+      iplanes = nplanes / procs_b[iband];
+      rem = nplanes - iplanes * procs_b[iband];
+      for (int iproc = 1; iproc <= procs_b[iband]; ++iproc) {
+        const int r = iproc - 1 + procs_b[iband - 1];
+        int from, to;
+        if (iproc <= rem) { from = (iplanes + 1) * (iproc - 1) + 1; to = (iplanes + 1) * iproc; }
+        else { from = iplanes * (iproc - rem - 1) + (iplanes + 1) * rem + 1; to = iplanes * (iproc - rem) + (iplanes + 1) * rem; }
+        lim[(g * 2) * np + r] = from + NBT(row, iband - 1);
+        lim[(g * 2 + 1) * np + r] = to + NBT(row, iband - 1);
+      }
+      procs_b[iband] = procs_b[iband - 1] + procs_b[iband];
+    }
+  }
+  return 0;
+}
+#undef NBT
+
+/* ops_in_planes' assembly (FOU3D.f90:1553-1613): on rank r the derivative
+   scratch holds the sums of the last plane of limPL_incw (planelim, the first
+   rank one plane lower, the last one plane higher: start.f90:1866-1869), and
+   every plane of limPL_excw (= planelim) receives them */
+static void lastplane_assign(or_state* s) {
+  const int np = s->cnp;
+  for (int g = 0; g < 2; ++g)
+    for (int r = 0; r < np; ++r) {
+      const int from = s->clim[(g * 2) * np + r], to = s->clim[(g * 2 + 1) * np + r];
+      const int inc_from = from - (r == 0), inc_to = to + (r == np - 1);
+      if (inc_from > inc_to) continue;
+      const int last = inc_to;
+#pragma omp parallel for schedule(static)
+      for (int col = 0; col < s->Ncol; ++col)
+        for (int j = from; j <= to; ++j) {
+          if (g == 0) {
+            NU1[fu(s, col, j)] = s->xz1[fu(s, col, last)];
+            NU3[fu(s, col, j)] = s->xz3[fu(s, col, last)];
+          } else {
+            NU2[fv(s, col, j)] = s->xz2[fv(s, col, last)];
+          }
+        }
+    }
+}
+
+int or_set_compat_lastplane(or_state* s, const int* N20, int np) {
+  free(s->clim); free(s->xz1); free(s->xz3); free(s->xz2);
+  s->clim = NULL; s->xz1 = s->xz3 = s->xz2 = NULL;
+  s->cnp = 0;
+  if (np == 0) return 0;
+  int* lim = (int*)malloc(sizeof(int) * 4 * (size_t)np);
+  if (or_plane_partition(N20, s->ngx, s->ngz, np, lim)) { free(lim); return 1; }
+  for (int g = 0; g < 2; ++g) {
+    const int lo = g == 0 ? s->jlo_u : s->jlo_v, hi = g == 0 ? s->jhi_u : s->jhi_v;
+    for (int r = 0; r < np; ++r) {
+      const int from = lim[(g * 2) * np + r], to = lim[(g * 2 + 1) * np + r];
+      if (from <= to && (from < lo + 1 || to > hi - 1)) { free(lim); return 2; }
+    }
+  }
+  s->clim = lim;
+  s->xz1 = (cplx*)calloc((size_t)s->Ncol * s->nyu, sizeof(cplx));
+  s->xz3 = (cplx*)calloc((size_t)s->Ncol * s->nyu, sizeof(cplx));
+  s->xz2 = (cplx*)calloc((size_t)s->Ncol * s->nyv, sizeof(cplx));
+  s->cnp = np;
+  return 0;
+}
+
 /* nonlinear (FOU3D.f90:41-259) with ops_in_planes (1529-1617) done per plane */
 void or_nonlinear(or_state* s, int kRK) {
   interp_u(s, s->u1i, U1);
@@ -891,7 +1008,11 @@ void or_nonlinear(or_state* s, int kRK) {
       for (int col = 0; col < s->Ncol; ++col) {
         const int i = col % s->Mx, k = col / s->Mx, sl = slot_of(s, k);
         s->vvc[fu(s, col, j)] = pl(s, Qvv, i, sl);
-        if (interior) {
+        if (s->cnp) { /* every plane's sums, walls included; assigned per rank below */
+          const int zero = (sl == ngz - s->N2 / 2);
+          s->xz1[fu(s, col, j)] = zero ? 0.0 : imul(s->k1x[i], pl(s, Quu, i, sl)) + imul(s->k1z[k], pl(s, Quw, i, sl));
+          s->xz3[fu(s, col, j)] = zero ? 0.0 : imul(s->k1x[i], pl(s, Quw, i, sl)) + imul(s->k1z[k], pl(s, Qww, i, sl));
+        } else if (interior) {
           const int zero = (sl == ngz - s->N2 / 2); /* der_x/der_z zero band (FOU3D.f90:329-333) */
           cplx a = 0.0, b = 0.0;
           if (!zero) {
@@ -935,7 +1056,10 @@ void or_nonlinear(or_state* s, int kRK) {
         const int i = col % s->Mx, k = col / s->Mx, sl = slot_of(s, k);
         s->uvf[fv(s, col, j)] = pl(s, Quv, i, sl);
         s->vwf[fv(s, col, j)] = pl(s, Qvw, i, sl);
-        if (interior) {
+        if (s->cnp) {
+          const int zero = (sl == ngz - s->N2 / 2);
+          s->xz2[fv(s, col, j)] = zero ? 0.0 : imul(s->k1x[i], pl(s, Quv, i, sl)) + imul(s->k1z[k], pl(s, Qvw, i, sl));
+        } else if (interior) {
           const int zero = (sl == ngz - s->N2 / 2);
           NU2[fv(s, col, j)] = zero ? 0.0 : imul(s->k1x[i], pl(s, Quv, i, sl)) + imul(s->k1z[k], pl(s, Qvw, i, sl));
         }
@@ -943,6 +1067,7 @@ void or_nonlinear(or_state* s, int kRK) {
     }
     free(B); free(w.w0); free(w.w1);
   }
+  if (s->cnp) lastplane_assign(s);
   /* y derivatives and assembly (FOU3D.f90:158-173, 453-493) */
 #pragma omp parallel for schedule(static)
   for (int col = 0; col < s->Ncol; ++col) {

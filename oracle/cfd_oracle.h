@@ -69,6 +69,11 @@ void or_get_scalars(const or_state* s, double* v);
 /* operators (reference names) */
 void or_rhs0(or_state* s, int kRK);                 /* RHS0_u1/u2/u3 */
 void or_nonlinear(or_state* s, int kRK);            /* nonlinear (NL part) */
+/* the as-shipped last-plane nonlinear term of np >= 3 ranks (0: per plane);
+   N20 the band table N(1:4, 0:4) */
+int  or_set_compat_lastplane(or_state* s, const int* N20, int np);
+/* proc_lims_planes (start.f90:1652-1866): planelim of the u and v grids */
+int  or_plane_partition(const int* N20, int ngalx, int ngalz, int np, int* lim);
 void or_solve_u(or_state* s, int comp, int kRK);    /* solveU(u1|u3) incl. IB */
 void or_solve_v(or_state* s, int kRK);              /* solveV(u2) incl. IB */
 void or_meanflow_ctU(or_state* s, int kRK);

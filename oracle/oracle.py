@@ -48,7 +48,8 @@ def load():
                      "or_set_field", "or_get_field", "or_set_scalars", "or_get_scalars", "or_rhs0",
                      "or_nonlinear", "or_solve_u", "or_solve_v", "or_meanflow_ctU", "or_solve_p", "or_vcorr",
                      "or_divergence", "or_substep", "or_step", "or_flowrate", "or_stats_init", "or_stats",
-                     "or_stats_dims", "or_stats_get", "or_stats_reset"):
+                     "or_stats_dims", "or_stats_get", "or_stats_reset", "or_set_compat_lastplane",
+                     "or_plane_partition"):
             getattr(lib, name).argtypes = None
         lib.or_flowrate.restype = ctypes.c_double
         _lib = lib
@@ -63,6 +64,18 @@ def to_or_params(p):
     for n in DBL_FIELDS:
         setattr(c, n, float(getattr(p, n)))
     return c
+
+
+def plane_partition(N20, ngalx, ngalz, np_ranks):
+    """proc_lims_planes (start.f90:1652-1866): {"u": [(first, last)], "v":
+    [...]} per rank, planes without walls; None when np_ranks < 3."""
+    lim = (ctypes.c_int * (4 * np_ranks))()
+    if load().or_plane_partition((ctypes.c_int * 20)(*[int(v) for v in N20]), ctypes.c_int(ngalx),
+                                 ctypes.c_int(ngalz), ctypes.c_int(np_ranks), lim):
+        return None
+    lim = list(lim)
+    return {g: [(lim[(k * 2) * np_ranks + r], lim[(k * 2 + 1) * np_ranks + r]) for r in range(np_ranks)]
+            for k, g in enumerate(("u", "v"))}
 
 
 def set_threads(n):
@@ -186,6 +199,14 @@ class Oracle:
     STATS = ("Um", "U2m", "Vm", "V2m", "Wm", "W2m", "Pm", "P2m", "UVm", "UWm", "VWm", "wxm", "wx2m")
     SPECTRA = ("spU", "spV", "spW", "spUV", "spP")
     _STAT_GRID = "uuvvuuppuuvvv"
+
+    def set_compat_lastplane(self, N20, np_ranks):
+        """The as-shipped last-plane nonlinear term of np_ranks ranks
+        (ops_in_planes, FOU3D.f90:1553-1613); 0: per plane."""
+        rc = self.lib.or_set_compat_lastplane(self.h, (ctypes.c_int * 20)(*[int(v) for v in N20]),
+                                              ctypes.c_int(int(np_ranks)))
+        if rc:
+            raise ValueError("oracle last-plane replay: no partition (%d)" % rc)
 
     def stats_init(self, N20):
         rc = self.lib.or_stats_init(self.h, (ctypes.c_int * 20)(*[int(v) for v in N20]))

@@ -76,3 +76,10 @@ def rel_l2(a, b):
 def max_rel(a, b):
     den = np.abs(b).max()
     return np.abs(a - b).max() / (den if den > 0 else 1.0)
+
+# rough walls whose y bands are strictly increasing (4 x 12 posts per wall on
+# a 48 x 33 x 72 tanh grid: bands -3, 14, 18, 35), so the reference's plane
+# partition over np ranks exists
+BANDED = Params(ngalx=48, ngalz=72, ny=33, dsty=3, ygrid=1, ytanh_beta=2.0, geometry_type=3, ntilex=4,
+                ntilez=12, extras24=0, flag_ctpress=0, Re=2800.0, alp=1.0, bet=2.0, posth=0.06, r=1.0,
+                c_i=6.0, c_k=3.0, stag=0.5, CFL=0.5)

@@ -1,0 +1,170 @@
+"""The reference's plane partition (proc_lims_planes, start.f90:1652-1866)
+and the as-shipped last-plane nonlinear term (ops_in_planes,
+FOU3D.f90:1553-1613) on the host: the library's partition (cfd_plane_partition)
+against the oracle's, the properties the partition's algorithm fixes, and the
+oracle's replay against closed forms (tests/lastplane_cases.py).
+
+What the algorithm fixes: the ranges of each grid tile its planes without
+walls in rank order, each rank inside one band, the bands split into ranges
+whose sizes differ by at most one, larger ranges first, and the two fine
+bands get the same number of ranks; with np = 3 the balance must give one rank
+per band (the central band gets a rank only once each fine band has one), and
+with np = the number of u planes one u plane per rank (one rank per fine
+plane leaves exactly the central band's plane count, and the balance step
+back returns there).
+"""
+import numpy as np
+import pytest
+
+from cfd_amd import fieldfile, workloads
+from oracle import oracle
+from oracle.oracle import Oracle
+
+from . import lastplane_cases as lc
+from .helpers import BANDED, EDGE, SHIPPED, SMOOTH
+
+CONFIGS = {"banded": BANDED, "smooth": SMOOTH, "edge": EDGE,
+           "rt395_384x192x192": workloads.get("rt395_384x192x192")[0],
+           "rt550_768x384x384": workloads.get("rt550_768x384x384")[0]}
+
+
+def _bands(p):
+    N = fieldfile.band_table(p)
+    return {"u": [N[b * 4 + 3] for b in range(4)], "v": [N[b * 4 + 2] for b in range(4)]}, N
+
+
+@pytest.mark.parametrize("name", list(CONFIGS))
+def test_partition_matches_oracle(name):
+    p = CONFIGS[name]
+    _, N = _bands(p)
+    for n in range(3, 121):
+        assert fieldfile.plane_partition(p, n) == oracle.plane_partition(N, p.ngalx, p.ngalz, n), n
+
+
+@pytest.mark.parametrize("name", list(CONFIGS))
+def test_partition_properties(name):
+    p = CONFIGS[name]
+    B, _ = _bands(p)
+    for n in list(range(3, 40)) + [64, 100]:
+        part = fieldfile.plane_partition(p, n)
+        per_band = {}
+        for g in ("u", "v"):
+            b = B[g]
+            nxt = b[0] + 1
+            band_of = []
+            for (a, z) in part[g]:
+                assert a == nxt, (g, n)
+                nxt = z + 1
+                size = z - a + 1
+                assert size >= 0
+                if size:
+                    k = [i for i in range(1, 4) if b[i - 1] < a <= b[i]]
+                    assert len(k) == 1 and z <= b[k[0]], (g, n, a, z)
+                    band_of.append((k[0], size))
+            assert nxt == b[3] + 1, (g, n)
+            for k in range(1, 4):
+                sizes = [s for kk, s in band_of if kk == k]
+                assert sizes == sorted(sizes, reverse=True) and sizes[0] - sizes[-1] <= 1, (g, n, k, sizes)
+            per_band[g] = band_of
+        # ranks per band from the u grid: bottom and top equal
+        ranks_u = [sum(1 for k, _ in per_band["u"] if k == kk) for kk in (1, 2, 3)]
+        if all(s > 0 for (a, z) in part["u"] for s in [z - a + 1]):
+            assert ranks_u[0] == ranks_u[2], (n, ranks_u)
+
+
+@pytest.mark.parametrize("name", list(CONFIGS))
+def test_partition_closed_forms(name):
+    p = CONFIGS[name]
+    B, _ = _bands(p)
+    part = fieldfile.plane_partition(p, 3)
+    for g in ("u", "v"):
+        b = B[g]
+        assert part[g] == [(b[0] + 1, b[1]), (b[1] + 1, b[2]), (b[2] + 1, b[3])], g
+    nplanes = B["u"][3] - B["u"][0]
+    part = fieldfile.plane_partition(p, nplanes)
+    assert part["u"] == [(j, j) for j in range(B["u"][0] + 1, B["u"][3] + 1)]
+
+
+def test_partition_refusals():
+    with pytest.raises(Exception):
+        fieldfile.plane_partition(BANDED, 2)
+    # the shipped geometry on its short grid: band limits out of order
+    B, N = _bands(SHIPPED)
+    assert not all(B["u"][i] < B["u"][i + 1] for i in range(3))
+    with pytest.raises(Exception):
+        fieldfile.plane_partition(SHIPPED, 4)
+    assert oracle.plane_partition(N, SHIPPED.ngalx, SHIPPED.ngalz, 4) is None
+    assert oracle.plane_partition(N, SHIPPED.ngalx, SHIPPED.ngalz, 2) is None
+
+
+def _oracle(p, np_ranks):
+    orc = Oracle(p)
+    orc.set_compat_lastplane(fieldfile.band_table(p), np_ranks)
+    return orc
+
+
+def _jlo(s):
+    return {"u": s.jlo_u, "v": s.jlo_v}
+
+
+@pytest.mark.parametrize("np_ranks", [3, 5, 9])
+def test_oracle_u_grid_against_closed_form(np_ranks):
+    p = BANDED
+    R = lc.ranks(p, np_ranks)["u"]
+    kx, kz = lc.wavenumbers(p)
+    orc = _oracle(p, np_ranks)
+    jlo = _jlo(orc)
+    multi = [r for r in R if r[1] > r[0]]
+    cases = [R[1][2], R[-1][2]] + ([multi[0][0]] if multi else [])  # a last plane, the top ghost, a first plane
+    for j0 in cases:
+        plane = lc.load_plane_field(orc, p, jlo, orc.nyu, orc.nyv, "u", j0, seed=j0 + 50)
+        orc.nonlinear(2)
+        nu1 = orc.get_field("Nu1")
+        expect = 1j * kz * plane
+        got = np.zeros(orc.nyu, bool)
+        for a, z, src in R:
+            for j in range(a, z + 1):
+                col = nu1[:, j - jlo["u"]]
+                if src == j0:
+                    assert np.abs(col - expect).max() <= 1e-12 * np.abs(expect).max(), (j0, j)
+                    got[j - jlo["u"]] = True
+                else:
+                    assert np.abs(col).max() <= 1e-14 * np.abs(expect).max(), (j0, j)
+        assert got.any() == any(src == j0 for _, _, src in R)
+    # np = 0: the per-plane term, i kz u3 on j0 alone
+    orc.set_compat_lastplane(fieldfile.band_table(p), 0)
+    j0 = R[1][0]
+    plane = lc.load_plane_field(orc, p, jlo, orc.nyu, orc.nyv, "u", j0, seed=3)
+    orc.nonlinear(2)
+    nu1 = orc.get_field("Nu1")
+    for j in range(jlo["u"] + 1, jlo["u"] + orc.nyu - 1):
+        e = 1j * kz * plane if j == j0 else 0 * plane
+        assert np.abs(nu1[:, j - jlo["u"]] - e).max() <= 1e-12 * np.abs(kz * plane).max(), j
+    orc.close()
+
+
+@pytest.mark.parametrize("np_ranks", [3, 6])
+def test_oracle_v_grid_against_closed_form(np_ranks):
+    p = BANDED
+    R = lc.ranks(p, np_ranks)["v"]
+    kx, kz = lc.wavenumbers(p)
+    orc = _oracle(p, np_ranks)
+    ref = Oracle(p)
+    jlo = _jlo(orc)
+    for j0 in (R[0][2], R[1][0] + (1 if R[1][1] > R[1][0] else 0)):
+        plane = lc.load_plane_field(orc, p, jlo, orc.nyu, orc.nyv, "v", j0, seed=j0 + 7)
+        lc.load_plane_field(ref, p, jlo, ref.nyu, ref.nyv, "v", j0, seed=j0 + 7)
+        orc.nonlinear(2)
+        ref.nonlinear(2)
+        d = orc.get_field("Nu2") - ref.get_field("Nu2")
+        xz = 1j * kx * plane
+        for a, z, src in R:
+            for j in range(a, z + 1):
+                e = xz * ((src == j0) - (j == j0))
+                assert np.abs(d[:, j - jlo["v"]] - e).max() <= 1e-12 * np.abs(xz).max(), (j0, j)
+    # the top rank's last v plane is the top wall face, where v = 0: its
+    # planes lose their x/z term
+    a, z, src = R[-1]
+    assert src == jlo["v"] + orc.nyv - 1
+    orc.close()
+    ref.close()
