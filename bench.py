@@ -180,7 +180,10 @@ def learned_leg(args, fine_value, fine_dt):
             if net is None:
                 net = cnn.CNN(ch.nyu, pc.ngalz, pc.ngalx)
                 net.set_params(w["params"])
-                net.set_scales(w["scales"])
+                sc = np.array(w["scales"], np.float32).copy()
+                if "gain" in w.files:  # the damped correction of the closure experiment
+                    sc[3:] *= float(w["gain"])
+                net.set_scales(sc)
             ch.closure_attach(net, every=every)
         for s in range(args.warmup):
             ch.substep(s % 3 + 1)
@@ -210,7 +213,8 @@ def learned_leg(args, fine_value, fine_dt):
                for lab in r if lab == "coarse" or lab.startswith("coarse_closure_every")},
             "coarse_closure_fallback": {k: r.get("coarse_closure_fallback", {}).get(k)
                                         for k in ("closure_every", "switch_t", "completed", "t_reached", "drag")},
-            "closure_every": r.get("closure_every"), "coarse_filter": r.get("coarse_filter")}
+            "closure_every": r.get("closure_every"), "closure_gain": r.get("closure_gain"),
+            "coarse_filter": r.get("coarse_filter"), "gain_scan": r.get("gain_scan")}
     return out
 
 

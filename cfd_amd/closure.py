@@ -187,14 +187,23 @@ def main(argv=None):
     ap.add_argument("--max-coarse-steps", type=int, default=20000)
     ap.add_argument("--coarse-start", type=float, default=0.0,
                     help="fraction of the fine run's time at which the coarse runs start from its filtered state")
+    ap.add_argument("--closure-gain", type=float, default=0.35,
+                    help="the network's output scales times this gain in the coarse runs")
+    ap.add_argument("--gain-scan", default="0,0.2,0.35,0.5,0.7",
+                    help="gains of the scan (closure every step) from the start and the validation start")
+    ap.add_argument("--validation-start", type=float, default=0.75,
+                    help="fraction of the fine run's time of the scan's second start")
     ap.add_argument("--filter", default="36,32,0",
                     help="modal filter of the coarse runs, alpha,order,eta_c (alpha 0: none)")
+    ap.add_argument("--weights", default=None,
+                    help="evaluate this trained network (closure_weights.npz) instead of training one")
     ap.add_argument("--out", default=".")
     ap.add_argument("--seed", type=int, default=0)
     a = ap.parse_args(argv)
     fa, fo, fe = a.filter.split(",")
     filt = (float(fa), int(fo), float(fe))
     every_list = [int(e) for e in a.closure_every.split(",")]
+    gains = [float(g) for g in a.gain_scan.split(",")] if a.gain_scan else []
     pf, _ = workloads.get("rt395_384x192x192")
     pc, _ = workloads.get("rt395_coarse_192x192x96")
     fine, scratch = Channel(pf), Channel(pc)
@@ -203,27 +212,36 @@ def main(argv=None):
     net.dataset(a.slots + a.held_out)
     rng = np.random.default_rng(a.seed)
     report = {"samples": a.samples, "every": a.every, "slots": a.slots, "train_per_sample": a.train_per_sample,
-              "closure_every": every_list, "coarse_filter": {"alpha": filt[0], "order": filt[1], "eta_c": filt[2]}}
-    # pass 1: the fine run alone, probing the sample magnitudes along the
-    # trajectory; the normalisation takes the largest RMS of each quantity
-    f0, q0 = start_fields(fine, pf, seed=0)
-    probe = cnn.CNN(scratch.nyu, pc.ngalz, pc.ngalx)
-    probe.dataset(1)
-    nsteps = a.samples * a.every
-    rmax = np.zeros(6)
-    for k in range(nsteps):
-        fine.step(1)
-        if (k + 1) % max(nsteps // 24, 1) == 0:
-            rmax = np.maximum(rmax, snapshot(fine, scratch, probe, 0))
-    probe.close()
-    report["scales_max_rms"] = rmax.tolist()
-    net.set_scales(rmax.astype(np.float32))
-    print("scales", rmax, flush=True)
+              "closure_every": every_list, "closure_gain": a.closure_gain, "coarse_filter": {"alpha": filt[0], "order": filt[1], "eta_c": filt[2]}}
+    train = a.weights is None
+    if train:
+        # pass 1: the fine run alone, probing the sample magnitudes along the
+        # trajectory; the normalisation takes the largest RMS of each quantity
+        f0, q0 = start_fields(fine, pf, seed=0)
+        probe = cnn.CNN(scratch.nyu, pc.ngalz, pc.ngalx)
+        probe.dataset(1)
+        nsteps = a.samples * a.every
+        rmax = np.zeros(6)
+        for k in range(nsteps):
+            fine.step(1)
+            if (k + 1) % max(nsteps // 24, 1) == 0:
+                rmax = np.maximum(rmax, snapshot(fine, scratch, probe, 0))
+        probe.close()
+        report["scales_max_rms"] = rmax.tolist()
+        net.set_scales(rmax.astype(np.float32))
+        print("scales", rmax, flush=True)
+    else:
+        wts = np.load(a.weights)
+        net.set_params(wts["params"])
+        net.set_scales(np.asarray(wts["scales"], np.float32))
+        report["weights"] = a.weights
     # pass 2: the same fine run from the same start: samples, online
     # training, and the judge's drag history
     f0, q0 = start_fields(fine, pf, seed=0)
     k_start = int(a.coarse_start * a.samples)
+    k_val = int(a.validation_start * a.samples)
     fs, ts = (f0, 0.0), None
+    fv = None
     hist_f = []
     losses = []
     filled = 0
@@ -238,6 +256,10 @@ def main(argv=None):
         wfine += time.perf_counter() - t1
         if k + 1 == k_start:
             fs = ({n: fine.get_field(n) for n in ("u1", "u2", "u3", "p")}, fine.scalars()["t"])
+        if k + 1 == k_val:
+            fv = ({n: fine.get_field(n) for n in ("u1", "u2", "u3", "p")}, fine.scalars()["t"])
+        if not train:
+            continue
         slot = k % a.slots
         rms = snapshot(fine, scratch, net, slot)
         filled = min(filled + 1, a.slots)
@@ -249,8 +271,9 @@ def main(argv=None):
         if k % 100 == 0:
             print("sample %5d  t %.3f  drag %.5f  train loss %.4f  (%.0f s)" % (
                 k, hist_f[-1][0], hist_f[-1][1], np.mean(losses[-100:]), time.perf_counter() - w0), flush=True)
-    report["train_seconds"] = time.perf_counter() - w0 - wfine
-    report["train_loss_last200"] = float(np.mean(losses[-200:]))
+    if train:
+        report["train_seconds"] = time.perf_counter() - w0 - wfine
+        report["train_loss_last200"] = float(np.mean(losses[-200:]))
     # held-out samples further along the fine run
     zero = cnn.CNN(scratch.nyu, pc.ngalz, pc.ngalx)
     zero.set_params(np.zeros(cnn.nparams(), np.float32))
@@ -276,7 +299,13 @@ def main(argv=None):
     report["fine"] = {"steps": len(hist_f), "t_end": t_end, "wall_s": wfine,
                       "sim_time_per_s": float(hist_f[a.samples * a.every - 1, 0] / max(wfine, 1e-9))}
     w = net.params()
-    np.savez(os.path.join(a.out, "closure_weights.npz"), params=w, scales=net.scales())
+    scales = net.scales()
+    np.savez(os.path.join(a.out, "closure_weights.npz"), params=w, scales=scales, gain=a.closure_gain)
+
+    def set_gain(g):
+        sc = np.array(scales, np.float32).copy()
+        sc[3:] *= g
+        net.set_scales(sc)
     zero.close()
     scratch.close()
     # the coarse runs from the filtered start field, without and with the
@@ -286,6 +315,7 @@ def main(argv=None):
     report["coarse_start_t"] = t_start
     cap = 50.0 * float(np.abs(hist_f[:, 1]).max())
     runs, prof = {}, {}
+    set_gain(a.closure_gain)
     cases = [("coarse", None, 0)] + [("coarse_closure_every%d" % e, "plain", e) for e in every_list]
     cases.append(("coarse_closure_clip_every%d" % every_list[0], "clip", every_list[0]))
     for label, attach, every in cases:
@@ -317,6 +347,39 @@ def main(argv=None):
             r["mean_profile_error"] = float(np.linalg.norm(prof[label] - uf) / np.linalg.norm(uf))
         print(label, json.dumps(r), flush=True)
     report.update(runs)
+    # the gain scan: closure every step, from the start and from the
+    # validation start, against the fine run over the last three quarters
+    scan = {}
+    for name, st in (("start", fs), ("validation_start", fv)):
+        if st is None or not gains:
+            continue
+        fcs = filtered_start(pc, st[0], pf)
+        res = []
+        for gval in gains:
+            ch = Channel(pc)
+            g = ch.ygrid()
+            for kname in ("u1", "u2", "u3", "p"):
+                ch.set_field(kname, fcs[kname])
+            ch.set_scalars(t=st[1], dt=0.0, mpgx=-float(np.interp(st[1], hist_f[:, 0], hist_f[:, 1])),
+                           QxT=ic.flowrate(fcs["u1"], g, pc.ny))
+            ch.set_mode_filter(*filt)
+            if gval:
+                set_gain(gval)
+                ch.closure_attach(net, every=1)
+            hist, _ = run_until(ch, t_end, a.max_coarse_steps, cap)
+            r = compare(hist, hist_f, st[1] + 0.25 * (t_end - st[1]), t_end)
+            r["gain"] = gval
+            r["t_reached"] = float(hist[-1, 0]) if len(hist) else st[1]
+            if r["t_reached"] >= t_end:
+                uf = fine.get_field("u1")[0].real
+                r["mean_profile_error"] = float(np.linalg.norm(ch.get_field("u1")[0].real - uf) / np.linalg.norm(uf))
+            res.append(r)
+            print("gain scan", name, json.dumps(r), flush=True)
+            ch.closure_attach(None)
+            ch.close()
+        scan[name] = {"t": st[1], "runs": res}
+    report["gain_scan"] = scan
+    set_gain(a.closure_gain)
     # the learned path with the full DNS as its fallback: the coarse run with
     # the closure until its drag leaves the bound, then the fine solver from
     # its last checkpoint, zero-padded onto the fine modes, to the same t_end
