@@ -10,7 +10,6 @@ Pins:
 * the buff_to_u copy rule for planes of another size (FOU3D.f90:2631-2654),
   restated in numpy.
 """
-import dataclasses
 import os
 import shutil
 import struct

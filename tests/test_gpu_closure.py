@@ -23,7 +23,6 @@ import numpy as np
 import pytest
 
 from cfd_amd import Channel, Params, closure, cnn
-from oracle import cnn_ref
 
 from .helpers import random_field
 

@@ -18,7 +18,6 @@ import pytest
 from cfd_amd import Channel, ensemble
 from cfd_amd._lib import CfdError
 from cfd_amd.params import Params
-from oracle.oracle import Oracle
 
 from .helpers import rel_l2, max_rel
 from .test_gpu_parity import _cmp, _pair

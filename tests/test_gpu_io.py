@@ -21,7 +21,7 @@ import pytest
 from cfd_amd import Channel, fieldfile, ic
 from cfd_amd._lib import CfdError
 
-from .helpers import SHIPPED, SMALLPOSTS, rel_l2
+from .helpers import SHIPPED, SMALLPOSTS
 from .test_gpu_parity import _pair
 
 pytestmark = pytest.mark.gpu
