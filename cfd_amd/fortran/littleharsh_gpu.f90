@@ -12,7 +12,9 @@
 ! statistics and spectra every nstat = min(20, nwrite) steps (start.f90:372;
 ! FOU3D.f90:56-61, 107-110, 185-221) and writes them with each record
 ! (FOU3D.f90:223-226), and appends a record to the history file hist_*.dat
-! with each field record (start.f90:245-259; FOU3D.f90:2504-2534).  The
+! with each field record (start.f90:245-259; FOU3D.f90:2504-2534).  With
+! CFD_LASTPLANE_NP=np in the environment it runs the nonlinear term of a run
+! of the reference on np ranks (the last-plane replay).  The
 ! reference counts the sampling interval from
 ! iter0, which it also resets when it samples; a counter of its own, iterst,
 ! does that here so iter0 keeps the start of the run for maxiter.
@@ -33,7 +35,8 @@ program littleharsh_gpu
   integer :: physlim_bot, physlim_top, nyu, nyv, nyp, ncol, jlo, nn, j
   real(8) :: Kib, maxerr, maxt, maxA, hny, Lfracx, Lfracz, sc(8), Qx
   character(len=90) :: fnameimb, dirin, dirout
-  character(len=32) :: arg
+  character(len=32) :: arg, envnp
+  integer :: envst, lastnp
   real(8), allocatable :: yu(:), yv(:), du(:), dv(:), d2u(:), d2v(:)
   complex(8), allocatable :: u1(:,:), u2(:,:), u3(:,:), pp(:,:)
 
@@ -150,6 +153,15 @@ program littleharsh_gpu
     stop
   end if
   rc = cfd_create(p, h);               call cfd_check(rc, h, 'cfd_create')
+  ! CFD_LASTPLANE_NP=np: the as-shipped nonlinear term of a run of the
+  ! reference on np ranks (ops_in_planes, FOU3D.f90:1553-1613), to compare
+  ! with an MPI run; unset: the per-plane term
+  call get_environment_variable('CFD_LASTPLANE_NP', envnp, status=envst)
+  if (envst == 0 .and. len_trim(envnp) > 0) then
+    read(envnp, *) lastnp
+    rc = cfd_set_compat_lastplane(h, lastnp); call cfd_check(rc, h, 'cfd_set_compat_lastplane')
+    write(*,'(a,i6,a)') ' nonlinear term of a run on', lastnp, ' ranks (last-plane replay)'
+  end if
   rc = cfd_dims(h, d);                 call cfd_check(rc, h, 'cfd_dims')
   ncol = d(4); nyu = d(5); nyv = d(6); nyp = d(7); jlo = d(8)
   allocate(yu(nyu), yv(nyv), du(nyu), dv(nyv), d2u(3*nyu), d2v(3*nyv))

@@ -1,6 +1,7 @@
 """The Fortran driver (cfd_amd/fortran/littleharsh_gpu) reads input.in in
 the reference layout and calls the hot path through the ISO_C_BINDING
 interface; its result must equal the same run driven from Python."""
+import dataclasses
 import os
 import shutil
 import subprocess
@@ -255,3 +256,59 @@ def test_fortran_driver_history_file(tmp_path):
     assert out.returncode == 0, out.stdout + out.stderr
     assert len(statsfile.records(str(tmp_path / name))) == 6 + 2
     ch.close()
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran driver not built")
+def test_fortran_driver_lastplane(tmp_path):
+    """CFD_LASTPLANE_NP=3 makes the driver run the as-shipped nonlinear term of
+    a 3-rank run (cfd_set_compat_lastplane).  From a perturbed field (smooth
+    walls, whose band table has a partition; flag_init = 2 from field files
+    written by record_out), 3 steps: the centreline value equals the same run
+    driven from Python with the replay, and differs from the per-plane run."""
+    from cfd_amd import ic
+    p = dataclasses.replace(_shipped_params(), geometry_type=0)
+    w = Channel(p)
+    g = w.ygrid()
+    f = ic.project(w, ic.initial_fields(p, g, seed=0, amp=0.1))
+    for k in ("u1", "u2", "u3", "p"):
+        w.set_field(k, f[k])
+    w.set_scalars(t=0.0, dt=0.0, mpgx=p.mpgx, QxT=ic.flowrate(f["u1"], g, p.ny))
+    u1 = os.path.basename(w.record_out(str(tmp_path) + "/", iter=0))
+    w.close()
+    for fn in ("input.in", "geom_circ_rough.txt"):
+        shutil.copy(os.path.join(FDIR, fn), tmp_path / fn)
+    lines = open(tmp_path / "input.in").read().splitlines()
+    for i, l in enumerate(lines):
+        if l.startswith("type "):
+            lines[i] = "type  =         0 ! smooth"
+        if l.startswith("flgin"):
+            lines[i] = "flgin =         2 ! continue from field files"
+        if l.startswith("fnmeb"):
+            lines[i] = "fnmeb =" + u1
+    open(tmp_path / "input.in", "w").write("\n".join(lines) + "\n")
+    vals = {}
+    for tag, env in (("replay", "3"), ("per_plane", None)):
+        run_dir = tmp_path / tag
+        run_dir.mkdir()
+        for fn in os.listdir(tmp_path):
+            if os.path.isfile(tmp_path / fn):
+                shutil.copy(tmp_path / fn, run_dir / fn)
+        e = dict(os.environ)
+        e.pop("CFD_LASTPLANE_NP", None)
+        if env:
+            e["CFD_LASTPLANE_NP"] = env
+        print("fortran driver, 3 steps,", tag, flush=True)
+        out = subprocess.run([EXE, "3"], cwd=run_dir, capture_output=True, text=True, timeout=120, env=e)
+        assert out.returncode == 0, out.stdout + out.stderr
+        assert ("last-plane replay" in out.stdout) == (env is not None)
+        line = [l for l in out.stdout.splitlines() if "centreline" in l][0]
+        vals[tag] = float(line.split()[-1].replace("D", "E"))
+    ch = Channel(p)
+    ch.set_scalars(t=0.0, dt=0.0, dti=0.0, mpgx=p.mpgx, dgx=0.0, QxT=0.0, Qx=0.0)
+    ch.read_in(str(tmp_path) + "/", u1, flag_init=2)
+    ch.set_compat_lastplane(3)
+    ch.step(3)
+    py = ch.get_field("u1")[0, ch.nyu // 2].real
+    ch.close()
+    assert abs(py - vals["replay"]) <= 1e-13 * abs(py), (py, vals)
+    assert vals["replay"] != vals["per_plane"], vals
