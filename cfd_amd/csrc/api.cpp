@@ -257,7 +257,10 @@ int create_impl(const cfd_params* pp, const cfd_tuning* tp, cfd_handle* out) {
     s->err = "hipSetDevice failed";
     return bail(CFD_EHIP);
   }
-  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming) != hipSuccess) {
     s->err = "hipStreamCreate failed";
     return bail(CFD_EHIP);
   }
@@ -379,7 +382,8 @@ int create_impl(const cfd_params* pp, const cfd_tuning* tp, cfd_handle* out) {
       (rc = upload(s, &s->d_k1x, s->k1x)) || (rc = upload(s, &s->d_k2x, s->k2x)) ||
       (rc = upload(s, &s->d_k1z, s->k1z)) || (rc = upload(s, &s->d_k2z, s->k2z)) || (rc = upload(s, &s->d_plo, plo)) ||
       (rc = upload(s, &s->d_pup, pup)) || (rc = upload(s, &s->d_pdg, pdg)) ||
-      (rc = dalloc(s, &s->d_mf, (size_t)4 * (s->ny + 2))) || (rc = dalloc(s, &s->d_sc, 1)))
+      (rc = dalloc(s, &s->d_mf, (size_t)4 * (s->ny + 2))) || (rc = dalloc(s, &s->d_sc, 1)) ||
+      (rc = dalloc(s, &s->d_u11, (size_t)s->nn + 4)))
     return bail(rc);
 
   // fields and scratch
@@ -636,6 +640,10 @@ extern "C" int cfd_destroy(cfd_handle h) {
   for (void* q : s->allocs) hipFree(q);
   for (auto& e : s->evpool)
     if (e) hipEventDestroy(e);
+  if (s->side) hipStreamSynchronize(s->side);
+  if (s->ev_fork) hipEventDestroy(s->ev_fork);
+  if (s->ev_join) hipEventDestroy(s->ev_join);
+  if (s->side) hipStreamDestroy(s->side);
   if (s->stream) hipStreamDestroy(s->stream);
   delete s;
   return CFD_OK;
@@ -1282,6 +1290,20 @@ int op_substep(cfd_solver* s, int kRK) {
   const int wrc = stats_req_end(s, req);
   if (s->cl.cnn)
     if (int rc = closure_apply(s, kRK)) return rc;
+  // the u11 solve of the flow-rate correction beside the IB and column
+  // solves (it needs dt only); serial while the stream is being captured
+  bool side = false;
+  if (s->P.flag_ctpress == 0) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIPCHK(s, hipStreamIsCapturing(s->stream, &cs));
+    side = cs == hipStreamCaptureStatusNone;
+    if (side) {
+      HIPCHK(s, hipEventRecord(s->ev_fork, s->stream));
+      HIPCHK(s, hipStreamWaitEvent(s->side, s->ev_fork, 0));
+      launch_mf_u11(*s, kRK, s->side);
+      HIPCHK(s, hipEventRecord(s->ev_join, s->side));
+    }
+  }
   {
     Timed t(s, KC_IB);
     launch_ib_rhs(*s, kRK);
@@ -1290,7 +1312,11 @@ int op_substep(cfd_solver* s, int kRK) {
   { Timed t(s, KC_YSOLVE); launch_rhs_solve(*s, kRK); }
   if (s->P.flag_ctpress == 0) {
     Timed t(s, KC_ELEM);
-    launch_meanflow_lds(*s, kRK);
+    if (side)
+      HIPCHK(s, hipStreamWaitEvent(s->stream, s->ev_join, 0));
+    else
+      launch_mf_u11(*s, kRK, s->stream);
+    launch_mf_apply(*s);
   }
   { Timed t(s, KC_PRESS); launch_pressure(*s, kRK); }
   if (kRK == 3 && s->filt) launch_mode_filter(*s);

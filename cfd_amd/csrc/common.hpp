@@ -85,6 +85,11 @@ struct Solver {
   std::string err;
   int dev = 0;
   hipStream_t stream = nullptr;
+  // the flow-rate correction's u11 solve runs on `side` beside the column
+  // solves (fork after the CFL step, join before the correction)
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  double* d_u11 = nullptr;
 
   // sizes
   int ngx, ngz, ny, dsty, N1, N2, Mx, Ncol;
@@ -235,6 +240,8 @@ void launch_ib_rhs(Solver& s, int kRK);
 void launch_rhs_solve(Solver& s, int kRK);
 void launch_pressure(Solver& s, int kRK);
 void launch_meanflow_lds(Solver& s, int kRK);
+void launch_mf_u11(Solver& s, int kRK, hipStream_t st);  // u11 and Q(u11) into d_u11
+void launch_mf_apply(Solver& s);                       // g from Q(u), u += g u11
 
 // device complex helpers
 __host__ __device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }

@@ -969,6 +969,136 @@ __global__ void __launch_bounds__(256) k_meanflow_lds(double2* __restrict__ u1, 
   for (int j = threadIdx.x; j < n; j += blockDim.x) u1[(size_t)(j - jlo_u) * Ncol].x = ur[j] + g * x[j];
 }
 
+
+// The flow-rate correction split in two (littleharsh.f90:411-537): u11 =
+// (I - beta D^2)^-1 (-dRK dt) and Q(u11) depend on dt and the RK
+// coefficients only, so k_mf_u11 can run beside the column solves; k_mf_apply
+// then takes Q(u), g = (QxT - Q(u)) / Q(u11), mpgx += g and u += g u11 for the
+// (0,0) column.  The same operations in the same order as k_meanflow_lds.
+// u11[0..n-1] the solution, u11[n] = Q(u11); one block.
+__global__ void __launch_bounds__(256) k_mf_u11(const double* __restrict__ d2u, const double* __restrict__ yv,
+                                                int jlo_u, int jlo_v, int nn, const DevScalars* sc, double bRK,
+                                                double dRK, double Re, double g0, double g1, double* __restrict__ u11) {
+  extern __shared__ double sm[];
+  const int n = nn + 3;
+  double *a1 = sm, *a2 = sm + n, *a3 = sm + 2 * n, *x = sm + 3 * n, *dy = sm + 4 * n;
+  const double dt = sc->dt;
+  const double beta = bRK * dt / Re;
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    dy[j] = (j >= 1 && j <= nn + 1) ? (yv[j - jlo_v] - yv[j - 1 - jlo_v]) : 0.0;
+    if (j == 0) {
+      a1[j] = 0.0; a2[j] = 1.0; a3[j] = g0; x[j] = 0.0;
+    } else if (j == nn + 2) {
+      a1[j] = g1; a2[j] = 1.0; a3[j] = 0.0; x[j] = 0.0;
+    } else {
+      const double* d = d2u + 3 * (j - jlo_u);
+      a1[j] = -beta * d[0];
+      a2[j] = 1.0 - beta * d[1];
+      a3[j] = -beta * d[2];
+      x[j] = -dRK * dt;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    constexpr int C = kMfChunk;
+    double p2 = 1.0 / a2[0], p3 = a3[0];
+    a2[0] = p2;
+    int j = 1;
+    for (; j + C <= n; j += C) {
+      double c1[C], c2[C], c3[C];
+#pragma unroll
+      for (int q = 0; q < C; ++q) {
+        c1[q] = a1[j + q];
+        c2[q] = a2[j + q];
+        c3[q] = a3[j + q];
+      }
+#pragma unroll
+      for (int q = 0; q < C; ++q) {
+        const double s3 = p3 * p2;
+        a3[j + q - 1] = s3;
+        p2 = 1.0 / (c2[q] - c1[q] * s3);
+        a2[j + q] = p2;
+        p3 = c3[q];
+      }
+    }
+    for (; j < n; ++j) {
+      const double c1 = a1[j], c2 = a2[j], c3 = a3[j];
+      const double s3 = p3 * p2;
+      a3[j - 1] = s3;
+      p2 = 1.0 / (c2 - c1 * s3);
+      a2[j] = p2;
+      p3 = c3;
+    }
+    double xp = x[0] * a2[0];
+    x[0] = xp;
+    for (j = 1; j + C <= n; j += C) {
+      double c1[C], c2[C], cx[C];
+#pragma unroll
+      for (int q = 0; q < C; ++q) {
+        c1[q] = a1[j + q];
+        c2[q] = a2[j + q];
+        cx[q] = x[j + q];
+      }
+#pragma unroll
+      for (int q = 0; q < C; ++q) {
+        xp = (cx[q] - c1[q] * xp) * c2[q];
+        x[j + q] = xp;
+      }
+    }
+    for (; j < n; ++j) {
+      xp = (x[j] - a1[j] * xp) * a2[j];
+      x[j] = xp;
+    }
+    for (j = n - 2; j - C + 1 >= 0; j -= C) {
+      double c3[C], cx[C];
+#pragma unroll
+      for (int q = 0; q < C; ++q) {
+        c3[q] = a3[j - q];
+        cx[q] = x[j - q];
+      }
+#pragma unroll
+      for (int q = 0; q < C; ++q) {
+        xp = cx[q] - c3[q] * xp;
+        x[j - q] = xp;
+      }
+    }
+    for (; j >= 0; --j) {
+      xp = x[j] - a3[j] * xp;
+      x[j] = xp;
+    }
+    double Qx1 = 0.0;
+    for (int jj = 1; jj <= nn + 1; ++jj) Qx1 = Qx1 + dy[jj] * x[jj];
+    u11[n] = Qx1;
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < n; j += blockDim.x) u11[j] = x[j];
+}
+
+__global__ void __launch_bounds__(256) k_mf_apply(double2* __restrict__ u1, size_t Ncol, int jlo_u, int nn,
+                                                  const double* __restrict__ yv, int jlo_v, DevScalars* sc,
+                                                  const double* __restrict__ u11) {
+  extern __shared__ double sm[];
+  const int n = nn + 3;
+  double *ur = sm, *dy = sm + n;
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    ur[j] = u1[(size_t)(j - jlo_u) * Ncol].x;
+    dy[j] = (j >= 1 && j <= nn + 1) ? (yv[j - jlo_v] - yv[j - 1 - jlo_v]) : 0.0;
+  }
+  __syncthreads();
+  __shared__ double gsh;
+  if (threadIdx.x == 0) {
+    double Qx0 = 0.0;
+    for (int j = 1; j <= nn + 1; ++j) Qx0 = Qx0 + dy[j] * ur[j];
+    const double g = (sc->QxT - Qx0) / u11[n];
+    sc->mpgx = sc->mpgx + g;
+    sc->dgx = g;
+    gsh = g;
+  }
+  __syncthreads();
+  const double g = gsh;
+  for (int j = threadIdx.x; j < n; j += blockDim.x) u1[(size_t)(j - jlo_u) * Ncol].x = ur[j] + g * u11[j];
+}
+
 }  // namespace
 
 void fused_args(Solver& s, int kRK, void* out) {
@@ -1095,6 +1225,19 @@ void launch_pressure(Solver& s, int kRK) {
   } else {
     hipLaunchKernelGGL(k_pressure, dim3((s.Ncol + 63) / 64), dim3(64), 0, s.stream, a);
   }
+}
+
+void launch_mf_u11(Solver& s, int kRK, hipStream_t st) {
+  const int n = s.nn + 3;
+  hipLaunchKernelGGL(k_mf_u11, dim3(1), dim3(256), sizeof(double) * 5 * n, st, (const double*)s.d_d2u,
+                     (const double*)s.d_yv, s.jlo_u, s.jlo_v, s.nn, (const DevScalars*)s.d_sc, s.bRK[kRK - 1],
+                     s.dRK[kRK - 1], s.P.Re, s.gw[0], s.gw[1], s.d_u11);
+}
+
+void launch_mf_apply(Solver& s) {
+  const int n = s.nn + 3;
+  hipLaunchKernelGGL(k_mf_apply, dim3(1), dim3(256), sizeof(double) * 2 * n, s.stream, s.F[CFD_U1], (size_t)s.Ncol,
+                     s.jlo_u, s.nn, (const double*)s.d_yv, s.jlo_v, s.d_sc, (const double*)s.d_u11);
 }
 
 void launch_meanflow_lds(Solver& s, int kRK) {
