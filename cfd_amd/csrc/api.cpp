@@ -636,11 +636,13 @@ extern "C" int cfd_destroy(cfd_handle h) {
   cfd_solver* s = static_cast<cfd_solver*>(h);
   if (!s) return CFD_OK;
   hipSetDevice(s->dev);
+  // both streams drain before any buffer goes: the side stream's u11 solve
+  // may still be writing when a substep returned between fork and join
   if (s->stream) hipStreamSynchronize(s->stream);
+  if (s->side) hipStreamSynchronize(s->side);
   for (void* q : s->allocs) hipFree(q);
   for (auto& e : s->evpool)
     if (e) hipEventDestroy(e);
-  if (s->side) hipStreamSynchronize(s->side);
   if (s->ev_fork) hipEventDestroy(s->ev_fork);
   if (s->ev_join) hipEventDestroy(s->ev_join);
   if (s->side) hipStreamDestroy(s->side);
@@ -1311,11 +1313,11 @@ int op_substep(cfd_solver* s, int kRK) {
   op_ib(s, 0, 3);
   { Timed t(s, KC_YSOLVE); launch_rhs_solve(*s, kRK); }
   if (s->P.flag_ctpress == 0) {
+    // the join comes before the class timer starts, so the elementwise class
+    // times k_mf_apply (and the serial u11 solve when captured), not the wait
+    if (side) HIPCHK(s, hipStreamWaitEvent(s->stream, s->ev_join, 0));
     Timed t(s, KC_ELEM);
-    if (side)
-      HIPCHK(s, hipStreamWaitEvent(s->stream, s->ev_join, 0));
-    else
-      launch_mf_u11(*s, kRK, s->stream);
+    if (!side) launch_mf_u11(*s, kRK, s->stream);
     launch_mf_apply(*s);
   }
   { Timed t(s, KC_PRESS); launch_pressure(*s, kRK); }

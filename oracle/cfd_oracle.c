@@ -1230,6 +1230,10 @@ static double flowrate_c(const or_state* s, const cplx* u0) {
 }
 double or_flowrate(const or_state* s) { return flowrate_c(s, U1); }
 
+/* meanflow_ctP (littleharsh.f90:504-520): with a constant mean pressure
+   gradient the flow rate of the (0,0) column of u1 is only recorded */
+void or_meanflow_ctP(or_state* s) { s->Qx = flowrate_c(s, U1); }
+
 /* meanflow_ctU -> flowrate_corr with LUsol0/LU_build0/LU_dec0
    (littleharsh.f90:452-484, 522-537; tridLU_3D_IB.f90:320-346, 493-519, 605-622) */
 void or_meanflow_ctU(or_state* s, int kRK) {
@@ -1374,7 +1378,7 @@ void or_step(or_state* s, int nsteps) {
 #endif
   for (int it = 0; it < nsteps; ++it) {
     for (int kRK = 1; kRK <= 3; ++kRK) or_substep(s, kRK);
-    if (s->P.flag_ctpress == 1) s->Qx = flowrate_c(s, U1);
+    if (s->P.flag_ctpress == 1) or_meanflow_ctP(s);
   }
 }
 

@@ -81,6 +81,7 @@ void or_solve_p(or_state* s, int kRK);
 void or_vcorr(or_state* s, int kRK);
 void or_divergence(or_state* s);
 double or_flowrate(const or_state* s);              /* flowrateIm on (0,0) u1 */
+void or_meanflow_ctP(or_state* s);                 /* meanflow_ctP: Qx = flowrateIm */
 void or_substep(or_state* s, int kRK);
 void or_step(or_state* s, int nsteps);
 void or_set_threads(int n);

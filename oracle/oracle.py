@@ -47,7 +47,7 @@ def load():
         for name in ("or_destroy", "or_dims", "or_get_ygrid", "or_get_metrics", "or_get_iblist", "or_get_dg",
                      "or_set_field", "or_get_field", "or_set_scalars", "or_get_scalars", "or_rhs0",
                      "or_nonlinear", "or_solve_u", "or_solve_v", "or_meanflow_ctU", "or_solve_p", "or_vcorr",
-                     "or_divergence", "or_substep", "or_step", "or_flowrate", "or_stats_init", "or_stats",
+                     "or_divergence", "or_substep", "or_step", "or_flowrate", "or_meanflow_ctP", "or_stats_init", "or_stats",
                      "or_stats_dims", "or_stats_get", "or_stats_reset", "or_set_compat_lastplane",
                      "or_plane_partition"):
             getattr(lib, name).argtypes = None
@@ -179,6 +179,9 @@ class Oracle:
 
     def meanflow_ctU(self, kRK):
         self.lib.or_meanflow_ctU(self.h, ctypes.c_int(kRK))
+
+    def meanflow_ctP(self):
+        self.lib.or_meanflow_ctP(self.h)
 
     def solveP(self, kRK):
         self.lib.or_solve_p(self.h, ctypes.c_int(kRK))

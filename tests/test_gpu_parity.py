@@ -209,6 +209,86 @@ def test_fused_substep_equals_operator_sequence(case):
     assert abs(sa["mpgx"] - sb["mpgx"]) <= 1e-13 * abs(sb["QxT"]) / sb["dt"], (sa["mpgx"], sb["mpgx"])
 
 
+CTP_CASES = ("shipped", "smallposts")
+
+
+def _ctp(case):
+    """The case with a constant mean pressure gradient (flag_ctpress = 1, the
+    reference's shipped mode, input.in flpress = 1)."""
+    import dataclasses
+    return dataclasses.replace(CASES[case], flag_ctpress=1)
+
+
+def _reference_sequence(s, kRK, ctpress):
+    """The reference's RK loop body (littleharsh.f90:165-187) operator by
+    operator: no meanflow_ctU with a constant pressure gradient."""
+    s.RHS0(kRK)
+    s.nonlinear(kRK)
+    s.solveU(1, kRK)
+    s.solveV(kRK)
+    s.solveU(3, kRK)
+    if not ctpress:
+        s.meanflow_ctU(kRK)
+    s.solveP(kRK)
+    s.v_corr(kRK)
+
+
+@pytest.mark.parametrize("case", CTP_CASES)
+@pytest.mark.parametrize("fused", [True, False])
+def test_substep_constant_pressure_gradient(case, fused):
+    """flag_ctpress = 1: the fixed mean pressure gradient enters RHS0 of the
+    (0,0) u1 column (littleharsh.f90:650-656), meanflow_ctU is skipped and
+    meanflow_ctP records the flow rate after the third substep
+    (littleharsh.f90:177-196, 504-520).  One RK3 step from identical state,
+    GPU (fused substeps or the operator entry points) against the oracle's
+    operator sequence: fields to 1e-11, mpgx unchanged, Qx to 1e-13."""
+    ch, orc = _pair(_ctp(case), seed=55)
+    mpgx0 = orc.scalars()["mpgx"]
+    for kRK in (1, 2, 3):
+        if fused:
+            ch.substep(kRK)
+        else:
+            _reference_sequence(ch, kRK, True)
+        _reference_sequence(orc, kRK, True)
+    ch.meanflow_ctP()
+    orc.meanflow_ctP()
+    _cmp(ch, orc, ("u1", "u2", "u3", "p"), tol=1e-11)
+    a, b = ch.scalars(), orc.scalars()
+    assert a["mpgx"] == mpgx0 and b["mpgx"] == mpgx0
+    assert abs(a["Qx"] - b["Qx"]) <= 1e-13 * abs(b["Qx"]), (a["Qx"], b["Qx"])
+    assert abs(a["t"] - b["t"]) <= 1e-13 * abs(b["t"])
+
+
+@pytest.mark.parametrize("case", CTP_CASES)
+def test_ten_steps_constant_pressure_gradient(case):
+    """10 RK3 steps (cfd_step: fused substeps and meanflow_ctP after each
+    step) with a constant mean pressure gradient of -3/Re (the laminar value
+    at unit bulk velocity) from the seeded projected field, against or_step:
+    rel-L2 <= 1e-10 per velocity component, the same t and Qx."""
+    from cfd_amd import ic
+    p = _ctp(case)
+    ch = Channel(p)
+    orc = Oracle(p)
+    f = ic.initial_fields(p, ch.ygrid(), seed=0, amp=0.1)
+    f = ic.project(orc, f)
+    mpgx = -3.0 / p.Re
+    for s in (ch, orc):
+        for k in ("u1", "u2", "u3", "p"):
+            s.set_field(k, f[k])
+        s.set_scalars(t=0.0, dt=0.0, mpgx=mpgx, QxT=0.0)
+    ch.step(10)
+    orc.step(10)
+    _cmp(ch, orc, ("u1", "u2", "u3"), tol=1e-10)
+    a, b = ch.scalars(), orc.scalars()
+    assert abs(a["t"] - b["t"]) <= 1e-12 * b["t"]
+    assert a["mpgx"] == mpgx and b["mpgx"] == mpgx
+    assert abs(a["Qx"] - b["Qx"]) <= 1e-12 * abs(b["Qx"]), (a["Qx"], b["Qx"])
+    # the flow rate moved: the run is not a trivial fixed point
+    assert abs(b["Qx"] - ic.flowrate(f["u1"], ch.ygrid(), p.ny)) > 1e-9
+    d = ch.get_field("div")
+    assert np.abs(d).max() <= 1e-10 * np.abs(ch.get_field("u1")).max()
+
+
 def _channel(p, fft=None):
     """A Channel whose FFT kernels are chosen by CFD_AMD_FFT at creation."""
     if fft:
