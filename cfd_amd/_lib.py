@@ -65,6 +65,7 @@ SIGNATURES = {
     "cfd_band_table": (ctypes.c_int, [ctypes.POINTER(CParams), _I]),
     "cfd_plane_partition": (ctypes.c_int, [ctypes.POINTER(CParams), ctypes.c_int, _I]),
     "cfd_four_to_phys": (ctypes.c_int, [_H, ctypes.c_int, _D]),
+    "cfd_four_to_phys_planes": (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
     "cfd_cnn_nparams": (ctypes.c_int, []),
     "cfd_cnn_create": (ctypes.c_int, [ctypes.c_int] * 4 + [ctypes.POINTER(ctypes.c_void_p)]),
     "cfd_cnn_destroy": (ctypes.c_int, [_H]),

@@ -215,6 +215,14 @@ class Channel:
         self._call("cfd_four_to_phys", FIELDS.index(name), out.ctypes.data_as(_D))
         return out
 
+    def four_to_phys_planes(self, name, plane0, nplanes, dev_ptr):
+        """Planes [plane0, plane0 + nplanes) of four_to_phys into the device
+        buffer at dev_ptr (nplanes x Ngalz x Ngalx doubles on this channel's
+        device), queued on the channel's stream without any device
+        allocation; call sync() before reading the buffer."""
+        self._call("cfd_four_to_phys_planes", FIELDS.index(name), int(plane0), int(nplanes),
+                   ctypes.c_void_p(int(dev_ptr)))
+
     # -- statistics and spectra -------------------------------------------------
     def stats(self):
         """Add one sample of the spectra and statistics of the current state

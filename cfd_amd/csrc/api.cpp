@@ -257,6 +257,8 @@ int create_impl(const cfd_params* pp, const cfd_tuning* tp, cfd_handle* out) {
     s->err = "hipSetDevice failed";
     return bail(CFD_EHIP);
   }
+  // the device address of the no-allocation scratch, once per device
+  if (int rc = noalloc_init(p.device, s->err)) return bail(rc);
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming) != hipSuccess ||
@@ -1727,6 +1729,22 @@ extern "C" int cfd_four_to_phys(cfd_handle h, int which, double* phys) {
   if (e == hipSuccess) e = hipMemcpy(phys, p, ppl * ny * sizeof(double), hipMemcpyDeviceToHost);
   cleanup();
   if (e != hipSuccess) return fail(s, CFD_EHIP, std::string("four_to_phys: ") + hipGetErrorString(e));
+  return CFD_OK;
+}
+
+// the same transforms without allocating: planes [plane0, plane0 + nplanes)
+// into the caller's device buffer, asynchronously on the handle's stream, the
+// half-transformed planes and job tables in the per-GPU scratch of noalloc.hip
+extern "C" int cfd_four_to_phys_planes(cfd_handle h, int which, int plane0, int nplanes, double* phys) {
+  cfd_solver* s = static_cast<cfd_solver*>(h);
+  if (!s || !phys || which < 0 || which >= CFD_NFIELDS) return fail(s, CFD_EINVAL, "bad field argument");
+  const int ny = ny_of(*s, which);
+  if (plane0 < 0 || nplanes < 1 || plane0 + nplanes > ny)
+    return fail(s, CFD_EINVAL, "planes [" + std::to_string(plane0) + ", " + std::to_string(plane0 + nplanes) +
+                                   ") outside the field's " + std::to_string(ny) + " planes");
+  HIPCHK(s, hipSetDevice(s->dev));
+  std::string err;
+  if (int rc = four_to_phys_noalloc(*s, which, plane0, nplanes, phys, err)) return fail(s, rc, err);
   return CFD_OK;
 }
 

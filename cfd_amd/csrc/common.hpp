@@ -242,6 +242,12 @@ void launch_pressure(Solver& s, int kRK);
 void launch_meanflow_lds(Solver& s, int kRK);
 void launch_mf_u11(Solver& s, int kRK, hipStream_t st);  // u11 and Q(u11) into d_u11
 void launch_mf_apply(Solver& s);                       // g from Q(u), u += g u11
+// no-allocation physical planes (noalloc.hip): the file-scope scratch's
+// device address is taken once per device (noalloc_init, device current);
+// four_to_phys_noalloc queues the planes [plane0, plane0 + nplanes) of field
+// `which` into the device buffer phys on the solver's stream
+int noalloc_init(int dev, std::string& err);
+int four_to_phys_noalloc(Solver& s, int which, int plane0, int nplanes, double* phys, std::string& err);
 
 // device complex helpers
 __host__ __device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }

@@ -198,6 +198,20 @@ int cfd_field_file_read(const char* path, cfd_file_header* hd, int ny_g, int nx,
  * conventions). */
 int cfd_four_to_phys(cfd_handle h, int which, double* phys);
 
+/* No counterpart in the reference: the variant of cfd_four_to_phys for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Planes [plane0,
+ * plane0 + nplanes) of field `which` are written to phys, a DEVICE buffer of
+ * nplanes x Ngalz x Ngalx doubles on the handle's device, asynchronously on
+ * the handle's stream (cfd_sync before reading it); the values are those of
+ * cfd_four_to_phys for the same planes.  No allocation: the half-transformed
+ * planes and job tables live in a fixed 16 MiB per-GPU scratch, so nplanes x
+ * Ngalz x (N1/2 + 1) x 16 bytes (plus under 100 bytes per plane of job tables) must
+ * fit in it (e.g. 40 planes of a 384 x 192 grid); larger requests fail with
+ * CFD_EINVAL and a message giving the limit.  Every call on one GPU shares
+ * the scratch: calls that overlap on one GPU race. */
+int cfd_four_to_phys_planes(cfd_handle h, int which, int plane0, int nplanes, double* phys);
+
 /* The band table N(1:4, 0:4) that record_out writes (start.f90:89-111;
  * y_grid_canopy.f90:250-275): x/z mode counts per band and the band limits
  * in y, the bottom band reaching h_ny above the lower tips (0.25 for smooth
