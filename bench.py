@@ -250,6 +250,47 @@ def learned_leg(args, fine_value, fine_dt):
     return out
 
 
+def fortran_leg(params, steps, fine_value):
+    """The drop-in Fortran driver (cfd_amd/fortran/littleharsh_gpu) on the same
+    workload: input.in written in the reference's layout with the driver's
+    optional records (tanh grid, no 24pt extras, FP64, fused stepping), field
+    files of the seeded projected field (flag_init = 1), `steps` time steps;
+    the driver times its own loop steps (those without a field record or a
+    statistics sample) and prints their rate."""
+    import shutil
+    import subprocess
+    import tempfile
+    from cfd_amd import Channel, ic, inputfile
+
+    exe = os.path.join(ROOT, "cfd_amd", "fortran", "littleharsh_gpu")
+    if not os.path.exists(exe):
+        return {"skipped": "driver not built"}
+    p = inputfile.representable(params)
+    d = tempfile.mkdtemp(prefix="cfd_fortran_leg_")
+    try:
+        for sub in ("in", "out"):
+            os.makedirs(os.path.join(d, sub))
+        ch = Channel(p)
+        g = ch.ygrid()
+        f = ic.project(ch, ic.initial_fields(p, g, seed=0, amp=0.1))
+        for k in ("u1", "u2", "u3", "p"):
+            ch.set_field(k, f[k])
+        ch.set_scalars(t=0.0, dt=0.0, mpgx=0.0, QxT=ic.flowrate(f["u1"], g, p.ny))
+        fn = os.path.basename(ch.record_out(os.path.join(d, "in") + "/", 0))
+        ch.close()
+        inputfile.write(d, p, flag_init=1, fnameimb=fn, dirin="./in/", dirout="./out/", nwrite=100000)
+        out = subprocess.run([exe, str(steps)], cwd=d, capture_output=True, text=True, timeout=600)
+        if out.returncode != 0:
+            return {"error": (out.stdout + out.stderr)[-2000:]}
+        line = [ln for ln in out.stdout.splitlines() if ln.strip().startswith("loop:")][0].split()
+        v = float(line[5].replace("D", "E"))
+        return {"substeps_per_s": v, "timed_steps": int(line[1]), "seconds": float(line[3].replace("D", "E")),
+                "vs_python_driven": v / fine_value,
+                "run": "littleharsh_gpu %d from input.in (flag_init 1, fused cfd_step, FP64)" % steps}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
 def ensemble_leg(fine_value, fine_points):
     """16 uneven channels, tuned, 50 RK3 steps (the first eager, 49 by the
     captured graphs)."""
@@ -346,6 +387,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes instead of the autotuner's")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-closure leg")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 16-channel ensemble leg")
+    ap.add_argument("--no-fortran", action="store_true", help="skip the Fortran driver leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy 3 times and print its bandwidth (the reference "
                          "kernel for scaling HBM byte counters)")
@@ -397,6 +439,7 @@ def main():
 
     learned = None if args.no_learned else learned_leg(args, value, sc["dt"])
     ens = None if args.no_ensemble else ensemble_leg(value, params.ngalx * params.ny * params.ngalz)
+    fortran = None if args.no_fortran else fortran_leg(params, 20, value)
 
     cpu = cpu_baseline(args, params, f, q0) if args.cpu_substeps > 0 else None
 
@@ -424,6 +467,7 @@ def main():
         "mixed": mixed,
         "learned": learned,
         "ensemble": ens,
+        "fortran_driver": fortran,
         "cpu_baseline": cpu,
         "sim": {"t": sc["t"], "dt": sc["dt"], "mpgx": sc["mpgx"], "finite": finite},
     }

@@ -298,13 +298,20 @@ __device__ __forceinline__ RowSys seg_build(const FusedArgs& a, const FieldPtrs&
   return s;
 }
 
+// Blocks are dealt to the 8 XCDs round-robin by block id; block b takes
+// column tile 8 (slot / 3) + b % 8 and field slot % 3, slot = b / 8, so the
+// three components of one tile run on one XCD at about the same time and the
+// pressure rows they all read come from that XCD's L2 after the first.
+// grid 24 ceil(ntiles / 8), ntiles = ceil(Ncol / CB)
 template <int SEG, typename TN>
 __global__ void __launch_bounds__(256) k_rhs_seg(FusedArgs a) {
   extern __shared__ double sm[];
   const int CB = blockDim.x, NT = blockDim.x * blockDim.y;
   const int cl = threadIdx.x, sg = threadIdx.y, me = sg * CB + cl;
-  const int f = blockIdx.y;
-  const int col = blockIdx.x * CB + cl;
+  const int slot = blockIdx.x >> 3;
+  const int tile = (slot / 3) * 8 + (blockIdx.x & 7), f = slot % 3;
+  if (tile * CB >= a.Ncol) return;  // the whole block: no barrier is reached
+  const int col = tile * CB + cl;
   const int n = (f == 1) ? a.nyv : a.nyu;
   const int S = (n + SEG + 1) / (SEG + 1);
   double2* Yb = reinterpret_cast<double2*>(sm);
@@ -1158,7 +1165,8 @@ void rhs_seg_launch(Solver& s, const FusedArgs& a, int sy) {
   // separator system (15 doubles per thread), then per row 4 doubles of
   // metadata and the IB flag (nyu rows: the longer of the u and v columns)
   const size_t lds = sizeof(double) * (15 * (size_t)cb * sy + 4 * (size_t)s.nyu) + (size_t)s.nyu;
-  const dim3 grid((s.Ncol + cb - 1) / cb, 3), block(cb, sy);
+  const int ntiles = (s.Ncol + cb - 1) / cb;
+  const dim3 grid(24 * ((ntiles + 7) / 8)), block(cb, sy);
   if (s.mixed)
     hipLaunchKernelGGL((k_rhs_seg<SEG, float2>), grid, block, lds, s.stream, a);
   else

@@ -18,6 +18,20 @@
 ! reference counts the sampling interval from
 ! iter0, which it also resets when it samples; a counter of its own, iterst,
 ! does that here so iter0 keeps the start of the run for maxiter.
+!
+! After the reference's last record (dstz) input.in may carry five more
+! records in the same layout, each optional (an absent record keeps the
+! default, so the reference's own input.in reads unchanged):
+!   ygrid =   0   y map: 0 the reference polynomial (y_grid_canopy.f90), 1 tanh
+!   ybeta =  2.d0 stretching of the tanh map
+!   extr24=   1   1: the 4 fixed forcing points of the linked 24pt geometry
+!   precis=   0   0 FP64; 1 mixed (FP32 transforms, products and nonlinear
+!                 scratch; FP64 fields, solves, projection, flow rate)
+!   fused =   1   1: one cfd_step per time step (fused column sweeps);
+!                 0: the reference's call sequence, one entry point per
+!                 subroutine (littleharsh.f90:165-196)
+! The time loop's own steps (those without a field record) are timed and the
+! rate printed as " loop: <steps> steps <seconds> s <substeps/s> substeps/s".
 program littleharsh_gpu
   use iso_c_binding
   use cfd_amd_mod
@@ -26,7 +40,10 @@ program littleharsh_gpu
   type(c_ptr) :: h
   integer(c_int) :: rc, d(16), kRK, iter0, dst(6), sflags
   integer :: i, Ngalx2, Ngalz2, flag_init, nwrite, iter, iwrite, maxiter, npeakx, npeakz, dstx, dstz
-  integer :: nstat, iterst
+  integer :: nstat, iterst, fused, ios, ival, nloop
+  integer(8) :: clk0, clk1, clkrate
+  real(8) :: tloop, rval
+  character(len=6) :: key
   integer(c_int) :: N20(20)
   real(8) :: hv(11)
   character(len=4) :: e1, e2, e3
@@ -124,6 +141,39 @@ program littleharsh_gpu
   read(40,20) p%dsty1
   read(40,20) dstx
   read(40,20) dstz
+  ! optional extension records (defaults: the reference's polynomial grid,
+  ! the 24pt extras, FP64, fused stepping)
+  p%ygrid = 0
+  p%ytanh_beta = 2d0
+  p%extras24 = 1
+  p%precision = 0
+  fused = 1
+  do
+    read(40,'(A6)', iostat=ios, advance='no') key
+    if (ios > 0) exit
+    if (is_iostat_end(ios)) exit
+    backspace(40)
+    select case (key)
+    case ('ygrid ')
+      read(40,20) ival
+      p%ygrid = ival
+    case ('ybeta ')
+      read(40,10) rval
+      p%ytanh_beta = rval
+    case ('extr24')
+      read(40,20) ival
+      p%extras24 = ival
+    case ('precis')
+      read(40,20) ival
+      p%precision = ival
+    case ('fused ')
+      read(40,20) ival
+      fused = ival
+    case default
+      read(40,*, iostat=ios)
+      if (ios /= 0) exit
+    end select
+  end do
   close(40)
   if (Ngalx2 /= p%ngalx .or. Ngalz2 /= p%ngalz) then
     write(*,*) 'one spectral resolution in all bands: Ngalx1 = Ngalx2, Ngalz1 = Ngalz2'
@@ -139,10 +189,6 @@ program littleharsh_gpu
   else
     p%r = 1d0; p%c_i = 2d0; p%c_k = 2d0; p%stag = 0d0
   end if
-  p%ygrid = 0
-  p%ytanh_beta = 2d0
-  p%extras24 = 1
-  p%precision = 0
   p%device = 0
 
   if (trim(arg) == 'parse') then
@@ -150,6 +196,7 @@ program littleharsh_gpu
     write(*,'(a,i4)') ' flag_ctpress', p%flag_ctpress
     write(*,'(a,7es24.16)') ' reals', p%Re, p%alp, p%bet, p%dyq, p%posth, p%CFL, p%mpgx
     write(*,'(a,4es24.16)') ' post', p%r, p%c_i, p%c_k, p%stag
+    write(*,'(a,4i4,es24.16)') ' ext', p%ygrid, p%extras24, p%precision, fused, p%ytanh_beta
     stop
   end if
   rc = cfd_create(p, h);               call cfd_check(rc, h, 'cfd_create')
@@ -236,6 +283,9 @@ program littleharsh_gpu
     nstat = huge(1)
   end if
   rc = cfd_get_scalars(h, sc);         call cfd_check(rc, h, 'get scalars')
+  call system_clock(count_rate=clkrate)
+  nloop = 0
+  tloop = 0d0
   do while (sc(CFD_T) < maxt .and. iter - iter0 < maxiter)
     sflags = 0
     if (iter - iterst >= nstat) then
@@ -253,28 +303,45 @@ program littleharsh_gpu
       rc = cfd_stats_request(h, sflags, trim(dirout)//c_null_char, iter)
       call cfd_check(rc, h, 'stats request')
     end if
-    do kRK = 1, 3
-      rc = cfd_rhs0(h, kRK);           call cfd_check(rc, h, 'RHS0')
-      rc = cfd_nonlinear(h, kRK);      call cfd_check(rc, h, 'nonlinear')
-      rc = cfd_solve_u(h, 1, kRK);     call cfd_check(rc, h, 'solveU(u1)')
-      rc = cfd_solve_v(h, kRK);        call cfd_check(rc, h, 'solveV(u2)')
-      rc = cfd_solve_u(h, 3, kRK);     call cfd_check(rc, h, 'solveU(u3)')
-      if (p%flag_ctpress == 0) then
-        rc = cfd_meanflow_ctU(h, kRK); call cfd_check(rc, h, 'meanflow_ctU')
+    ! the previous step ended with a synchronous read of the scalars, so the
+    ! device is idle here
+    call system_clock(clk0)
+    if (fused == 1) then
+      ! three fused substeps and meanflow_ctP (flag_ctpress = 1)
+      rc = cfd_step(h, 1);             call cfd_check(rc, h, 'step')
+    else
+      do kRK = 1, 3
+        rc = cfd_rhs0(h, kRK);         call cfd_check(rc, h, 'RHS0')
+        rc = cfd_nonlinear(h, kRK);    call cfd_check(rc, h, 'nonlinear')
+        rc = cfd_solve_u(h, 1, kRK);   call cfd_check(rc, h, 'solveU(u1)')
+        rc = cfd_solve_v(h, kRK);      call cfd_check(rc, h, 'solveV(u2)')
+        rc = cfd_solve_u(h, 3, kRK);   call cfd_check(rc, h, 'solveU(u3)')
+        if (p%flag_ctpress == 0) then
+          rc = cfd_meanflow_ctU(h, kRK); call cfd_check(rc, h, 'meanflow_ctU')
+        end if
+        rc = cfd_solve_p(h, kRK);      call cfd_check(rc, h, 'solveP')
+        rc = cfd_vcorr(h, kRK);        call cfd_check(rc, h, 'v_corr')
+      end do
+      if (p%flag_ctpress == 1) then
+        rc = cfd_meanflow_ctP(h);      call cfd_check(rc, h, 'meanflow_ctP')
       end if
-      rc = cfd_solve_p(h, kRK);        call cfd_check(rc, h, 'solveP')
-      rc = cfd_vcorr(h, kRK);          call cfd_check(rc, h, 'v_corr')
-    end do
-    if (p%flag_ctpress == 1) then
-      rc = cfd_meanflow_ctP(h);        call cfd_check(rc, h, 'meanflow_ctP')
     end if
     iter = iter + 1
     rc = cfd_get_scalars(h, sc);       call cfd_check(rc, h, 'get scalars')
+    call system_clock(clk1)
+    ! steps that wrote statistics are left out of the rate
+    if (sflags == 0) then
+      nloop = nloop + 1
+      tloop = tloop + real(clk1 - clk0, 8) / real(clkrate, 8)
+    end if
     if (mod(iter, max(nwrite, 1)) == 0 .or. iter - iter0 == maxiter) then
       write(*,'(a,i8,a,es14.7,a,es12.5,a,es14.7)') ' iter', iter, '  t', sc(CFD_T), '  dt', sc(CFD_DT), &
                                                    '  mpgx', sc(CFD_MPGX)
     end if
   end do
+  if (nloop > 0 .and. tloop > 0d0) then
+    write(*,'(a,i8,a,es12.5,a,es12.5,a)') ' loop:', nloop, ' steps', tloop, ' s', 3d0 * nloop / tloop, ' substeps/s'
+  end if
   ! finalize: one more record of the final state
   rc = cfd_record_out(h, trim(dirout)//c_null_char, iter)
   call cfd_check(rc, h, 'record_out')

@@ -45,3 +45,46 @@ def test_implied_decimal(tmp_path):
     # point implies one decimal place (start.f90:434, read(40,10))
     v = parse(tmp_path, {"alp": "     128d0", "bet": "     256d0"})
     assert abs(v["reals"][1] - 12.8) < 1e-12 and abs(v["reals"][2] - 25.6) < 1e-12
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran driver not built")
+@pytest.mark.parametrize("workload", ["rt180_64x96x64", "rt395_384x192x192", "rt550_768x384x384"])
+def test_inputfile_round_trip(tmp_path, workload):
+    """cfd_amd.inputfile writes a workload's input.in and geom_circ_rough.txt
+    in the reference layout plus the optional records; the driver reads back
+    exactly the values inputfile.representable reports (every real within
+    the 10-character field, the tanh grid, no 24pt extras)."""
+    import dataclasses
+
+    from cfd_amd import inputfile, workloads
+    p, _ = workloads.get(workload)
+    p = dataclasses.replace(p, precision=1)
+    q = inputfile.write(str(tmp_path), p)
+    out = subprocess.run([EXE, "parse"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stdout + out.stderr
+    v = {}
+    for line in out.stdout.splitlines():
+        parts = line.split()
+        if parts:
+            v[parts[0]] = [float(x.replace("D", "E")) for x in parts[1:]]
+    assert v["ints"] == [q.ngalx, q.ngalz, q.ny, q.dsty, q.dsty1, q.ppp, q.geometry_type, q.ntilex, q.ntilez]
+    assert v["reals"] == [q.Re, q.alp, q.bet, q.dyq, q.posth, q.CFL, q.mpgx]
+    assert v["post"] == [q.r, q.c_i, q.c_k, q.stag]
+    assert v["ext"] == [1, 0, 1, 1, 2.0]
+    # the 10-character fields keep 8 significant digits of these values
+    for k in ("posth", "r"):
+        assert abs(getattr(q, k) - getattr(p, k)) <= 1e-7 * abs(getattr(p, k))
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran driver not built")
+def test_optional_records_default(tmp_path):
+    """Without the optional records the driver keeps the reference's choices
+    (polynomial grid, 24pt extras, FP64) and fused stepping; a record that is
+    present sets its value, in any order."""
+    v = parse(tmp_path)
+    assert v["ext"] == [0, 1, 0, 1, 2.0]
+    with open(tmp_path / "input.in", "a") as fh:
+        fh.write("fused =         0 ! per entry point\nybeta =     3.5d0 ! tanh\nygrid =         1 ! tanh\n")
+    out = subprocess.run([EXE, "parse"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    ext = [float(x.replace("D", "E")) for x in [l for l in out.stdout.splitlines() if l.split()[:1] == ["ext"]][0].split()[1:]]
+    assert ext == [1, 1, 0, 0, 3.5]
