@@ -45,6 +45,50 @@ def _same(a, b, what):
         assert sa[k] == sb[k], "%s: scalar %s %r != %r" % (what, k, sa[k], sb[k])
 
 
+def test_graph_stepped_members_against_oracle():
+    """The three smallest members of the 16-member set (one of them the
+    96 x 96 x 64 end member), one eager substep, then kRK = 2 and 3 and one
+    full RK3 step from the captured graphs, against the oracle's substeps
+    from the same state: relative L2 <= 1e-11 per field, the same t and
+    mean pressure gradient."""
+    from oracle.oracle import Oracle, set_threads
+    import os
+    set_threads(min(16, len(os.sched_getaffinity(0))))
+    ms = sorted(ensemble.members(16, seed=1),
+                key=lambda m: m.params.ngalx * m.params.ny * m.params.ngalz)[:3]
+    print("3 members, graphs vs oracle:", [(m.params.ngalx, m.params.ny, m.params.ngalz) for m in ms], flush=True)
+    A = _started(ms)
+    orcs = []
+    for m, ch in zip(ms, A):
+        o = Oracle(m.params)
+        for n in FIELDS:
+            o.set_field(n, ch.get_field(n))
+        o.set_scalars(**{k: v for k, v in ch.scalars().items()})
+        orcs.append(o)
+    for ch, o in zip(A, orcs):
+        ch.substep(1)  # the eager warm-up substep the graphs need
+        o.substep(1)
+        ch.sync()
+    with ensemble.Ensemble(A) as ens:
+        for kRK in (2, 3):
+            ens.substep(kRK)
+        ens.step(1)
+        ens.sync()
+    for o, m in zip(orcs, ms):
+        for kRK in (2, 3, 1, 2, 3):
+            o.substep(kRK)
+        assert m.params.flag_ctpress == 0
+    for i, (ch, o) in enumerate(zip(A, orcs)):
+        for n in FIELDS:
+            e = rel_l2(ch.get_field(n), o.get_field(n))
+            assert e <= 1e-11, "member %d %s rel-L2 %.3e" % (i, n, e)
+        sa, sb = ch.scalars(), o.scalars()
+        assert abs(sa["t"] - sb["t"]) <= 1e-13 * sb["t"]
+        assert abs(sa["mpgx"] - sb["mpgx"]) <= 1e-13 * abs(sb["QxT"]) / sb["dt"]
+        ch.close()
+        o.close()
+
+
 def test_graph_replays_equal_eager_substeps():
     ms = ensemble.members(16, seed=1)
     print("16 members, graph vs eager", flush=True)
