@@ -2,8 +2,9 @@
 
 ``Params`` mirrors the values ``start`` reads from ``input.in``
 (start.f90:62-227) plus the post geometry of ``geom_circ_rough.txt``
-(geom_circ_rough_mixed_forcing_24pt.f90:113-123).  One spectral resolution
-for all y-bands (Ngalx1 == Ngalx2, Ngalz1 == Ngalz2).
+(geom_circ_rough_mixed_forcing_24pt.f90:113-123).  ngalx/ngalz are the wall
+bands' resolution (Ngalx1, Ngalz1); ngalx2/ngalz2 the central band's
+(Ngalx2, Ngalz2), 0 for the same resolution in every band.
 """
 import dataclasses
 import math
@@ -27,6 +28,8 @@ class Params:
     flag_ctpress: int = 0     # 0 constant flow rate, 1 constant mpgx
     precision: int = 0
     device: int = 0
+    ngalx2: int = 0           # Ngalx2, Ngalz2 of the central y band (input.in:16, 18);
+    ngalz2: int = 0           # 0 = as the wall bands (one resolution)
     Re: float = 2800.0
     alp: float = 1.0          # Lx = 2 pi / alp
     bet: float = 2.0          # Lz = 2 pi / bet
@@ -68,6 +71,30 @@ class Params:
     @property
     def Ncol(self):
         return self.Mx * self.N2
+
+    @property
+    def multiband(self):
+        """True when the central band is transformed on a coarser grid."""
+        return ((self.ngalx2 or self.ngalx) != self.ngalx) or ((self.ngalz2 or self.ngalz) != self.ngalz)
+
+    @property
+    def N1c(self):
+        return 2 * ((self.ngalx2 or self.ngalx) // 3)
+
+    @property
+    def N2c(self):
+        return 2 * ((self.ngalz2 or self.ngalz) // 3)
+
+    def short_columns(self):
+        """Boolean per column (column = k Mx + i, the wall bands' numbering):
+        True for the columns outside the central band's N1c x N2c modes,
+        which exist in the wall bands only (proc_lims_columns,
+        start.f90:1424-1501)."""
+        import numpy as np
+        i = np.arange(self.Ncol) % self.Mx
+        k = np.arange(self.Ncol) // self.Mx
+        kin = (k < self.N2c // 2) | (k >= self.N2 - self.N2c // 2)
+        return ~((i <= self.N1c // 2) & kin)
 
     @property
     def nyu(self):

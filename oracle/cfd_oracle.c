@@ -185,6 +185,14 @@ struct or_state {
   int cnp;
   int* clim;             /* planelim(grid, 1:2, 0:np-1), u then v: [(g*2+e)*np + r] */
   cplx *xz1, *xz3, *xz2; /* x/z derivative sums of every plane */
+  /* multi-band resolution (start.f90:89-111, 1420-1557): mb = 1 when the
+     central band is transformed on the coarser Ngal(1:2,2) grid */
+  int mb, ngx2, ngz2, N1c, N2c;
+  int N3[4], N4[4];         /* band limits N(3,0:3), N(4,0:3) (y_grid_canopy.f90:250-275) */
+  unsigned char* shortc;    /* per column: 1 = a short column (wall bands only) */
+  double gw12, gw31, gwi12, gwi31; /* gridweighting(_interp) at the band interfaces */
+  fplan pz2, px22;          /* central-band transforms */
+  cplx* twx2;
 };
 
 static int g_threads = 0;
@@ -201,6 +209,12 @@ void or_set_threads(int n) { g_threads = n; }
 static inline size_t fu(const or_state* s, int col, int j) { return (size_t)col * s->nyu + (j - s->jlo_u); }
 static inline size_t fv(const or_state* s, int col, int j) { return (size_t)col * s->nyv + (j - s->jlo_v); }
 static inline size_t fp(const or_state* s, int col, int j) { return (size_t)col * s->nyp + (j - s->jlo_p); }
+/* band segments of a column and the gridweighting at their ends (below) */
+static int col_segs(const or_state* s, int col, int g, int lo[2], int hi[2]);
+static double gw_lo(const or_state* s, int lo);
+static double gw_hi(const or_state* s, int hi);
+static double gwi_lo(const or_state* s, int lo);
+static double gwi_hi(const or_state* s, int hi);
 
 /* x**e for an integer exponent e >= 0, by binary exponentiation (the
    usual lowering of Fortran real**integer) */
@@ -605,6 +619,7 @@ static int build_ib(or_state* s, char* err, int errlen) {
 /* create / destroy                                                          */
 /* ------------------------------------------------------------------------ */
 static void lu_build_p(or_state* s);
+static int central_plane(const or_state* s, int g, int j);
 
 or_state* or_create(const or_params* p, char* err, int errlen) {
   char dummy[8];
@@ -663,8 +678,66 @@ or_state* or_create(const or_params* p, char* err, int errlen) {
     long double ang = -2.0L * 3.14159265358979323846264338327950288L * k / (long double)s->ngx;
     s->twx[k] = (double)cosl(ang) + I * (double)sinl(ang);
   }
+  /* band table y limits (y_grid_canopy.f90:250-275), the bottom band h_ny
+     above the lower tips (start.f90:199-226: 0.25 for smooth walls, two post
+     spacings for canopies) */
+  {
+    const double h = p->geometry_type == 3 ? 2.0 * (2.0 * 3.14159265358979323846 / p->bet) / p->ntilez : 0.25;
+    const double target = YU(s, 0) + h;
+    int j = -p->dsty;
+    while (j < s->jhi_u && YU(s, j) < target) ++j;
+    s->N3[0] = -p->dsty; s->N3[1] = j; s->N3[2] = (p->ny - 2) - j; s->N3[3] = p->ny - 2 + p->dsty;
+    s->N4[0] = -p->dsty; s->N4[1] = j + 1; s->N4[2] = s->N3[2]; s->N4[3] = s->N3[3] + 1;
+  }
+  s->ngx2 = p->ngalx2 > 0 ? p->ngalx2 : s->ngx;
+  s->ngz2 = p->ngalz2 > 0 ? p->ngalz2 : s->ngz;
+  s->N1c = 2 * (s->ngx2 / 3); s->N2c = 2 * (s->ngz2 / 3);
+  s->mb = (s->ngx2 != s->ngx || s->ngz2 != s->ngz);
+  s->shortc = (unsigned char*)calloc(s->Ncol, 1);
+  if (s->mb) {
+    if (s->ngx2 > s->ngx || s->ngz2 > s->ngz || s->ngx2 < 4 || s->ngz2 < 4 || s->ngx2 % 2 || s->ngz2 % 2) {
+      snprintf(err, errlen, "central band resolution must be even, at least 4 and at most the wall bands'");
+      or_destroy(s); return NULL;
+    }
+    /* each band needs its own rows: the short columns' interface ghost rows
+       N(4,1)+1 and N(4,2) must differ */
+    if (!(s->N3[0] < s->N3[1] && s->N3[1] + 2 < s->N3[2] && s->N3[2] < s->N3[3])) {
+      snprintf(err, errlen, "band limits leave no central band");
+      or_destroy(s); return NULL;
+    }
+    for (int g = 0; g < 2; ++g)
+      for (int w = 0; w < 2; ++w)
+        for (int q = 0; q < s->nib[g][w]; ++q) {
+          const int j = s->ibl[g][w][9 * q + 2];
+          if (central_plane(s, g, j)) { snprintf(err, errlen, "forcing planes in the central band"); or_destroy(s); return NULL; }
+        }
+    /* long columns: i <= N(1,2)/2 and kz in the central band's N(2,2) modes
+       (proc_lims_columns, start.f90:1424-1501) */
+    for (int col = 0; col < s->Ncol; ++col) {
+      const int i = col % s->Mx, k = col / s->Mx;
+      const int kin = k < s->N2c / 2 || k >= s->N2 - s->N2c / 2;
+      s->shortc[col] = !(i <= s->N1c / 2 && kin);
+    }
+    s->gw12 = (YU(s, s->N4[1] + 1) - YV(s, s->N3[1] + 1)) / (YV(s, s->N3[1] + 1) - YU(s, s->N4[1]));
+    s->gw31 = -(YV(s, s->N3[2]) - YU(s, s->N4[2])) / (YV(s, s->N3[2]) - YU(s, s->N4[2] + 1));
+    s->gwi12 = (YU(s, s->N4[1] + 1) - YU(s, s->N4[1])) / (YV(s, s->N3[1] + 1) - YU(s, s->N4[1]));
+    s->gwi31 = (YU(s, s->N4[2]) - YU(s, s->N4[2] + 1)) / (YV(s, s->N3[2]) - YU(s, s->N4[2] + 1));
+    fplan_init(&s->pz2, s->ngz2);
+    fplan_init(&s->px22, s->ngx2 / 2);
+    s->twx2 = malloc(sizeof(cplx) * (s->ngx2 / 2 + 1));
+    for (int k = 0; k <= s->ngx2 / 2; ++k) {
+      long double ang = -2.0L * 3.14159265358979323846264338327950288L * k / (long double)s->ngx2;
+      s->twx2[k] = (double)cosl(ang) + I * (double)sinl(ang);
+    }
+  }
   lu_build_p(s);
   return s;
+}
+
+void or_band_info(const or_state* s, int* b, unsigned char* shortc) {
+  b[0] = s->mb; b[1] = s->N1c; b[2] = s->N2c;
+  for (int q = 0; q < 4; ++q) { b[3 + q] = s->N3[q]; b[7 + q] = s->N4[q]; }
+  if (shortc) memcpy(shortc, s->shortc, s->Ncol);
 }
 
 static void or_stats_free(or_state* s);
@@ -682,6 +755,10 @@ void or_destroy(or_state* s) {
   free(s->twx);
   or_stats_free(s);
   free(s->clim); free(s->xz1); free(s->xz3); free(s->xz2);
+  free(s->shortc);
+  if (s->pz2.tw) fplan_free(&s->pz2);
+  if (s->px22.tw) fplan_free(&s->px22);
+  free(s->twx2);
   free(s);
 }
 
@@ -757,11 +834,14 @@ static void rhs0_u(or_state* s, cplx* du, const cplx* u, const cplx* Nu, int com
     const double k2x = s->k2x[i], k2z = s->k2z[k];
     const double kk = (comp == 1) ? s->k1x[i] : s->k1z[k];
     const double c3 = -s->dRK[kRK - 1] * kk; /* C3 = -dRK*k1F (imaginary) */
-    for (int j = s->jlo_u + 1; j <= s->jhi_u - 1; ++j) {
-      const cplx Lu = DYU2(s, 1, j) * u[fu(s, col, j - 1)] + (k2x + k2z + DYU2(s, 2, j)) * u[fu(s, col, j)] +
-                      DYU2(s, 3, j) * u[fu(s, col, j + 1)];
-      du[fu(s, col, j)] = C1 * Lu + C2 * Nu[fu(s, col, j)] + imul(c3, PP[fp(s, col, j)]);
-    }
+    int lo[2], hi[2];
+    const int ns = col_segs(s, col, 0, lo, hi);
+    for (int g = 0; g < ns; ++g)
+      for (int j = lo[g] + 1; j <= hi[g] - 1; ++j) {
+        const cplx Lu = DYU2(s, 1, j) * u[fu(s, col, j - 1)] + (k2x + k2z + DYU2(s, 2, j)) * u[fu(s, col, j)] +
+                        DYU2(s, 3, j) * u[fu(s, col, j + 1)];
+        du[fu(s, col, j)] = C1 * Lu + C2 * Nu[fu(s, col, j)] + imul(c3, PP[fp(s, col, j)]);
+      }
   }
   if (comp == 1) {
     const double C = -s->dRK[kRK - 1] * s->mpgx;
@@ -776,12 +856,15 @@ static void rhs0_v(or_state* s, int kRK) {
   for (int col = 0; col < s->Ncol; ++col) {
     const int i = col % s->Mx, k = col / s->Mx;
     const double k2x = s->k2x[i], k2z = s->k2z[k];
-    for (int j = s->jlo_v + 1; j <= s->jhi_v - 1; ++j) {
-      const double C3 = -s->dRK[kRK - 1] * s->ddthetavi * DTHV(s, j);
-      const cplx Lv = DYV2(s, 1, j) * U2[fv(s, col, j - 1)] + (k2x + k2z + DYV2(s, 2, j)) * U2[fv(s, col, j)] +
-                      DYV2(s, 3, j) * U2[fv(s, col, j + 1)];
-      DU2[fv(s, col, j)] = C1 * Lv + C2 * NU2[fv(s, col, j)] + C3 * (PP[fp(s, col, j + 1)] - PP[fp(s, col, j)]);
-    }
+    int lo[2], hi[2];
+    const int ns = col_segs(s, col, 1, lo, hi);
+    for (int g = 0; g < ns; ++g)
+      for (int j = lo[g] + 1; j <= hi[g] - 1; ++j) {
+        const double C3 = -s->dRK[kRK - 1] * s->ddthetavi * DTHV(s, j);
+        const cplx Lv = DYV2(s, 1, j) * U2[fv(s, col, j - 1)] + (k2x + k2z + DYV2(s, 2, j)) * U2[fv(s, col, j)] +
+                        DYV2(s, 3, j) * U2[fv(s, col, j + 1)];
+        DU2[fv(s, col, j)] = C1 * Lv + C2 * NU2[fv(s, col, j)] + C3 * (PP[fp(s, col, j + 1)] - PP[fp(s, col, j)]);
+      }
   }
 }
 
@@ -796,58 +879,133 @@ void or_rhs0(or_state* s, int kRK) {
 /* ------------------------------------------------------------------------ */
 typedef struct { cplx *w0, *w1; } scratch;
 
+/* the resolution a plane is transformed at: the wall bands' (fine) or, with
+   multi-band resolution, the central band's */
+typedef struct {
+  int ngx, ngz, N1, N2, coarse;
+  const fplan *pz, *px2;
+  const cplx* twx;
+} pview;
+static pview view_of(const or_state* s, int coarse) {
+  pview v;
+  if (coarse) { v.ngx = s->ngx2; v.ngz = s->ngz2; v.N1 = s->N1c; v.N2 = s->N2c; v.pz = &s->pz2; v.px2 = &s->px22; v.twx = s->twx2; }
+  else { v.ngx = s->ngx; v.ngz = s->ngz; v.N1 = s->N1; v.N2 = s->N2; v.pz = &s->pz; v.px2 = &s->px2; v.twx = s->twx; }
+  v.coarse = coarse;
+  return v;
+}
+/* plane j of the u (g 0) or v (g 1) grid lies in the central band
+   (planes N(4,1)+1..N(4,2), N(3,1)+1..N(3,2): the plane ranges of the ranks
+   of band 2 in proc_lims_planes, start.f90:1650-1951) */
+static int central_plane(const or_state* s, int g, int j) {
+  if (!s->mb) return 0;
+  return g == 0 ? (j > s->N4[1] && j <= s->N4[2]) : (j > s->N3[1] && j <= s->N3[2]);
+}
+/* slot of reference mode k (fine column numbering, 0-based) in the view's
+   z transform (modes_to_planes with dk, start.f90:1561-1573) */
+static inline int vslot(const or_state* s, const pview* v, int k) { return (k < s->N2 / 2) ? k : k - s->N2 + v->ngz; }
+
 /* modes_to_planes: column values of plane j into xPL (zero elsewhere)
-   (FOU3D.f90:825-842; k slot = k - dk, start.f90:1561-1573) */
-static void to_plane(const or_state* s, const cplx* f, int ny_g, int jlo_g, int j, double* P) {
-  const int igal = s->ngx + 2;
-  memset(P, 0, sizeof(double) * igal * s->ngz);
+   (FOU3D.f90:825-842; k slot = k - dk, start.f90:1561-1573); a central
+   plane takes the long columns only */
+static void to_plane_v(const or_state* s, const pview* v, const cplx* f, int ny_g, int jlo_g, int j, double* P) {
+  const int igal = v->ngx + 2;
+  memset(P, 0, sizeof(double) * igal * v->ngz);
   for (int col = 0; col < s->Ncol; ++col) {
+    if (v->coarse && s->shortc[col]) continue;
     const int i = col % s->Mx, k = col / s->Mx; /* k 0-based */
-    const int slot = (k < s->N2 / 2) ? k : k - s->N2 + s->ngz;
-    const cplx v = f[(size_t)col * ny_g + (j - jlo_g)];
-    P[slot * igal + 2 * i] = creal(v);
-    P[slot * igal + 2 * i + 1] = cimag(v);
+    const int slot = vslot(s, v, k);
+    const cplx val = f[(size_t)col * ny_g + (j - jlo_g)];
+    P[slot * igal + 2 * i] = creal(val);
+    P[slot * igal + 2 * i + 1] = cimag(val);
   }
 }
 static inline int slot_of(const or_state* s, int k) { return (k < s->N2 / 2) ? k : k - s->N2 + s->ngz; }
 
 /* four_to_phys_u (zero_nyq=1) / four_to_phys_du (FOU3D.f90:546-592) */
-static void four_to_phys(const or_state* s, double* P, int zero_nyq, scratch* w) {
-  const int igal = s->ngx + 2;
-  if (zero_nyq) memset(P + (s->ngz / 2) * igal, 0, sizeof(double) * igal);
-  for (int i = 0; i < (s->N1 + 2) / 2; ++i) cft_line(&s->pz, P + 2 * i, igal, +1, w->w0, w->w1);
-  for (int k = 0; k < s->ngz; ++k) rft_line(&s->px2, s->twx, P + k * igal, +1, w->w0, w->w1);
+static void four_to_phys_v(const pview* v, double* P, int zero_nyq, scratch* w) {
+  const int igal = v->ngx + 2;
+  if (zero_nyq) memset(P + (v->ngz / 2) * igal, 0, sizeof(double) * igal);
+  for (int i = 0; i < (v->N1 + 2) / 2; ++i) cft_line(v->pz, P + 2 * i, igal, +1, w->w0, w->w1);
+  for (int k = 0; k < v->ngz; ++k) rft_line(v->px2, v->twx, P + k * igal, +1, w->w0, w->w1);
 }
 /* phys_to_four_du (FOU3D.f90:1619-1639) */
-static void phys_to_four(const or_state* s, double* P, scratch* w) {
-  const int igal = s->ngx + 2;
-  for (int k = 0; k < s->ngz; ++k) rft_line(&s->px2, s->twx, P + k * igal, -1, w->w0, w->w1);
-  for (int i = 0; i < (s->N1 + 2) / 2; ++i) cft_line(&s->pz, P + 2 * i, igal, -1, w->w0, w->w1);
+static void phys_to_four_v(const pview* v, double* P, scratch* w) {
+  const int igal = v->ngx + 2;
+  for (int k = 0; k < v->ngz; ++k) rft_line(v->px2, v->twx, P + k * igal, -1, w->w0, w->w1);
+  for (int i = 0; i < (v->N1 + 2) / 2; ++i) cft_line(v->pz, P + 2 * i, igal, -1, w->w0, w->w1);
 }
-
-static inline cplx pl(const or_state* s, const double* P, int i, int slot) {
-  const int igal = s->ngx + 2;
+static inline cplx pl_v(const pview* v, const double* P, int i, int slot) {
+  const int igal = v->ngx + 2;
   return P[slot * igal + 2 * i] + I * P[slot * igal + 2 * i + 1];
 }
+/* the same at the wall bands' resolution */
+static void to_plane(const or_state* s, const cplx* f, int ny_g, int jlo_g, int j, double* P) {
+  const pview v = view_of(s, 0);
+  to_plane_v(s, &v, f, ny_g, jlo_g, j, P);
+}
+static void four_to_phys(const or_state* s, double* P, int zero_nyq, scratch* w) {
+  const pview v = view_of(s, 0);
+  four_to_phys_v(&v, P, zero_nyq, w);
+}
+static void phys_to_four(const or_state* s, double* P, scratch* w) {
+  const pview v = view_of(s, 0);
+  phys_to_four_v(&v, P, w);
+}
+static inline cplx pl(const or_state* s, const double* P, int i, int slot) {
+  const pview v = view_of(s, 0);
+  return pl_v(&v, P, i, slot);
+}
+
+/* rows of column col on grid g (0 u, 1 v, 2 p), inclusive, boundary rows
+   included: one segment for a long column, two for a short one, the wall
+   bands' jlim (start.f90:1536-1557); returns the segment count */
+static int col_segs(const or_state* s, int col, int g, int lo[2], int hi[2]) {
+  const int jl = g == 0 ? s->jlo_u : (g == 1 ? s->jlo_v : s->jlo_p);
+  const int jh = g == 0 ? s->jhi_u : (g == 1 ? s->jhi_v : s->jhi_p);
+  lo[0] = jl;
+  if (!s->mb || !s->shortc[col]) {
+    hi[0] = jh;
+    return 1;
+  }
+  hi[0] = g == 0 ? s->N4[1] + 1 : (g == 1 ? s->N3[1] + 1 : s->N4[1]);
+  lo[1] = g == 0 ? s->N4[2] : (g == 1 ? s->N3[2] : s->N4[2] + 1);
+  hi[1] = jh;
+  return 2;
+}
+/* gridweighting of the u-grid boundary row at the start (lo) or end of a
+   segment: the wall's, or the band interface's (start.f90:494-500) */
+static double gw_lo(const or_state* s, int lo) { return lo == s->jlo_u ? s->gw[0] : s->gw31; }
+static double gw_hi(const or_state* s, int hi) { return hi == s->jhi_u ? s->gw[1] : s->gw12; }
+static double gwi_lo(const or_state* s, int lo) { return lo == s->jlo_u ? s->gwi[0] : s->gwi31; }
+static double gwi_hi(const or_state* s, int hi) { return hi == s->jhi_u ? s->gwi[1] : s->gwi12; }
 
 /* interp_u (FOU3D.f90:261-281): u/w centres -> v faces, all faces */
 static void interp_u(or_state* s, cplx* ui, const cplx* u) {
 #pragma omp parallel for schedule(static)
-  for (int col = 0; col < s->Ncol; ++col)
-    for (int j = s->jlo_v; j <= s->jhi_v; ++j)
-      ui[fv(s, col, j)] = ((YV(s, j) - YU(s, j)) * u[fu(s, col, j + 1)] + (YU(s, j + 1) - YV(s, j)) * u[fu(s, col, j)]) /
-                          (YU(s, j + 1) - YU(s, j));
+  for (int col = 0; col < s->Ncol; ++col) {
+    int lo[2], hi[2];
+    const int ns = col_segs(s, col, 1, lo, hi);
+    for (int g = 0; g < ns; ++g)
+      for (int j = lo[g]; j <= hi[g]; ++j)
+        ui[fv(s, col, j)] = ((YV(s, j) - YU(s, j)) * u[fu(s, col, j + 1)] + (YU(s, j + 1) - YV(s, j)) * u[fu(s, col, j)]) /
+                            (YU(s, j + 1) - YU(s, j));
+  }
 }
 /* interp_v (FOU3D.f90:283-308): v faces -> centres, ghosts extrapolated */
 static void interp_v(or_state* s, cplx* ui, const cplx* v) {
 #pragma omp parallel for schedule(static)
   for (int col = 0; col < s->Ncol; ++col) {
-    for (int j = s->jlo_u + 1; j <= s->jhi_u - 1; ++j)
-      ui[fu(s, col, j)] = ((YU(s, j) - YV(s, j - 1)) * v[fv(s, col, j)] + (YV(s, j) - YU(s, j)) * v[fv(s, col, j - 1)]) /
-                          (YV(s, j) - YV(s, j - 1));
-    const cplx a = ui[fu(s, col, s->jlo_u + 1)], b = ui[fu(s, col, s->jhi_u - 1)];
-    ui[fu(s, col, s->jlo_u)] = s->gwi[0] * (v[fv(s, col, s->jlo_v)] - a) + a;
-    ui[fu(s, col, s->jhi_u)] = s->gwi[1] * (v[fv(s, col, s->jhi_v)] - b) + b;
+    int lo[2], hi[2], vlo[2], vhi[2];
+    const int ns = col_segs(s, col, 0, lo, hi);
+    col_segs(s, col, 1, vlo, vhi);
+    for (int g = 0; g < ns; ++g) {
+      for (int j = lo[g] + 1; j <= hi[g] - 1; ++j)
+        ui[fu(s, col, j)] = ((YU(s, j) - YV(s, j - 1)) * v[fv(s, col, j)] + (YV(s, j) - YU(s, j)) * v[fv(s, col, j - 1)]) /
+                            (YV(s, j) - YV(s, j - 1));
+      const cplx a = ui[fu(s, col, lo[g] + 1)], b = ui[fu(s, col, hi[g] - 1)];
+      ui[fu(s, col, lo[g])] = gwi_lo(s, lo[g]) * (v[fv(s, col, vlo[g])] - a) + a;
+      ui[fu(s, col, hi[g])] = gwi_hi(s, hi[g]) * (v[fv(s, col, vhi[g])] - b) + b;
+    }
   }
 }
 
@@ -948,6 +1106,7 @@ int or_set_compat_lastplane(or_state* s, const int* N20, int np) {
   s->clim = NULL; s->xz1 = s->xz3 = s->xz2 = NULL;
   s->cnp = 0;
   if (np == 0) return 0;
+  if (s->mb) return 4; /* the partition here is for one resolution in every band */
   int* lim = (int*)malloc(sizeof(int) * 4 * (size_t)np);
   if (or_plane_partition(N20, s->ngx, s->ngz, np, lim)) { free(lim); return 1; }
   for (int g = 0; g < 2; ++g) {
@@ -970,26 +1129,31 @@ void or_nonlinear(or_state* s, int kRK) {
   interp_u(s, s->u1i, U1);
   interp_v(s, s->u2i, U2);
   interp_u(s, s->u3i, U3);
-  const int igal = s->ngx + 2, ngz = s->ngz, ngx = s->ngx;
-  const size_t psz = (size_t)igal * ngz;
-  double m1 = 0.0, m3 = 0.0, m2 = 1e-6;
+  const size_t psz = (size_t)(s->ngx + 2) * s->ngz;
+  /* CFL maxima per resolution: [0] the wall bands, [1] the central band */
+  double m1f = 0.0, m3f = 0.0, m1c = 0.0, m3c = 0.0, m2 = 1e-6;
   const int nU = s->nyu, nV = s->nyv;
   /* u planes: uu, uw, vv, ww */
-#pragma omp parallel reduction(max : m1, m3)
+#pragma omp parallel reduction(max : m1f, m3f, m1c, m3c)
   {
     double* B = (double*)malloc(sizeof(double) * psz * 7);
-    scratch w = {malloc(sizeof(cplx) * (ngx + ngz + 8)), malloc(sizeof(cplx) * (ngx + ngz + 8))};
+    scratch w = {malloc(sizeof(cplx) * (s->ngx + s->ngz + 8)), malloc(sizeof(cplx) * (s->ngx + s->ngz + 8))};
 #pragma omp for schedule(dynamic, 1)
     for (int jj = 0; jj < nU; ++jj) {
       const int j = s->jlo_u + jj;
-      double *Pu = B, *Pv = B + psz, *Pw = B + 2 * psz, *Quu = B + 3 * psz, *Quw = B + 4 * psz, *Qvv = B + 5 * psz, *Qww = B + 6 * psz;
-      to_plane(s, U1, nU, s->jlo_u, j, Pu);
-      to_plane(s, s->u2i, nU, s->jlo_u, j, Pv);
-      to_plane(s, U3, nU, s->jlo_u, j, Pw);
-      four_to_phys(s, Pu, 1, &w);
-      four_to_phys(s, Pv, 1, &w);
-      four_to_phys(s, Pw, 1, &w);
-      memset(Quu, 0, sizeof(double) * psz * 4);
+      const int cen = central_plane(s, 0, j);
+      const pview v = view_of(s, cen);
+      const int igal = v.ngx + 2, ngz = v.ngz, ngx = v.ngx;
+      const size_t vsz = (size_t)igal * ngz;
+      double *Pu = B, *Pv = B + vsz, *Pw = B + 2 * vsz, *Quu = B + 3 * vsz, *Quw = B + 4 * vsz, *Qvv = B + 5 * vsz, *Qww = B + 6 * vsz;
+      to_plane_v(s, &v, U1, nU, s->jlo_u, j, Pu);
+      to_plane_v(s, &v, s->u2i, nU, s->jlo_u, j, Pv);
+      to_plane_v(s, &v, U3, nU, s->jlo_u, j, Pw);
+      four_to_phys_v(&v, Pu, 1, &w);
+      four_to_phys_v(&v, Pv, 1, &w);
+      four_to_phys_v(&v, Pw, 1, &w);
+      memset(Quu, 0, sizeof(double) * vsz * 4);
+      double m1 = 0.0, m3 = 0.0;
       for (int k = 0; k < ngz; ++k)
         for (int i = 0; i < ngx; ++i) {
           const size_t o = (size_t)k * igal + i;
@@ -1000,24 +1164,30 @@ void or_nonlinear(or_state* s, int kRK) {
           m1 = dmax(m1, fabs(Pu[o]));
           m3 = dmax(m3, fabs(Pw[o]));
         }
-      phys_to_four(s, Quu, &w);
-      phys_to_four(s, Quw, &w);
-      phys_to_four(s, Qvv, &w);
-      phys_to_four(s, Qww, &w);
+      if (cen) { m1c = dmax(m1c, m1); m3c = dmax(m3c, m3); }
+      else { m1f = dmax(m1f, m1); m3f = dmax(m3f, m3); }
+      phys_to_four_v(&v, Quu, &w);
+      phys_to_four_v(&v, Quw, &w);
+      phys_to_four_v(&v, Qvv, &w);
+      phys_to_four_v(&v, Qww, &w);
       const int interior = (j > s->jlo_u && j < s->jhi_u);
       for (int col = 0; col < s->Ncol; ++col) {
-        const int i = col % s->Mx, k = col / s->Mx, sl = slot_of(s, k);
-        s->vvc[fu(s, col, j)] = pl(s, Qvv, i, sl);
+        if (cen && s->shortc[col]) { /* no short column on a central plane: its flux rows there stay zero */
+          s->vvc[fu(s, col, j)] = 0.0;
+          continue;
+        }
+        const int i = col % s->Mx, k = col / s->Mx, sl = vslot(s, &v, k);
+        s->vvc[fu(s, col, j)] = pl_v(&v, Qvv, i, sl);
         if (s->cnp) { /* every plane's sums, walls included; assigned per rank below */
-          const int zero = (sl == ngz - s->N2 / 2);
-          s->xz1[fu(s, col, j)] = zero ? 0.0 : imul(s->k1x[i], pl(s, Quu, i, sl)) + imul(s->k1z[k], pl(s, Quw, i, sl));
-          s->xz3[fu(s, col, j)] = zero ? 0.0 : imul(s->k1x[i], pl(s, Quw, i, sl)) + imul(s->k1z[k], pl(s, Qww, i, sl));
+          const int zero = (sl == ngz - v.N2 / 2);
+          s->xz1[fu(s, col, j)] = zero ? 0.0 : imul(s->k1x[i], pl_v(&v, Quu, i, sl)) + imul(s->k1z[k], pl_v(&v, Quw, i, sl));
+          s->xz3[fu(s, col, j)] = zero ? 0.0 : imul(s->k1x[i], pl_v(&v, Quw, i, sl)) + imul(s->k1z[k], pl_v(&v, Qww, i, sl));
         } else if (interior) {
-          const int zero = (sl == ngz - s->N2 / 2); /* der_x/der_z zero band (FOU3D.f90:329-333) */
+          const int zero = (sl == ngz - v.N2 / 2); /* der_x/der_z zero band (FOU3D.f90:329-333) */
           cplx a = 0.0, b = 0.0;
           if (!zero) {
-            a = imul(s->k1x[i], pl(s, Quu, i, sl)) + imul(s->k1z[k], pl(s, Quw, i, sl));
-            b = imul(s->k1x[i], pl(s, Quw, i, sl)) + imul(s->k1z[k], pl(s, Qww, i, sl));
+            a = imul(s->k1x[i], pl_v(&v, Quu, i, sl)) + imul(s->k1z[k], pl_v(&v, Quw, i, sl));
+            b = imul(s->k1x[i], pl_v(&v, Quw, i, sl)) + imul(s->k1z[k], pl_v(&v, Qww, i, sl));
           }
           NU1[fu(s, col, j)] = a;
           NU3[fu(s, col, j)] = b;
@@ -1030,18 +1200,22 @@ void or_nonlinear(or_state* s, int kRK) {
 #pragma omp parallel reduction(max : m2)
   {
     double* B = (double*)malloc(sizeof(double) * psz * 5);
-    scratch w = {malloc(sizeof(cplx) * (ngx + ngz + 8)), malloc(sizeof(cplx) * (ngx + ngz + 8))};
+    scratch w = {malloc(sizeof(cplx) * (s->ngx + s->ngz + 8)), malloc(sizeof(cplx) * (s->ngx + s->ngz + 8))};
 #pragma omp for schedule(dynamic, 1)
     for (int jj = 0; jj < nV; ++jj) {
       const int j = s->jlo_v + jj;
-      double *Pu = B, *Pv = B + psz, *Pw = B + 2 * psz, *Quv = B + 3 * psz, *Qvw = B + 4 * psz;
-      to_plane(s, s->u1i, nV, s->jlo_v, j, Pu);
-      to_plane(s, U2, nV, s->jlo_v, j, Pv);
-      to_plane(s, s->u3i, nV, s->jlo_v, j, Pw);
-      four_to_phys(s, Pu, 1, &w);
-      four_to_phys(s, Pv, 1, &w);
-      four_to_phys(s, Pw, 1, &w);
-      memset(Quv, 0, sizeof(double) * psz * 2);
+      const int cen = central_plane(s, 1, j);
+      const pview v = view_of(s, cen);
+      const int igal = v.ngx + 2, ngz = v.ngz, ngx = v.ngx;
+      const size_t vsz = (size_t)igal * ngz;
+      double *Pu = B, *Pv = B + vsz, *Pw = B + 2 * vsz, *Quv = B + 3 * vsz, *Qvw = B + 4 * vsz;
+      to_plane_v(s, &v, s->u1i, nV, s->jlo_v, j, Pu);
+      to_plane_v(s, &v, U2, nV, s->jlo_v, j, Pv);
+      to_plane_v(s, &v, s->u3i, nV, s->jlo_v, j, Pw);
+      four_to_phys_v(&v, Pu, 1, &w);
+      four_to_phys_v(&v, Pv, 1, &w);
+      four_to_phys_v(&v, Pw, 1, &w);
+      memset(Quv, 0, sizeof(double) * vsz * 2);
       const int interior = (j > s->jlo_v && j < s->jhi_v);
       for (int k = 0; k < ngz; ++k)
         for (int i = 0; i < ngx; ++i) {
@@ -1050,41 +1224,58 @@ void or_nonlinear(or_state* s, int kRK) {
           Qvw[o] = Pw[o] * Pv[o];
           if (interior) m2 = dmax(m2, fabs(Pv[o]) * DTHV(s, j)); /* dtc_calc (FOU3D.f90:530-536) */
         }
-      phys_to_four(s, Quv, &w);
-      phys_to_four(s, Qvw, &w);
+      phys_to_four_v(&v, Quv, &w);
+      phys_to_four_v(&v, Qvw, &w);
       for (int col = 0; col < s->Ncol; ++col) {
-        const int i = col % s->Mx, k = col / s->Mx, sl = slot_of(s, k);
-        s->uvf[fv(s, col, j)] = pl(s, Quv, i, sl);
-        s->vwf[fv(s, col, j)] = pl(s, Qvw, i, sl);
+        if (cen && s->shortc[col]) {
+          s->uvf[fv(s, col, j)] = 0.0;
+          s->vwf[fv(s, col, j)] = 0.0;
+          continue;
+        }
+        const int i = col % s->Mx, k = col / s->Mx, sl = vslot(s, &v, k);
+        s->uvf[fv(s, col, j)] = pl_v(&v, Quv, i, sl);
+        s->vwf[fv(s, col, j)] = pl_v(&v, Qvw, i, sl);
         if (s->cnp) {
-          const int zero = (sl == ngz - s->N2 / 2);
-          s->xz2[fv(s, col, j)] = zero ? 0.0 : imul(s->k1x[i], pl(s, Quv, i, sl)) + imul(s->k1z[k], pl(s, Qvw, i, sl));
+          const int zero = (sl == ngz - v.N2 / 2);
+          s->xz2[fv(s, col, j)] = zero ? 0.0 : imul(s->k1x[i], pl_v(&v, Quv, i, sl)) + imul(s->k1z[k], pl_v(&v, Qvw, i, sl));
         } else if (interior) {
-          const int zero = (sl == ngz - s->N2 / 2);
-          NU2[fv(s, col, j)] = zero ? 0.0 : imul(s->k1x[i], pl(s, Quv, i, sl)) + imul(s->k1z[k], pl(s, Qvw, i, sl));
+          const int zero = (sl == ngz - v.N2 / 2);
+          NU2[fv(s, col, j)] = zero ? 0.0 : imul(s->k1x[i], pl_v(&v, Quv, i, sl)) + imul(s->k1z[k], pl_v(&v, Qvw, i, sl));
         }
       }
     }
     free(B); free(w.w0); free(w.w1);
   }
   if (s->cnp) lastplane_assign(s);
-  /* y derivatives and assembly (FOU3D.f90:158-173, 453-493) */
+  /* y derivatives and assembly (FOU3D.f90:158-173, 453-493), per band */
 #pragma omp parallel for schedule(static)
   for (int col = 0; col < s->Ncol; ++col) {
-    for (int j = s->jlo_u + 1; j <= s->jhi_u - 1; ++j) {
-      NU1[fu(s, col, j)] += (s->uvf[fv(s, col, j)] - s->uvf[fv(s, col, j - 1)]) * DTHU(s, j) * s->ddthetavi;
-      NU3[fu(s, col, j)] += (s->vwf[fv(s, col, j)] - s->vwf[fv(s, col, j - 1)]) * DTHU(s, j) * s->ddthetavi;
-    }
-    for (int j = s->jlo_v + 1; j <= s->jhi_v - 1; ++j)
-      NU2[fv(s, col, j)] += (s->vvc[fu(s, col, j + 1)] - s->vvc[fu(s, col, j)]) * DTHV(s, j) * s->ddthetavi;
+    int lo[2], hi[2];
+    int ns = col_segs(s, col, 0, lo, hi);
+    for (int g = 0; g < ns; ++g)
+      for (int j = lo[g] + 1; j <= hi[g] - 1; ++j) {
+        NU1[fu(s, col, j)] += (s->uvf[fv(s, col, j)] - s->uvf[fv(s, col, j - 1)]) * DTHU(s, j) * s->ddthetavi;
+        NU3[fu(s, col, j)] += (s->vwf[fv(s, col, j)] - s->vwf[fv(s, col, j - 1)]) * DTHU(s, j) * s->ddthetavi;
+      }
+    ns = col_segs(s, col, 1, lo, hi);
+    for (int g = 0; g < ns; ++g)
+      for (int j = lo[g] + 1; j <= hi[g] - 1; ++j)
+        NU2[fv(s, col, j)] += (s->vvc[fu(s, col, j + 1)] - s->vvc[fu(s, col, j)]) * DTHV(s, j) * s->ddthetavi;
   }
-  /* CFL (FOU3D.f90:177-183, 516-544) */
+  /* CFL (FOU3D.f90:177-183, 516-544): each band's limit with its own mode
+     counts, the smallest over the bands (the MIN all-reduce over the ranks,
+     each rank in one band) */
   if (kRK == 1) {
-    const double dtl1 = 1.0 / (s->P.alp * (s->N1 / 2) * m1);
-    const double dtl3 = 1.0 / (s->P.bet * (s->N2 / 2) * m3);
+    const double dtl1 = 1.0 / (s->P.alp * (s->N1 / 2) * m1f);
+    const double dtl3 = 1.0 / (s->P.bet * (s->N2 / 2) * m3f);
     const double dtl2 = 1.0 / (m2 * s->dthetavi);
     double dtc = dtl1 < dtl3 ? dtl1 : dtl3;
     dtc = dtc < dtl2 ? dtc : dtl2;
+    if (s->mb) {
+      const double c1 = 1.0 / (s->P.alp * (s->N1c / 2) * m1c), c3 = 1.0 / (s->P.bet * (s->N2c / 2) * m3c);
+      const double dc = c1 < c3 ? c1 : c3;
+      dtc = dtc < dc ? dtc : dc;
+    }
     s->dtc = dtc;
     s->dt = (2.5 * s->dtv < s->P.CFL * dtc) ? 2.5 * s->dtv : s->P.CFL * dtc;
     s->dti = 1.0 / s->dt;
@@ -1094,11 +1285,16 @@ void or_nonlinear(or_state* s, int kRK) {
   const double C1 = -s->gRK[kRK - 1];
 #pragma omp parallel for schedule(static)
   for (int col = 0; col < s->Ncol; ++col) {
-    for (int j = s->jlo_u + 1; j <= s->jhi_u - 1; ++j) {
-      DU1[fu(s, col, j)] += C1 * NU1[fu(s, col, j)];
-      DU3[fu(s, col, j)] += C1 * NU3[fu(s, col, j)];
-    }
-    for (int j = s->jlo_v + 1; j <= s->jhi_v - 1; ++j) DU2[fv(s, col, j)] += C1 * NU2[fv(s, col, j)];
+    int lo[2], hi[2];
+    int ns = col_segs(s, col, 0, lo, hi);
+    for (int g = 0; g < ns; ++g)
+      for (int j = lo[g] + 1; j <= hi[g] - 1; ++j) {
+        DU1[fu(s, col, j)] += C1 * NU1[fu(s, col, j)];
+        DU3[fu(s, col, j)] += C1 * NU3[fu(s, col, j)];
+      }
+    ns = col_segs(s, col, 1, lo, hi);
+    for (int g = 0; g < ns; ++g)
+      for (int j = lo[g] + 1; j <= hi[g] - 1; ++j) DU2[fv(s, col, j)] += C1 * NU2[fv(s, col, j)];
   }
 }
 
@@ -1169,14 +1365,20 @@ static void lu_sol(const double* a1, const double* a2, const double* a3, cplx* x
 
 /* solveU / solveV predictor (littleharsh.f90:732-794) + LUsolU/LUsolV
    (tridLU_3D_IB.f90:50-160) with LU_buildU/LU_buildV (349-413) */
-static void predictor(or_state* s, cplx* du, const cplx* u, int ny_g) {
+static void predictor(or_state* s, cplx* du, const cplx* u, int grid) {
+  const int ny_g = grid == 0 ? s->nyu : s->nyv, jlo = grid == 0 ? s->jlo_u : s->jlo_v;
 #pragma omp parallel for schedule(static)
   for (int col = 0; col < s->Ncol; ++col) {
     cplx* d = du + (size_t)col * ny_g;
     const cplx* uu = u + (size_t)col * ny_g;
-    d[0] = 0.0;
-    for (int q = 1; q < ny_g - 1; ++q) d[q] = uu[q] + s->dt * d[q];
-    d[ny_g - 1] = 0.0;
+    int lo[2], hi[2];
+    const int ns = col_segs(s, col, grid, lo, hi);
+    for (int g = 0; g < ns; ++g) {
+      const int a = lo[g] - jlo, b = hi[g] - jlo;
+      d[a] = 0.0;
+      for (int q = a + 1; q < b; ++q) d[q] = uu[q] + s->dt * d[q];
+      d[b] = 0.0;
+    }
   }
 }
 
@@ -1191,19 +1393,26 @@ static void lusol_uv(or_state* s, cplx* u, const cplx* rhs, int grid, int kRK) {
     for (int col = 0; col < s->Ncol; ++col) {
       const int i = col % s->Mx, k = col / s->Mx;
       const double k2x = s->k2x[i], k2z = s->k2z[k];
-      a1[0] = 0.0; a2[0] = 1.0; a3[0] = grid == 0 ? s->gw[0] : 0.0;
-      for (int q = 1; q < n - 1; ++q) {
-        const int j = jlo + q;
-        const double d1 = grid == 0 ? DYU2(s, 1, j) : DYV2(s, 1, j);
-        const double d2 = grid == 0 ? DYU2(s, 2, j) : DYV2(s, 2, j);
-        const double d3 = grid == 0 ? DYU2(s, 3, j) : DYV2(s, 3, j);
-        a1[q] = -beta * d1;
-        a2[q] = 1.0 - beta * (d2 + k2x + k2z);
-        a3[q] = -beta * d3;
+      int lo[2], hi[2];
+      const int ns = col_segs(s, col, grid, lo, hi);
+      for (int g = 0; g < ns; ++g) {
+        /* one band's system, boundary rows at its ends (LU_buildU/V with the
+           band's gridweighting, tridLU_3D_IB.f90:349-413) */
+        const int q0 = lo[g] - jlo, m = hi[g] - lo[g] + 1;
+        a1[0] = 0.0; a2[0] = 1.0; a3[0] = grid == 0 ? gw_lo(s, lo[g]) : 0.0;
+        for (int q = 1; q < m - 1; ++q) {
+          const int j = lo[g] + q;
+          const double d1 = grid == 0 ? DYU2(s, 1, j) : DYV2(s, 1, j);
+          const double d2 = grid == 0 ? DYU2(s, 2, j) : DYV2(s, 2, j);
+          const double d3 = grid == 0 ? DYU2(s, 3, j) : DYV2(s, 3, j);
+          a1[q] = -beta * d1;
+          a2[q] = 1.0 - beta * (d2 + k2x + k2z);
+          a3[q] = -beta * d3;
+        }
+        a1[m - 1] = grid == 0 ? gw_hi(s, hi[g]) : 0.0; a2[m - 1] = 1.0; a3[m - 1] = 0.0;
+        lu_dec(a1, a2, a3, m);
+        lu_sol(a1, a2, a3, u + (size_t)col * n + q0, rhs + (size_t)col * n + q0, m);
       }
-      a1[n - 1] = grid == 0 ? s->gw[1] : 0.0; a2[n - 1] = 1.0; a3[n - 1] = 0.0;
-      lu_dec(a1, a2, a3, n);
-      lu_sol(a1, a2, a3, u + (size_t)col * n, rhs + (size_t)col * n, n);
     }
     free(a);
   }
@@ -1212,12 +1421,12 @@ static void lusol_uv(or_state* s, cplx* u, const cplx* rhs, int grid, int kRK) {
 void or_solve_u(or_state* s, int comp, int kRK) {
   cplx* u = comp == 1 ? U1 : U3;
   cplx* du = comp == 1 ? DU1 : DU3;
-  predictor(s, du, u, s->nyu);
+  predictor(s, du, u, 0);
   immersed(s, du, u, 0);
   lusol_uv(s, u, du, 0, kRK);
 }
 void or_solve_v(or_state* s, int kRK) {
-  predictor(s, DU2, U2, s->nyv);
+  predictor(s, DU2, U2, 1);
   immersed(s, DU2, U2, 1);
   lusol_uv(s, U2, DU2, 1, kRK);
 }
@@ -1271,68 +1480,86 @@ void or_divergence(or_state* s) {
 #pragma omp parallel for schedule(static)
   for (int col = 0; col < s->Ncol; ++col) {
     const int i = col % s->Mx, k = col / s->Mx;
-    for (int j = s->jlo_p; j <= s->jhi_p; ++j)
-      DIV[fp(s, col, j)] = imul(s->k1x[i], U1[fu(s, col, j)]) + imul(s->k1z[k], U3[fu(s, col, j)]) +
-                           s->ddthetavi * DTHU(s, j) * (U2[fv(s, col, j)] - U2[fv(s, col, j - 1)]);
+    int lo[2], hi[2];
+    const int ns = col_segs(s, col, 2, lo, hi);
+    for (int g = 0; g < ns; ++g)
+      for (int j = lo[g]; j <= hi[g]; ++j)
+        DIV[fp(s, col, j)] = imul(s->k1x[i], U1[fu(s, col, j)]) + imul(s->k1z[k], U3[fu(s, col, j)]) +
+                             s->ddthetavi * DTHU(s, j) * (U2[fv(s, col, j)] - U2[fv(s, col, j - 1)]);
   }
 }
 
-/* LU_buildP + LU_decP (tridLU_3D_IB.f90:417-491, 562-603), once at start */
+/* LU_buildP + LU_decP (tridLU_3D_IB.f90:417-491, 562-603), once at start;
+   per band of a column, the first and last rows without the flux through
+   the band's end faces.  Rows of a short column between its bands are
+   identity rows. */
 static void lu_build_p(or_state* s) {
-  const int ns = s->jlo_p, ne = s->jhi_p, n = s->nyp;
+  const int n = s->nyp;
   for (int col = 0; col < s->Ncol; ++col) {
     const int i = col % s->Mx, k = col / s->Mx;
     const double k2x = s->k2x[i], k2z = s->k2z[k];
     double* dg = s->DG + (size_t)3 * n * col; /* dg[3*q + m] */
+    for (int q = 0; q < 3 * n; ++q) dg[q] = 0.0;
+    for (int q = 0; q < n; ++q) dg[3 * q + 1] = 1.0;
+    int lo[2], hi[2];
+    const int nseg = col_segs(s, col, 2, lo, hi);
+    for (int g = 0; g < nseg; ++g) {
+      const int ns = lo[g], ne = hi[g], q0 = ns - s->jlo_p, m = ne - ns + 1;
+      double* d = dg + 3 * q0;
 #define DVB(j) (-s->ddthetavi * DTHU(s, (j) + 1))
 #define GVB(j) (-s->ddthetavi * DTHV(s, (j)))
 #define DVT(j) (s->ddthetavi * DTHU(s, (j)))
 #define GVT(j) (s->ddthetavi * DTHV(s, (j) - 1))
-    for (int q = 0; q < 3 * n; ++q) dg[q] = 0.0;
-    dg[3 * 0 + 1] = (DVT(ns) * GVB(ns)) + k2x + k2z;
-    dg[3 * 0 + 2] = (DVT(ns) * GVT(ns + 1));
-    dg[3 * (n - 1) + 1] = (DVB(ne - 1) * GVT(ne)) + k2x + k2z;
-    dg[3 * (n - 1) + 0] = (DVB(ne - 1) * GVB(ne - 1));
-    for (int j = ns + 1; j <= ne - 1; ++j) {
-      const int q = j - ns;
-      dg[3 * q + 1] = (DVB(j - 1) * GVT(j)) + (DVT(j) * GVB(j)) + k2x + k2z;
-      dg[3 * q + 0] = (DVB(j - 1) * GVB(j - 1));
-      dg[3 * q + 2] = (DVT(j) * GVT(j + 1));
-    }
+      d[3 * 0 + 1] = (DVT(ns) * GVB(ns)) + k2x + k2z;
+      d[3 * 0 + 2] = (DVT(ns) * GVT(ns + 1));
+      d[3 * (m - 1) + 1] = (DVB(ne - 1) * GVT(ne)) + k2x + k2z;
+      d[3 * (m - 1) + 0] = (DVB(ne - 1) * GVB(ne - 1));
+      for (int j = ns + 1; j <= ne - 1; ++j) {
+        const int q = j - ns;
+        d[3 * q + 1] = (DVB(j - 1) * GVT(j)) + (DVT(j) * GVB(j)) + k2x + k2z;
+        d[3 * q + 0] = (DVB(j - 1) * GVB(j - 1));
+        d[3 * q + 2] = (DVT(j) * GVT(j + 1));
+      }
 #undef DVB
 #undef GVB
 #undef DVT
 #undef GVT
-    if (col == 0) { /* (0,0) pin (tridLU_3D_IB.f90:468-474) */
-      dg[0] = 0.0;
-      dg[1] = 1.0 / ((YU(s, 2) - YU(s, 1)) * (YU(s, 2) - YU(s, 1)));
-      dg[2] = 0.0;
-    }
-    dg[1] = 1.0 / dg[1];
-    for (int q = 1; q < n; ++q) {
-      dg[3 * (q - 1) + 2] = dg[3 * (q - 1) + 2] * dg[3 * (q - 1) + 1];
-      dg[3 * q + 1] = 1.0 / (dg[3 * q + 1] - dg[3 * q + 0] * dg[3 * (q - 1) + 2]);
+      if (col == 0) { /* (0,0) pin (tridLU_3D_IB.f90:468-474) */
+        d[0] = 0.0;
+        d[1] = 1.0 / ((YU(s, 2) - YU(s, 1)) * (YU(s, 2) - YU(s, 1)));
+        d[2] = 0.0;
+      }
+      d[1] = 1.0 / d[1];
+      for (int q = 1; q < m; ++q) {
+        d[3 * (q - 1) + 2] = d[3 * (q - 1) + 2] * d[3 * (q - 1) + 1];
+        d[3 * q + 1] = 1.0 / (d[3 * q + 1] - d[3 * q + 0] * d[3 * (q - 1) + 2]);
+      }
     }
   }
 }
 
-/* solveP (littleharsh.f90:556-611) + LUsolP (tridLU_3D_IB.f90:284-318) */
+/* solveP (littleharsh.f90:556-611) + LUsolP (tridLU_3D_IB.f90:284-318), per band */
 void or_solve_p(or_state* s, int kRK) {
   const double dtirk = s->dti / (s->aRK[kRK - 1] + s->bRK[kRK - 1]);
   or_divergence(s);
   const int n = s->nyp;
 #pragma omp parallel for schedule(static)
   for (int col = 0; col < s->Ncol; ++col) {
-    cplx* x = PSI + (size_t)col * n;
-    const cplx* dv = DIV + (size_t)col * n;
-    for (int q = 0; q < n; ++q) x[q] = dtirk * dv[q];
-    if (col == 0) x[0] = 0.0;
-    const double* dg = s->DG + (size_t)3 * n * col;
-    x[0] = x[0] * dg[1];
-    for (int q = 1; q < n; ++q) x[q] = (x[q] - dg[3 * q] * x[q - 1]) * dg[3 * q + 1];
-    for (int q = n - 2; q >= 0; --q) x[q] = x[q] - dg[3 * q + 2] * x[q + 1];
-    cplx* pp = PP + (size_t)col * n;
-    for (int q = 0; q < n; ++q) pp[q] = pp[q] + x[q];
+    int lo[2], hi[2];
+    const int nseg = col_segs(s, col, 2, lo, hi);
+    for (int g = 0; g < nseg; ++g) {
+      const int q0 = lo[g] - s->jlo_p, m = hi[g] - lo[g] + 1;
+      cplx* x = PSI + (size_t)col * n + q0;
+      const cplx* dv = DIV + (size_t)col * n + q0;
+      for (int q = 0; q < m; ++q) x[q] = dtirk * dv[q];
+      if (col == 0) x[0] = 0.0;
+      const double* dg = s->DG + (size_t)3 * n * col + 3 * q0;
+      x[0] = x[0] * dg[1];
+      for (int q = 1; q < m; ++q) x[q] = (x[q] - dg[3 * q] * x[q - 1]) * dg[3 * q + 1];
+      for (int q = m - 2; q >= 0; --q) x[q] = x[q] - dg[3 * q + 2] * x[q + 1];
+      cplx* pp = PP + (size_t)col * n + q0;
+      for (int q = 0; q < m; ++q) pp[q] = pp[q] + x[q];
+    }
   }
 }
 
@@ -1343,18 +1570,26 @@ void or_vcorr(or_state* s, int kRK) {
   for (int col = 0; col < s->Ncol; ++col) {
     const int i = col % s->Mx, k = col / s->Mx;
     const cplx kx = imul(s->k1x[i], dtrk), kz = imul(s->k1z[k], dtrk);
-    for (int j = s->jlo_u + 1; j <= s->jhi_u - 1; ++j) {
-      U1[fu(s, col, j)] = U1[fu(s, col, j)] - kx * PSI[fp(s, col, j)];
-      U3[fu(s, col, j)] = U3[fu(s, col, j)] - kz * PSI[fp(s, col, j)];
-    }
+    int lo[2], hi[2];
+    int ns = col_segs(s, col, 0, lo, hi);
+    for (int g = 0; g < ns; ++g)
+      for (int j = lo[g] + 1; j <= hi[g] - 1; ++j) {
+        U1[fu(s, col, j)] = U1[fu(s, col, j)] - kx * PSI[fp(s, col, j)];
+        U3[fu(s, col, j)] = U3[fu(s, col, j)] - kz * PSI[fp(s, col, j)];
+      }
+    /* ghosts at the walls only: the bottom of bands 1, 2 and the top of
+       bands 2, 3; a short column's interface ghosts keep their solve values */
     U1[fu(s, col, s->jlo_u)] = -s->gw[0] * U1[fu(s, col, s->jlo_u + 1)];
     U1[fu(s, col, s->jhi_u)] = -s->gw[1] * U1[fu(s, col, s->jhi_u - 1)];
     U3[fu(s, col, s->jlo_u)] = -s->gw[0] * U3[fu(s, col, s->jlo_u + 1)];
     U3[fu(s, col, s->jhi_u)] = -s->gw[1] * U3[fu(s, col, s->jhi_u - 1)];
-    for (int j = s->jlo_v + 1; j <= s->jhi_v - 1; ++j)
-      U2[fv(s, col, j)] = U2[fv(s, col, j)] - dtrk * s->ddthetavi * DTHV(s, j) * (PSI[fp(s, col, j + 1)] - PSI[fp(s, col, j)]);
-    U2[fv(s, col, s->jlo_v)] = 0.0;
-    U2[fv(s, col, s->jhi_v)] = 0.0;
+    ns = col_segs(s, col, 1, lo, hi);
+    for (int g = 0; g < ns; ++g) {
+      for (int j = lo[g] + 1; j <= hi[g] - 1; ++j)
+        U2[fv(s, col, j)] = U2[fv(s, col, j)] - dtrk * s->ddthetavi * DTHV(s, j) * (PSI[fp(s, col, j + 1)] - PSI[fp(s, col, j)]);
+      U2[fv(s, col, lo[g])] = 0.0;
+      U2[fv(s, col, hi[g])] = 0.0;
+    }
   }
   or_divergence(s);
 }
@@ -1449,6 +1684,7 @@ static int jlo_grid(const or_state* s, int g) { return g == 0 ? s->jlo_u : g == 
 int or_stats_init(or_state* s, const int* N20) {
   or_stats_free(s);
   if (s->P.ntilex <= 0 || s->P.ntilez <= 0) return 1;
+  if (s->mb) return 3; /* statistics of one resolution in every band only */
   struct or_stats* t = (struct or_stats*)calloc(1, sizeof *t);
   memcpy(t->N, N20, sizeof t->N);
 #define NB(r, b) (t->N[4 * (b) + (r) - 1])

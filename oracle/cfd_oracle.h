@@ -13,9 +13,15 @@
  *       (intent of FOU3D.f90:1553-1613), not from the last plane.
  *   D8  y-grid: the polynomial grid of y_grid_canopy.f90 (ygrid = 0) and a
  *       tanh tip-to-tip grid (ygrid = 1) with the same canopy extension.
- * One spectral band resolution (Ngal equal in all bands), i.e. the whole
- * column set of proc_lims_columns is "long" (start.f90:1420-1527), which is
- * the case for every configuration this project runs.
+ * Multi-band resolution (start.f90:89-111, 1420-1557): with ngalx2/ngalz2
+ * below ngalx/ngalz the central y band (N(4,1) < j <= N(4,2) on the u grid)
+ * is transformed on the coarser Ngal(1:2,2) grid and holds only the "long"
+ * columns (the N(1:2,2) modes, which span every plane); the other "short"
+ * columns live only in the two wall bands, each band a closed system with
+ * its own boundary rows at the band interface (gridweighting(1,2),
+ * gridweighting(3,1), start.f90:494-508).  Fields keep one array per grid
+ * over all fine columns and all planes; a short column's rows between its
+ * two bands are not part of the system and stay as they are (zero).
  */
 #ifndef CFD_ORACLE_H
 #define CFD_ORACLE_H
@@ -32,6 +38,7 @@ typedef struct {
   int ntilex, ntilez;
   int extras24;                /* geom_circ_rough_mixed_forcing_24pt.f90:219-259 */
   int flag_ctpress;            /* 0 constant flow rate, 1 constant mpgx */
+  int ngalx2, ngalz2;          /* Ngal(1,2), Ngal(2,2) of the central band; 0: as band 1 */
   double Re, alp, bet, dyq;
   double posth, posth1, ytanh_beta;
   double CFL, mpgx;
@@ -43,7 +50,8 @@ typedef struct or_state or_state;
 or_state* or_create(const or_params* p, char* err, int errlen);
 void      or_destroy(or_state* s);
 
-/* dims[0..15]: N1 N2 Mx Ncol nyu nyv nyp jlo_u jlo_v jlo_p ngx ngz ny dsty nib_u nib_v */
+/* dims[0..15]: N1 N2 Mx Ncol nyu nyv nyp jlo_u jlo_v jlo_p ngx ngz ny dsty nib_u nib_v
+   (the wall bands' resolution; or_band_info gives the rest) */
 void or_dims(const or_state* s, int* dims);
 /* y grid, index j - jlo: yu/dthdyu/dyu2i(3,.) over nyu, yv/dthdyv/dyv2i over nyv */
 void or_get_ygrid(const or_state* s, double* yu, double* yv, double* dthdyu,
@@ -56,6 +64,10 @@ void or_get_metrics(const or_state* s, double* gw, double* gwi, double* dtv);
 int  or_get_iblist(const or_state* s, int grid, int wall, int* l9, double* w3);
 /* pressure LU (3, nyp, Ncol) after LU_decP */
 void or_get_dg(const or_state* s, double* dg);
+/* multi-band description: b[0] 1 when the central band is coarser, b[1..2]
+   N1, N2 of the central band, b[3..10] N(3,0:3), N(4,0:3) (the band table's
+   y limits); shortc[Ncol] (may be null) 1 for the wall-band-only columns */
+void or_band_info(const or_state* s, int* b, unsigned char* shortc);
 
 /* fields in reference column layout: complex(8) f(j, column), y fastest,
    interleaved (re, im).  which: 0 u1 1 u2 2 u3 3 p 4 Nu1 5 Nu2 6 Nu3

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "libcfd_oracle.so")
 REF_LIB = os.path.join(HERE, "_ref", "libref.so")
 
-INT_FIELDS = "ngalx ngalz ny dsty dsty1 ppp ygrid geometry_type ntilex ntilez extras24 flag_ctpress".split()
+INT_FIELDS = "ngalx ngalz ny dsty dsty1 ppp ygrid geometry_type ntilex ntilez extras24 flag_ctpress ngalx2 ngalz2".split()
 DBL_FIELDS = "Re alp bet dyq posth posth1 ytanh_beta CFL mpgx r c_i c_k stag".split()
 FIELDS = ("u1", "u2", "u3", "p", "Nu1", "Nu2", "Nu3", "du1", "du2", "du3", "psi", "div")
 GRID = {"u1": "u", "u2": "v", "u3": "u", "p": "p", "Nu1": "u", "Nu2": "v", "Nu3": "u",
@@ -49,7 +49,7 @@ def load():
                      "or_nonlinear", "or_solve_u", "or_solve_v", "or_meanflow_ctU", "or_solve_p", "or_vcorr",
                      "or_divergence", "or_substep", "or_step", "or_flowrate", "or_meanflow_ctP", "or_stats_init", "or_stats",
                      "or_stats_dims", "or_stats_get", "or_stats_reset", "or_set_compat_lastplane",
-                     "or_plane_partition"):
+                     "or_plane_partition", "or_band_info"):
             getattr(lib, name).argtypes = None
         lib.or_flowrate.restype = ctypes.c_double
         _lib = lib
@@ -57,10 +57,11 @@ def load():
 
 
 def to_or_params(p):
-    """From any object with the cfd_params attribute names."""
+    """From any object with the cfd_params attribute names (ngalx2, ngalz2
+    default to 0: one resolution in every band)."""
     c = OrParams()
     for n in INT_FIELDS:
-        setattr(c, n, int(getattr(p, n)))
+        setattr(c, n, int(getattr(p, n, 0)))
     for n in DBL_FIELDS:
         setattr(c, n, float(getattr(p, n)))
     return c

@@ -86,12 +86,14 @@ def test_chebyshev_orr_sommerfeld_reproduces_orszag():
     assert abs(c.real - 0.23752649) < 1e-8 and abs(c.imag - 0.00373967) < 1e-8, c
 
 
-def _os_run(ny, T1, T2, R=1.0e4, eps=1e-6):
+def _os_run(ny, T1, T2, R=1.0e4, eps=1e-6, ngalx=6, ngalx2=0, dt=None):
     """Growth rate and angular frequency of the alpha = 1 mode of the
     oracle's time stepping on a smooth tanh grid of ny points, measured on
-    [T1, T2] from the L2 norm and the phase of its v column."""
-    p = Params(ngalx=6, ngalz=6, ny=ny, dsty=0, ygrid=1, ytanh_beta=2.0, geometry_type=0, ntilex=2, ntilez=2,
-               Re=R, alp=1.0, bet=2.0, posth=0.0, CFL=0.5, flag_ctpress=0)
+    [T1, T2] from the L2 norm and the phase of its v column (ngalx2: a
+    coarser central band in x; dt: a fixed time step in place of the CFL
+    one, the reference's call sequence with dt set after nonlinear)."""
+    p = Params(ngalx=ngalx, ngalz=6, ny=ny, dsty=0, ygrid=1, ytanh_beta=2.0, geometry_type=0, ntilex=2, ntilez=2,
+               Re=R, alp=1.0, bet=2.0, posth=0.0, CFL=0.5, flag_ctpress=0, ngalx2=ngalx2)
     lam, x, v, D = orr_sommerfeld(R, 1.0)
     u = 1j * (D @ v)  # continuity: i alpha u + v' = 0
     cu = np.polynomial.chebyshev.chebfit(x, u, len(x) - 1)
@@ -111,9 +113,23 @@ def _os_run(ny, T1, T2, R=1.0e4, eps=1e-6):
     ts, E, A = [], [], []
     jm = len(yv) // 2
     while not ts or ts[-1] < T2:
-        o.step(1)
+        if dt is None:
+            o.step(1)
+            ts.append(o.scalars()["t"])
+        else:
+            for kRK in (1, 2, 3):
+                o.RHS0(kRK)
+                o.nonlinear(kRK)
+                if kRK == 1:
+                    o.set_scalars(dt=dt, dti=1.0 / dt)
+                o.solveU(1, kRK)
+                o.solveV(kRK)
+                o.solveU(3, kRK)
+                o.meanflow_ctU(kRK)
+                o.solveP(kRK)
+                o.v_corr(kRK)
+            ts.append(len(ts) * dt + dt)
         a = o.get_field("u2")[1]
-        ts.append(o.scalars()["t"])
         E.append(np.linalg.norm(a))
         A.append(a[jm])
     o.close()
