@@ -11,7 +11,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libcfd_amd.so")
 
 INT_FIELDS = ("ngalx ngalz ny dsty dsty1 ppp ygrid geometry_type ntilex ntilez "
-              "extras24 flag_ctpress precision device").split()
+              "extras24 flag_ctpress precision device ngalx2 ngalz2").split()
 DBL_FIELDS = "Re alp bet dyq posth posth1 ytanh_beta CFL mpgx r c_i c_k stag".split()
 
 

@@ -199,6 +199,7 @@ int tuning_check(const cfd_tuning& t, std::string& err) {
   return CFD_EINVAL;
 }
 int create_impl(const cfd_params* pp, const cfd_tuning* tp, cfd_handle* out);
+int setup_bands(cfd_solver* s);
 }  // namespace
 
 extern "C" int cfd_create(const cfd_params* pp, cfd_handle* out) { return create_impl(pp, nullptr, out); }
@@ -225,6 +226,8 @@ extern "C" int cfd_set_tuning(cfd_handle h, const cfd_tuning* t) {
   std::string err;
   if (tuning_check(*t, err)) return fail(s, CFD_EINVAL, err);
   if ((t->fft == 1) != s->fft_stockham) return fail(s, CFD_EINVAL, "tuning: the FFT engine is fixed at creation");
+  if (s->mb && (s->nyu + t->seg_rows + 1) / (t->seg_rows + 1) > 256)
+    return fail(s, CFD_EINVAL, "tuning: multi-band resolution needs the segmented solves (more rows per segment)");
   s->seg_rows = t->seg_rows;
   s->solve_threads = t->solve_threads;
   return CFD_OK;
@@ -275,6 +278,12 @@ int create_impl(const cfd_params* pp, const cfd_tuning* tp, cfd_handle* out) {
   s->Mx = s->N1 / 2 + 1;
   s->Ncol = s->Mx * s->N2;
   s->nn = p.ny - 2;
+  // the central band's resolution (start.f90:89-111): multi-band when coarser
+  s->ngx2 = p.ngalx2 > 0 ? p.ngalx2 : s->ngx;
+  s->ngz2 = p.ngalz2 > 0 ? p.ngalz2 : s->ngz;
+  s->N1c = 2 * (s->ngx2 / 3);
+  s->N2c = 2 * (s->ngz2 / 3);
+  s->mb = s->ngx2 != s->ngx || s->ngz2 != s->ngz;
   std::string err;
   if (!build_ygrid(p, s->yg, err) || !build_iblists(p, s->ib, err)) {
     s->err = err;
@@ -578,6 +587,9 @@ int create_impl(const cfd_params* pp, const cfd_tuning* tp, cfd_handle* out) {
     }
     if ((rc = upload(s, &s->d_ibmask[f], mask)) || (rc = upload(s, &s->d_ibq[f], q))) return bail(rc);
   }
+  // multi-band resolution: short columns, the central solver, CFL plane sets
+  if (s->mb)
+    if ((rc = setup_bands(s))) return bail(rc);
   // scalars
   {
     DevScalars h{};
@@ -593,6 +605,93 @@ int create_impl(const cfd_params* pp, const cfd_tuning* tp, cfd_handle* out) {
     }
   HIPCHK(s, hipDeviceSynchronize());
   *out = s;
+  return CFD_OK;
+}
+
+// Multi-band resolution (start.f90:89-111, 1420-1557): the band limits, the
+// short columns (the wall bands' modes outside the central band's N1c x N2c,
+// proc_lims_columns start.f90:1424-1501), the interface boundary weights
+// (start.f90:494-500), and the central solver: the same y grid at the central
+// band's x/z resolution, on this solver's stream, whose nonlinear transforms
+// give the central planes' products over the long columns.
+int setup_bands(cfd_solver* s) {
+  const cfd_params& p = s->P;
+  if (s->mixed) return fail(s, CFD_EINVAL, "multi-band resolution runs the FP64 path (precision 0) only");
+  if (s->ngx2 > s->ngx || s->ngz2 > s->ngz || s->ngx2 % 2 || s->ngz2 % 2 || s->ngx2 < 6 || s->ngz2 < 6)
+    return fail(s, CFD_EINVAL, "the central band's Ngalx2, Ngalz2 must be even, at least 6 and at most Ngalx, Ngalz");
+  int N[20];
+  band_table(p, s->yg, N);
+  for (int b = 0; b < 4; ++b) {
+    s->N3[b] = N[4 * b + 2];
+    s->N4[b] = N[4 * b + 3];
+  }
+  // each wall band keeps its own rows: the short columns' interface ghost
+  // rows N(4,1)+1 and N(4,2) must differ
+  if (!(s->N3[0] < s->N3[1] && s->N3[1] + 2 < s->N3[2] && s->N3[2] < s->N3[3]))
+    return fail(s, CFD_EINVAL, "the band limits leave no central band for a second resolution");
+  for (int g = 0; g < 2; ++g)
+    for (int j : s->ibplanes[g])
+      if (g == 0 ? (j > s->N4[1] && j <= s->N4[2]) : (j > s->N3[1] && j <= s->N3[2]))
+        return fail(s, CFD_EINVAL, "immersed-boundary planes in the central band");
+  const int R = s->seg_rows;
+  if ((s->nyu + R + 1) / (R + 1) > 256) return fail(s, CFD_EINVAL, "multi-band resolution: too many planes");
+  const YGrid& g = s->yg;
+  s->gw12 = (g.YU(s->N4[1] + 1) - g.YV(s->N3[1] + 1)) / (g.YV(s->N3[1] + 1) - g.YU(s->N4[1]));
+  s->gw31 = -(g.YV(s->N3[2]) - g.YU(s->N4[2])) / (g.YV(s->N3[2]) - g.YU(s->N4[2] + 1));
+  // column sets
+  s->h_short.assign(s->Ncol, 0);
+  std::vector<int> ccol(s->Ncol, -1), lcol;
+  const int Mxc = s->N1c / 2 + 1;
+  for (int col = 0; col < s->Ncol; ++col) {
+    const int i = col % s->Mx, k = col / s->Mx;
+    const bool kin = k < s->N2c / 2 || k >= s->N2 - s->N2c / 2;
+    if (i <= s->N1c / 2 && kin) {
+      const int kc = k < s->N2c / 2 ? k : k - s->N2 + s->N2c;  // column order of the central solver
+      ccol[col] = kc * Mxc + i;
+    } else {
+      s->h_short[col] = 1;
+    }
+  }
+  lcol.assign((size_t)Mxc * s->N2c, -1);
+  for (int col = 0; col < s->Ncol; ++col)
+    if (ccol[col] >= 0) lcol[ccol[col]] = col;
+  // CFL plane sets: the wall bands' planes here, the central planes there
+  std::vector<unsigned char> fu(s->nyu), fv(s->nyv), cu(s->nyu), cv(s->nyv);
+  for (int q = 0; q < s->nyu; ++q) {
+    const int j = s->jlo_u + q;
+    cu[q] = j > s->N4[1] && j <= s->N4[2];
+    fu[q] = !cu[q];
+  }
+  for (int q = 0; q < s->nyv; ++q) {
+    const int j = s->jlo_v + q;
+    cv[q] = j > s->N3[1] && j <= s->N3[2];
+    fv[q] = !cv[q];
+  }
+  int rc;
+  if ((rc = upload(s, &s->d_short, s->h_short)) || (rc = upload(s, &s->d_ccol, ccol)) ||
+      (rc = upload(s, &s->d_lcol, lcol)) || (rc = upload(s, &s->d_cfu, fu)) || (rc = upload(s, &s->d_cfv, fv)))
+    return rc;
+  // the central solver
+  cfd_params pc = p;
+  pc.ngalx = s->ngx2;
+  pc.ngalz = s->ngz2;
+  pc.ngalx2 = pc.ngalz2 = 0;
+  pc.geometry_type = 0;  // its IB planes are none of its business
+  const cfd_tuning t{s->fft_stockham ? 1 : 0, s->seg_rows, s->solve_threads};
+  cfd_handle hc = nullptr;
+  if ((rc = create_impl(&pc, &t, &hc))) return fail(s, rc, std::string("central band solver: ") + g_err);
+  cfd_solver* c = static_cast<cfd_solver*>(hc);
+  s->mbc = c;
+  if (c->nyu != s->nyu || c->nyv != s->nyv || c->Ncol != Mxc * s->N2c)
+    return fail(s, CFD_EINVAL, "central band solver: planes or columns differ");
+  HIPCHK(s, hipSetDevice(s->dev));
+  HIPCHK(s, hipStreamSynchronize(c->stream));
+  HIPCHK(s, hipStreamDestroy(c->stream));
+  c->stream = s->stream;
+  c->borrowed_stream = true;
+  if ((rc = upload(c, &c->d_cfu, cu)) || (rc = upload(c, &c->d_cfv, cv)))
+    return fail(s, rc, std::string("central band solver: ") + c->err);
+  s->alloc_bytes += c->alloc_bytes;
   return CFD_OK;
 }
 }  // namespace
@@ -638,6 +737,11 @@ extern "C" int cfd_destroy(cfd_handle h) {
   cfd_solver* s = static_cast<cfd_solver*>(h);
   if (!s) return CFD_OK;
   hipSetDevice(s->dev);
+  if (s->mbc) {  // the central solver runs on this stream: it goes first
+    if (s->stream) hipStreamSynchronize(s->stream);
+    cfd_destroy(static_cast<cfd_solver*>(s->mbc));
+    s->mbc = nullptr;
+  }
   // both streams drain before any buffer goes: the side stream's u11 solve
   // may still be writing when a substep returned between fork and join
   if (s->stream) hipStreamSynchronize(s->stream);
@@ -648,7 +752,7 @@ extern "C" int cfd_destroy(cfd_handle h) {
   if (s->ev_fork) hipEventDestroy(s->ev_fork);
   if (s->ev_join) hipEventDestroy(s->ev_join);
   if (s->side) hipStreamDestroy(s->side);
-  if (s->stream) hipStreamDestroy(s->stream);
+  if (s->stream && !s->borrowed_stream) hipStreamDestroy(s->stream);
   delete s;
   return CFD_OK;
 }
@@ -701,6 +805,34 @@ extern "C" int cfd_get_iblist(cfd_handle h, int grid, int wall, int* l9, double*
   return CFD_OK;
 }
 
+namespace {
+// multi-band resolution: the planes [g0, g1) of field `which` that a short
+// column does not have (the rows between its wall bands); none when g0 == g1
+void gap_planes(const Solver& s, int which, int& g0, int& g1) {
+  g0 = g1 = 0;
+  if (!s.mb) return;
+  const int g = grid_of(which);
+  if (g == 0) {
+    g0 = s.N4[1] + 2 - s.jlo_u;
+    g1 = s.N4[2] - s.jlo_u;
+  } else if (g == 1) {
+    g0 = s.N3[1] + 2 - s.jlo_v;
+    g1 = s.N3[2] - s.jlo_v;
+  } else {
+    g0 = s.N4[1] + 1 - s.jlo_p;
+    g1 = s.N4[2] + 1 - s.jlo_p;
+  }
+}
+void zero_gaps(const Solver& s, int which, std::vector<double2>& t) {
+  int g0, g1;
+  gap_planes(s, which, g0, g1);
+  const size_t N = s.Ncol;
+  for (int q = g0; q < g1; ++q)
+    for (size_t c = 0; c < N; ++c)
+      if (s.h_short[c]) t[q * N + c] = make_double2(0.0, 0.0);
+}
+}  // namespace
+
 // reference column layout f(j, column) <-> device f[plane][column]
 extern "C" int cfd_set_field(cfd_handle h, int which, const double* src) {
   cfd_solver* s = static_cast<cfd_solver*>(h);
@@ -710,6 +842,7 @@ extern "C" int cfd_set_field(cfd_handle h, int which, const double* src) {
   std::vector<double2> t((size_t)ny * N);
   for (size_t c = 0; c < N; ++c)
     for (int q = 0; q < ny; ++q) t[q * N + c] = make_double2(src[2 * (c * ny + q)], src[2 * (c * ny + q) + 1]);
+  zero_gaps(*s, which, t);
   HIPCHK(s, hipSetDevice(s->dev));
   HIPCHK(s, hipStreamSynchronize(s->stream));
   HIPCHK(s, hipMemcpy(s->F[which], t.data(), t.size() * sizeof(double2), hipMemcpyHostToDevice));
@@ -725,6 +858,7 @@ extern "C" int cfd_get_field(cfd_handle h, int which, double* dst) {
   HIPCHK(s, hipSetDevice(s->dev));
   HIPCHK(s, hipStreamSynchronize(s->stream));
   HIPCHK(s, hipMemcpy(t.data(), s->F[which], t.size() * sizeof(double2), hipMemcpyDeviceToHost));
+  zero_gaps(*s, which, t);
   for (size_t c = 0; c < N; ++c)
     for (int q = 0; q < ny; ++q) {
       dst[2 * (c * ny + q)] = t[q * N + c].x;
@@ -771,6 +905,14 @@ int check(cfd_solver* s, int kRK) {
   if (!s) return fail(s, CFD_EINVAL, "null handle");
   if (kRK < 1 || kRK > 3) return fail(s, CFD_EINVAL, "kRK must be 1, 2 or 3");
   if (hipSetDevice(s->dev) != hipSuccess) return fail(s, CFD_EHIP, "hipSetDevice failed");
+  return CFD_OK;
+}
+
+// entry points that exist for one resolution in every band
+int one_band(cfd_solver* s, const char* what) {
+  if (s && s->mb)
+    return fail(s, CFD_EINVAL, std::string(what) + ": not available with a coarser central band (multi-band runs "
+                                                   "through cfd_substep / cfd_step)");
   return CFD_OK;
 }
 
@@ -1233,10 +1375,21 @@ int stats_req_end(cfd_solver* s, const StatsReq& r) {
   return stats_write(s, s->st.pending_dir.c_str(), s->st.pending_iter, d1.t, r.d0.mpgx, r.d0.t);
 }
 
-int op_nonlinear(cfd_solver* s, int kRK) {
-  StatsReq req;
-  if (int rc = stats_req_begin(s, kRK, req)) return rc;
+// The nonlinear term's transforms (nonlinear, littleharsh.f90:520-700): every
+// plane at the solver's resolution; with multi-band resolution the central
+// solver transforms the long columns of the central planes at the central
+// band's resolution and its products replace this solver's on those planes
+// (short columns carry none there), and dt takes both resolutions' limits.
+void nl_transforms(cfd_solver* s, int kRK) {
   if (kRK == 1) launch_reset_max(*s);
+  if (s->mb) {
+    cfd_solver* c = static_cast<cfd_solver*>(s->mbc);
+    if (kRK == 1) launch_reset_max(*c);
+    { Timed t(s, KC_ELEM); launch_band_gather(*s); }
+    { Timed t(s, KC_ZINV); launch_zinv(*c, c->d_nljobs, c->n_nljobs, true); }
+    { Timed t(s, KC_XPASS); launch_xpass_nl(*c, kRK == 1); }
+    { Timed t(s, KC_ZFWD); launch_zfwd_nl(*c); }
+  }
   { Timed t(s, KC_ZINV); launch_zinv(*s, s->d_nljobs, s->n_nljobs, true); }
   { Timed t(s, KC_XPASS); launch_xpass_nl(*s, kRK == 1); }
   {
@@ -1244,7 +1397,18 @@ int op_nonlinear(cfd_solver* s, int kRK) {
     launch_zfwd_nl(*s);
     launch_plane_map(*s);
   }
-  if (kRK == 1) launch_cfl(*s);
+  if (s->mb) {
+    { Timed t(s, KC_ELEM); launch_band_scatter(*s); }
+    if (kRK == 1) launch_cfl_bands(*s);
+  } else if (kRK == 1) {
+    launch_cfl(*s);
+  }
+}
+
+int op_nonlinear(cfd_solver* s, int kRK) {
+  StatsReq req;
+  if (int rc = stats_req_begin(s, kRK, req)) return rc;
+  nl_transforms(s, kRK);
   const int wrc = stats_req_end(s, req);
   if (s->cl.cnn)
     if (int rc = closure_apply(s, kRK)) return rc;
@@ -1281,15 +1445,7 @@ void op_solve(cfd_solver* s, int f, int kRK) {
 int op_substep(cfd_solver* s, int kRK) {
   StatsReq req;
   if (int rc = stats_req_begin(s, kRK, req)) return rc;
-  if (kRK == 1) launch_reset_max(*s);
-  { Timed t(s, KC_ZINV); launch_zinv(*s, s->d_nljobs, s->n_nljobs, true); }
-  { Timed t(s, KC_XPASS); launch_xpass_nl(*s, kRK == 1); }
-  {
-    Timed t(s, KC_ZFWD);
-    launch_zfwd_nl(*s);
-    launch_plane_map(*s);
-  }
-  if (kRK == 1) launch_cfl(*s);
+  nl_transforms(s, kRK);
   // a failed write (say, a missing directory) still completes the substep
   const int wrc = stats_req_end(s, req);
   if (s->cl.cnn)
@@ -1333,18 +1489,21 @@ int op_substep(cfd_solver* s, int kRK) {
 extern "C" int cfd_rhs0(cfd_handle h, int kRK) {
   cfd_solver* s = static_cast<cfd_solver*>(h);
   if (int rc = check(s, kRK)) return rc;
+  if (int rc = one_band(s, "cfd_rhs0")) return rc;
   launch_rhs0(*s, kRK);
   return launched(s);
 }
 extern "C" int cfd_nonlinear(cfd_handle h, int kRK) {
   cfd_solver* s = static_cast<cfd_solver*>(h);
   if (int rc = check(s, kRK)) return rc;
+  if (int rc = one_band(s, "cfd_nonlinear")) return rc;
   if (int rc = op_nonlinear(s, kRK)) return rc;
   return launched(s);
 }
 extern "C" int cfd_solve_u(cfd_handle h, int comp, int kRK) {
   cfd_solver* s = static_cast<cfd_solver*>(h);
   if (int rc = check(s, kRK)) return rc;
+  if (int rc = one_band(s, "cfd_solve_u")) return rc;
   if (comp != 1 && comp != 3) return fail(s, CFD_EINVAL, "comp must be 1 or 3");
   op_solve(s, comp - 1, kRK);
   return launched(s);
@@ -1352,24 +1511,28 @@ extern "C" int cfd_solve_u(cfd_handle h, int comp, int kRK) {
 extern "C" int cfd_solve_v(cfd_handle h, int kRK) {
   cfd_solver* s = static_cast<cfd_solver*>(h);
   if (int rc = check(s, kRK)) return rc;
+  if (int rc = one_band(s, "cfd_solve_v")) return rc;
   op_solve(s, 1, kRK);
   return launched(s);
 }
 extern "C" int cfd_meanflow_ctU(cfd_handle h, int kRK) {
   cfd_solver* s = static_cast<cfd_solver*>(h);
   if (int rc = check(s, kRK)) return rc;
+  if (int rc = one_band(s, "cfd_meanflow_ctU")) return rc;
   launch_meanflow_lds(*s, kRK);
   return launched(s);
 }
 extern "C" int cfd_solve_p(cfd_handle h, int kRK) {
   cfd_solver* s = static_cast<cfd_solver*>(h);
   if (int rc = check(s, kRK)) return rc;
+  if (int rc = one_band(s, "cfd_solve_p")) return rc;
   launch_solve_p(*s, kRK);
   return launched(s);
 }
 extern "C" int cfd_vcorr(cfd_handle h, int kRK) {
   cfd_solver* s = static_cast<cfd_solver*>(h);
   if (int rc = check(s, kRK)) return rc;
+  if (int rc = one_band(s, "cfd_vcorr")) return rc;
   launch_vcorr(*s, kRK);
   launch_divergence(*s);
   if (kRK == 3 && s->filt) launch_mode_filter(*s);
@@ -1439,6 +1602,7 @@ extern "C" int cfd_hist(cfd_handle h, double* v) {
 extern "C" int cfd_set_mode_filter(cfd_handle h, double alpha, int order, double eta_c) {
   cfd_solver* s = static_cast<cfd_solver*>(h);
   if (!s) return fail(s, CFD_EINVAL, "null handle");
+  if (int rc = one_band(s, "cfd_set_mode_filter")) return rc;
   if (!(alpha >= 0.0) || order < 1 || !(eta_c >= 0.0 && eta_c < 1.0))
     return fail(s, CFD_EINVAL, "mode filter: alpha >= 0, order >= 1, 0 <= eta_c < 1");
   if (alpha == 0.0) {
@@ -1472,6 +1636,7 @@ extern "C" int cfd_set_mode_filter(cfd_handle h, double alpha, int order, double
 extern "C" int cfd_set_compat_lastplane(cfd_handle h, int np) {
   cfd_solver* s = static_cast<cfd_solver*>(h);
   if (!s) return fail(s, CFD_EINVAL, "null handle");
+  if (int rc = one_band(s, "cfd_set_compat_lastplane")) return rc;
   if (np == 0) {
     s->compat_np = 0;
     return CFD_OK;
@@ -1563,6 +1728,27 @@ struct FileField {
 };
 const FileField kFileFields[4] = {{CFD_U1, 1, 0, "u1"}, {CFD_U2, 2, 1, "u2"}, {CFD_U3, 3, 0, "u3"}, {CFD_P, 4, 2, "p"}};
 
+// the planes of grid g that the central band's ranks write (limPL_incw of
+// the midband ranks, start.f90:1758-1866): u N(4,1)+1 .. N(4,2), v N(3,1)+1
+// .. N(3,2), p N(4,1) .. N(4,2)+1, at N(1,2)+2 x N(2,2)
+BandPlanes central_planes(const Solver& s, int g) {
+  BandPlanes b;
+  if (!s.mb) return b;
+  if (g == 0) {
+    b.q0 = s.N4[1] + 1 - s.jlo_u;
+    b.q1 = s.N4[2] + 1 - s.jlo_u;
+  } else if (g == 1) {
+    b.q0 = s.N3[1] + 1 - s.jlo_v;
+    b.q1 = s.N3[2] + 1 - s.jlo_v;
+  } else {
+    b.q0 = s.N4[1] - s.jlo_p;
+    b.q1 = s.N4[2] + 2 - s.jlo_p;
+  }
+  b.nx = s.N1c + 2;
+  b.nz = s.N2c;
+  return b;
+}
+
 }  // namespace
 
 extern "C" int cfd_record_out(cfd_handle h, const char* dirout, int iter) {
@@ -1600,7 +1786,8 @@ extern "C" int cfd_record_out(cfd_handle h, const char* dirout, int iter) {
     const int jlo = ff.grid == 0 ? s->jlo_u : (ff.grid == 1 ? s->jlo_v : s->jlo_p);
     const std::string path = std::string(dirout) + "/" + ff.name + name_tail;
     const std::string e = write_field_file(path.c_str(), hd, ff.id, 0.0, y.data(), dy.data(), (int)y.size(),
-                                           v ? s->jlo_v : s->jlo_u, ny_g, jlo, nx, nz, planes.data());
+                                           v ? s->jlo_v : s->jlo_u, ny_g, jlo, nx, nz, planes.data(),
+                                           central_planes(*s, ff.grid));
     if (!e.empty()) return fail(s, CFD_EINVAL, e);
   }
   return CFD_OK;
@@ -1624,7 +1811,8 @@ extern "C" int cfd_read_in(cfd_handle h, const char* dirin, const char* fnameimb
     cfd_file_header hd{};
     int mism = 0;
     const std::string path = std::string(dirin) + "/" + ff.name + tail;
-    const std::string e = read_field_file(path.c_str(), hd, ny_g, nx, nz, s->ngz, planes.data(), mism);
+    const std::string e =
+        read_field_file(path.c_str(), hd, ny_g, nx, nz, s->ngz, planes.data(), mism, central_planes(*s, ff.grid));
     if (!e.empty()) return fail(s, CFD_EINVAL, e);
     if (ff.which == CFD_U1) {
       h1 = hd;
@@ -1674,6 +1862,7 @@ extern "C" int cfd_device_bytes(cfd_handle h, double* bytes) {
 extern "C" int cfd_four_to_phys(cfd_handle h, int which, double* phys) {
   cfd_solver* s = static_cast<cfd_solver*>(h);
   if (!s || !phys || which < 0 || which >= CFD_NFIELDS) return fail(s, CFD_EINVAL, "bad field argument");
+  if (int rc = one_band(s, "cfd_four_to_phys")) return rc;
   const int ny = ny_of(*s, which);
   const size_t zpl = (size_t)s->ngz * s->Mx, ppl = (size_t)s->ngz * s->ngx;
   HIPCHK(s, hipSetDevice(s->dev));
@@ -1738,6 +1927,7 @@ extern "C" int cfd_four_to_phys(cfd_handle h, int which, double* phys) {
 extern "C" int cfd_four_to_phys_planes(cfd_handle h, int which, int plane0, int nplanes, double* phys) {
   cfd_solver* s = static_cast<cfd_solver*>(h);
   if (!s || !phys || which < 0 || which >= CFD_NFIELDS) return fail(s, CFD_EINVAL, "bad field argument");
+  if (int rc = one_band(s, "cfd_four_to_phys_planes")) return rc;
   const int ny = ny_of(*s, which);
   if (plane0 < 0 || nplanes < 1 || plane0 + nplanes > ny)
     return fail(s, CFD_EINVAL, "planes [" + std::to_string(plane0) + ", " + std::to_string(plane0 + nplanes) +
@@ -1754,12 +1944,14 @@ extern "C" int cfd_four_to_phys_planes(cfd_handle h, int which, int plane0, int 
 extern "C" int cfd_stats(cfd_handle h) {
   cfd_solver* s = static_cast<cfd_solver*>(h);
   if (int rc = check(s, 1)) return rc;
+  if (int rc = one_band(s, "cfd_stats")) return rc;
   return stats_sample(s);
 }
 
 extern "C" int cfd_write_stats(cfd_handle h, const char* dirout, int iter) {
   cfd_solver* s = static_cast<cfd_solver*>(h);
   if (int rc = check(s, 1)) return rc;
+  if (int rc = one_band(s, "cfd_write_stats")) return rc;
   if (!dirout) return fail(s, CFD_EINVAL, "null dirout");
   DevScalars d{};
   if (int rc = read_scalars(s, d)) return rc;
@@ -1769,6 +1961,7 @@ extern "C" int cfd_write_stats(cfd_handle h, const char* dirout, int iter) {
 extern "C" int cfd_stats_request(cfd_handle h, int flags, const char* dirout, int iter) {
   cfd_solver* s = static_cast<cfd_solver*>(h);
   if (int rc = check(s, 1)) return rc;
+  if (int rc = one_band(s, "cfd_stats_request")) return rc;
   if (flags < 0 || flags > 3) return fail(s, CFD_EINVAL, "flags must be 0..3 (bit 0 sample, bit 1 write)");
   if ((flags & 2) && !dirout) return fail(s, CFD_EINVAL, "null dirout");
   if (flags)
@@ -1782,6 +1975,7 @@ extern "C" int cfd_stats_request(cfd_handle h, int flags, const char* dirout, in
 extern "C" int cfd_stats_dims(cfd_handle h, int* d) {
   cfd_solver* s = static_cast<cfd_solver*>(h);
   if (int rc = check(s, 1)) return rc;
+  if (int rc = one_band(s, "cfd_stats_dims")) return rc;
   if (!d) return fail(s, CFD_EINVAL, "null argument");
   if (int rc = stats_setup(s)) return rc;
   const auto& S = s->st;
@@ -1800,6 +1994,7 @@ extern "C" int cfd_stats_dims(cfd_handle h, int* d) {
 extern "C" int cfd_closure_attach(cfd_handle h, cfd_cnn c, int every) {
   cfd_solver* s = static_cast<cfd_solver*>(h);
   if (int rc = check(s, 1)) return rc;
+  if (int rc = one_band(s, "cfd_closure_attach")) return rc;
   if (!c) {
     s->cl.cnn = nullptr;
     s->cl.have = false;
@@ -1835,6 +2030,8 @@ extern "C" int cfd_closure_snapshot(cfd_handle hf, cfd_handle hc, cfd_cnn c, int
   cfd_solver* s = static_cast<cfd_solver*>(hc);
   if (int rc = check(f, 2)) return rc;
   if (int rc = check(s, 2)) return rc;
+  if (int rc = one_band(f, "cfd_closure_snapshot")) return rc;
+  if (int rc = one_band(s, "cfd_closure_snapshot")) return rc;
   if (!c || slot < 0 || slot >= c->nslots) return fail(s, CFD_EINVAL, "closure: bad network or slot");
   if (f->ny != s->ny || f->dsty != s->dsty || f->P.ygrid != s->P.ygrid || s->N1 > f->N1 || s->N2 > f->N2)
     return fail(s, CFD_EINVAL, "closure: the coarse solver must share the y grid and have fewer x/z modes");
@@ -1893,6 +2090,7 @@ extern "C" int cfd_closure_snapshot(cfd_handle hf, cfd_handle hc, cfd_cnn c, int
 extern "C" int cfd_closure_phys(cfd_handle h, int which, double* out) {
   cfd_solver* s = static_cast<cfd_solver*>(h);
   if (int rc = check(s, 1)) return rc;
+  if (int rc = one_band(s, "cfd_closure_phys")) return rc;
   if (!out || which < 0 || which > 5 || !s->cl.ready) return fail(s, CFD_EINVAL, "closure: nothing to read");
   const int g = which % 3, ny = g == 1 ? s->nyv : s->nyu;
   HIPCHK(s, hipStreamSynchronize(s->stream));
@@ -1959,6 +2157,7 @@ extern "C" int cfd_ensemble_create(cfd_handle* hs, int n, cfd_ensemble* out) {
     if (i && s->dev != e->dev) return bail(CFD_EINVAL, "ensemble: members on different devices");
     if (s->prof || s->cl.cnn || s->st.pending)
       return bail(CFD_EINVAL, "ensemble: a member has profiling, a closure or a statistics request");
+    if (s->mb) return bail(CFD_EINVAL, "ensemble: a member has a coarser central band");
     if (!s->nsub) return bail(CFD_EINVAL, "ensemble: every member must run one substep before capture");
     for (cfd_solver* q : e->m)
       if (q == s) return bail(CFD_EINVAL, "ensemble: a member appears twice");

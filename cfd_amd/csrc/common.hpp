@@ -166,6 +166,25 @@ struct Solver {
   int* d_ibseg[2] = {};
   int nibseg[2] = {0, 0};
 
+  // multi-band resolution (start.f90:89-111, 1420-1557): with mb the central
+  // y band is transformed on the coarser ngx2 x ngz2 grid and holds only the
+  // long columns (the N1c x N2c modes); the short columns live in the wall
+  // bands only, each band a closed system whose boundary rows sit at the
+  // band interface.  The central planes' nonlinear products come from mbc, a
+  // solver of the central band's resolution over the long columns that
+  // shares this solver's stream.
+  bool mb = false;
+  int ngx2 = 0, ngz2 = 0, N1c = 0, N2c = 0;
+  int N3[4] = {0, 0, 0, 0}, N4[4] = {0, 0, 0, 0};  // band limits N(3,0:3), N(4,0:3)
+  double gw12 = 0.0, gw31 = 0.0;  // gridweighting(1,2), (3,1): the interface boundary rows
+  Solver* mbc = nullptr;
+  bool borrowed_stream = false;      // a central solver runs on its parent's stream
+  int* d_ccol = nullptr;             // per column: its column in mbc, -1 for a short column
+  int* d_lcol = nullptr;             // per mbc column: the column here
+  unsigned char* d_short = nullptr;  // per column: 1 = short
+  unsigned char *d_cfu = nullptr, *d_cfv = nullptr;  // per u / v plane: 1 = in this solver's CFL maxima
+  std::vector<unsigned char> h_short;
+
   // profiling
   bool prof = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -242,6 +261,10 @@ void launch_pressure(Solver& s, int kRK);
 void launch_meanflow_lds(Solver& s, int kRK);
 void launch_mf_u11(Solver& s, int kRK, hipStream_t st);  // u11 and Q(u11) into d_u11
 void launch_mf_apply(Solver& s);                       // g from Q(u), u += g u11
+// multi-band resolution (kernels_band.hip)
+void launch_band_gather(Solver& s);   // long columns of u1, u2, u3 -> the central solver's fields
+void launch_band_scatter(Solver& s);  // central planes of the nonlinear scratch <- the central solver
+void launch_cfl_bands(Solver& s);     // dt from the wall bands' and the central band's maxima
 // no-allocation physical planes (noalloc.hip): the file-scope scratch's
 // device address is taken once per device (noalloc_init, device current);
 // four_to_phys_noalloc queues the planes [plane0, plane0 + nplanes) of field

@@ -95,7 +95,7 @@ Rec grid_rec(const std::vector<double>& y, double dtheta, const std::vector<doub
 
 std::string write_field_file(const char* path, const cfd_file_header& hd, int id, double dthetavi, const double* y,
                              const double* dthdy, int ny_arr, int jy0, int ny_g, int j0, int nx, int nz,
-                             const double* planes) {
+                             const double* planes, const BandPlanes& bp) {
   if (j0 < jy0 || j0 + ny_g > jy0 + ny_arr) return "plane range outside the grid array";
   File F;
   F.f = fopen(path, "wb");
@@ -107,21 +107,35 @@ std::string write_field_file(const char* path, const cfd_file_header& hd, int id
   r3.put(dthdy, (size_t)ny_arr);
   if (!write_rec(F.f, r1) || !write_rec(F.f, r2) || !write_rec(F.f, r3)) return std::string("write error on ") + path;
   const size_t pl = (size_t)nx * nz;
+  std::vector<double> cb;
   for (int q = 0; q < ny_g; ++q) {
     Rec r;
+    const bool cen = q >= bp.q0 && q < bp.q1;
     r.put((int32_t)(j0 + q));
     r.put((int32_t)id);
-    r.put((int32_t)nx);
-    r.put((int32_t)nz);
+    r.put((int32_t)(cen ? bp.nx : nx));
+    r.put((int32_t)(cen ? bp.nz : nz));
     r.put(y[j0 + q - jy0]);
-    r.put(planes + q * pl, pl);
+    if (cen) {
+      // the central band's modes of the plane: kx reals 1..bp.nx, the
+      // bp.nz/2 lowest z modes and the bp.nz/2 most negative ones kept
+      cb.assign((size_t)bp.nx * bp.nz, 0.0);
+      const double* u = planes + q * pl;
+      for (int kc = 0; kc < bp.nz; ++kc) {
+        const int k = kc < bp.nz / 2 ? kc : kc - bp.nz + nz;
+        for (int i = 0; i < bp.nx; ++i) cb[(size_t)kc * bp.nx + i] = u[(size_t)k * nx + i];
+      }
+      r.put(cb.data(), cb.size());
+    } else {
+      r.put(planes + q * pl, pl);
+    }
     if (!write_rec(F.f, r)) return std::string("write error on ") + path;
   }
   return "";
 }
 
 std::string read_field_file(const char* path, cfd_file_header& hd, int ny_g, int nx, int nz, int ngz, double* planes,
-                            int& mismatch) {
+                            int& mismatch, const BandPlanes& bp) {
   mismatch = 0;
   File F;
   F.f = fopen(path, "rb");
@@ -145,7 +159,8 @@ std::string read_field_file(const char* path, cfd_file_header& hd, int ny_g, int
     const int nxf = get<int32_t>(b, 8), nzf = get<int32_t>(b, 12);
     if (nxf < 0 || nzf < 0 || b.size() != 4 * 4 + 8 + (size_t)nxf * nzf * 8)
       return std::string("bad plane record in ") + path;
-    if (nxf != nx || nzf != nz) ++mismatch;
+    const bool cen = q >= bp.q0 && q < bp.q1;
+    if (nxf != (cen ? bp.nx : nx) || nzf != (cen ? bp.nz : nz)) ++mismatch;
     const size_t d0 = 4 * 4 + 8;
     double* u = planes + q * pl;
     // buff_to_u into a plane of ngz z-slots, then the slots that carry one of
@@ -206,6 +221,8 @@ std::string write_spectrum_file(const char* path, const cfd_file_header& hd, con
 void band_table(const cfd_params& p, const YGrid& g, int N[20]) {
   // N(1:4, 0:nband+1), nband = 3 (start.f90:89-111, y_grid_canopy.f90:250-275)
   const int N1 = 2 * (p.ngalx / 3), N2 = 2 * (p.ngalz / 3), dny = p.dsty, nn = p.ny - 2;
+  // the central band's modes (Ngal(1:2, 2), 0: as the wall bands')
+  const int N1c = p.ngalx2 > 0 ? 2 * (p.ngalx2 / 3) : N1, N2c = p.ngalz2 > 0 ? 2 * (p.ngalz2 / 3) : N2;
   // height of the fine bottom band (start.f90:199-226 for the geometry types built here)
   const double pi = 3.14159265358979323846;
   const double h = p.geometry_type == 3 ? 2.0 * (2.0 * pi / p.bet) / p.ntilez : 0.25;
@@ -213,15 +230,14 @@ void band_table(const cfd_params& p, const YGrid& g, int N[20]) {
   int j = -dny;
   while (j < g.jlo_u + g.nyu - 1 && g.YU(j) < target) ++j;
   const int N31 = j, N32 = nn - j, N33 = nn + dny;
-  const int tab[5][4] = {{0, 0, -dny, -dny}, {N1, N2, N31, N31 + 1}, {N1, N2, N32, N32}, {N1, N2, N33, N33 + 1},
+  const int tab[5][4] = {{0, 0, -dny, -dny}, {N1, N2, N31, N31 + 1}, {N1c, N2c, N32, N32}, {N1, N2, N33, N33 + 1},
                          {-2, 0, 0, 0}};
   for (int b = 0; b < 5; ++b)
     for (int r = 0; r < 4; ++r) N[b * 4 + r] = tab[b][r];
 }
 
 // The reference's plane partition over np ranks (start.f90:1652-1866),
-// restated for one resolution in every band (Ngal(1:2, iband) = Ngalx,
-// Ngalz): ranks per band from its load balance, then contiguous ranges of
+// each band's load from its own Ngal(1:2, iband): ranks per band from its load balance, then contiguous ranges of
 // planes per rank, the ranks with one plane more first.  lim[(g * 2 + e) *
 // np + r] is planelim(grid, e + 1, r) for g 0 the u grid (band limits N(4,
 // .)), 1 the v grid (N(3, .)); walls excluded.  False when np < 3 (the
@@ -236,7 +252,10 @@ bool plane_partition(const cfd_params& p, const int N[20], int np, int* lim) {
     if (!(NB(r, 0) < NB(r, 1) && NB(r, 1) < NB(r, 2) && NB(r, 2) < NB(r, 3))) return false;
   const int bot = 1, mid = 2, top = 3;
   double load[4] = {0, 0, 0, 0}, il[4] = {0, 0, 0, 0};
-  for (int b = 1; b <= 3; ++b) load[b] = (double)(p.ngalx + 2) * p.ngalz * (NB(4, b) - NB(4, b - 1));
+  for (int b = 1; b <= 3; ++b) {
+    const int gx = b == 2 && p.ngalx2 > 0 ? p.ngalx2 : p.ngalx, gz = b == 2 && p.ngalz2 > 0 ? p.ngalz2 : p.ngalz;
+    load[b] = (double)(gx + 2) * gz * (NB(4, b) - NB(4, b - 1));
+  }
   auto iceil = [](double x) { return (int)std::ceil(x); };
   int procs[4] = {0, 0, 0, 0};
   // ranks of the two fine bands with iplanes planes each, the rest to the

@@ -1,0 +1,113 @@
+// kernels_band.hip -- multi-band resolution: moving the long columns between
+// the solver (all columns, the wall bands' resolution) and the central
+// solver (long columns only, the central band's resolution), and the time
+// step from both resolutions' CFL maxima.
+//
+// The reference keeps one column array per band (x(sband:eband),
+// declaration.f90:140-150): band 2 holds the long columns over every plane,
+// bands 1 and 3 the short columns over their own planes, and the central
+// planes are transformed on the coarser Ngal(1:2,2) grid (modes_to_planes
+// with bandPL, FOU3D.f90:808-906).  Here the fields keep every column over
+// every plane; the central solver sees the long columns of u1, u2, u3 and
+// returns the x/z derivative sums and y fluxes of the central planes.
+#include "common.hpp"
+
+namespace cfd {
+
+namespace {
+
+constexpr int kTB = 256;
+
+// grid (ceil(Ncolc / kTB), nyu, 3): field f, plane q, central column c
+__global__ void k_band_gather(const double2* __restrict__ u1, const double2* __restrict__ u2,
+                              const double2* __restrict__ u3, double2* __restrict__ c1, double2* __restrict__ c2,
+                              double2* __restrict__ c3, const int* __restrict__ lcol, int Ncol, int Ncolc, int nyu,
+                              int nyv) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = blockIdx.y, f = blockIdx.z;
+  if (c >= Ncolc || q >= (f == 1 ? nyv : nyu)) return;
+  const double2* src = f == 0 ? u1 : (f == 1 ? u2 : u3);
+  double2* dst = f == 0 ? c1 : (f == 1 ? c2 : c3);
+  dst[(size_t)q * Ncolc + c] = src[(size_t)q * Ncol + lcol[c]];
+}
+
+struct ScatterArgs {
+  double2* out[6];        // nxz1, nxz3, fvv (u planes), nxz2, fuv, fvw (v planes)
+  const double2* in[6];   // the same arrays of the central solver
+  const int* ccol;        // per column: central column or -1
+  int Ncol, Ncolc;
+  int u0, nu, v0, nv;     // central u planes [u0, u0 + nu), v planes [v0, v0 + nv)
+};
+
+// grid (ceil(Ncol / kTB), max(nu, nv), 6): array a, central plane t
+__global__ void k_band_scatter(ScatterArgs a) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blockIdx.y, w = blockIdx.z;
+  const bool onu = w < 3;
+  if (col >= a.Ncol || t >= (onu ? a.nu : a.nv)) return;
+  const int q = (onu ? a.u0 : a.v0) + t;
+  const int c = a.ccol[col];
+  // a short column has no value on a central plane (its band arrays stop
+  // at the band interface): its fluxes there are zero
+  const double2 v = c >= 0 ? a.in[w][(size_t)q * a.Ncolc + c] : make_double2(0.0, 0.0);
+  a.out[w][(size_t)q * a.Ncol + col] = v;
+}
+
+// dtc_calc (FOU3D.f90:516-544) on every rank with its band's mode counts and
+// the MIN over the ranks (FOU3D.f90:177-183): the wall bands' maxima with N1,
+// N2, the central band's with N1c, N2c; the v limit has no mode count
+__global__ void k_cfl_bands(DevScalars* sc, const DevScalars* cc, double alp, double bet, int N1h, int N2h, int N1ch,
+                            int N2ch, double dtv, double CFL) {
+  const double m1 = __longlong_as_double((long long)sc->m1);
+  const double m3 = __longlong_as_double((long long)sc->m3);
+  const double c1 = __longlong_as_double((long long)cc->m1);
+  const double c3 = __longlong_as_double((long long)cc->m3);
+  const double m2 = fmax(1e-6, fmax(__longlong_as_double((long long)sc->m2), __longlong_as_double((long long)cc->m2)));
+  const double d1 = 1.0 / (alp * N1h * m1), d3 = 1.0 / (bet * N2h * m3), d2 = 1.0 / m2;
+  double dtc = d1 < d3 ? d1 : d3;
+  dtc = dtc < d2 ? dtc : d2;
+  const double e1 = 1.0 / (alp * N1ch * c1), e3 = 1.0 / (bet * N2ch * c3);
+  const double dc = e1 < e3 ? e1 : e3;
+  dtc = dtc < dc ? dtc : dc;
+  sc->dtc = dtc;
+  sc->dt = (2.5 * dtv < CFL * dtc) ? 2.5 * dtv : CFL * dtc;
+  sc->dti = 1.0 / sc->dt;
+  sc->t = sc->t + sc->dt;
+}
+
+}  // namespace
+
+void launch_band_gather(Solver& s) {
+  Solver& c = *s.mbc;
+  const dim3 grid((c.Ncol + kTB - 1) / kTB, s.nyu, 3);
+  hipLaunchKernelGGL(k_band_gather, grid, dim3(kTB), 0, s.stream, s.F[CFD_U1], s.F[CFD_U2], s.F[CFD_U3], c.F[CFD_U1],
+                     c.F[CFD_U2], c.F[CFD_U3], (const int*)s.d_lcol, s.Ncol, c.Ncol, s.nyu, s.nyv);
+}
+
+void launch_band_scatter(Solver& s) {
+  Solver& c = *s.mbc;
+  ScatterArgs a;
+  void* out[6] = {s.nxz1, s.nxz3, s.fvv, s.nxz2, s.fuv, s.fvw};
+  void* in[6] = {c.nxz1, c.nxz3, c.fvv, c.nxz2, c.fuv, c.fvw};
+  for (int w = 0; w < 6; ++w) {
+    a.out[w] = static_cast<double2*>(out[w]);
+    a.in[w] = static_cast<const double2*>(in[w]);
+  }
+  a.ccol = s.d_ccol;
+  a.Ncol = s.Ncol;
+  a.Ncolc = c.Ncol;
+  // central planes: u N(4,1)+1 .. N(4,2), v N(3,1)+1 .. N(3,2)
+  a.u0 = s.N4[1] + 1 - s.jlo_u;
+  a.nu = s.N4[2] - s.N4[1];
+  a.v0 = s.N3[1] + 1 - s.jlo_v;
+  a.nv = s.N3[2] - s.N3[1];
+  const dim3 grid((s.Ncol + kTB - 1) / kTB, a.nu > a.nv ? a.nu : a.nv, 6);
+  hipLaunchKernelGGL(k_band_scatter, grid, dim3(kTB), 0, s.stream, a);
+}
+
+void launch_cfl_bands(Solver& s) {
+  hipLaunchKernelGGL(k_cfl_bands, dim3(1), dim3(1), 0, s.stream, s.d_sc, (const DevScalars*)s.mbc->d_sc, s.P.alp,
+                     s.P.bet, s.N1 / 2, s.N2 / 2, s.N1c / 2, s.N2c / 2, s.dtv, s.P.CFL);
+}
+
+}  // namespace cfd

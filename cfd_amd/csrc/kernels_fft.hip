@@ -165,6 +165,9 @@ struct XNLArgs {
   int nyu, nyv, Mx, ngz, cfl;
   int zs;   // Z row stride (elements)
   int top;  // also the products of the top ghost u plane (the last-plane replay reads them)
+  // per u / v plane: 1 when the plane's maxima count in this solver's CFL
+  // (multi-band resolution: each resolution takes its own planes); null: all
+  const unsigned char *cfu, *cfv;
 };
 
 template <int LC>
@@ -211,6 +214,8 @@ __global__ void __launch_bounds__(kThreads) k_xpass_nl(XNLArgs a, FftPlan P, con
     m3 = fmax(m3, dabs(ww));
     m2 = fmax(m2, dabs(vf) * dv);
   }
+  if (a.cfu && !a.cfu[q]) m1 = m3 = 0.0;
+  if (a.cfv && !(hasv && a.cfv[q])) m2 = 0.0;
   if (a.cfl) {
     m1 = block_max(m1, red);
     if (threadIdx.x == 0) atomic_max_pos(&a.sc->m1, m1);
@@ -805,6 +810,8 @@ __global__ void __launch_bounds__(3 * xp_width(P, Q, R)) k_xpass4(XNLArgs a, con
     double da = ma, db = mb;
     if (!act || z0 + r >= ngz) da = db = 0.0;
     if (tr == 1) db *= vint ? a.dthdyv[q] : 0.0;
+    if (tr == 0 && a.cfu && !a.cfu[q]) da = db = 0.0;
+    if (tr == 1 && a.cfv && !(hasv && a.cfv[q])) db = 0.0;
     da = wave_max(da);
     db = wave_max(db);
     if ((lane & 63) == 0) {
@@ -1111,6 +1118,8 @@ void launch_xpass_nl(Solver& s, int cfl) {
   a.cfl = cfl;
   a.zs = s.zs;
   a.top = s.compat_np > 0;
+  a.cfu = s.d_cfu;
+  a.cfv = s.d_cfv;
   if (try_xpass4(s, a)) return;
   switch (s.px.L) {
 #define C(n) case n: xpass_nl_t<n>(s, a); break;

@@ -42,6 +42,13 @@ struct FusedArgs {
   double C1, C2, C1g, dRK, bRK, Re, ddthetavi, g0, g1;
   int Mx, Ncol, nyu, nyv;
   int ib_only;  // 1: k_ib_rhs mode
+  // multi-band resolution: per column 1 = short (null: none); a short
+  // column's rows t1..b3 of field f (plane indices of its grid) are its
+  // band-1 top boundary row, rows between its bands and its band-3 bottom
+  // boundary row; gw12 / gw31 their u-grid weights (start.f90:494-500)
+  const unsigned char* shortc;
+  int t1[3], b3[3];
+  double gw12, gw31;
 };
 
 // pointers of one field, selected without indexing the kernel-argument
@@ -248,6 +255,10 @@ struct RowCtx {
   const unsigned char* ibl;  // LDS: 1 on the IB rows
   int n, col;
   size_t N;
+  // a short column: rows t1..b3 are its band ends and the rows between
+  // (t1 > n: a long column); gt1 / gb3 the band-end rows' coupling weights
+  int t1, b3;
+  double gt1, gb3;
 };
 
 // The same row in two halves, so that the loads of row q+1 can be issued
@@ -278,7 +289,15 @@ __device__ __forceinline__ RowSys seg_build(const FusedArgs& a, const FieldPtrs&
     s.lo = 0.0; s.dg = 1.0; s.up = 0.0; s.r = make_double2(0.0, 0.0);
     return s;
   }
-  if (q == 0) {
+  if (q >= c.t1 && q <= c.b3) {
+    // a short column: its band-1 top boundary row [gw(1,2), 1, 0], the rows
+    // between its bands (identity, zero) and its band-3 bottom boundary row
+    // [0, 1, gw(3,1)] (LU_buildU/V per band, tridLU_3D_IB.f90:349-413)
+    s.lo = q == c.t1 ? c.gt1 : 0.0;
+    s.dg = 1.0;
+    s.up = q == c.b3 ? c.gb3 : 0.0;
+    s.r = make_double2(0.0, 0.0);
+  } else if (q == 0) {
     s.lo = 0.0; s.dg = 1.0; s.up = c.g0; s.r = make_double2(0.0, 0.0);
   } else if (q == c.n - 1) {
     s.lo = c.g1; s.dg = 1.0; s.up = 0.0; s.r = make_double2(0.0, 0.0);
@@ -363,6 +382,11 @@ __global__ void __launch_bounds__(256) k_rhs_seg(FusedArgs a) {
     c.n = n;
     c.col = col;
     c.N = N;
+    const bool shc = a.shortc && a.shortc[col];
+    c.t1 = shc ? (f == 0 ? a.t1[0] : (f == 1 ? a.t1[1] : a.t1[2])) : n + 1;
+    c.b3 = shc ? (f == 0 ? a.b3[0] : (f == 1 ? a.b3[1] : a.b3[2])) : n + 1;
+    c.gt1 = (f == 1) ? 0.0 : a.gw12;
+    c.gb3 = (f == 1) ? 0.0 : a.gw31;
     double2 um = b >= 1 ? u[(size_t)(b - 1) * N + col] : make_double2(0.0, 0.0);
     double2 u0 = u[(size_t)b * N + col];
     // forward elimination inside the segment
@@ -489,6 +513,10 @@ struct PressArgs {
   const DevScalars* sc;
   double apb, ddthetavi, pin, g0, g1;
   int Mx, Ncol, nyp;
+  // multi-band resolution: per column 1 = short (null: none); a short
+  // column's band 1 ends at p row P1 and its band 3 starts at p row P3
+  const unsigned char* shortc;
+  int P1, P3;
 };
 
 __global__ void __launch_bounds__(64) k_pressure(PressArgs a) {
@@ -661,6 +689,11 @@ __global__ void __launch_bounds__(256) k_press_seg(PressArgs a) {
   const size_t N = a.Ncol;
   const int b = sg * (SEG + 1);
   const int i = col % a.Mx, k = col / a.Mx;
+  const bool shc = act && a.shortc && a.shortc[col];
+  // v faces a short column keeps at zero: its band interfaces
+  auto iface = [&](int face) { return shc && (face == a.P1 + 1 || face == a.P3); };
+  // p rows between a short column's bands: no divergence there
+  auto gap = [&](int q) { return shc && q > a.P1 && q < a.P3; };
   double kx = 0.0, kz = 0.0;
   double2 y[SEG];
   double g[SEG], h[SEG];
@@ -687,8 +720,9 @@ __global__ void __launch_bounds__(256) k_press_seg(PressArgs a) {
       return v;
     };
     auto row = [&](int q, const Vel& v, double& lo, double& dg, double& up, double2& rhs) {
-      if (q >= n) {
+      if (q >= n || (shc && q > a.P1 && q < a.P3)) {
         lo = 0.0; dg = 1.0; up = 0.0; rhs = make_double2(0.0, 0.0);
+        if (q < n) vq = v.V1;  // the face chain goes on through the rows between the bands
         return;
       }
       const double2 U = v.U, W = v.W, V1 = v.V1;
@@ -705,6 +739,16 @@ __global__ void __launch_bounds__(256) k_press_seg(PressArgs a) {
         lo = 0.0;
         dg = a.pin;
         up = 0.0;
+      }
+      // a short column's band ends drop the flux through the interface face
+      // (LU_buildP per band, tridLU_3D_IB.f90:417-491)
+      if (shc && q == a.P1) {
+        dg = -(a.ddthetavi * ldu[q + 1]) * (a.ddthetavi * ldv[q]) + k2;
+        up = 0.0;
+      }
+      if (shc && q == a.P3) {
+        dg = -(a.ddthetavi * ldu[q + 1]) * (a.ddthetavi * ldv[q + 1]) + k2;
+        lo = 0.0;
       }
     };
     double2 yp = make_double2(0.0, 0.0);
@@ -802,8 +846,8 @@ __global__ void __launch_bounds__(256) k_press_seg(PressArgs a) {
       a.u1[(size_t)(nyu - 1) * N + col] = make_double2(-a.g1 * un.x, -a.g1 * un.y);
       a.u3[(size_t)(nyu - 1) * N + col] = make_double2(-a.g1 * wn.x, -a.g1 * wn.y);
     }
-    double2 vf = make_double2(0.0, 0.0);  // bottom wall face
-    if (q > 0) {
+    double2 vf = make_double2(0.0, 0.0);  // bottom wall face, a short column's interface faces
+    if (q > 0 && !iface(q)) {
       const double2 V = a.u2[(size_t)q * N + col];
       const double cc = dtrk * a.ddthetavi * ldv[q];
       vf = make_double2(V.x - cc * (ps.x - psm.x), V.y - cc * (ps.y - psm.y));
@@ -813,7 +857,8 @@ __global__ void __launch_bounds__(256) k_press_seg(PressArgs a) {
       const double2 du = imulk(kx, unp), dw = imulk(kz, wnp);
       const double m = a.ddthetavi * ldu[q];
       a.div[(size_t)(q - 1) * N + col] =
-          make_double2(du.x + dw.x + m * (vf.x - vprev.x), du.y + dw.y + m * (vf.y - vprev.y));
+          gap(q - 1) ? make_double2(0.0, 0.0)
+                     : make_double2(du.x + dw.x + m * (vf.x - vprev.x), du.y + dw.y + m * (vf.y - vprev.y));
     }
     unp = un;
     wnp = wn;
@@ -837,6 +882,8 @@ __global__ void __launch_bounds__(256) k_press_seg(PressArgs a) {
   double2 vf = make_double2(0.0, 0.0);
   if (last == n - 1) {
     a.u2[(size_t)n * N + col] = vf;
+  } else if (iface(last + 1)) {
+    // the next segment writes this face (zero)
   } else {
     const int B = me + CB;
     const double2 xr2 = sg + 1 < S - 1 ? Xs[B] : make_double2(0.0, 0.0);
@@ -846,7 +893,9 @@ __global__ void __launch_bounds__(256) k_press_seg(PressArgs a) {
   }
   const double2 du = imulk(kx, unp), dw = imulk(kz, wnp);
   const double m = a.ddthetavi * ldu[last + 1];
-  a.div[(size_t)last * N + col] = make_double2(du.x + dw.x + m * (vf.x - vprev.x), du.y + dw.y + m * (vf.y - vprev.y));
+  a.div[(size_t)last * N + col] = gap(last) ? make_double2(0.0, 0.0)
+                                            : make_double2(du.x + dw.x + m * (vf.x - vprev.x),
+                                                           du.y + dw.y + m * (vf.y - vprev.y));
 }
 
 // (0,0) flow-rate correction staged through LDS: the column, metrics and
@@ -1139,6 +1188,15 @@ void fused_args(Solver& s, int kRK, void* out) {
   a.nyu = s.nyu;
   a.nyv = s.nyv;
   a.ib_only = 0;
+  a.shortc = s.mb ? s.d_short : nullptr;
+  // short columns' band ends in plane indices: u rows N(4,1)+1 and N(4,2),
+  // v rows N(3,1)+1 and N(3,2)
+  a.t1[0] = a.t1[2] = s.N4[1] + 1 - s.jlo_u;
+  a.b3[0] = a.b3[2] = s.N4[2] - s.jlo_u;
+  a.t1[1] = s.N3[1] + 1 - s.jlo_v;
+  a.b3[1] = s.N3[2] - s.jlo_v;
+  a.gw12 = s.gw12;
+  a.gw31 = s.gw31;
 }
 
 void launch_ib_rhs(Solver& s, int kRK) {
@@ -1221,6 +1279,9 @@ void launch_pressure(Solver& s, int kRK) {
   a.Mx = s.Mx;
   a.Ncol = s.Ncol;
   a.nyp = s.nyp;
+  a.shortc = s.mb ? s.d_short : nullptr;
+  a.P1 = s.N4[1] - s.jlo_p;      // p row N(4,1): the end of band 1
+  a.P3 = s.N4[2] + 1 - s.jlo_p;  // p row N(4,2)+1: the start of band 3
   const int R = s.seg_rows;
   const int sy = (s.nyp + R + 1) / (R + 1);  // S of k_press_seg
   if (sy <= 256) {

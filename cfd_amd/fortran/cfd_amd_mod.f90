@@ -14,6 +14,7 @@ module cfd_amd_mod
   type, bind(C) :: cfd_params
     integer(c_int) :: ngalx, ngalz, ny, dsty, dsty1, ppp, ygrid, geometry_type
     integer(c_int) :: ntilex, ntilez, extras24, flag_ctpress, precision, device
+    integer(c_int) :: ngalx2, ngalz2
     real(c_double) :: Re, alp, bet, dyq, posth, posth1, ytanh_beta, CFL, mpgx
     real(c_double) :: r, c_i, c_k, stag
   end type cfd_params

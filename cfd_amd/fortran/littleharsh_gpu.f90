@@ -175,10 +175,9 @@ program littleharsh_gpu
     end select
   end do
   close(40)
-  if (Ngalx2 /= p%ngalx .or. Ngalz2 /= p%ngalz) then
-    write(*,*) 'one spectral resolution in all bands: Ngalx1 = Ngalx2, Ngalz1 = Ngalz2'
-    error stop 1
-  end if
+  ! the central band's resolution (multi-band when coarser than band 1)
+  p%ngalx2 = Ngalx2
+  p%ngalz2 = Ngalz2
   if (p%geometry_type == 3) then
     open(69, file='geom_circ_rough.txt', form='formatted', status='old')
     read(69,10) p%r
@@ -197,6 +196,7 @@ program littleharsh_gpu
     write(*,'(a,7es24.16)') ' reals', p%Re, p%alp, p%bet, p%dyq, p%posth, p%CFL, p%mpgx
     write(*,'(a,4es24.16)') ' post', p%r, p%c_i, p%c_k, p%stag
     write(*,'(a,4i4,es24.16)') ' ext', p%ygrid, p%extras24, p%precision, fused, p%ytanh_beta
+    write(*,'(a,2i8)') ' band2', p%ngalx2, p%ngalz2
     stop
   end if
   rc = cfd_create(p, h);               call cfd_check(rc, h, 'cfd_create')

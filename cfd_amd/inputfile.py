@@ -80,9 +80,9 @@ def input_text(p, flag_init=3, fnameimb="none", dirin="./", dirout="./", nwrite=
          _r("alp", p.alp, "Lx = 2*pi/alp"),
          _r("bet", p.bet, "Lz = 2*pi/bet"),
          _i("Ngalx1", p.ngalx, "collocation points in x, band 1"),
-         _i("Ngalx2", p.ngalx, "collocation points in x, band 2"),
+         _i("Ngalx2", p.ngalx2 or p.ngalx, "collocation points in x, band 2"),
          _i("Ngalz1", p.ngalz, "collocation points in z, band 1"),
-         _i("Ngalz2", p.ngalz, "collocation points in z, band 2"),
+         _i("Ngalz2", p.ngalz2 or p.ngalz, "collocation points in z, band 2"),
          _i("ny", p.ny, "grid points in y, y(0)=-1, y(ny-1)=1"),
          _r("dyq", p.dyq, "dymax/dymin of the polynomial map"),
          _i("pp", p.ppp, "y polynomial exponent"),

@@ -35,8 +35,12 @@ typedef struct cfd_solver* cfd_handle;
 
 /* Parameters of input.in (start.f90:62-227) and geom_circ_rough.txt
    (geom_circ_rough_mixed_forcing_24pt.f90:113-123).  Integers first, then
-   doubles.  One spectral resolution in all y-bands: Ngalx1 == Ngalx2 and
-   Ngalz1 == Ngalz2. */
+   doubles.  ngalx/ngalz are the wall bands' resolution (Ngalx1, Ngalz1);
+   ngalx2/ngalz2 the central band's (Ngalx2, Ngalz2, input.in:16, 18), 0 for
+   the same resolution in every band.  With a coarser central band (multi-
+   band resolution, start.f90:89-111, 1420-1557) the fields keep every
+   column of the wall bands over every plane (see cfd_set_field); the path
+   is FP64 (precision 0) and runs through cfd_substep / cfd_step. */
 typedef struct cfd_params {
   int ngalx, ngalz;     /* physical grid points in x and z (Ngalx1, Ngalz1)      */
   int ny;               /* y points tip to tip, y(0)=-1 .. y(ny-1)=1 (input.in:19) */
@@ -51,6 +55,7 @@ typedef struct cfd_params {
                         /* products and nonlinear scratch, FP64 fields, solves, */
                         /* projection and flow rate (four-step FFT lengths only) */
   int device;           /* HIP device ordinal                                     */
+  int ngalx2, ngalz2;   /* central band's x and z points (0: as ngalx, ngalz)   */
   double Re, alp, bet;  /* bulk Reynolds number; Lx = 2 pi/alp, Lz = 2 pi/bet     */
   double dyq;           /* dymax/dymin of the polynomial map                      */
   double posth, posth1; /* canopy depth (outer units); turning point             */
@@ -108,12 +113,22 @@ int cfd_get_ygrid(cfd_handle h, double* yu, double* yv, double* dthdyu,
    Lap, w2, w3.  *n receives the count; l9/w3 may be NULL to query it. */
 int cfd_get_iblist(cfd_handle h, int grid, int wall, int* l9, double* w3, int* n);
 
+/* Fields as (Re, Im) pairs f(plane, column), the plane index fastest:
+   src[2 * (c * ny_g + q)], column c = k * Mx + i (kx index i, kz index k in
+   column order), plane q = j - jlo of the field's grid (u/w, v or p).  With
+   a coarser central band a short column (i > N1c/2, or kz outside the
+   central band's N2c modes; proc_lims_columns, start.f90:1424-1501) has no
+   rows between its two wall bands: u/w planes N(4,1)+2 .. N(4,2)-1, v planes
+   N(3,1)+2 .. N(3,2)-1, p planes N(4,1)+1 .. N(4,2) (the band table of
+   start.f90:1420-1557) are read as zero and returned as zero there. */
 int cfd_set_field(cfd_handle h, int which, const double* src);
 int cfd_get_field(cfd_handle h, int which, double* dst);
 int cfd_set_scalars(cfd_handle h, const double* v);
 int cfd_get_scalars(cfd_handle h, double* v);
 
-/* Hot-path operators.  kRK = 1, 2, 3. */
+/* Hot-path operators.  kRK = 1, 2, 3.  One at a time they exist for one
+   resolution in every band; with a coarser central band only cfd_substep
+   and cfd_step run (the others return CFD_EINVAL). */
 int cfd_rhs0(cfd_handle h, int kRK);             /* RHS0_u1/u2/u3   littleharsh.f90:613-730 */
 int cfd_nonlinear(cfd_handle h, int kRK);        /* nonlinear       FOU3D.f90:41-259        */
 int cfd_solve_u(cfd_handle h, int comp, int kRK);/* solveU(u1|u3)   littleharsh.f90:732-762 */

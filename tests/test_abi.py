@@ -25,9 +25,9 @@ def test_exports_every_declared_symbol():
 
 
 def test_params_struct_layout():
-    # 14 ints then 13 doubles, as cfd_params declares them
-    assert ctypes.sizeof(_lib.CParams) == 14 * 4 + 13 * 8
-    assert _lib.CParams.Re.offset == 56
+    # 16 ints then 13 doubles, as cfd_params declares them
+    assert ctypes.sizeof(_lib.CParams) == 16 * 4 + 13 * 8
+    assert _lib.CParams.Re.offset == 64
 
 
 def test_host_setup_errors_are_reported():
