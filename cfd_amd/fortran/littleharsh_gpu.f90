@@ -30,6 +30,8 @@
 !   fused =   1   1: one cfd_step per time step (fused column sweeps);
 !                 0: the reference's call sequence, one entry point per
 !                 subroutine (littleharsh.f90:165-196)
+! Ngalx2/Ngalz2 below Ngalx1/Ngalz1 (multi-band resolution) run the FP64
+! path with fused = 1 and without statistics.
 ! The time loop's own steps (those without a field record) are timed and the
 ! rate printed as " loop: <steps> steps <seconds> s <substeps/s> substeps/s".
 program littleharsh_gpu
@@ -175,9 +177,14 @@ program littleharsh_gpu
     end select
   end do
   close(40)
-  ! the central band's resolution (multi-band when coarser than band 1)
+  ! the central band's resolution (multi-band when coarser than band 1):
+  ! stepped by cfd_step only
   p%ngalx2 = Ngalx2
   p%ngalz2 = Ngalz2
+  if ((Ngalx2 /= p%ngalx .or. Ngalz2 /= p%ngalz) .and. fused /= 1) then
+    write(*,*) 'a coarser central band (Ngalx2, Ngalz2) runs with fused = 1 only'
+    error stop 1
+  end if
   if (p%geometry_type == 3) then
     open(69, file='geom_circ_rough.txt', form='formatted', status='old')
     read(69,10) p%r
@@ -279,7 +286,7 @@ program littleharsh_gpu
   ! the conditional statistics need wall bands (band limits in order)
   rc = cfd_stats_dims(h, dst)
   if (rc /= 0) then
-    write(*,*) 'statistics off: the band table has no wall bands for this grid'
+    write(*,*) 'statistics off: no wall bands in the band table, or a coarser central band'
     nstat = huge(1)
   end if
   rc = cfd_get_scalars(h, sc);         call cfd_check(rc, h, 'get scalars')

@@ -363,3 +363,54 @@ def test_fortran_driver_rt395_384x192x192(tmp_path, precision):
         a = (outd / (name + last[2:])).read_bytes()
         b = (pyd / (name + last[2:])).read_bytes()
         assert a == b, name
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran driver not built")
+def test_fortran_driver_multiband_shipped_grid(tmp_path):
+    """The grid of the reference's shipped input.in with its coarser central
+    band (Ngalx 24/12, Ngalz 12/6, ny 25, alp 128, bet 256, constant pressure
+    gradient; a post that fits its tiles), started from field files of a
+    seeded projected field (flag_init = 1; the central planes stored at the
+    central band's size): after 3 time steps the driver's field files equal,
+    byte for byte, those of the same run driven from Python."""
+    from cfd_amd import ic, inputfile
+    from oracle.oracle import Oracle
+    from .test_gpu_multiband import SHIPPED_GRID, mask_bands
+    p = inputfile.representable(SHIPPED_GRID)
+    ind, outd, pyd = tmp_path / "in", tmp_path / "out", tmp_path / "py"
+    for d in (ind, outd, pyd):
+        d.mkdir()
+    print("case multi-band shipped grid driver", flush=True)
+    ch = Channel(p)
+    g = ch.ygrid()
+    f = ic.initial_fields(p, g, seed=1, amp=0.1)
+    f = ic.project(Oracle(p), {k: mask_bands(p, k, v) for k, v in f.items()})
+    for k in ("u1", "u2", "u3", "p"):
+        ch.set_field(k, f[k])
+    ch.set_scalars(t=0.0, dt=0.0, mpgx=p.mpgx, QxT=ic.flowrate(f["u1"], g, p.ny))
+    fn = os.path.basename(ch.record_out(str(ind) + "/", 0))
+    ch.close()
+    inputfile.write(str(tmp_path), p, flag_init=1, fnameimb=fn, dirin="./in/", dirout="./out/", nwrite=1000)
+    out = subprocess.run([EXE, "3"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    py = Channel(p)
+    py.read_in(str(ind) + "/", fn, 1)
+    py.step(3)
+    last = os.path.basename(py.record_out(str(pyd) + "/", 3))
+    py.close()
+    for name in ("u1", "u2", "u3", "p"):
+        a = (outd / (name + last[2:])).read_bytes()
+        b = (pyd / (name + last[2:])).read_bytes()
+        assert a == b, name
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran driver not built")
+def test_fortran_driver_multiband_refuses_operator_sequence(tmp_path):
+    """fused = 0 (one entry point per subroutine) has no multi-band path: the
+    driver stops with a message before creating a solver."""
+    from cfd_amd import inputfile
+    from .test_gpu_multiband import SHIPPED_GRID
+    inputfile.write(str(tmp_path), SHIPPED_GRID, flag_init=3, fused=0)
+    out = subprocess.run([EXE, "1"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert out.returncode != 0
+    assert "fused = 1 only" in out.stdout + out.stderr
