@@ -477,6 +477,7 @@ int create_impl(const cfd_params* pp, const cfd_tuning* tp, cfd_handle* out) {
     }
     s->n_nljobs = (int)J.size();
     if ((rc = upload(s, &s->d_nljobs, J))) return bail(rc);
+    s->h_nljobs = J;
     for (const ZInvJob& z : J) {
       if (z.dst == s->Z[1]) s->nl_u2i.push_back(z);
       if (z.dst == s->Z[5]) s->nl_u3i.push_back(z);
@@ -668,6 +669,28 @@ int setup_bands(cfd_solver* s) {
     fv[q] = !cv[q];
   }
   int rc;
+  // the planes whose products all come from the central solver drop out of
+  // this solver's transforms: x-pass planes holding a central u and a central
+  // v plane (j = N(3,1)+2 .. N(3,2); plane j = N(3,1)+1 keeps its fine u
+  // products), the z-forward's central u and v planes, and the z-inverse
+  // jobs of the x-pass planes left out
+  {
+    const int x0 = std::max(s->N4[1] + 1 - s->jlo_u, s->N3[1] + 1 - s->jlo_v);
+    const int x1 = std::min(s->N4[2] + 1 - s->jlo_u, s->N3[2] + 1 - s->jlo_v);
+    if (x1 > x0) {
+      s->nl_skip[0] = x0;
+      s->nl_skip[1] = x1;
+    }
+    s->nl_skip[2] = s->N4[1] + 1 - s->jlo_u;
+    s->nl_skip[3] = s->N4[2] + 1 - s->jlo_u;
+    s->nl_skip[4] = s->N3[1] + 1 - s->jlo_v;
+    s->nl_skip[5] = s->N3[2] + 1 - s->jlo_v;
+    std::vector<ZInvJob> J;
+    for (const ZInvJob& z : s->h_nljobs)
+      if (z.pd < s->nl_skip[0] || z.pd >= s->nl_skip[1]) J.push_back(z);
+    s->n_nljobs = (int)J.size();
+    if ((rc = upload(s, &s->d_nljobs, J))) return rc;
+  }
   if ((rc = upload(s, &s->d_short, s->h_short)) || (rc = upload(s, &s->d_ccol, ccol)) ||
       (rc = upload(s, &s->d_lcol, lcol)) || (rc = upload(s, &s->d_cfu, fu)) || (rc = upload(s, &s->d_cfv, fv)))
     return rc;

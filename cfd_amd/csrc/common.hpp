@@ -184,6 +184,11 @@ struct Solver {
   unsigned char* d_short = nullptr;  // per column: 1 = short
   unsigned char *d_cfu = nullptr, *d_cfv = nullptr;  // per u / v plane: 1 = in this solver's CFL maxima
   std::vector<unsigned char> h_short;
+  // planes the nonlinear transforms leave out (multi-band: the central
+  // band's): [0..1] x-pass planes, [2..3] z-forward u planes, [4..5] z-forward
+  // v planes; equal pairs leave out none
+  int nl_skip[6] = {0, 0, 0, 0, 0, 0};
+  std::vector<ZInvJob> h_nljobs;  // the z-inverse jobs of the nonlinear term (host copy)
 
   // profiling
   bool prof = false;
