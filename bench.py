@@ -291,6 +291,24 @@ def fortran_leg(params, steps, fine_value):
         shutil.rmtree(d, ignore_errors=True)
 
 
+def multiband_leg(args, f, q0, fine_value, nprof, tuning):
+    """The 384 x 192 x 192 case with a 192 x 96 central band (multi-band
+    resolution), FP64, the launch shapes of the one-resolution run, from the
+    same projected field (the short columns' rows between their bands
+    dropped)."""
+    from cfd_amd import fieldfile, workloads
+
+    pmb, _ = workloads.get("rt395_mb_384x192x192_c192x96")
+    ch, v, ms, prof, st = timed(pmb, f, q0, args, nprof, tuning)
+    ch.close()
+    N = fieldfile.band_table(pmb)
+    return {"workload": "rt395_mb_384x192x192_c192x96", "value": v, "unit": "substeps/s", "ms_per_step": ms,
+            "speedup_vs_one_resolution": v / fine_value, "central_u_planes": N[11] - N[7], "u_planes": pmb.nyu,
+            "central_grid": [pmb.ngalx2, pmb.ngalz2],
+            "kernel_ms_per_substep": {c: t / nprof for c, (t, cnt) in prof.items()},
+            "finite": bool(np.isfinite(st["u1"]).all()), "dt": st["scalars"]["dt"]}
+
+
 def ensemble_leg(fine_value, fine_points):
     """16 uneven channels, tuned, 50 RK3 steps (the first eager, 49 by the
     captured graphs)."""
@@ -388,6 +406,7 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-closure leg")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 16-channel ensemble leg")
     ap.add_argument("--no-fortran", action="store_true", help="skip the Fortran driver leg")
+    ap.add_argument("--no-multiband", action="store_true", help="skip the coarser-central-band leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy 3 times and print its bandwidth (the reference "
                          "kernel for scaling HBM byte counters)")
@@ -440,6 +459,9 @@ def main():
     learned = None if args.no_learned else learned_leg(args, value, sc["dt"])
     ens = None if args.no_ensemble else ensemble_leg(value, params.ngalx * params.ny * params.ngalz)
     fortran = None if args.no_fortran else fortran_leg(params, 20, value)
+    multiband = None
+    if not args.no_multiband and args.workload == "rt395_384x192x192":
+        multiband = multiband_leg(args, f, q0, value, nprof, tun64)
 
     cpu = cpu_baseline(args, params, f, q0) if args.cpu_substeps > 0 else None
 
@@ -468,6 +490,7 @@ def main():
         "learned": learned,
         "ensemble": ens,
         "fortran_driver": fortran,
+        "multiband": multiband,
         "cpu_baseline": cpu,
         "sim": {"t": sc["t"], "dt": sc["dt"], "mpgx": sc["mpgx"], "finite": finite},
     }

@@ -299,6 +299,8 @@ int create_impl(const cfd_params* pp, const cfd_tuning* tp, cfd_handle* out) {
   s->nyu = s->yg.nyu;
   s->nyv = s->yg.nyv;
   s->nyp = s->nyu - 2;
+  s->nl_x[1] = s->nl_u[1] = s->nyu;  // every plane through the nonlinear transforms
+  s->nl_v[1] = s->nyv;
   s->jlo_u = s->yg.jlo_u;
   s->jlo_v = s->yg.jlo_v;
   s->jlo_p = s->jlo_u + 1;
@@ -669,28 +671,38 @@ int setup_bands(cfd_solver* s) {
     fv[q] = !cv[q];
   }
   int rc;
-  // the planes whose products all come from the central solver drop out of
-  // this solver's transforms: x-pass planes holding a central u and a central
-  // v plane (j = N(3,1)+2 .. N(3,2); plane j = N(3,1)+1 keeps its fine u
-  // products), the z-forward's central u and v planes, and the z-inverse
-  // jobs of the x-pass planes left out
-  {
-    const int x0 = std::max(s->N4[1] + 1 - s->jlo_u, s->N3[1] + 1 - s->jlo_v);
-    const int x1 = std::min(s->N4[2] + 1 - s->jlo_u, s->N3[2] + 1 - s->jlo_v);
-    if (x1 > x0) {
-      s->nl_skip[0] = x0;
-      s->nl_skip[1] = x1;
-    }
-    s->nl_skip[2] = s->N4[1] + 1 - s->jlo_u;
-    s->nl_skip[3] = s->N4[2] + 1 - s->jlo_u;
-    s->nl_skip[4] = s->N3[1] + 1 - s->jlo_v;
-    s->nl_skip[5] = s->N3[2] + 1 - s->jlo_v;
+  // The planes split between the two resolutions.  Central u planes [u0,
+  // u1) and v planes [v0, v1) take their products from the central solver:
+  // this solver's z-forward leaves them out, and its x-pass the planes [x0,
+  // x1) where both the u and the v plane are central (plane N(3,1)+1 keeps
+  // its fine u products); the central solver's x-pass covers [c0, c1), the
+  // planes with a central u or v plane, its z-forward only the central
+  // planes, and its fields take the long columns of the planes the
+  // interpolations of those read, [c0 - 1, c1 + 1).  Each z-inverse job list
+  // keeps the jobs of its x-pass planes.
+  const int u0 = s->N4[1] + 1 - s->jlo_u, u1 = s->N4[2] + 1 - s->jlo_u;
+  const int v0 = s->N3[1] + 1 - s->jlo_v, v1 = s->N3[2] + 1 - s->jlo_v;
+  const int x0 = std::max(u0, v0), x1 = std::max(x0, std::min(u1, v1));
+  const int c0 = std::min(u0, v0), c1 = std::max(u1, v1);
+  auto set_win = [](int* w, int lo, int hi, int s0, int s1) {
+    w[0] = lo;
+    w[1] = hi;
+    w[2] = s0;
+    w[3] = s1;
+  };
+  set_win(s->nl_x, 0, s->nyu, x0, x1);
+  set_win(s->nl_u, 0, s->nyu, u0, u1);
+  set_win(s->nl_v, 0, s->nyv, v0, v1);
+  s->gather_q[0] = std::max(c0 - 1, 0);
+  s->gather_q[1] = std::min(c1 + 1, s->nyu);
+  auto keep_jobs = [&](cfd_solver* t, int lo, int hi, int s0, int s1) {
     std::vector<ZInvJob> J;
-    for (const ZInvJob& z : s->h_nljobs)
-      if (z.pd < s->nl_skip[0] || z.pd >= s->nl_skip[1]) J.push_back(z);
-    s->n_nljobs = (int)J.size();
-    if ((rc = upload(s, &s->d_nljobs, J))) return rc;
-  }
+    for (const ZInvJob& z : t->h_nljobs)
+      if (z.pd >= lo && z.pd < hi && (z.pd < s0 || z.pd >= s1)) J.push_back(z);
+    t->n_nljobs = (int)J.size();
+    return upload(t, &t->d_nljobs, J);
+  };
+  if ((rc = keep_jobs(s, 0, s->nyu, x0, x1))) return rc;
   if ((rc = upload(s, &s->d_short, s->h_short)) || (rc = upload(s, &s->d_ccol, ccol)) ||
       (rc = upload(s, &s->d_lcol, lcol)) || (rc = upload(s, &s->d_cfu, fu)) || (rc = upload(s, &s->d_cfv, fv)))
     return rc;
@@ -712,8 +724,11 @@ int setup_bands(cfd_solver* s) {
   HIPCHK(s, hipStreamDestroy(c->stream));
   c->stream = s->stream;
   c->borrowed_stream = true;
-  if ((rc = upload(c, &c->d_cfu, cu)) || (rc = upload(c, &c->d_cfv, cv)))
+  if ((rc = upload(c, &c->d_cfu, cu)) || (rc = upload(c, &c->d_cfv, cv)) || (rc = keep_jobs(c, c0, c1, 0, 0)))
     return fail(s, rc, std::string("central band solver: ") + c->err);
+  set_win(c->nl_x, c0, c1, 0, 0);
+  set_win(c->nl_u, u0, u1, 0, 0);
+  set_win(c->nl_v, v0, v1, 0, 0);
   s->alloc_bytes += c->alloc_bytes;
   return CFD_OK;
 }

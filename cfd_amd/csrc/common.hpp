@@ -184,11 +184,13 @@ struct Solver {
   unsigned char* d_short = nullptr;  // per column: 1 = short
   unsigned char *d_cfu = nullptr, *d_cfv = nullptr;  // per u / v plane: 1 = in this solver's CFL maxima
   std::vector<unsigned char> h_short;
-  // planes the nonlinear transforms leave out (multi-band: the central
-  // band's): [0..1] x-pass planes, [2..3] z-forward u planes, [4..5] z-forward
-  // v planes; equal pairs leave out none
-  int nl_skip[6] = {0, 0, 0, 0, 0, 0};
+  // plane windows {lo, hi, s0, s1} of the nonlinear transforms: planes [lo,
+  // hi) without [s0, s1) of the x-pass (nl_x) and of the z-forward's u and v
+  // planes (nl_u, nl_v); all planes unless multi-band resolution splits them
+  // between the two resolutions
+  int nl_x[4] = {0, 0, 0, 0}, nl_u[4] = {0, 0, 0, 0}, nl_v[4] = {0, 0, 0, 0};
   std::vector<ZInvJob> h_nljobs;  // the z-inverse jobs of the nonlinear term (host copy)
+  int gather_q[2] = {0, 0};       // multi-band: planes [q0, q1) whose long columns mbc takes
 
   // profiling
   bool prof = false;

@@ -18,13 +18,13 @@ namespace {
 
 constexpr int kTB = 256;
 
-// grid (ceil(Ncolc / kTB), nyu, 3): field f, plane q, central column c
+// grid (ceil(Ncolc / kTB), q1 - q0, 3): field f, plane q0 + y, central column c
 __global__ void k_band_gather(const double2* __restrict__ u1, const double2* __restrict__ u2,
                               const double2* __restrict__ u3, double2* __restrict__ c1, double2* __restrict__ c2,
                               double2* __restrict__ c3, const int* __restrict__ lcol, int Ncol, int Ncolc, int nyu,
-                              int nyv) {
+                              int nyv, int q0) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  const int q = blockIdx.y, f = blockIdx.z;
+  const int q = q0 + blockIdx.y, f = blockIdx.z;
   if (c >= Ncolc || q >= (f == 1 ? nyv : nyu)) return;
   const double2* src = f == 0 ? u1 : (f == 1 ? u2 : u3);
   double2* dst = f == 0 ? c1 : (f == 1 ? c2 : c3);
@@ -79,9 +79,10 @@ __global__ void k_cfl_bands(DevScalars* sc, const DevScalars* cc, double alp, do
 
 void launch_band_gather(Solver& s) {
   Solver& c = *s.mbc;
-  const dim3 grid((c.Ncol + kTB - 1) / kTB, s.nyu, 3);
+  // the planes the central solver's transforms read
+  const dim3 grid((c.Ncol + kTB - 1) / kTB, s.gather_q[1] - s.gather_q[0], 3);
   hipLaunchKernelGGL(k_band_gather, grid, dim3(kTB), 0, s.stream, s.F[CFD_U1], s.F[CFD_U2], s.F[CFD_U3], c.F[CFD_U1],
-                     c.F[CFD_U2], c.F[CFD_U3], (const int*)s.d_lcol, s.Ncol, c.Ncol, s.nyu, s.nyv);
+                     c.F[CFD_U2], c.F[CFD_U3], (const int*)s.d_lcol, s.Ncol, c.Ncol, s.nyu, s.nyv, s.gather_q[0]);
 }
 
 void launch_band_scatter(Solver& s) {

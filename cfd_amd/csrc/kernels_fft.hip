@@ -168,14 +168,18 @@ struct XNLArgs {
   // per u / v plane: 1 when the plane's maxima count in this solver's CFL
   // (multi-band resolution: each resolution takes its own planes); null: all
   const unsigned char *cfu, *cfv;
-  // planes [sk0, sk1) are left out (multi-band resolution: the planes whose u
-  // and v products both come from the central band's transforms); the grid
-  // covers nyu - (sk1 - sk0) planes
-  int sk0, sk1;
+  // plane window {lo, hi, s0, s1}: the grid covers planes [lo, hi) without
+  // [s0, s1) (multi-band resolution: each resolution transforms its own
+  // planes); {0, nyu, 0, 0} covers all
+  int win[4];
 };
 
-// plane of y-block b when planes [s0, s1) are left out
-__device__ __forceinline__ int skip_plane(int b, int s0, int s1) { return b < s0 ? b : b + (s1 - s0); }
+// plane of y-block b of the window {lo, hi, s0, s1}
+__device__ __forceinline__ int win_plane(int b, const int* w) {
+  const int q = w[0] + b;
+  return q < w[2] ? q : q + (w[3] - w[2]);
+}
+inline int win_count(const int* w) { return (w[1] - w[0]) - (w[3] - w[2]); }
 
 template <int LC>
 __global__ void __launch_bounds__(kThreads) k_xpass_nl(XNLArgs a, FftPlan P, const double2* __restrict__ twg) {
@@ -186,7 +190,7 @@ __global__ void __launch_bounds__(kThreads) k_xpass_nl(XNLArgs a, FftPlan P, con
   double2* A = lds + L;
   double2* B = A + 3 * st;
   __shared__ double red[kThreads / 64];
-  const int z = blockIdx.x, q = skip_plane(blockIdx.y, a.sk0, a.sk1);
+  const int z = blockIdx.x, q = win_plane(blockIdx.y, a.win);
   const int ngz = gridDim.x;
   const bool hasv = q < a.nyv;
   const bool uint = q > 0 && (q < a.nyu - 1 || (a.top && q == a.nyu - 1));
@@ -263,9 +267,9 @@ struct ZNLArgs {
   int nyu, nyv, N2, Mx;
   int zs;   // Z row stride (elements)
   int top;  // also the x/z terms of the top ghost u plane and the top wall v plane
-  // u planes [su0, su1) and v planes [sv0, sv1) are left out (multi-band
-  // resolution: the central band's planes); the grid covers the others
-  int su0, su1, sv0, sv1;
+  // plane windows {lo, hi, s0, s1} of the u and v planes transformed
+  // (multi-band resolution: each resolution transforms its own planes)
+  int winu[4], winv[4];
 };
 
 constexpr int kMaxOut = 8;  // outputs (k,t) per thread: N2*TI <= kMaxOut*kThreads
@@ -279,9 +283,9 @@ __global__ void __launch_bounds__(kThreads) k_zfwd_nl(ZNLArgs a, FftPlan P, cons
   double2* A = lds + L;
   double2* B = A + TI * st;
   const int i0 = blockIdx.x * TI;
-  const int nue = a.nyu - (a.su1 - a.su0);
+  const int nue = (a.winu[1] - a.winu[0]) - (a.winu[3] - a.winu[2]);
   const bool isu = (int)blockIdx.y < nue;
-  const int q = isu ? skip_plane(blockIdx.y, a.su0, a.su1) : skip_plane(blockIdx.y - nue, a.sv0, a.sv1);
+  const int q = isu ? win_plane(blockIdx.y, a.winu) : win_plane(blockIdx.y - nue, a.winv);
   const int ny = isu ? a.nyu : a.nyv;
   const bool inter = q > 0 && (q < ny - 1 || (a.top && q == ny - 1));
   if (isu && !inter) return;  // ghost u planes: no product is needed there (but see a.top)
@@ -647,9 +651,9 @@ __global__ void __launch_bounds__(TI * cmax(P, Q)) k_zfwd4(ZNLArgs a, const cplx
   using C = cplx_t<T>;
   static_assert(P * Q == L, "P*Q");
   __shared__ C X[P * Q * TI];
-  const int nue = a.nyu - (a.su1 - a.su0);
+  const int nue = (a.winu[1] - a.winu[0]) - (a.winu[3] - a.winu[2]);
   const bool isu = (int)blockIdx.y < nue;
-  const int pl = isu ? skip_plane(blockIdx.y, a.su0, a.su1) : skip_plane(blockIdx.y - nue, a.sv0, a.sv1);
+  const int pl = isu ? win_plane(blockIdx.y, a.winu) : win_plane(blockIdx.y - nue, a.winv);
   const int ny = isu ? a.nyu : a.nyv;
   const bool inter = pl > 0 && (pl < ny - 1 || (a.top && pl == ny - 1));
   if (isu && !inter) return;  // ghost u planes: no product is needed there (but see a.top)
@@ -730,7 +734,7 @@ __global__ void __launch_bounds__(3 * xp_width(P, Q, R)) k_xpass4(XNLArgs a, con
   __shared__ C S[R * 3 * SP];
   const int tid = threadIdx.x;
   const int tr = tid / W, lane = tid - tr * W;
-  const int q = skip_plane(blockIdx.y, a.sk0, a.sk1), z0 = blockIdx.x * R, ngz = a.ngz;
+  const int q = win_plane(blockIdx.y, a.win), z0 = blockIdx.x * R, ngz = a.ngz;
   const bool hasv = q < a.nyv;
   const bool uint = q > 0 && (q < a.nyu - 1 || (a.top && q == a.nyu - 1));
   const bool vint = hasv && q > 0 && q < a.nyv - 1;
@@ -904,7 +908,7 @@ template <int LC>
 void xpass_nl_t(Solver& s, const XNLArgs& a) {
   const size_t lds = lds_bytes<LC>(s.px.L, 3);
   set_lds((const void*)k_xpass_nl<LC>, lds);
-  hipLaunchKernelGGL((k_xpass_nl<LC>), dim3(s.ngz, s.nyu - (a.sk1 - a.sk0)), dim3(kThreads), lds, s.stream, a, s.px,
+  hipLaunchKernelGGL((k_xpass_nl<LC>), dim3(s.ngz, win_count(a.win)), dim3(kThreads), lds, s.stream, a, s.px,
                      s.d_twx);
 }
 
@@ -914,7 +918,7 @@ void zfwd_nl_t(Solver& s, const ZNLArgs& a) {
   static_assert(TI <= 16, "tile");
   const size_t lds = lds_bytes<LC>(s.pz.L, TI);
   set_lds((const void*)k_zfwd_nl<LC, TI>, lds);
-  dim3 grid((s.Mx + TI - 1) / TI, s.nyu + s.nyv - (a.su1 - a.su0) - (a.sv1 - a.sv0));
+  dim3 grid((s.Mx + TI - 1) / TI, win_count(a.winu) + win_count(a.winv));
   hipLaunchKernelGGL((k_zfwd_nl<LC, TI>), grid, dim3(kThreads), lds, s.stream, a, s.pz, s.d_twz);
 }
 
@@ -999,7 +1003,7 @@ bool try_zinv4(Solver& s, const ZInvJob* jobs, int njobs, bool nl) {
 #define CFD_ZF_CASE(T, L, P, Q, TI)                                                                \
   case L:                                                                                       \
     hipLaunchKernelGGL((k_zfwd4<T, L, P, Q, TI>),                                              \
-                       dim3((s.Mx + TI - 1) / TI, s.nyu + s.nyv - (a.su1 - a.su0) - (a.sv1 - a.sv0)), \
+                       dim3((s.Mx + TI - 1) / TI, win_count(a.winu) + win_count(a.winv)),      \
                        dim3(TI * cmax(P, Q)), 0, s.stream, a, tw);                             \
     return true;
 
@@ -1028,7 +1032,7 @@ bool try_zfwd4(Solver& s, const ZNLArgs& a) {
 
 #define CFD_XP_CASE(T, L, P, Q, R)                                                                  \
   case L:                                                                                         \
-    hipLaunchKernelGGL((k_xpass4<T, L, P, Q, R>), dim3((s.ngz + R - 1) / R, s.nyu - (a.sk1 - a.sk0)), \
+    hipLaunchKernelGGL((k_xpass4<T, L, P, Q, R>), dim3((s.ngz + R - 1) / R, win_count(a.win)), \
                        dim3(3 * xp_width(P, Q, R)), 0, s.stream, a, tw);                            \
     return true;
 
@@ -1134,8 +1138,7 @@ void launch_xpass_nl(Solver& s, int cfl) {
   a.top = s.compat_np > 0;
   a.cfu = s.d_cfu;
   a.cfv = s.d_cfv;
-  a.sk0 = s.nl_skip[0];
-  a.sk1 = s.nl_skip[1];
+  for (int w = 0; w < 4; ++w) a.win[w] = s.nl_x[w];
   if (try_xpass4(s, a)) return;
   switch (s.px.L) {
 #define C(n) case n: xpass_nl_t<n>(s, a); break;
@@ -1162,10 +1165,10 @@ void launch_zfwd_nl(Solver& s) {
   a.Mx = s.Mx;
   a.zs = s.zs;
   a.top = s.compat_np > 0;
-  a.su0 = s.nl_skip[2];
-  a.su1 = s.nl_skip[3];
-  a.sv0 = s.nl_skip[4];
-  a.sv1 = s.nl_skip[5];
+  for (int w = 0; w < 4; ++w) {
+    a.winu[w] = s.nl_u[w];
+    a.winv[w] = s.nl_v[w];
+  }
   if (try_zfwd4(s, a)) return;
   switch (s.pz.L) {
 #define C(n) case n: zfwd_nl_t<n>(s, a); break;

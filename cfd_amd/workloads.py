@@ -44,6 +44,15 @@ WORKLOADS = {
                       r=_radius(16.0, 395.0, 192), c_i=6.25, c_k=6.25, stag=0.5, extras24=0, CFL=0.5,
                       flag_ctpress=0),
         steps=5000, re_tau=395.0),
+    # the 384 x 192 x 192 case with a coarser central band (multi-band
+    # resolution): 192 x 96 points in x/z between the wall bands, whose
+    # height is two post spacings in z (17 of the 209 u planes central)
+    "rt395_mb_384x192x192_c192x96": dict(
+        params=Params(ngalx=384, ngalz=192, ny=192, dsty=8, ygrid=1, ytanh_beta=2.0, Re=round(_rb(395.0)),
+                      alp=1.0, bet=2.0, posth=16.0 / 395.0, ntilex=16, ntilez=8,
+                      r=_radius(16.0, 395.0, 384), c_i=12.0, c_k=12.0, stag=0.5, extras24=0, CFL=0.5,
+                      flag_ctpress=0, ngalx2=192, ngalz2=96),
+        steps=200, re_tau=395.0),
     # 768 x 384 x 384, Re_tau 550, 32 x 16 posts, D+ = 16, k+ = 20
     "rt550_768x384x384": dict(
         params=Params(ngalx=768, ngalz=384, ny=384, dsty=16, ygrid=1, ytanh_beta=2.0, Re=round(_rb(550.0)),
