@@ -194,8 +194,10 @@ def test_hist_record_against_numpy():
     assert h["Umax"] == max(0.0, u[d:d + p.ny + 1].max())
     uslip = (u[1 + d] * (yv[d] - yu[d]) + u[d] * (yu[1 + d] - yv[d])) / (yu[1 + d] - yu[d])
     assert abs(h["Uslip"] - uslip) <= 1e-15 * abs(uslip)
+    # |div| of a complex mode on the device and in numpy: the two moduli may
+    # differ in the last bit, so two units in the last place
     err = np.abs(ch.get_field("div")).max()
-    assert abs(h["err"] - err) <= 2e-16 * err and err > 0
+    assert abs(h["err"] - err) <= 4.5e-16 * err and err > 0
     for k in ("t", "dt", "dtc", "mpgx", "dgx", "QxT"):
         assert h[k] == sc[k], k
     ch.close()
