@@ -213,3 +213,52 @@ def test_multiband_record_out_read_in(tmp_path):
         b = b.copy()
         b[np.ix_(sh, rows)] = 0.0
         assert np.array_equal(a, b), n
+
+
+@pytest.mark.parametrize("tuning", [dict(fft=1), dict(seg_rows=5, solve_threads=64),
+                                    dict(seg_rows=21, solve_threads=128)])
+def test_multiband_launch_shapes(tuning):
+    """The Stockham engine and other segment lengths / block sizes of the
+    column solves (the short columns' band rows fall in different segments)
+    give the oracle's three substeps to 1e-11."""
+    p = MB
+    print("multi-band launch shapes", tuning, flush=True)
+    ch0, orc = _pair(p, seed=74)
+    ch = Channel(p, tuning=tuning)
+    for name in STATE:
+        ch.set_field(name, ch0.get_field(name))
+    ch.set_scalars(**ch0.scalars())
+    for kRK in (1, 2, 3):
+        ch.substep(kRK)
+        orc.substep(kRK)
+    _cmp(ch, orc, ("u1", "u2", "u3", "p"), 1e-11)
+
+
+def test_multiband_workload_384x192x192_two_steps():
+    """The 384 x 192 x 192 workload with a 192 x 96 central band (17 central
+    u planes of 209) at full size: two RK3 steps from a projected start
+    against the oracle, rel-L2 <= 1e-10 per velocity component, divergence at
+    round-off, the same time and mean pressure gradient."""
+    from cfd_amd import workloads
+    from oracle.oracle import set_threads
+    p, _ = workloads.get("rt395_mb_384x192x192_c192x96")
+    print("case rt395_mb_384x192x192_c192x96 two steps vs oracle", flush=True)
+    set_threads(min(16, len(os.sched_getaffinity(0))))
+    ch = Channel(p)
+    orc = Oracle(p)
+    g = ch.ygrid()
+    f = ic.initial_fields(p, g, seed=0, amp=0.1)
+    f = ic.project(orc, {k: mask_bands(p, k, v) for k, v in f.items()})
+    q0 = ic.flowrate(f["u1"], g, p.ny)
+    for s in (ch, orc):
+        for k in ("u1", "u2", "u3", "p"):
+            s.set_field(k, f[k])
+        s.set_scalars(t=0.0, dt=0.0, mpgx=0.0, QxT=q0)
+    ch.step(2)
+    orc.step(2)
+    _cmp(ch, orc, ("u1", "u2", "u3"), 1e-10)
+    sa, sb = ch.scalars(), orc.scalars()
+    assert abs(sa["t"] - sb["t"]) <= 1e-12 * sb["t"]
+    assert abs(sa["mpgx"] - sb["mpgx"]) <= 1e-9 * abs(sb["mpgx"])
+    d = np.abs(ch.get_field("div")).max() / np.abs(ch.get_field("u1")).max()
+    assert d <= 1e-12, d
