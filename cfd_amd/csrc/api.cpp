@@ -1462,7 +1462,7 @@ void op_ib(cfd_solver* s, int f0, int f1) {
   Timed t(s, KC_IB);
   launch_zinv(*s, s->d_ibz + 2 * z0, 2 * (z1 - z0));
   launch_xpass_inv2(*s, s->d_ibxi + x0, x1 - x0);
-  for (int f = f0; f < f1; ++f) launch_ib_apply(*s, f);
+  launch_ib_apply(*s, f0, f1);
   launch_xpass_fwd2(*s, s->d_ibxf + x0, x1 - x0);
   launch_zfwd_plain(*s, s->d_ibzf + x0, x1 - x0);
 }

@@ -218,7 +218,7 @@ void launch_nlcombine(Solver& s, int kRK);
 void launch_reset_max(Solver& s);
 void launch_cfl(Solver& s);
 void launch_predictor(Solver& s, int field);
-void launch_ib_apply(Solver& s, int field);
+void launch_ib_apply(Solver& s, int f0, int f1);  // IB forcing of fields [f0, f1) in one launch
 void launch_ysolve(Solver& s, int field, int kRK);
 void launch_meanflow(Solver& s, int kRK);
 void launch_solve_p(Solver& s, int kRK);

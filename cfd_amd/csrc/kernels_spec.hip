@@ -165,8 +165,27 @@ __global__ void __launch_bounds__(kTB) k_predictor(double2* du, const double2* u
 // tridLU_3D_IB.f90:188-211, 248-271): one thread per distinct point; repeated
 // entries of a point are applied in list order.
 // ---------------------------------------------------------------------------
-__global__ void k_ib_apply(const IBDev* __restrict__ L, const int* __restrict__ seg, int nseg, double* pr,
-                           const double* pu, size_t plane_sz) {
+struct IBApplyArgs {
+  const IBDev* L[3];
+  const int* seg[3];
+  int nseg[3];
+  double* pr[3];
+  const double* pu[3];
+};
+
+// grid (ceil(max nseg / 128), fields): blockIdx.y takes field f0 + y
+__global__ void k_ib_apply(IBApplyArgs a, size_t plane_sz) {
+  const int y = blockIdx.y;
+  const IBDev* __restrict__ L = a.L[0];
+  const int* __restrict__ seg = a.seg[0];
+  int nseg = a.nseg[0];
+  double* pr = a.pr[0];
+  const double* pu = a.pu[0];
+  if (y == 1) {
+    L = a.L[1]; seg = a.seg[1]; nseg = a.nseg[1]; pr = a.pr[1]; pu = a.pu[1];
+  } else if (y == 2) {
+    L = a.L[2]; seg = a.seg[2]; nseg = a.nseg[2]; pr = a.pr[2]; pu = a.pu[2];
+  }
   const int sgi = blockIdx.x * blockDim.x + threadIdx.x;
   if (sgi >= nseg) return;
   const int e0 = seg[sgi], e1 = seg[sgi + 1];
@@ -477,12 +496,21 @@ void launch_predictor(Solver& s, int field) {
   hipLaunchKernelGGL(k_predictor, grid, dim3(kTB), 0, s.stream, du, u, s.Ncol, ny, (const DevScalars*)s.d_sc);
 }
 
-void launch_ib_apply(Solver& s, int field) {
-  const int g = field == 1 ? 1 : 0;
-  if (s.nibseg[g] == 0) return;
+void launch_ib_apply(Solver& s, int f0, int f1) {
+  IBApplyArgs a{};
+  int m = 0;
+  for (int f = f0; f < f1; ++f) {
+    const int g = f == 1 ? 1 : 0, y = f - f0;
+    a.L[y] = s.d_ibl[g];
+    a.seg[y] = s.d_ibseg[g];
+    a.nseg[y] = s.nibseg[g];
+    a.pr[y] = s.pib_r[f];
+    a.pu[y] = (const double*)s.pib_u[f];
+    m = std::max(m, s.nibseg[g]);
+  }
+  if (m == 0) return;
   const size_t psz = (size_t)s.ngz * s.ngx;
-  hipLaunchKernelGGL(k_ib_apply, dim3((s.nibseg[g] + 127) / 128), dim3(128), 0, s.stream, s.d_ibl[g], s.d_ibseg[g],
-                     s.nibseg[g], s.pib_r[field], (const double*)s.pib_u[field], psz);
+  hipLaunchKernelGGL(k_ib_apply, dim3((m + 127) / 128, f1 - f0), dim3(128), 0, s.stream, a, psz);
 }
 
 void launch_ysolve(Solver& s, int field, int kRK) {
