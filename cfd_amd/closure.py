@@ -198,7 +198,9 @@ def main(argv=None):
     ap.add_argument("--weights", default=None,
                     help="evaluate this trained network (closure_weights.npz) instead of training one")
     ap.add_argument("--out", default=".")
-    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--seed", type=int, default=0, help="network initialisation and sample draws")
+    ap.add_argument("--fine-seed", type=int, default=0,
+                    help="seed of the fine run's initial field (another trajectory for choosing the gain)")
     a = ap.parse_args(argv)
     fa, fo, fe = a.filter.split(",")
     filt = (float(fa), int(fo), float(fe))
@@ -211,13 +213,13 @@ def main(argv=None):
     net.set_params(cnn.init_params(seed=a.seed))
     net.dataset(a.slots + a.held_out)
     rng = np.random.default_rng(a.seed)
-    report = {"samples": a.samples, "every": a.every, "slots": a.slots, "train_per_sample": a.train_per_sample,
+    report = {"fine_seed": a.fine_seed, "seed": a.seed, "samples": a.samples, "every": a.every, "slots": a.slots, "train_per_sample": a.train_per_sample,
               "closure_every": every_list, "closure_gain": a.closure_gain, "coarse_filter": {"alpha": filt[0], "order": filt[1], "eta_c": filt[2]}}
     train = a.weights is None
     if train:
         # pass 1: the fine run alone, probing the sample magnitudes along the
         # trajectory; the normalisation takes the largest RMS of each quantity
-        f0, q0 = start_fields(fine, pf, seed=0)
+        f0, q0 = start_fields(fine, pf, seed=a.fine_seed)
         probe = cnn.CNN(scratch.nyu, pc.ngalz, pc.ngalx)
         probe.dataset(1)
         nsteps = a.samples * a.every
@@ -237,7 +239,7 @@ def main(argv=None):
         report["weights"] = a.weights
     # pass 2: the same fine run from the same start: samples, online
     # training, and the judge's drag history
-    f0, q0 = start_fields(fine, pf, seed=0)
+    f0, q0 = start_fields(fine, pf, seed=a.fine_seed)
     k_start = int(a.coarse_start * a.samples)
     k_val = int(a.validation_start * a.samples)
     fs, ts = (f0, 0.0), None
