@@ -619,7 +619,6 @@ int create_impl(const cfd_params* pp, const cfd_tuning* tp, cfd_handle* out) {
 // give the central planes' products over the long columns.
 int setup_bands(cfd_solver* s) {
   const cfd_params& p = s->P;
-  if (s->mixed) return fail(s, CFD_EINVAL, "multi-band resolution runs the FP64 path (precision 0) only");
   if (s->ngx2 > s->ngx || s->ngz2 > s->ngz || s->ngx2 % 2 || s->ngz2 % 2 || s->ngx2 < 6 || s->ngz2 < 6)
     return fail(s, CFD_EINVAL, "the central band's Ngalx2, Ngalz2 must be even, at least 6 and at most Ngalx, Ngalz");
   int N[20];

@@ -32,14 +32,16 @@ __global__ void k_band_gather(const double2* __restrict__ u1, const double2* __r
 }
 
 struct ScatterArgs {
-  double2* out[6];        // nxz1, nxz3, fvv (u planes), nxz2, fuv, fvw (v planes)
-  const double2* in[6];   // the same arrays of the central solver
+  void* out[6];           // nxz1, nxz3, fvv (u planes), nxz2, fuv, fvw (v planes)
+  const void* in[6];      // the same arrays of the central solver
   const int* ccol;        // per column: central column or -1
   int Ncol, Ncolc;
   int u0, nu, v0, nv;     // central u planes [u0, u0 + nu), v planes [v0, v0 + nv)
 };
 
-// grid (ceil(Ncol / kTB), max(nu, nv), 6): array a, central plane t
+// grid (ceil(Ncol / kTB), max(nu, nv), 6): array a, central plane t; C the
+// nonlinear-term element (double2 FP64, float2 on the mixed path)
+template <typename C>
 __global__ void k_band_scatter(ScatterArgs a) {
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
   const int t = blockIdx.y, w = blockIdx.z;
@@ -49,8 +51,11 @@ __global__ void k_band_scatter(ScatterArgs a) {
   const int c = a.ccol[col];
   // a short column has no value on a central plane (its band arrays stop
   // at the band interface): its fluxes there are zero
-  const double2 v = c >= 0 ? a.in[w][(size_t)q * a.Ncolc + c] : make_double2(0.0, 0.0);
-  a.out[w][(size_t)q * a.Ncol + col] = v;
+  C v;
+  v.x = 0;
+  v.y = 0;
+  if (c >= 0) v = static_cast<const C*>(a.in[w])[(size_t)q * a.Ncolc + c];
+  static_cast<C*>(a.out[w])[(size_t)q * a.Ncol + col] = v;
 }
 
 // dtc_calc (FOU3D.f90:516-544) on every rank with its band's mode counts and
@@ -91,8 +96,8 @@ void launch_band_scatter(Solver& s) {
   void* out[6] = {s.nxz1, s.nxz3, s.fvv, s.nxz2, s.fuv, s.fvw};
   void* in[6] = {c.nxz1, c.nxz3, c.fvv, c.nxz2, c.fuv, c.fvw};
   for (int w = 0; w < 6; ++w) {
-    a.out[w] = static_cast<double2*>(out[w]);
-    a.in[w] = static_cast<const double2*>(in[w]);
+    a.out[w] = out[w];
+    a.in[w] = in[w];
   }
   a.ccol = s.d_ccol;
   a.Ncol = s.Ncol;
@@ -103,7 +108,10 @@ void launch_band_scatter(Solver& s) {
   a.v0 = s.N3[1] + 1 - s.jlo_v;
   a.nv = s.N3[2] - s.N3[1];
   const dim3 grid((s.Ncol + kTB - 1) / kTB, a.nu > a.nv ? a.nu : a.nv, 6);
-  hipLaunchKernelGGL(k_band_scatter, grid, dim3(kTB), 0, s.stream, a);
+  if (s.mixed)
+    hipLaunchKernelGGL(k_band_scatter<float2>, grid, dim3(kTB), 0, s.stream, a);
+  else
+    hipLaunchKernelGGL(k_band_scatter<double2>, grid, dim3(kTB), 0, s.stream, a);
 }
 
 void launch_cfl_bands(Solver& s) {

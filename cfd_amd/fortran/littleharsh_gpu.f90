@@ -30,8 +30,8 @@
 !   fused =   1   1: one cfd_step per time step (fused column sweeps);
 !                 0: the reference's call sequence, one entry point per
 !                 subroutine (littleharsh.f90:165-196)
-! Ngalx2/Ngalz2 below Ngalx1/Ngalz1 (multi-band resolution) run the FP64
-! path with fused = 1 and without statistics.
+! Ngalx2/Ngalz2 below Ngalx1/Ngalz1 (multi-band resolution) run with
+! fused = 1 and without statistics.
 ! The time loop's own steps (those without a field record) are timed and the
 ! rate printed as " loop: <steps> steps <seconds> s <substeps/s> substeps/s".
 program littleharsh_gpu

@@ -39,8 +39,8 @@ typedef struct cfd_solver* cfd_handle;
    ngalx2/ngalz2 the central band's (Ngalx2, Ngalz2, input.in:16, 18), 0 for
    the same resolution in every band.  With a coarser central band (multi-
    band resolution, start.f90:89-111, 1420-1557) the fields keep every
-   column of the wall bands over every plane (see cfd_set_field); the path
-   is FP64 (precision 0) and runs through cfd_substep / cfd_step. */
+   column of the wall bands over every plane (see cfd_set_field), and the
+   path runs through cfd_substep / cfd_step (FP64 or mixed). */
 typedef struct cfd_params {
   int ngalx, ngalz;     /* physical grid points in x and z (Ngalx1, Ngalz1)      */
   int ny;               /* y points tip to tip, y(0)=-1 .. y(ny-1)=1 (input.in:19) */

@@ -178,9 +178,39 @@ def test_multiband_per_operator_entry_points_refused():
             call()
 
 
-def test_multiband_mixed_refused():
-    with pytest.raises(CfdError, match="FP64|mixed"):
-        Channel(MB_SMOOTH.replace(precision=1))
+# four-step lengths in both resolutions, as the mixed path needs
+MB96 = Params(ngalx=96, ngalz=96, ngalx2=48, ngalz2=48, ny=33, dsty=3, ygrid=1, ytanh_beta=2.0, geometry_type=3,
+              ntilex=8, ntilez=8, r=2.5, c_i=6.0, c_k=6.0, stag=0.5, extras24=0, Re=2000.0, alp=1.0, bet=4.0,
+              posth=0.06, CFL=0.5)
+
+
+def test_multiband_mixed_path():
+    """The mixed path (FP32 transforms, products and nonlinear scratch in
+    both resolutions) with a coarser central band: ten RK3 steps from a
+    projected state within 1e-5 rel-L2 of the FP64 oracle and of the FP64
+    GPU path, per velocity component, flow rate exact to 1e-12 relative,
+    divergence at FP64 round-off; FP64 itself to 1e-10."""
+    p = MB96
+    print("multi-band mixed path", flush=True)
+    chs = {prec: Channel(p.replace(precision=prec)) for prec in (0, 1)}
+    orc = Oracle(p)
+    g = chs[0].ygrid()
+    f = ic.initial_fields(p, g, seed=4, amp=0.1)
+    f = ic.project(orc, {k: mask_bands(p, k, v) for k, v in f.items()})
+    q0 = ic.flowrate(f["u1"], g, p.ny)
+    for s in (chs[0], chs[1], orc):
+        for k in ("u1", "u2", "u3", "p"):
+            s.set_field(k, f[k])
+        s.set_scalars(t=0.0, dt=0.0, mpgx=0.0, QxT=q0)
+    for s in (chs[0], chs[1], orc):
+        s.step(10)
+    _cmp(chs[0], orc, ("u1", "u2", "u3"), 1e-10)
+    for n in ("u1", "u2", "u3"):
+        a, b, c = chs[1].get_field(n), orc.get_field(n), chs[0].get_field(n)
+        assert rel_l2(a, b) <= 1e-5 and rel_l2(a, c) <= 1e-5, (n, rel_l2(a, b), rel_l2(a, c))
+    assert abs(ic.flowrate(chs[1].get_field("u1"), g, p.ny) - q0) <= 1e-12 * abs(q0)
+    d = np.abs(chs[1].get_field("div")).max() / np.abs(chs[1].get_field("u1")).max()
+    assert d <= 1e-12, d
 
 
 def test_multiband_record_out_read_in(tmp_path):
