@@ -2194,7 +2194,6 @@ extern "C" int cfd_ensemble_create(cfd_handle* hs, int n, cfd_ensemble* out) {
     if (i && s->dev != e->dev) return bail(CFD_EINVAL, "ensemble: members on different devices");
     if (s->prof || s->cl.cnn || s->st.pending)
       return bail(CFD_EINVAL, "ensemble: a member has profiling, a closure or a statistics request");
-    if (s->mb) return bail(CFD_EINVAL, "ensemble: a member has a coarser central band");
     if (!s->nsub) return bail(CFD_EINVAL, "ensemble: every member must run one substep before capture");
     for (cfd_solver* q : e->m)
       if (q == s) return bail(CFD_EINVAL, "ensemble: a member appears twice");

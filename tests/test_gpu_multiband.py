@@ -292,3 +292,38 @@ def test_multiband_workload_384x192x192_two_steps():
     assert abs(sa["mpgx"] - sb["mpgx"]) <= 1e-9 * abs(sb["mpgx"])
     d = np.abs(ch.get_field("div")).max() / np.abs(ch.get_field("u1")).max()
     assert d <= 1e-12, d
+
+
+def test_multiband_member_in_an_ensemble():
+    """A multi-band channel stepped by the ensemble's captured graphs (its
+    central solver's kernels are captured on its stream) beside a
+    one-resolution channel: every field and scalar bit-identical to a copy
+    stepped eagerly, after a substep of each kRK and a whole step, and both
+    against the oracle to 1e-11."""
+    from cfd_amd import ensemble
+    print("multi-band member in an ensemble", flush=True)
+    a, orc = _pair(MB, seed=78)
+    b = Channel(MB)
+    for name in STATE:
+        b.set_field(name, a.get_field(name))
+    b.set_scalars(**a.scalars())
+    c, _ = _pair(MB_SMOOTH.replace(ngalx2=0, ngalz2=0), seed=79)
+    for ch in (a, b, c):
+        ch.substep(1)
+    orc.substep(1)
+    with ensemble.Ensemble([a, c]) as ens:
+        for kRK in (2, 3):
+            ens.substep(kRK)
+            ens.sync()
+            b.substep(kRK)
+            orc.substep(kRK)
+            for n in ("u1", "u2", "u3", "p"):
+                assert np.array_equal(a.get_field(n), b.get_field(n)), (kRK, n)
+        ens.step(1)
+        ens.sync()
+    b.step(1)
+    orc.step(1)
+    for n in ("u1", "u2", "u3", "p"):
+        assert np.array_equal(a.get_field(n), b.get_field(n)), n
+    assert a.scalars() == b.scalars()
+    _cmp(a, orc, ("u1", "u2", "u3", "p"), 1e-11)
