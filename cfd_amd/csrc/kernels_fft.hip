@@ -861,7 +861,13 @@ __global__ void __launch_bounds__(3 * xp_width(P, Q, R)) k_xpass4(XNLArgs a, con
     C v[P];
     sfor<0, P>([&](auto c) { v[decltype(c)::value] = s4[decltype(c)::value]; });
     dft<false, P>(v);
-    sfor<0, P>([&](auto c) { s4[decltype(c)::value] = v[decltype(c)::value]; });
+    // only the kept modes m < Mx and their mirrors L - m are read below:
+    // outputs m1 with m2 + Q m1 in [Mx, L - Mx] for every m2 are dropped,
+    // and the compiler drops the arithmetic that only they need
+    sfor<0, P>([&](auto c) {
+      constexpr int m1 = decltype(c)::value;
+      if constexpr (!(Q * m1 >= Mx && Q * m1 + Q - 1 <= L - Mx)) s4[m1] = v[m1];
+    });
   }
   __syncthreads();
 
