@@ -651,7 +651,9 @@ __global__ void __launch_bounds__(64) k_pressure(PressArgs a) {
 // above its last row (the next segment's first face) is recomputed from that
 // segment's LDS record to close the divergence of its last row, with the old
 // v of that face read before the first barrier.
-template <int SEG>
+// CU: the first pass's u1 rows are kept for the projection too (the v faces
+// always are); a register budget matter, chosen per segment length
+template <int SEG, bool CU>
 __global__ void __launch_bounds__(256) k_press_seg(PressArgs a) {
   extern __shared__ double sm[];
   const int CB = blockDim.x, NT = blockDim.x * blockDim.y;
@@ -698,12 +700,16 @@ __global__ void __launch_bounds__(256) k_press_seg(PressArgs a) {
   double2 y[SEG];
   double g[SEG], h[SEG];
   double2 vnext = make_double2(0.0, 0.0);  // old v of the face above the separator
+  // the rows' old v (face b+j) and, with CU, u1 (above p row b+j) from the
+  // first pass, kept for the projection so that it does not read them again
+  double2 cu[CU ? SEG + 1 : 1], cv[SEG + 1];
   if (act) {
     kx = a.k1x[i];
     kz = a.k1z[k];
     const double k2 = a.k2x[i] + a.k2z[k];
     const double dtirk = a.sc->dti / a.apb;
     double2 vq = a.u2[(size_t)b * N + col];  // old v at the face below row b
+    cv[0] = vq;
     // row q: divergence of the old field as right-hand side; fetch issues
     // the loads of a row (u1, u3 above p row q and v of the face above it)
     // so that they can be in flight during the previous row's arithmetic
@@ -759,6 +765,8 @@ __global__ void __launch_bounds__(256) k_press_seg(PressArgs a) {
       double lo, dg, up;
       double2 r;
       const Vel nxt = fetch(b + j + 1);
+      if constexpr (CU) cu[j] = cur.U;
+      cv[j + 1] = cur.V1;
       row(b + j, cur, lo, dg, up, r);
       cur = nxt;
       const double inv = 1.0 / (dg - lo * gp);
@@ -772,6 +780,7 @@ __global__ void __launch_bounds__(256) k_press_seg(PressArgs a) {
     if (sg < S - 1) {
       double lo, dg, up;
       double2 r;
+      if constexpr (CU) cu[SEG] = cur.U;
       row(b + SEG, cur, lo, dg, up, r);
       vnext = vq;  // old v of face b+SEG+1
       As[me] = lo;
@@ -828,11 +837,17 @@ __global__ void __launch_bounds__(256) k_press_seg(PressArgs a) {
   double2 unp = make_double2(0.0, 0.0), wnp = unp, vprev = unp;  // corrected u, w, v of the row below
   // psi, p, projection of the u/w row above p row q and of face q, and the
   // divergence of p row q-1 once both its faces are known
-  auto proc = [&](int q, double2 ps) {
+  auto proc = [&](int j, double2 ps) {
+    const int q = b + j;
     a.psi[(size_t)q * N + col] = ps;
     const double2 P = a.p[(size_t)q * N + col];
     a.p[(size_t)q * N + col] = make_double2(P.x + ps.x, P.y + ps.y);
-    const double2 U = a.u1[(size_t)(q + 1) * N + col], W = a.u3[(size_t)(q + 1) * N + col];
+    double2 U;
+    if constexpr (CU)
+      U = cu[j];
+    else
+      U = a.u1[(size_t)(q + 1) * N + col];
+    const double2 W = a.u3[(size_t)(q + 1) * N + col];
     const double2 gx = imulk(kxd, ps), gz = imulk(kzd, ps);
     const double2 un = make_double2(U.x - gx.x, U.y - gx.y);
     const double2 wn = make_double2(W.x - gz.x, W.y - gz.y);
@@ -848,7 +863,7 @@ __global__ void __launch_bounds__(256) k_press_seg(PressArgs a) {
     }
     double2 vf = make_double2(0.0, 0.0);  // bottom wall face, a short column's interface faces
     if (q > 0 && !iface(q)) {
-      const double2 V = a.u2[(size_t)q * N + col];
+      const double2 V = cv[j];
       const double cc = dtrk * a.ddthetavi * ldv[q];
       vf = make_double2(V.x - cc * (ps.x - psm.x), V.y - cc * (ps.y - psm.y));
     }
@@ -869,12 +884,12 @@ __global__ void __launch_bounds__(256) k_press_seg(PressArgs a) {
   for (int j = 0; j < SEG; ++j) {
     const int q = b + j;
     if (q < n)
-      proc(q, make_double2(y[j].x + g[j] * xr.x + h[j] * xl.x, y[j].y + g[j] * xr.y + h[j] * xl.y));
+      proc(j, make_double2(y[j].x + g[j] * xr.x + h[j] * xl.x, y[j].y + g[j] * xr.y + h[j] * xl.y));
   }
   int last;
   if (sg < S - 1) {
     last = b + SEG;
-    proc(last, xr);
+    proc(SEG, xr);
   } else {
     last = n - 1;
   }
@@ -1238,7 +1253,8 @@ void press_seg_launch(Solver& s, const PressArgs& a, int sy) {
   // separator system (15 doubles per thread), then 5 nyp + 3 doubles of
   // per-row metadata
   const size_t lds = sizeof(double) * (15 * (size_t)cb * sy + 5 * (size_t)s.nyp + 3);
-  hipLaunchKernelGGL(k_press_seg<SEG>, dim3((s.Ncol + cb - 1) / cb), dim3(cb, sy), lds, s.stream, a);
+  // the u1 rows fit the register budget (two waves per SIMD) up to 9 rows
+  hipLaunchKernelGGL((k_press_seg<SEG, (SEG <= 9)>), dim3((s.Ncol + cb - 1) / cb), dim3(cb, sy), lds, s.stream, a);
 }
 
 bool seg_rows_supported(int r) { return r == 5 || r == 9 || r == 13 || r == 21; }
