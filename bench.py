@@ -160,7 +160,7 @@ def copy_calibration():
 
 
 def realised_dt(report):
-    """Mean time step of the main runs of the closure report (simulated
+    """Mean time step of the evaluated runs of the closure report (simulated
     time span / steps): the fine FP64 run over its whole span from t = 0 and
     each coarse run from its start to its end.  At the seeded near-laminar
     start the coarse dt is twice the fine dt, but in the developed flow both
@@ -180,9 +180,9 @@ def learned_leg(args, fine_value, fine_dt):
     36, order 32, as in the closure experiment) with the trained closure
     network (cfd_amd/data/closure_rt395.npz) evaluated every ``every`` steps,
     timed like the FP64 leg from a seeded start.  Simulated time per second
-    against the fine FP64 run's = (substeps/s x realised mean dt of the judged
-    coarse run) / (fine substeps/s x realised mean dt of the judged fine run),
-    from the closure report; the report's own wall-clock ratio of the judged
+    against the fine FP64 run's = (substeps/s x realised mean dt of the evaluated
+    coarse run) / (fine substeps/s x realised mean dt of the evaluated fine run),
+    from the closure report; the report's own wall-clock ratio of the evaluated
     runs (speedup_vs_fine) is quoted beside it.  The dt of the timed substeps
     (a near-laminar start, where the coarse dt is twice the fine dt) is
     recorded but not used for the ratio."""

@@ -10,7 +10,7 @@ to the coarse solver, which then adds the correction in its nonlinear terms.
 
 runs the experiment at Re_tau 395: the fine DNS on 384 x 192 x 192 produces
 the samples (one every `every` steps, trained on online from a device ring
-buffer) and is the judge; the coarse solver on 192 x 192 x 96 then runs from
+buffer) and is the reference; the coarse solver on 192 x 192 x 96 then runs from
 the box-filtered start field over the same simulated time without and with
 the closure (as predicted, and backscatter-clipped: components that would feed
 energy into the resolved field set to zero), and once more with the closure
@@ -238,7 +238,7 @@ def main(argv=None):
         net.set_scales(np.asarray(wts["scales"], np.float32))
         report["weights"] = a.weights
     # pass 2: the same fine run from the same start: samples, online
-    # training, and the judge's drag history
+    # training, and the reference drag history
     f0, q0 = start_fields(fine, pf, seed=a.fine_seed)
     k_start = int(a.coarse_start * a.samples)
     k_val = int(a.validation_start * a.samples)
