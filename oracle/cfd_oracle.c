@@ -893,11 +893,12 @@ static pview view_of(const or_state* s, int coarse) {
   v.coarse = coarse;
   return v;
 }
-/* plane j of the u (g 0) or v (g 1) grid lies in the central band
-   (planes N(4,1)+1..N(4,2), N(3,1)+1..N(3,2): the plane ranges of the ranks
-   of band 2 in proc_lims_planes, start.f90:1650-1951) */
+/* plane j of the u (g 0), v (g 1) or p (g 2) grid lies in the central band
+   (planes N(4,1)+1..N(4,2), N(3,1)+1..N(3,2), N(4,1)..N(4,2)+1: the plane
+   ranges of the ranks of band 2 in proc_lims_planes, start.f90:1650-1951) */
 static int central_plane(const or_state* s, int g, int j) {
   if (!s->mb) return 0;
+  if (g == 2) return j >= s->N4[1] && j <= s->N4[2] + 1;
   return g == 0 ? (j > s->N4[1] && j <= s->N4[2]) : (j > s->N3[1] && j <= s->N3[2]);
 }
 /* slot of reference mode k (fine column numbering, 0-based) in the view's
@@ -1684,7 +1685,6 @@ static int jlo_grid(const or_state* s, int g) { return g == 0 ? s->jlo_u : g == 
 int or_stats_init(or_state* s, const int* N20) {
   or_stats_free(s);
   if (s->P.ntilex <= 0 || s->P.ntilez <= 0) return 1;
-  if (s->mb) return 3; /* statistics of one resolution in every band only */
   struct or_stats* t = (struct or_stats*)calloc(1, sizeof *t);
   memcpy(t->N, N20, sizeof t->N);
 #define NB(r, b) (t->N[4 * (b) + (r) - 1])
@@ -1817,9 +1817,11 @@ int or_stats(or_state* s) {
 #endif
   interp_v(s, s->u2i, U2); /* u2_itp (FOU3D.f90:80) */
   interp_u(s, s->u3i, U3); /* u3_itp (FOU3D.f90:81) */
-  /* buildsp / buildspUV (spectra.f90:17-59) */
+  /* buildsp / buildspUV (spectra.f90:17-59): the midband's columns, with a
+     coarser central band the long columns only */
 #pragma omp parallel for schedule(static)
   for (int col = 0; col < s->Ncol; ++col) {
+    if (s->mb && s->shortc[col]) continue;
     for (int j = s->jlo_u; j <= s->jhi_u; ++j) {
       const size_t e = fu(s, col, j);
       t->sp[0][e] = t->sp[0][e] + creal(U1[e]) * creal(U1[e]) + cimag(U1[e]) * cimag(U1[e]);
@@ -1847,16 +1849,19 @@ int or_stats(or_state* s) {
     for (int jj = 0; jj < s->nyu; ++jj) {
       const int j = s->jlo_u + jj;
       double *Pu = B, *Pv = B + psz, *Pw = B + 2 * psz;
-      to_plane(s, U1, s->nyu, s->jlo_u, j, Pu);
-      to_plane(s, s->u2i, s->nyu, s->jlo_u, j, Pv);
-      to_plane(s, U3, s->nyu, s->jlo_u, j, Pw);
-      four_to_phys(s, Pu, 1, &w);
-      four_to_phys(s, Pv, 1, &w);
-      four_to_phys(s, Pw, 1, &w);
-      sum1(&t->prof[ST_U][jj], &t->prof[ST_U2][jj], Pu, igal, ngx, ngz);
-      sum1(&t->prof[ST_W][jj], &t->prof[ST_W2][jj], Pw, igal, ngx, ngz);
-      sum2(&t->prof[ST_UV][jj], Pu, Pv, igal, ngx, ngz);
-      sum2(&t->prof[ST_UW][jj], Pu, Pw, igal, ngx, ngz);
+      /* a central plane on the central band's grid, the long columns only */
+      const pview v = view_of(s, central_plane(s, 0, j));
+      const int vig = v.ngx + 2;
+      to_plane_v(s, &v, U1, s->nyu, s->jlo_u, j, Pu);
+      to_plane_v(s, &v, s->u2i, s->nyu, s->jlo_u, j, Pv);
+      to_plane_v(s, &v, U3, s->nyu, s->jlo_u, j, Pw);
+      four_to_phys_v(&v, Pu, 1, &w);
+      four_to_phys_v(&v, Pv, 1, &w);
+      four_to_phys_v(&v, Pw, 1, &w);
+      sum1(&t->prof[ST_U][jj], &t->prof[ST_U2][jj], Pu, vig, v.ngx, v.ngz);
+      sum1(&t->prof[ST_W][jj], &t->prof[ST_W2][jj], Pw, vig, v.ngx, v.ngz);
+      sum2(&t->prof[ST_UV][jj], Pu, Pv, vig, v.ngx, v.ngz);
+      sum2(&t->prof[ST_UW][jj], Pu, Pw, vig, v.ngx, v.ngz);
       int j1;
       const int b = band_of(t, 0, j, &j1);
       if (b) {
@@ -1880,42 +1885,50 @@ int or_stats(or_state* s) {
     for (int jj = 0; jj < s->nyv; ++jj) {
       const int j = s->jlo_v + jj;
       double *Pv = B, *Pw = B + psz, *Wx = B + 2 * psz, *D = B + 3 * psz;
-      to_plane(s, U2, s->nyv, s->jlo_v, j, Pv);
+      const int cen = central_plane(s, 1, j);
+      const pview v = view_of(s, cen);
+      const int vig = v.ngx + 2, vmx = v.N1 / 2 + 1;
+      const size_t vsz = (size_t)vig * v.ngz;
+      to_plane_v(s, &v, U2, s->nyv, s->jlo_v, j, Pv);
       /* der_z of the u2 plane (FOU3D.f90:373-413): modes 0..N1/2 of the
          slots below N2/2 and above Ngz-N2/2, zero elsewhere (the slot of
-         kz = -N2/2 included) */
-      memset(Wx, 0, sizeof(double) * psz);
-      for (int sl = 0; sl < ngz; ++sl) {
+         kz = -N2/2 included); k1z by the mode's index in the wall bands'
+         numbering */
+      memset(Wx, 0, sizeof(double) * vsz);
+      for (int sl = 0; sl < v.ngz; ++sl) {
         int k;
-        if (sl < s->N2 / 2) k = sl;
-        else if (sl >= ngz - s->N2 / 2 + 1) k = sl - ngz + s->N2;
+        if (sl < v.N2 / 2) k = sl;
+        else if (sl >= v.ngz - v.N2 / 2 + 1) k = sl - v.ngz + s->N2;
         else continue;
-        for (int i = 0; i < s->Mx; ++i) {
-          const cplx v = imul(s->k1z[k], pl(s, Pv, i, sl));
-          Wx[sl * igal + 2 * i] = creal(v);
-          Wx[sl * igal + 2 * i + 1] = cimag(v);
+        for (int i = 0; i < vmx; ++i) {
+          const cplx z = imul(s->k1z[k], pl_v(&v, Pv, i, sl));
+          Wx[sl * vig + 2 * i] = creal(z);
+          Wx[sl * vig + 2 * i + 1] = cimag(z);
         }
       }
       /* der_yv_h_wx (FOU3D.f90:495-514) */
       for (int col = 0; col < s->Ncol; ++col)
         d[col] = (U3[fu(s, col, j + 1)] - U3[fu(s, col, j)]) * DTHV(s, j) * s->ddthetavi;
-      /* modes_to_planes of the columns into a zeroed plane; only the u2
-         placement wrote other slots, and those were zero */
-      memset(D, 0, sizeof(double) * psz);
+      /* modes_to_planes of the columns into a zeroed plane (a central plane:
+         the long columns); only the u2 placement wrote other slots, and those
+         were zero */
+      memset(D, 0, sizeof(double) * vsz);
       for (int col = 0; col < s->Ncol; ++col) {
-        const int i = col % s->Mx, k = col / s->Mx, sl = slot_of(s, k);
-        D[sl * igal + 2 * i] = creal(d[col]);
-        D[sl * igal + 2 * i + 1] = cimag(d[col]);
+        if (cen && s->shortc[col]) continue;
+        const int i = col % s->Mx, k = col / s->Mx, sl = vslot(s, &v, k);
+        D[sl * vig + 2 * i] = creal(d[col]);
+        D[sl * vig + 2 * i + 1] = cimag(d[col]);
       }
-      for (size_t e = 0; e < psz; ++e) Wx[e] = -Wx[e] + D[e];
-      four_to_phys(s, Wx, 0, &w);
-      to_plane(s, s->u3i, s->nyv, s->jlo_v, j, Pw);
-      four_to_phys(s, Pv, 1, &w);
-      four_to_phys(s, Pw, 1, &w);
-      sum1(&t->prof[ST_V][jj], &t->prof[ST_V2][jj], Pv, igal, ngx, ngz);
-      /* wx sums run over i = 1..N1, k = 1..N2 of the physical plane (stats.f90:56-62) */
-      sum1(&t->prof[ST_WX][jj], &t->prof[ST_WX2][jj], Wx, igal, s->N1, s->N2);
-      sum2(&t->prof[ST_VW][jj], Pv, Pw, igal, ngx, ngz);
+      for (size_t e = 0; e < vsz; ++e) Wx[e] = -Wx[e] + D[e];
+      four_to_phys_v(&v, Wx, 0, &w);
+      to_plane_v(s, &v, s->u3i, s->nyv, s->jlo_v, j, Pw);
+      four_to_phys_v(&v, Pv, 1, &w);
+      four_to_phys_v(&v, Pw, 1, &w);
+      sum1(&t->prof[ST_V][jj], &t->prof[ST_V2][jj], Pv, vig, v.ngx, v.ngz);
+      /* wx sums run over i = 1..N1, k = 1..N2 of the physical plane
+         (stats.f90:56-62), the band's N1, N2 */
+      sum1(&t->prof[ST_WX][jj], &t->prof[ST_WX2][jj], Wx, vig, v.N1, v.N2);
+      sum2(&t->prof[ST_VW][jj], Pv, Pw, vig, v.ngx, v.ngz);
       int j1;
       const int b = band_of(t, 1, j, &j1);
       if (b) {
@@ -1936,9 +1949,10 @@ int or_stats(or_state* s) {
 #pragma omp for schedule(dynamic, 1)
     for (int jj = 0; jj < s->nyp; ++jj) {
       const int j = s->jlo_p + jj;
-      to_plane(s, PP, s->nyp, s->jlo_p, j, Pp);
-      four_to_phys(s, Pp, 0, &w);
-      sum1(&t->prof[ST_P][jj], &t->prof[ST_P2][jj], Pp, igal, ngx, ngz);
+      const pview v = view_of(s, central_plane(s, 2, j));
+      to_plane_v(s, &v, PP, s->nyp, s->jlo_p, j, Pp);
+      four_to_phys_v(&v, Pp, 0, &w);
+      sum1(&t->prof[ST_P][jj], &t->prof[ST_P2][jj], Pp, v.ngx + 2, v.ngx, v.ngz);
       int j1;
       const int b = band_of(t, 2, j, &j1);
       if (b) cond1(t, b == 1 ? t->cb[ST_P] : t->ct[ST_P], b == 1 ? t->cb[ST_P2] : t->ct[ST_P2], Pp, igal, j1, 1.0,

@@ -105,7 +105,10 @@ void or_set_threads(int n);
    band plane counts of the u, v, p grids and istat; or_stats_get copies one
    accumulator (part 0 profile, 1 bottom / 2 top conditional sums
    (dnx, dnz, j1), 3 spectrum sp(j, column) of U V W UV P); or_stats_reset
-   zeroes everything. */
+   zeroes everything.  With a coarser central band its planes are summed on
+   the central band's grid (the long columns only), and the spectra cover the
+   long columns (the midband's, spectra.f90:1-59); the short columns' spectra
+   stay zero. */
 int  or_stats_init(or_state* s, const int* N20);
 int  or_stats(or_state* s);
 void or_stats_dims(const or_state* s, int* d);

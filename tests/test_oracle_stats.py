@@ -201,3 +201,84 @@ def test_unordered_band_table_is_refused():
     assert N[7] > N[11]
     with pytest.raises(ValueError):
         o.stats_init(N)
+
+
+def _phys_band(p, f, coarse):
+    """(Ncol, ny) spectral (the wall bands' column numbering) -> (ny, Ngz_b,
+    Ngx_b) physical planes on a band's grid: the wall bands' (every column)
+    or, coarse, the central band's (the long columns at their slots of the
+    Ngalz2 transform, start.f90:1561-1573)."""
+    if not coarse:
+        return _phys(p, f)
+    gx, gz = p.ngalx2, p.ngalz2
+    k = np.arange(p.N2)
+    slot = np.where(k < p.N2 // 2, k, k - p.N2 + gz)
+    C = np.zeros((f.shape[1], gz, gx // 2 + 1), dtype=complex)
+    long_ = ~p.short_columns()
+    for col in np.nonzero(long_)[0]:
+        i, kk = col % p.Mx, col // p.Mx
+        C[:, slot[kk], i] = f[col]
+    return np.fft.irfft(np.fft.ifft(C, axis=1) * gz, n=gx, axis=2) * gx
+
+
+def test_multiband_plane_sums_spectra_and_omega_x():
+    """With a coarser central band (24 x 48 / 12 x 24, tanh grid, posts) the
+    central planes are summed on the central band's grid from the long
+    columns (u planes N(4,1)+1..N(4,2), v N(3,1)+1..N(3,2), p
+    N(4,1)..N(4,2)+1, the band-2 ranks' planes), the other planes on the wall
+    bands' grid from every column: plane sums of u, w, v, p and their squares
+    against numpy's inverse FFTs on each plane's grid; omega_x of a central
+    plane (-i kz v of the long columns, kz = -N2c/2 dropped by the central
+    band's der_z, plus the y derivative of u3) over the central band's N1c x
+    N2c corner; the spectra of the long columns, zero for the short ones."""
+    from .test_gpu_multiband import MB, mask_bands
+    p = MB
+    o = Oracle(p)
+    rng = np.random.default_rng(7)
+    f = {n: mask_bands(p, n, random_field(p, n, rng)) for n in ("u1", "u2", "u3", "p")}
+    for n, a in f.items():
+        o.set_field(n, a)
+    N = fieldfile.band_table(p)
+    o.stats_init(N)
+    o.stats()
+    st = o.stats_get()
+    N3 = [N[4 * b + 2] for b in range(4)]
+    N4 = [N[4 * b + 3] for b in range(4)]
+    short = p.short_columns()
+    for s_, fld in (("spU", "u1"), ("spV", "u2"), ("spW", "u3"), ("spP", "p")):
+        sp = st["spec"][s_]
+        assert np.allclose(sp[~short], np.abs(f[fld][~short]) ** 2, rtol=1e-15, atol=0), s_
+        assert (sp[short] == 0).all(), s_
+    central = {"u": lambda j: N4[1] < j <= N4[2], "v": lambda j: N3[1] < j <= N3[2],
+               "p": lambda j: N4[1] <= j <= N4[2] + 1}
+    jlo = {"u": -p.dsty, "v": -p.dsty, "p": -p.dsty + 1}
+    ph = {c: {n: _phys_band(p, f[n], c) for n in f} for c in (False, True)}
+    seen = 0
+    for m, m2, fld, g in (("Um", "U2m", "u1", "u"), ("Wm", "W2m", "u3", "u"), ("Vm", "V2m", "u2", "v"),
+                          ("Pm", "P2m", "p", "p")):
+        for q in range(f[fld].shape[1]):
+            j = jlo[g] + q
+            c = central[g](j)
+            seen += c
+            x = ph[c][fld][q]
+            scale = np.abs(x).sum() + 1e-300
+            assert abs(st["prof"][m][q] - x.sum()) <= 1e-12 * scale, (m, j, c)
+            assert abs(st["prof"][m2][q] - (x ** 2).sum()) <= 1e-12 * (x ** 2).sum() + 1e-300, (m2, j, c)
+    assert seen > 0
+    # omega_x on the central v planes
+    dthdyv = o.ygrid()["dthdyv"]
+    kk = np.arange(p.Ncol) // p.Mx
+    kz = p.bet * np.where(kk < p.N2 // 2, kk, kk - p.N2)
+    dz = -1j * kz[:, None] * f["u2"]
+    dz[kk == p.N2 - p.N2c // 2] = 0.0  # kz = -N2c/2: no central der_z term
+    dz[kk == p.N2 // 2] = 0.0
+    W = dz + (f["u3"][:, 1:] - f["u3"][:, :-1]) * dthdyv[None, :]
+    wxc = _phys_band(p, W, True)
+    for q in range(W.shape[1]):
+        j = jlo["v"] + q
+        if not central["v"](j):
+            continue
+        corner = wxc[q, :p.N2c, :p.N1c]
+        scale = np.abs(wxc[q]).max() * corner.size
+        assert abs(st["prof"]["wxm"][q] - corner.sum()) <= 1e-12 * scale, j
+        assert abs(st["prof"]["wx2m"][q] - (corner ** 2).sum()) <= 1e-12 * (corner ** 2).sum(), j
