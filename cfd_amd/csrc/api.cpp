@@ -716,6 +716,9 @@ int setup_bands(cfd_solver* s) {
   if ((rc = create_impl(&pc, &t, &hc))) return fail(s, rc, std::string("central band solver: ") + g_err);
   cfd_solver* c = static_cast<cfd_solver*>(hc);
   s->mbc = c;
+  // its band table (statistics) is this solver's: same y grid, same
+  // geometry type (its IB lists, built at creation, stay empty)
+  c->P.geometry_type = p.geometry_type;
   if (c->nyu != s->nyu || c->nyv != s->nyv || c->Ncol != Mxc * s->N2c)
     return fail(s, CFD_EINVAL, "central band solver: planes or columns differ");
   HIPCHK(s, hipSetDevice(s->dev));
@@ -984,7 +987,9 @@ int stats_setup(cfd_solver* s) {
   auto& S = s->st;
   if (S.ready) return CFD_OK;
   const cfd_params& p = s->P;
-  if (p.ntilex < 1 || p.ntilez < 1 || s->ngx / p.ntilex < 1 || s->ngz / p.ntilez < 1)
+  // a central solver (multi-band) sums profiles only: no tiles of its own
+  const bool central = s->borrowed_stream;
+  if (!central && (p.ntilex < 1 || p.ntilez < 1 || s->ngx / p.ntilex < 1 || s->ngz / p.ntilez < 1))
     return fail(s, CFD_EINVAL, "statistics: ntilex and ntilez must give tiles of at least one point");
   band_table(p, s->yg, S.N);
   auto NB = [&](int r, int b) { return S.N[4 * b + r - 1]; };
@@ -993,8 +998,8 @@ int stats_setup(cfd_solver* s) {
       return fail(s, CFD_EINVAL,
                   "statistics: the band limits N(r,0:3) are not in order (the wall bands overlap), so the "
                   "reference's plane partition has no wall bands to condition on");
-  S.dnx = s->ngx / p.ntilex;  // start.f90:456-457
-  S.dnz = s->ngz / p.ntilez;
+  S.dnx = central ? 1 : s->ngx / p.ntilex;  // start.f90:456-457
+  S.dnz = central ? 1 : s->ngz / p.ntilez;
   S.bot[0][0] = NB(4, 0), S.bot[0][1] = NB(4, 1), S.top[0][0] = NB(4, 2) + 1, S.top[0][1] = NB(4, 3) + 1;
   S.bot[1][0] = NB(3, 0), S.bot[1][1] = NB(3, 1), S.top[1][0] = NB(3, 2) + 1, S.top[1][1] = NB(3, 3) + 1;
   S.bot[2][0] = NB(4, 0) + 1, S.bot[2][1] = NB(4, 1) - 1, S.top[2][0] = NB(4, 2) + 2, S.top[2][1] = NB(4, 3);
@@ -1122,9 +1127,18 @@ int stats_setup(cfd_solver* s) {
   return CFD_OK;
 }
 
-// one sample of spectra and statistics, queued on the solver's stream
+// one sample of spectra and statistics, queued on the solver's stream.  With
+// a coarser central band the central solver samples the long columns of
+// every plane too; its profiles give the central planes' (stats_write).
 int stats_sample(cfd_solver* s) {
   if (int rc = stats_setup(s)) return rc;
+  if (s->mb) {
+    launch_band_gather(*s, 0, std::max(s->nyu, std::max(s->nyv, s->nyp)), true);
+    if (int rc = stats_sample(static_cast<cfd_solver*>(s->mbc))) return fail(s, rc, s->mbc->err);
+  }
+  // a central solver gives profiles only: the spectra and the wall bands'
+  // conditional sums are its parent's
+  const bool central = s->borrowed_stream;
   auto& S = s->st;
   StatsSpecArgs A{};
   A.u1 = s->F[CFD_U1];
@@ -1141,7 +1155,7 @@ int stats_sample(cfd_solver* s) {
   A.nyu = s->nyu;
   A.nyv = s->nyv;
   A.nyp = s->nyp;
-  launch_spectra(*s, A);
+  if (!central) launch_spectra(*s, A);
   for (int g = 0; g < 3; ++g) {
     const int ny = ny_grid(*s, g), nf = g == 2 ? 1 : 3;
     for (int q0 = 0; q0 < ny; q0 += S.chunk) {
@@ -1153,20 +1167,43 @@ int stats_sample(cfd_solver* s) {
       launch_zinv(*s, S.d_zj[g] + (size_t)q0 * nf, len * nf, false);
       launch_xpass_inv2(*s, S.d_xj[g] + (size_t)nf * (q0 / 2), nf * ((len + 1) / 2));
       launch_stat_sums(*s, S.d_sum[g], S.nsum[g], len, q0);
-      launch_stat_cond(*s, S.d_cond[g], S.ncond[g], len, S.dnx, S.dnz, jlo_grid(*s, g) + q0);
+      if (!central) launch_stat_cond(*s, S.d_cond[g], S.ncond[g], len, S.dnx, S.dnz, jlo_grid(*s, g) + q0);
     }
   }
   S.istat += 1;
   return launched(s);
 }
 
+// zero the accumulators (stats.f90:396-426, spectra.f90:124-128)
+int stats_zero(cfd_solver* s) {
+  auto& S = s->st;
+  const size_t cell = (size_t)S.dnx * S.dnz;
+  for (int q = 0; q < SQ_N; ++q) {
+    const int g = kStatGrid[q];
+    HIPCHK(s, hipMemsetAsync(S.prof[q], 0, ny_grid(*s, g) * sizeof(double), s->stream));
+    HIPCHK(s, hipMemsetAsync(S.cb[q], 0, cell * S.nyc[g] * sizeof(double), s->stream));
+    HIPCHK(s, hipMemsetAsync(S.ct[q], 0, cell * S.nyc[g] * sizeof(double), s->stream));
+  }
+  for (int q = 0; q < 5; ++q)
+    HIPCHK(s, hipMemsetAsync(S.sp[q], 0, (size_t)s->Ncol * ny_grid(*s, kSpecGrid[q]) * sizeof(double), s->stream));
+  S.istat = 0;
+  return CFD_OK;
+}
+
 // write_spect + write_stats (spectra.f90:62-182, stats.f90:255-442), then
-// zero the accumulators.  The header carries t_hdr and mpgx_hdr; the file
-// names end in _<N1>x<N2>x<ny>_<int(10 t_name)>.dat, the stats file directly
-// after dirout ('stats_', no separator, stats.f90:357) and the spectra after
-// dirout/ ('/spU_' ..., spectra.f90:99-108).
+// zero the accumulators.  With a coarser central band the central planes'
+// profiles are the central solver's (their sums run over the ngx2 x ngz2
+// grid) and the spectra cover the long columns, folded into the central
+// band's N1c/2+1 x N2c/2+1 modes (buffSP of the midband, spectra.f90:
+// 122-170).  The header carries t_hdr and mpgx_hdr; the file names end in
+// _<N1>x<N2>x<ny>_<int(10 t_name)>.dat, the stats file directly after dirout
+// ('stats_', no separator, stats.f90:357) and the spectra after dirout/
+// ('/spU_' ..., spectra.f90:99-108).
 int stats_write(cfd_solver* s, const char* dirout, int iter, double t_hdr, double mpgx_hdr, double t_name) {
   if (int rc = stats_setup(s)) return rc;
+  cfd_solver* c = static_cast<cfd_solver*>(s->mbc);
+  if (s->mb)
+    if (int rc = stats_setup(c)) return fail(s, rc, c->err);
   auto& S = s->st;
   HIPCHK(s, hipStreamSynchronize(s->stream));
   const size_t cell = (size_t)S.dnx * S.dnz;
@@ -1180,6 +1217,16 @@ int stats_write(cfd_solver* s, const char* dirout, int iter, double t_hdr, doubl
     HIPCHK(s, hipMemcpy(t.data(), S.ct[q], t.size() * sizeof(double), hipMemcpyDeviceToHost));
     cond[q].resize(b.size());
     for (size_t e = 0; e < b.size(); ++e) cond[q][e] = b[e] + t[e];  // xC = xC + buffC (recvXC)
+    if (s->mb) {
+      // the central planes: u N(4,1)+1..N(4,2), v N(3,1)+1..N(3,2), p
+      // N(4,1)..N(4,2)+1 (the plane ranges of band 2's ranks,
+      // start.f90:1650-1951)
+      const int j0 = g == 0 ? s->N4[1] + 1 : g == 1 ? s->N3[1] + 1 : s->N4[1];
+      const int j1 = g == 0 ? s->N4[2] : g == 1 ? s->N3[2] : s->N4[2] + 1;
+      const int q0 = j0 - jlo_grid(*s, g), n = j1 - j0 + 1;
+      if (q0 < 0 || q0 + n > (int)prof[q].size()) return fail(s, CFD_EINVAL, "statistics: central planes off the grid");
+      HIPCHK(s, hipMemcpy(prof[q].data() + q0, c->st.prof[q] + q0, n * sizeof(double), hipMemcpyDeviceToHost));
+    }
   }
   cfd_file_header hd{};
   hd.t = t_hdr;
@@ -1193,15 +1240,18 @@ int stats_write(cfd_solver* s, const char* dirout, int iter, double t_hdr, doubl
   int Ngal[20];
   std::memcpy(Ngal, S.N, sizeof Ngal);
   for (int b = 1; b <= 3; ++b) {  // Ngal(1:2, band) = the physical sizes (start.f90:92-100)
-    Ngal[4 * b] = s->ngx;
-    Ngal[4 * b + 1] = s->ngz;
+    Ngal[4 * b] = s->mb && b == 2 ? s->ngx2 : s->ngx;
+    Ngal[4 * b + 1] = s->mb && b == 2 ? s->ngz2 : s->ngz;
   }
   char tag[64];
   snprintf(tag, sizeof tag, "%04dx%04dx%04d_%05d.dat", s->N1, s->N2, s->ny, (int)(10.0 * t_name));
-  // spectra: fold -kz onto kz (k > N2/2 -> N2 + 2 - k, so kz = -N2/2 lands in
-  // the last row), double every kx > 0 (recvwrspec, spectra.f90:142-170)
+  // spectra: fold -kz onto kz (k > N2/2 -> N2 + 2 - k with the wall bands'
+  // N2, so kz = -N2/2, or -N2c/2 of a coarser central band, lands in the last
+  // row), double every kx > 0 (recvwrspec, spectra.f90:129-170); the rows and
+  // columns are the midband's (N1c/2+1 x N2c/2+1 with a coarser central
+  // band, whose short columns have no spectra)
   const char* spname[5] = {"spU", "spV", "spW", "spUV", "spP"};
-  const int mx = s->Mx, nzh = s->N2 / 2 + 1;
+  const int mx = s->mb ? s->N1c / 2 + 1 : s->Mx, nzh = (s->mb ? s->N2c : s->N2) / 2 + 1;
   for (int q = 0; q < 5; ++q) {
     const int ny = ny_grid(*s, kSpecGrid[q]);
     std::vector<double> raw((size_t)s->Ncol * ny), buf((size_t)mx * nzh * ny, 0.0);
@@ -1211,10 +1261,11 @@ int stats_write(cfd_solver* s, const char* dirout, int iter, double t_hdr, doubl
         const bool neg = k >= s->N2 / 2;
         if (neg != (pass == 1)) continue;
         const int kk = neg ? s->N2 - k : k;  // 0-based row of buffSP(., 1:N2/2+1, .)
+        if (neg ? kk > nzh - 1 : k >= nzh - 1) continue;  // a short column's kz (coarser central band)
         for (int jq = 0; jq < ny; ++jq)
           for (int i = 0; i < mx; ++i) {
             double& d = buf[((size_t)jq * nzh + kk) * mx + i];
-            const double v = raw[(size_t)jq * s->Ncol + (size_t)k * mx + i];
+            const double v = raw[(size_t)jq * s->Ncol + (size_t)k * s->Mx + i];
             d = pass == 0 ? v : d + v;
           }
       }
@@ -1227,17 +1278,9 @@ int stats_write(cfd_solver* s, const char* dirout, int iter, double t_hdr, doubl
   const std::string path = std::string(dirout) + "stats_" + tag;
   const std::string e = write_stats_file(path.c_str(), hd, Ngal, s->yg, S.istat, prof, cond);
   if (!e.empty()) return fail(s, CFD_EINVAL, "write_stats: " + e);
-  // zero (stats.f90:396-426, spectra.f90:124-128)
-  for (int q = 0; q < SQ_N; ++q) {
-    const int g = kStatGrid[q];
-    HIPCHK(s, hipMemsetAsync(S.prof[q], 0, ny_grid(*s, g) * sizeof(double), s->stream));
-    HIPCHK(s, hipMemsetAsync(S.cb[q], 0, cell * S.nyc[g] * sizeof(double), s->stream));
-    HIPCHK(s, hipMemsetAsync(S.ct[q], 0, cell * S.nyc[g] * sizeof(double), s->stream));
-  }
-  for (int q = 0; q < 5; ++q)
-    HIPCHK(s, hipMemsetAsync(S.sp[q], 0, (size_t)s->Ncol * ny_grid(*s, kSpecGrid[q]) * sizeof(double), s->stream));
-  S.istat = 0;
-  return CFD_OK;
+  if (s->mb)
+    if (int rc = stats_zero(c)) return fail(s, rc, c->err);
+  return stats_zero(s);
 }
 
 int read_scalars(cfd_solver* s, DevScalars& d) {
@@ -1981,14 +2024,12 @@ extern "C" int cfd_four_to_phys_planes(cfd_handle h, int which, int plane0, int 
 extern "C" int cfd_stats(cfd_handle h) {
   cfd_solver* s = static_cast<cfd_solver*>(h);
   if (int rc = check(s, 1)) return rc;
-  if (int rc = one_band(s, "cfd_stats")) return rc;
   return stats_sample(s);
 }
 
 extern "C" int cfd_write_stats(cfd_handle h, const char* dirout, int iter) {
   cfd_solver* s = static_cast<cfd_solver*>(h);
   if (int rc = check(s, 1)) return rc;
-  if (int rc = one_band(s, "cfd_write_stats")) return rc;
   if (!dirout) return fail(s, CFD_EINVAL, "null dirout");
   DevScalars d{};
   if (int rc = read_scalars(s, d)) return rc;
@@ -1998,7 +2039,6 @@ extern "C" int cfd_write_stats(cfd_handle h, const char* dirout, int iter) {
 extern "C" int cfd_stats_request(cfd_handle h, int flags, const char* dirout, int iter) {
   cfd_solver* s = static_cast<cfd_solver*>(h);
   if (int rc = check(s, 1)) return rc;
-  if (int rc = one_band(s, "cfd_stats_request")) return rc;
   if (flags < 0 || flags > 3) return fail(s, CFD_EINVAL, "flags must be 0..3 (bit 0 sample, bit 1 write)");
   if ((flags & 2) && !dirout) return fail(s, CFD_EINVAL, "null dirout");
   if (flags)
@@ -2012,7 +2052,6 @@ extern "C" int cfd_stats_request(cfd_handle h, int flags, const char* dirout, in
 extern "C" int cfd_stats_dims(cfd_handle h, int* d) {
   cfd_solver* s = static_cast<cfd_solver*>(h);
   if (int rc = check(s, 1)) return rc;
-  if (int rc = one_band(s, "cfd_stats_dims")) return rc;
   if (!d) return fail(s, CFD_EINVAL, "null argument");
   if (int rc = stats_setup(s)) return rc;
   const auto& S = s->st;

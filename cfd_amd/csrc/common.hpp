@@ -270,6 +270,9 @@ void launch_mf_u11(Solver& s, int kRK, hipStream_t st);  // u11 and Q(u11) into 
 void launch_mf_apply(Solver& s);                       // g from Q(u), u += g u11
 // multi-band resolution (kernels_band.hip)
 void launch_band_gather(Solver& s);   // long columns of u1, u2, u3 -> the central solver's fields
+// the same over planes [q0, q1) of every field (q1 at most the largest plane
+// count; each field stops at its own), p too with with_p
+void launch_band_gather(Solver& s, int q0, int q1, bool with_p);
 void launch_band_scatter(Solver& s);  // central planes of the nonlinear scratch <- the central solver
 void launch_cfl_bands(Solver& s);     // dt from the wall bands' and the central band's maxima
 // no-allocation physical planes (noalloc.hip): the file-scope scratch's

@@ -18,17 +18,21 @@ namespace {
 
 constexpr int kTB = 256;
 
-// grid (ceil(Ncolc / kTB), q1 - q0, 3): field f, plane q0 + y, central column c
-__global__ void k_band_gather(const double2* __restrict__ u1, const double2* __restrict__ u2,
-                              const double2* __restrict__ u3, double2* __restrict__ c1, double2* __restrict__ c2,
-                              double2* __restrict__ c3, const int* __restrict__ lcol, int Ncol, int Ncolc, int nyu,
-                              int nyv, int q0) {
+struct GatherArgs {
+  const double2* in[4];  // u1, u2, u3, p
+  double2* out[4];       // the same fields of the central solver
+  const int* lcol;       // per central column: the column here
+  int Ncol, Ncolc;
+  int ny[4];             // planes of each field
+  int q0;
+};
+
+// grid (ceil(Ncolc / kTB), q1 - q0, nf): field f, plane q0 + y, central column c
+__global__ void k_band_gather(GatherArgs a) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  const int q = q0 + blockIdx.y, f = blockIdx.z;
-  if (c >= Ncolc || q >= (f == 1 ? nyv : nyu)) return;
-  const double2* src = f == 0 ? u1 : (f == 1 ? u2 : u3);
-  double2* dst = f == 0 ? c1 : (f == 1 ? c2 : c3);
-  dst[(size_t)q * Ncolc + c] = src[(size_t)q * Ncol + lcol[c]];
+  const int q = a.q0 + blockIdx.y, f = blockIdx.z;
+  if (c >= a.Ncolc || q >= a.ny[f]) return;
+  a.out[f][(size_t)q * a.Ncolc + c] = a.in[f][(size_t)q * a.Ncol + a.lcol[c]];
 }
 
 struct ScatterArgs {
@@ -82,12 +86,28 @@ __global__ void k_cfl_bands(DevScalars* sc, const DevScalars* cc, double alp, do
 
 }  // namespace
 
-void launch_band_gather(Solver& s) {
+void launch_band_gather(Solver& s, int q0, int q1, bool with_p) {
   Solver& c = *s.mbc;
+  GatherArgs a;
+  const int fid[4] = {CFD_U1, CFD_U2, CFD_U3, CFD_P};
+  for (int f = 0; f < 4; ++f) {
+    a.in[f] = s.F[fid[f]];
+    a.out[f] = c.F[fid[f]];
+  }
+  a.lcol = s.d_lcol;
+  a.Ncol = s.Ncol;
+  a.Ncolc = c.Ncol;
+  a.ny[0] = a.ny[2] = s.nyu;
+  a.ny[1] = s.nyv;
+  a.ny[3] = s.nyp;
+  a.q0 = q0;
+  const dim3 grid((c.Ncol + kTB - 1) / kTB, q1 - q0, with_p ? 4 : 3);
+  hipLaunchKernelGGL(k_band_gather, grid, dim3(kTB), 0, s.stream, a);
+}
+
+void launch_band_gather(Solver& s) {
   // the planes the central solver's transforms read
-  const dim3 grid((c.Ncol + kTB - 1) / kTB, s.gather_q[1] - s.gather_q[0], 3);
-  hipLaunchKernelGGL(k_band_gather, grid, dim3(kTB), 0, s.stream, s.F[CFD_U1], s.F[CFD_U2], s.F[CFD_U3], c.F[CFD_U1],
-                     c.F[CFD_U2], c.F[CFD_U3], (const int*)s.d_lcol, s.Ncol, c.Ncol, s.nyu, s.nyv, s.gather_q[0]);
+  launch_band_gather(s, s.gather_q[0], s.gather_q[1], false);
 }
 
 void launch_band_scatter(Solver& s) {

@@ -31,7 +31,8 @@
 !                 0: the reference's call sequence, one entry point per
 !                 subroutine (littleharsh.f90:165-196)
 ! Ngalx2/Ngalz2 below Ngalx1/Ngalz1 (multi-band resolution) run with
-! fused = 1 and without statistics.
+! fused = 1; their statistics sum the central planes on the Ngalx2 x Ngalz2
+! grid and their spectra cover the long columns (stats.f90, spectra.f90).
 ! The time loop's own steps (those without a field record) are timed and the
 ! rate printed as " loop: <steps> steps <seconds> s <substeps/s> substeps/s".
 program littleharsh_gpu
@@ -286,7 +287,7 @@ program littleharsh_gpu
   ! the conditional statistics need wall bands (band limits in order)
   rc = cfd_stats_dims(h, dst)
   if (rc /= 0) then
-    write(*,*) 'statistics off: no wall bands in the band table, or a coarser central band'
+    write(*,*) 'statistics off: no wall bands in the band table'
     nstat = huge(1)
   end if
   rc = cfd_get_scalars(h, sc);         call cfd_check(rc, h, 'get scalars')

@@ -76,12 +76,14 @@ def read_stats(path, dnx, dnz):
 
 
 def read_spectrum(path):
-    """Header dict (with N as 'table') and buffSP as (planes, N2/2+1, N1/2+1)."""
+    """Header dict (with N as 'table') and buffSP as (planes, N2/2+1, N1/2+1)
+    with N1, N2 of the central band (the midband, whose columns the spectra
+    cover)."""
     recs = records(path)
     if len(recs) != 6:
         raise ValueError("%s: %d records, a spectrum file has 6" % (path, len(recs)))
     nyu = (len(recs[2]) // 8 - 1) // 2
     h = _header(recs, nyu)
-    N1, N2 = h["table"][4], h["table"][5]
+    N1, N2 = h["table"][8], h["table"][9]  # the midband's modes (spectra.f90:80-82)
     sp = np.frombuffer(recs[5], dtype="<f8").reshape(-1, N2 // 2 + 1, N1 // 2 + 1).copy()
     return h, sp

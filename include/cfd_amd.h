@@ -271,13 +271,19 @@ int cfd_device_bytes(cfd_handle h, double* bytes);
  *
  * cfd_stats_request(h, flags, dirout, iter): bit 0 sample, bit 1 write; the
  * next nonlinear term with kRK = 1 (in cfd_nonlinear, cfd_substep or
- * cfd_step) serves it where the reference's nonlinear does: the sample of the fields at the start of the
- * substep, the files after the time-step update, with the new t and the
- * start-of-substep mpgx in the headers and the start-of-substep t in the
- * names.
+ * cfd_step) serves it where the reference's nonlinear does: the sample of
+ * the fields at the start of the substep, the files after the time-step
+ * update, with the new t and the start-of-substep mpgx in the headers and
+ * the start-of-substep t in the names.
  *
  * cfd_stats_dims: d[6] = dnx, dnz, wall-band plane counts of the u, v and p
  * grids (the third extent of the conditional records), istat.
+ *
+ * With a coarser central band (Ngalx2/Ngalz2) the central planes (u
+ * N(4,1)+1..N(4,2), v N(3,1)+1..N(3,2), p N(4,1)..N(4,2)+1) are summed on
+ * the Ngalx2 x Ngalz2 grid from the long columns, the stats file's Ngal
+ * table carries Ngalx2, Ngalz2 in band 2, and the spectra cover the long
+ * columns (the midband's) in N1c/2+1 x N2c/2+1 modes (spectra.f90:122-170).
  *
  * All four return CFD_EINVAL when ntilex or ntilez is below 1 or the band
  * table's limits are not in order (no wall bands to condition on). */

@@ -407,6 +407,59 @@ def test_fortran_driver_multiband_shipped_grid(tmp_path):
 
 
 @pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran driver not built")
+def test_fortran_driver_multiband_statistics(tmp_path):
+    """The shipped grid's coarser central band with statistics: nwrite = 2,
+    3 steps (a write of empty sums at iter 0, a sample and a write at iter
+    2).  Every name collides (int(10 t) = 0), so the surviving files are those
+    of iter 2; they equal, byte for byte, the statistics and spectrum files of
+    the same requests and steps driven from Python, hold one sample and carry
+    Ngalx2, Ngalz2 in band 2 of the Ngal table."""
+    from cfd_amd import ic, inputfile, statsfile
+    from oracle.oracle import Oracle
+    from .test_gpu_multiband import SHIPPED_GRID, mask_bands
+    p = inputfile.representable(SHIPPED_GRID)
+    ind, outd, pyd = tmp_path / "in", tmp_path / "out", tmp_path / "py"
+    for d in (ind, outd, pyd):
+        d.mkdir()
+    print("case multi-band shipped grid driver statistics", flush=True)
+    ch = Channel(p)
+    g = ch.ygrid()
+    f = ic.initial_fields(p, g, seed=2, amp=0.1)
+    f = ic.project(Oracle(p), {k: mask_bands(p, k, v) for k, v in f.items()})
+    for k in ("u1", "u2", "u3", "p"):
+        ch.set_field(k, f[k])
+    ch.set_scalars(t=0.0, dt=0.0, mpgx=p.mpgx, QxT=ic.flowrate(f["u1"], g, p.ny))
+    fn = os.path.basename(ch.record_out(str(ind) + "/", 0))
+    ch.close()
+    inputfile.write(str(tmp_path), p, flag_init=1, fnameimb=fn, dirin="./in/", dirout="./out/", nwrite=2)
+    out = subprocess.run([EXE, "3"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "statistics off" not in out.stdout
+    py = Channel(p)
+    try:
+        py.read_in(str(ind) + "/", fn, 1)
+        for it in range(3):
+            if it == 0:
+                py.stats_request(sample=False, write_dir=str(pyd) + "/", iter=0)
+            if it == 2:
+                py.stats_request(sample=True, write_dir=str(pyd) + "/", iter=2)
+            py.step(1)
+        dims = py.stats_dims()
+    finally:
+        py.close()
+    names = sorted(x for x in os.listdir(outd) if x.startswith(("stats_", "sp")))
+    assert len(names) == 6, names
+    assert names == sorted(os.listdir(pyd)), (names, sorted(os.listdir(pyd)))
+    for x in names:
+        assert (outd / x).read_bytes() == (pyd / x).read_bytes(), x
+    st = [x for x in names if x.startswith("stats_")][0]
+    h, prof, _ = statsfile.read_stats(str(outd / st), dims["dnx"], dims["dnz"])
+    assert h["istat"] == 1 and h["iter"] == 2
+    assert h["table"][8:10] == [p.ngalx2, p.ngalz2] and h["table"][4:6] == [p.ngalx, p.ngalz]
+    assert all(np.isfinite(v).all() for v in prof.values()) and np.abs(prof["U2m"]).max() > 0
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran driver not built")
 def test_fortran_driver_multiband_refuses_operator_sequence(tmp_path):
     """fused = 0 (one entry point per subroutine) has no multi-band path: the
     driver stops with a message before creating a solver."""

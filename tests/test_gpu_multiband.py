@@ -172,8 +172,7 @@ def test_multiband_per_operator_entry_points_refused():
     """The operator-by-operator entry points exist for one resolution; with a
     coarser central band they return CFD_EINVAL with a message."""
     ch = Channel(MB_SMOOTH)
-    for call in (lambda: ch.nonlinear(1), lambda: ch.solveP(1), lambda: ch.RHS0(1), lambda: ch.four_to_phys("u1"),
-                 lambda: ch.stats()):
+    for call in (lambda: ch.nonlinear(1), lambda: ch.solveP(1), lambda: ch.RHS0(1), lambda: ch.four_to_phys("u1")):
         with pytest.raises(CfdError, match="central band"):
             call()
 
@@ -327,3 +326,92 @@ def test_multiband_member_in_an_ensemble():
         assert np.array_equal(a.get_field(n), b.get_field(n)), n
     assert a.scalars() == b.scalars()
     _cmp(a, orc, ("u1", "u2", "u3", "p"), 1e-11)
+
+
+def _fold_long(raw, p):
+    """recvwrspec (spectra.f90:129-170) for the midband's columns: (Ncol, ny)
+    per-column sums of the long columns -> buffSP (ny, N2c/2+1, N1c/2+1);
+    kz >= 0 at row kz, kz < 0 at row -kz (N(2,1)+2-k with the wall bands'
+    N2), kx > 0 doubled."""
+    ny = raw.shape[1]
+    r = raw.T.reshape(ny, p.N2, p.Mx)
+    mx, h = p.N1c // 2 + 1, p.N2c // 2
+    out = np.zeros((ny, h + 1, mx))
+    out[:, :h] = r[:, :h, :mx]
+    for k in range(p.N2 - h, p.N2):
+        out[:, p.N2 - k] += r[:, k, :mx]
+    out[:, :, 1:] *= 2.0
+    return out
+
+
+def _check_stats(p, d, tag, dims, ref, istat, it, tol_sp, tol):
+    from cfd_amd import fieldfile, statsfile
+    from .test_gpu_stats import _scale
+    N = fieldfile.band_table(p)
+    h, prof, cond = statsfile.read_stats(d + "stats_" + tag, dims["dnx"], dims["dnz"])
+    assert h["istat"] == istat and h["iter"] == it
+    ngal = list(N)
+    for b in (1, 2, 3):
+        ngal[4 * b:4 * b + 2] = [p.ngalx2, p.ngalz2] if b == 2 else [p.ngalx, p.ngalz]
+    assert h["table"] == ngal
+    for s_ in ("spU", "spV", "spW", "spUV", "spP"):
+        hs, sp = statsfile.read_spectrum(d + s_ + "_" + tag)
+        assert hs["table"] == N and hs["istat"] == istat
+        e = _fold_long(ref["spec"][s_], p)
+        assert sp.shape == e.shape, s_
+        t = tol_sp if tol_sp is not None else (1e-14 if s_ == "spUV" else 0.0)
+        assert np.abs(sp - e).max() <= t * np.abs(e).max(), (s_, np.abs(sp - e).max() / np.abs(e).max())
+    R = ref["prof"]
+    npts = p.ngalx * p.ngalz
+    for k, v in prof.items():
+        sc = _scale(R, k, npts)
+        assert np.abs(v - R[k]).max() <= tol * sc, (k, np.abs(v - R[k]).max() / sc)
+    C = {k[:-1]: cb + ct for k, (cb, ct) in ref["cond"].items()}
+    for k, v in cond.items():
+        e = C[k[:-1]]
+        assert v.shape == e.shape, k
+        sc = _scale(C, k[:-1], 2 * p.ntilex * p.ntilez)
+        assert np.abs(v - e).max() <= tol * sc, (k, np.abs(v - e).max() / sc)
+
+
+@pytest.mark.parametrize("case", ["mb_rough", "shipped_grid"])
+def test_multiband_statistics_against_oracle(tmp_path, case):
+    """stats.f90 / spectra.f90 with a coarser central band, written and read
+    back.  One sample from identical state: profiles on every plane (the
+    central planes from the central band's grid) and the wall bands'
+    conditional sums within 1e-12 of their natural scale, spectra of the long
+    columns in the central band's N1c/2+1 x N2c/2+1 layout bit for bit (spUV
+    to 1e-14); the stats file's Ngal table carries Ngalx2, Ngalz2 in band 2
+    and the spectrum files' N table N1c, N2c.  Then, the write having zeroed
+    both resolutions' sums, one RK3 step and a second sample (the two sides'
+    fields then differ at the 1e-12 level): within 1e-10."""
+    from cfd_amd import fieldfile
+
+    p = CASES[case]
+    ch, orc = _pair(p, 400)
+    try:
+        N = fieldfile.band_table(p)
+        assert N[8:10] == [p.N1c, p.N2c]
+        orc.stats_init(N)
+        ch.stats()
+        orc.stats()
+        t = ch.scalars()["t"]
+        d = str(tmp_path) + "/"
+        ch.write_stats(d, iter=5)
+        tag = "%04dx%04dx%04d_%05d.dat" % (ch.N1, ch.N2, p.ny, int(10.0 * t))
+        dims = ch.stats_dims()
+        _check_stats(p, d, tag, dims, orc.stats_get(), 1, 5, None, 1e-12)
+        ch.step(1)
+        orc.step(1)
+        orc.stats_reset()
+        ch.stats()
+        orc.stats()
+        t = ch.scalars()["t"]
+        d2 = str(tmp_path / "second") + "/"
+        os.makedirs(d2)
+        ch.write_stats(d2, iter=6)
+        tag = "%04dx%04dx%04d_%05d.dat" % (ch.N1, ch.N2, p.ny, int(10.0 * t))
+        _check_stats(p, d2, tag, dims, orc.stats_get(), 1, 6, 1e-10, 1e-10)
+    finally:
+        ch.close()
+        orc.close()
