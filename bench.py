@@ -322,6 +322,8 @@ def ensemble_leg(fine_value, fine_points):
             "point_substeps_per_s_single_channel": fine_value * fine_points,
             "points": rep["points"], "tuning_seconds": rep["tuning_seconds"],
             "all_finite": all(m["finite"] for m in rep["members"]),
+            "all_healthy": all(m["healthy"] for m in rep["members"]),
+            "min_dt_over_first_step_dt": min(m["dt"] / m["dt_after_first_step"] for m in rep["members"]),
             "max_div_over_max_u": max(m["max_div_over_max_u"] for m in rep["members"]),
             "tunings": [m["tuning"] for m in rep["members"]]}
 

@@ -13,8 +13,15 @@ uniform in [8, 20], 4 x 2 to 16 x 8 posts per wall, a seeded perturbation
 per member.  The box is Lx = 2 pi, Lz = 2 pi Ngalz / Ngalx (square cells in
 x-z, so that the posts, whose radius is in grid units, are circular).  A post
 narrower than 1.2 cells is widened to 1.2 cells, the smallest post that the
-5-point forcing cross resolves; ``members`` records the D+ drawn and the D+
-simulated.
+5-point forcing cross resolves, and a radius whose forcing stencils
+extrapolate (lin_interp, geom_circ_rough_mixed_forcing_24pt.f90:634-738:
+weights above 1 in magnitude, e.g. 4.3 at r = 1.507 cells and 450 just above
+sqrt(2) for a post centred between grid points, where the forced point's two
+fluid neighbours line up with its boundary point) is moved to the nearest
+radius whose stencils interpolate: with extrapolating weights the forcing
+amplifies the velocity every substep and the run blows up within a few
+steps, on the CPU oracle as on the GPU.  ``members`` records the D+ drawn
+and the D+ simulated.
 
 ``tune`` is the autotuner: per member it times every launch-shape choice
 (FFT engine, rows per segment and threads per block of the segmented y
@@ -75,6 +82,32 @@ def _fits(p):
         return False
 
 
+def ib_max_weight(p):
+    """Largest |weight| of the forcing stencils of both grids (0 without
+    posts); above 1 the stencil extrapolates."""
+    w = 0.0
+    for grid in (0, 1):
+        _, w3 = build_iblist(p, grid, 0)
+        if len(w3):
+            w = max(w, float(np.abs(w3[:, 1:]).max()))
+    return w
+
+
+def _stable_radius(p, step=0.005, span=1.0):
+    """p with the radius nearest to p.r (not below R_MIN) whose post fits
+    its tile and whose forcing stencils interpolate; None when no radius
+    within span does."""
+    for n in range(int(span / step) + 1):
+        for d in ((0.0,) if n == 0 else (n * step, -n * step)):
+            r = p.r + d
+            if r < R_MIN:
+                continue
+            q = p.replace(r=r)
+            if _fits(q) and ib_max_weight(q) <= 1.0:
+                return q
+    return None
+
+
 @dataclasses.dataclass
 class Member:
     params: Params
@@ -99,9 +132,9 @@ def member_params(ngalx, ny, ngalz, re_tau, dplus, kplus, arrays):
         if ngalx % ntx or ngalz % ntz:
             continue
         dnx, dnz = ngalx // ntx, ngalz // ntz
-        p = base.replace(ntilex=ntx, ntilez=ntz, c_i=(dnx + 1) / 2.0, c_k=(dnz + 1) / 2.0)
-        if _fits(p):
-            return p, 2.0 * r * dxp
+        p = _stable_radius(base.replace(ntilex=ntx, ntilez=ntz, c_i=(dnx + 1) / 2.0, c_k=(dnz + 1) / 2.0))
+        if p is not None:
+            return p, 2.0 * p.r * dxp
     raise ValueError("no post array of %s fits a post of r = %.3g cells on %dx%d" % (arrays, r, ngalx, ngalz))
 
 
@@ -265,13 +298,14 @@ def run(n=16, seed=1, steps=50, autotune=True, log=print):
             ch.step(1)
         for ch in chs:
             ch.sync()
+        dt1 = [ch.scalars()["dt"] for ch in chs]
         with Ensemble(chs) as ens:
             t1 = time.perf_counter()
             ens.step(steps - 1)
             ens.sync()
             T = time.perf_counter() - t1
         out = []
-        for m, ch, r in zip(ms, chs, rows):
+        for m, ch, r, d1 in zip(ms, chs, rows, dt1):
             p = m.params
             u1 = ch.get_field("u1")
             sc = ch.scalars()
@@ -283,7 +317,9 @@ def run(n=16, seed=1, steps=50, autotune=True, log=print):
                             t=sc["t"], dt=sc["dt"], mpgx=sc["mpgx"], wall_shear=[bw, tw],
                             max_div_over_max_u=float(div / np.abs(u1).max()),
                             device_bytes=ch.device_bytes(),
-                            finite=bool(np.isfinite(u1).all())))
+                            finite=bool(np.isfinite(u1).all()), dt_after_first_step=d1,
+                            # a run that blows up drives dt towards zero
+                            healthy=bool(np.isfinite(u1).all() and 0.2 * d1 <= sc["dt"] <= 5.0 * d1)))
     finally:
         for ch in chs:
             ch.close()

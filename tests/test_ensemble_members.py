@@ -32,10 +32,16 @@ def test_ranges(ms):
         assert abs(p.posth * m.re_tau - m.kplus) < 1e-12 * m.kplus
         # square cells in x-z: Lx / Ngalx == Lz / Ngalz
         assert abs((2 * np.pi / p.alp) / p.ngalx - (2 * np.pi / p.bet) / p.ngalz) < 1e-12
-        # the post diameter simulated is the one drawn unless the floor applies
+        # the post diameter simulated is the one drawn unless the floor
+        # applies or the drawn radius has extrapolating forcing stencils
         dxp = 2 * np.pi / p.alp * m.re_tau / p.ngalx
         assert abs(2 * p.r * dxp - m.dplus_sim) < 1e-9
-        assert p.r == pytest.approx(max(0.5 * m.dplus / dxp, ensemble.R_MIN), rel=1e-14)
+        r0 = max(0.5 * m.dplus / dxp, ensemble.R_MIN)
+        assert ensemble.ib_max_weight(p) <= 1.0
+        if ensemble.ib_max_weight(p.replace(r=r0)) <= 1.0:
+            assert p.r == pytest.approx(r0, rel=1e-14)
+        else:
+            assert p.r != r0 and abs(p.r - r0) < 0.2
         shapes.add((p.ngalx, p.ny, p.ngalz))
     assert len(shapes) >= 12  # uneven
 
@@ -57,6 +63,25 @@ def test_y_grids_monotonic_and_posts_listed(ms):
             for wall in (0, 1):
                 l9, w3 = build_iblist(p, grid, wall)
                 assert len(l9) > 0
+
+
+def test_extrapolating_radius_is_moved():
+    """A post centred between grid points with r just above sqrt(2) cells:
+    the forced points at distance sqrt(2.5) take two fluid neighbours nearly
+    in line with their boundary point, and lin_interp's weights extrapolate
+    (450 at r = 1.415, 4.3 at r = 1.507); the member draw moves such a radius
+    to the nearest one whose stencils interpolate."""
+    p = ensemble.members(16, seed=1)[-1].params
+    assert (p.ngalx // p.ntilex, p.ngalz // p.ntilez, p.c_i, p.c_k) == (16, 16, 8.5, 8.5)
+    assert ensemble.ib_max_weight(p.replace(r=1.415)) > 100.0
+    assert ensemble.ib_max_weight(p.replace(r=1.507)) > 4.0
+    assert ensemble.ib_max_weight(p.replace(r=1.41)) < 0.1
+    q = ensemble._stable_radius(p.replace(r=1.507))
+    assert q.r == pytest.approx(1.582) and ensemble.ib_max_weight(q) <= 1.0
+    # every member of a few other draws is stable by this measure
+    for seed in (2, 3):
+        for m in ensemble.members(16, seed=seed):
+            assert ensemble.ib_max_weight(m.params) <= 1.0
 
 
 def test_fold_check_catches_a_folded_grid():

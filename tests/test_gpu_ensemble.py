@@ -97,6 +97,7 @@ def test_graph_replays_equal_eager_substeps():
         ch.step(1)  # warm-up substeps, eager on both sides
     for ch in A + B:
         ch.sync()
+    dt1 = [ch.scalars()["dt"] for ch in A]
     with ensemble.Ensemble(A) as ens:
         for rep in range(3):
             ens.step(1)
@@ -115,6 +116,10 @@ def test_graph_replays_equal_eager_substeps():
                 ch.sync()
             for i, (a, b) in enumerate(zip(A, B)):
                 _same(a, b, "substep %d member %d" % (kRK, i))
+    # no member blows up (its CFL time step would collapse within a few
+    # steps, as it does for posts whose forcing stencils extrapolate)
+    for i, (ch, d1) in enumerate(zip(A, dt1)):
+        assert 0.2 * d1 <= ch.scalars()["dt"] <= 5.0 * d1, (i, d1, ch.scalars()["dt"])
     # the members run on by themselves after the ensemble is gone
     A[0].step(1)
     B[0].step(1)
