@@ -359,7 +359,8 @@ int create_impl(const cfd_params* pp, const cfd_tuning* tp, cfd_handle* out) {
   if (s->mixed) s->fft_stockham = false;
   int rc;
   if ((rc = upload(s, &s->d_twz, twiddles(s->ngz))) || (rc = upload(s, &s->d_twx, twiddles(s->ngx))) ||
-      (rc = upload(s, &s->d_twzf, twiddles_f(s->ngz))) || (rc = upload(s, &s->d_twxf, twiddles_f(s->ngx))))
+      (rc = upload(s, &s->d_twzf, twiddles_f(s->ngz))) || (rc = upload(s, &s->d_twxf, twiddles_f(s->ngx))) ||
+      (rc = dalloc(s, &s->d_zrow, (size_t)s->ngx)))
     return bail(rc);
 
   // metrics

@@ -146,6 +146,7 @@ struct Solver {
   int *d_umap = nullptr, *d_vmap = nullptr;
   double2 *d_twz = nullptr, *d_twx = nullptr;
   float2 *d_twzf = nullptr, *d_twxf = nullptr;  // the same tables rounded to FP32
+  double2* d_zrow = nullptr;                      // Ngx zeros: the x-pass's stand-in for an absent row
 
   // job tables
   ZInvJob* d_nljobs = nullptr;

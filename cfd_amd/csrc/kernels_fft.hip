@@ -172,6 +172,10 @@ struct XNLArgs {
   // [s0, s1) (multi-band resolution: each resolution transforms its own
   // planes); {0, nyu, 0, 0} covers all
   int win[4];
+  // a zeroed row of at least Mx elements, read in place of an absent row
+  // (a z beyond Ngz, the v-plane pairs of the last u plane), so that every
+  // load of the four-step x-pass is unconditional
+  const void* zrow;
 };
 
 // plane of y-block b of the window {lo, hi, s0, s1}
@@ -669,27 +673,9 @@ __global__ void __launch_bounds__(TI * cmax(P, Q)) k_zfwd4(ZNLArgs a, const cplx
   }
 }
 
-// Hermitian pair spectrum of a + i b (as pair_mode), any precision
-template <typename C>
-__device__ __forceinline__ C pair_mode_t(const C* __restrict__ A, const C* __restrict__ B, int m, int L, int Mx) {
-  if (m == 0) {
-    const real_t<C> x = A ? A[0].x : 0, y = B ? B[0].x : 0;
-    return mk<C>(x, y);
-  }
-  if (m < Mx) {
-    const C a = A ? A[m] : mk<C>(0, 0), b = B ? B[m] : mk<C>(0, 0);
-    return mk<C>(a.x - b.y, a.y + b.x);
-  }
-  if (m > L - Mx) {
-    const int q = L - m;
-    const C a = A ? A[q] : mk<C>(0, 0), b = B ? B[q] : mk<C>(0, 0);
-    return mk<C>(a.x + b.y, -a.y + b.x);
-  }
-  return mk<C>(0, 0);
-}
-
 // one Hermitian pair mode with the range of m known at compile time where
-// the whole group of Q consecutive m shares it
+// the whole group of Q consecutive m shares it; A and B are valid rows (a
+// zeroed row stands in for an absent one), so the loads are unconditional
 template <int L, int Q, int P0, typename C>
 __device__ __forceinline__ C pair_mode_q(const C* __restrict__ A, const C* __restrict__ B, int m) {
   constexpr int Mx = Dealias<L>::Mx;
@@ -697,13 +683,22 @@ __device__ __forceinline__ C pair_mode_q(const C* __restrict__ A, const C* __res
   if constexpr (lo >= Mx && hi <= L - Mx) {
     return mk<C>(0, 0);
   } else if constexpr (lo > 0 && hi < Mx) {
-    const C a = A ? A[m] : mk<C>(0, 0), b = B ? B[m] : mk<C>(0, 0);
+    const C a = A[m], b = B[m];
     return mk<C>(a.x - b.y, a.y + b.x);
   } else if constexpr (lo > L - Mx) {
-    const C a = A ? A[L - m] : mk<C>(0, 0), b = B ? B[L - m] : mk<C>(0, 0);
+    const C a = A[L - m], b = B[L - m];
     return mk<C>(a.x + b.y, -a.y + b.x);
+  } else if constexpr (lo == 0 && hi < Mx) {
+    // m = 0 keeps the real parts only (rfftb drops the imaginary ones)
+    const C a = A[m], b = B[m];
+    return m == 0 ? mk<C>(a.x, b.x) : mk<C>(a.x - b.y, a.y + b.x);
   } else {
-    return pair_mode_t<C>(A, B, m, L, Mx);
+    // a group that straddles Mx or L - Mx: a clamped load, then a select
+    const bool low = m < Mx, high = m > L - Mx;
+    const int i = low ? m : (high ? L - m : 0);
+    const C a = A[i], b = B[i];
+    const C f = low ? mk<C>(a.x - b.y, a.y + b.x) : mk<C>(a.x + b.y, -a.y + b.x);
+    return (low || high) ? f : mk<C>(0, 0);
   }
 }
 
@@ -746,7 +741,8 @@ __global__ void __launch_bounds__(3 * xp_width(P, Q, R)) k_xpass4(XNLArgs a, con
     const int qq = j % Q, r = j / Q;
     const int z = z0 + r;
     C* s = S + (r * 3 + tr) * SP;
-    const C *A = nullptr, *B = nullptr;
+    const C* const zr = static_cast<const C*>(a.zrow);
+    const C *A = zr, *B = zr;
     if (z < ngz) {
       const size_t ru = ((size_t)q * ngz + z) * a.zs;
       if (tr == 0) {
@@ -754,7 +750,7 @@ __global__ void __launch_bounds__(3 * xp_width(P, Q, R)) k_xpass4(XNLArgs a, con
         B = zi[2] + ru;
       } else if (tr == 1) {
         A = zi[1] + ru;
-        B = hasv ? zi[4] + ru : nullptr;
+        if (hasv) B = zi[4] + ru;
       } else if (hasv) {
         A = zi[3] + ru;
         B = zi[5] + ru;
@@ -1145,6 +1141,7 @@ void launch_xpass_nl(Solver& s, int cfl) {
   a.cfu = s.d_cfu;
   a.cfv = s.d_cfv;
   for (int w = 0; w < 4; ++w) a.win[w] = s.nl_x[w];
+  a.zrow = s.d_zrow;
   if (try_xpass4(s, a)) return;
   switch (s.px.L) {
 #define C(n) case n: xpass_nl_t<n>(s, a); break;
