@@ -736,36 +736,58 @@ __global__ void __launch_bounds__(3 * xp_width(P, Q, R)) k_xpass4(XNLArgs a, con
   const C* const* zi = reinterpret_cast<const C* const*>(a.zi);
 
   // phase 1: tasks (qq, r) of transform tr; inverse P-point DFTs over p of
-  // c[Q p + qq]
-  for (int j = lane; j < Q * R; j += W) {
-    const int qq = j % Q, r = j / Q;
-    const int z = z0 + r;
-    C* s = S + (r * 3 + tr) * SP;
-    const C* const zr = static_cast<const C*>(a.zrow);
-    const C *A = zr, *B = zr;
-    if (z < ngz) {
-      const size_t ru = ((size_t)q * ngz + z) * a.zs;
-      if (tr == 0) {
-        A = zi[0] + ru;
-        B = zi[2] + ru;
-      } else if (tr == 1) {
-        A = zi[1] + ru;
-        if (hasv) B = zi[4] + ru;
-      } else if (hasv) {
-        A = zi[3] + ru;
-        B = zi[5] + ru;
+  // c[Q p + qq].  The Q R tasks are dealt to the lanes in NS = ceil(Q R / W)
+  // slots; a lane without a task in the last slot repeats the last task and
+  // stores nothing.  Every slot's inputs are loaded before any slot's DFT, so
+  // all of the loads are in flight together (the LDS, not the registers,
+  // bounds this kernel's residency)
+  {
+    constexpr int NS = (Q * R + W - 1) / W;
+    C v[NS][P];
+    int qs[NS], rs[NS];
+    sfor<0, NS>([&](auto sc) {
+      constexpr int t = decltype(sc)::value;
+      const int jt = t * W + lane;
+      const int j = (t * W + W <= Q * R || jt < Q * R) ? jt : Q * R - 1;
+      const int qq = j % Q, r = j / Q;
+      qs[t] = qq;
+      rs[t] = r;
+      const int z = z0 + r;
+      const C* const zr = static_cast<const C*>(a.zrow);
+      const C *A = zr, *B = zr;
+      if (z < ngz) {
+        const size_t ru = ((size_t)q * ngz + z) * a.zs;
+        if (tr == 0) {
+          A = zi[0] + ru;
+          B = zi[2] + ru;
+        } else if (tr == 1) {
+          A = zi[1] + ru;
+          if (hasv) B = zi[4] + ru;
+        } else if (hasv) {
+          A = zi[3] + ru;
+          B = zi[5] + ru;
+        }
       }
-    }
-    C v[P];
-    sfor<0, P>([&](auto pc) {
-      constexpr int p = decltype(pc)::value;
-      v[p] = pair_mode_q<L, Q, p>(A, B, Q * p + qq);
+      sfor<0, P>([&](auto pc) {
+        constexpr int p = decltype(pc)::value;
+        v[t][p] = pair_mode_q<L, Q, p>(A, B, Q * p + qs[t]);
+      });
     });
-    dft<true, P>(v);
-    s[qq] = v[0];
-    sfor<1, P>([&](auto kc) {
-      constexpr int k1 = decltype(kc)::value;
-      s[k1 * (Q + 1) + qq] = mul_tw<true>(v[k1], twg[qq * k1]);
+    // keep the scheduler from sinking the later slots' loads below the
+    // first slot's DFT
+    __builtin_amdgcn_sched_barrier(0);
+    sfor<0, NS>([&](auto sc) {
+      constexpr int t = decltype(sc)::value;
+      const int qq = qs[t];
+      dft<true, P>(v[t]);
+      if (t * W + W <= Q * R || t * W + lane < Q * R) {
+        C* s = S + (rs[t] * 3 + tr) * SP;
+        s[qq] = v[t][0];
+        sfor<1, P>([&](auto kc) {
+          constexpr int k1 = decltype(kc)::value;
+          s[k1 * (Q + 1) + qq] = mul_tw<true>(v[t][k1], twg[qq * k1]);
+        });
+      }
     });
   }
   __syncthreads();
@@ -872,17 +894,29 @@ __global__ void __launch_bounds__(3 * xp_width(P, Q, R)) k_xpass4(XNLArgs a, con
   const T hs = (T)(0.5 / L);
   C* oa = (C*)(tr == 0 ? a.zo[0] : (tr == 1 ? a.zo[3] : a.zo[4]));  // uu, vv, uv
   C* ob = (C*)(tr == 0 ? a.zo[1] : (tr == 1 ? a.zo[2] : a.zo[5]));  // uw, ww, vw
-  for (int j = lane; j < Mx * R; j += W) {
-    const int m = j % Mx, r5 = j / Mx;
-    const int z = z0 + r5;
-    if (z >= ngz) continue;
-    const C* F = S + (r5 * 3 + tr) * SP;
-    const int mb = m == 0 ? 0 : L - m;
-    const C f = F[(m % Q) * (P + 1) + m / Q], g = F[(mb % Q) * (P + 1) + mb / Q];
-    const size_t ro = ((size_t)q * ngz + z) * a.zs + m;
-    oa[ro] = mk<C>(hs * (f.x + g.x), hs * (f.y - g.y));
-    ob[ro] = mk<C>(hs * (f.y + g.y), -hs * (f.x - g.x));
-  }
+  // lane -> mode m of a chunk of W modes, all R rows per lane; F[m] sits at
+  // slot (m % Q)(P+1) + m / Q, and its mirror L - m at (Q - m % Q)(P+1) +
+  // P-1 - m / Q (at P - m / Q when m % Q = 0; m = 0 pairs with itself)
+  sfor<0, (Mx + W - 1) / W>([&](auto cc) {
+    constexpr int c0 = decltype(cc)::value * W;
+    const int m = c0 + lane;
+    if (c0 + W <= Mx || m < Mx) {
+      const int m2 = m % Q, m1 = m / Q;
+      const int fa = m2 * (P + 1) + m1;
+      const int gb = m == 0 ? 0 : (m2 == 0 ? P - m1 : (Q - m2) * (P + 1) + P - 1 - m1);
+      sfor<0, R>([&](auto rc) {
+        constexpr int r5 = decltype(rc)::value;
+        const int z = z0 + r5;
+        if (z < ngz) {
+          const C* F = S + (r5 * 3 + tr) * SP;
+          const C f = F[fa], g = F[gb];
+          const size_t ro = ((size_t)q * ngz + z) * a.zs + m;
+          oa[ro] = mk<C>(hs * (f.x + g.x), hs * (f.y - g.y));
+          ob[ro] = mk<C>(hs * (f.y + g.y), -hs * (f.x - g.x));
+        }
+      });
+    }
+  });
 }
 
 // raise the dynamic-LDS limit of a kernel once per size
