@@ -793,16 +793,22 @@ __global__ void __launch_bounds__(3 * xp_width(P, Q, R)) k_xpass4(XNLArgs a, con
   __syncthreads();
 
   // phase 2: task (k1, r) of transform tr per thread; inverse Q-point DFT ->
-  // physical values at x = k1 + P k2, kept in registers and written back in
-  // place for the products of the next transform
+  // physical values at x = k1 + P k2, kept in registers; the imaginary parts
+  // of transforms 0 and 1 (w, then v) are written back in place for the
+  // products of the next transform.  In FP32 the twiddles of phase 3 are
+  // loaded first, so that they arrive during the DFT (in FP64 they would
+  // push the registers past the 256 that two blocks per CU allow).
+  constexpr bool kTw3Early = sizeof(T) == 4;
   const bool act = lane < P * R;
   const int k1 = lane % P, r = act ? lane / P : 0;
   C* s = S + (r * 3 + tr) * SP + k1 * (Q + 1);
-  C u[Q];
+  C u[Q], tw3[Q];
   if (act) {
+    if constexpr (kTw3Early) sfor<1, Q>([&](auto c) { tw3[decltype(c)::value] = twg[k1 * decltype(c)::value]; });
     sfor<0, Q>([&](auto c) { u[decltype(c)::value] = s[decltype(c)::value]; });
     dft<true, Q>(u);
-    sfor<0, Q>([&](auto c) { s[decltype(c)::value] = u[decltype(c)::value]; });
+    if (tr < 2)
+      sfor<0, Q>([&](auto c) { reinterpret_cast<T*>(s + decltype(c)::value)[1] = u[decltype(c)::value].y; });
   }
   __syncthreads();
 
@@ -866,7 +872,7 @@ __global__ void __launch_bounds__(3 * xp_width(P, Q, R)) k_xpass4(XNLArgs a, con
     sf[k1] = u[0];
     sfor<1, Q>([&](auto c) {
       constexpr int m2c = decltype(c)::value;
-      sf[m2c * (P + 1) + k1] = mul_tw<false>(u[m2c], twg[k1 * m2c]);
+      sf[m2c * (P + 1) + k1] = mul_tw<false>(u[m2c], kTw3Early ? tw3[m2c] : twg[k1 * m2c]);
     });
   }
   __syncthreads();
