@@ -311,12 +311,18 @@ def multiband_leg(args, f, q0, fine_value, nprof, tuning):
 
 def ensemble_leg(fine_value, fine_points):
     """16 uneven channels, tuned, 50 RK3 steps (the first eager, 49 by the
-    captured graphs)."""
+    captured graphs, one launch per member kernel); the same members stepped
+    by the graphs with batched launches over the member table (the tunings'
+    column-sweep shapes, four-step transforms), for comparison."""
     from cfd_amd import ensemble
 
     rep = ensemble.run(16, 1, 50, autotune=True, log=None)
+    bat = ensemble.run(16, 1, 50, log=None, batched=True,
+                       tunings=[dict(m["tuning"], fft=0) for m in rep["members"]])
     return {"members": 16, "seed": 1, "steps": 50,
             "member_substeps_per_s": rep["member_substeps_per_s"],
+            "point_substeps_per_s_batched": bat["point_substeps_per_s"],
+            "batched_over_per_member": bat["point_substeps_per_s"] / rep["point_substeps_per_s"],
             "ensemble_substeps_per_s": rep["ensemble_substeps_per_s"],
             "point_substeps_per_s": rep["point_substeps_per_s"],
             "point_substeps_per_s_single_channel": fine_value * fine_points,

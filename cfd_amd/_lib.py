@@ -100,6 +100,8 @@ SIGNATURES = {
     "cfd_set_compat_lastplane": (ctypes.c_int, [_H, ctypes.c_int]),
     "cfd_ensemble_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
                                            ctypes.POINTER(ctypes.c_void_p)]),
+    "cfd_ensemble_create_batched": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
+                                                   ctypes.POINTER(ctypes.c_void_p)]),
     "cfd_ensemble_destroy": (ctypes.c_int, [_H]),
     "cfd_ensemble_last_error": (ctypes.c_char_p, [_H]),
     "cfd_ensemble_substep": (ctypes.c_int, [_H, ctypes.c_int]),

@@ -2188,12 +2188,18 @@ struct cfd_ensemble_s {
   hipGraph_t g[3] = {};
   hipGraphExec_t ge[3] = {};
   std::string err;
+  // batched kernels: per kRK and kernel class the launches over the member
+  // table, whose arguments live in dargs
+  bool batched = false;
+  void* dargs = nullptr;
+  std::vector<BatchLaunch> bl[3][BC_N];
 };
 
 namespace {
 thread_local std::string g_ens_err;
 
 void ens_free(cfd_ensemble_s* e) {
+  if (e->dargs) hipFree(e->dargs);
   for (int k = 0; k < 3; ++k) {
     if (e->ge[k]) hipGraphExecDestroy(e->ge[k]);
     if (e->g[k]) hipGraphDestroy(e->g[k]);
@@ -2212,17 +2218,234 @@ int ens_fail(cfd_ensemble_s* e, int rc, const std::string& msg) {
 }
 }  // namespace
 
+namespace {
+// The member tables of the batched kernels: for every kRK and class, each
+// member's part (its arguments and block count), grouped by kernel variant
+// and block width into launches of at most kMaxBatch members; the arguments
+// of all launches go to one device buffer.
+int ens_build_batches(cfd_ensemble_s* e, std::string& msg) {
+  struct Pending {
+    BatchLaunch b;
+    size_t off;  // argument offset in the blob
+  };
+  std::vector<char> blob;
+  std::vector<Pending> pend[3][BC_N];
+  for (int k = 0; k < 3; ++k) {
+    for (int cls = 0; cls < BC_N; ++cls) {
+      std::vector<BatchPart> parts(e->m.size());
+      for (size_t i = 0; i < e->m.size(); ++i) {
+        std::string err;
+        const bool fft = cls == BC_ZINV_NL || cls == BC_XPASS || cls == BC_ZFWD;
+        const bool ok = fft ? batch_part_fft(*e->m[i], cls, k + 1, parts[i], err)
+                            : batch_part_fused(*e->m[i], cls, k + 1, parts[i], err);
+        if (!ok) {
+          msg = "ensemble: member " + std::to_string(i) + ": " + err;
+          return CFD_EINVAL;
+        }
+      }
+      std::vector<bool> done(parts.size(), false);
+      for (size_t i = 0; i < parts.size(); ++i) {
+        if (done[i]) continue;
+        // members of the same variant and block width, in member order
+        std::vector<size_t> grp;
+        for (size_t j = i; j < parts.size(); ++j)
+          if (!done[j] && parts[j].key == parts[i].key && parts[j].block.x == parts[i].block.x &&
+              grp.size() < (size_t)kMaxBatch) {
+            grp.push_back(j);
+            done[j] = true;
+          }
+        Pending pd;
+        BatchLaunch& b = pd.b;
+        b.cls = cls;
+        b.key = parts[i].key;
+        b.lead = (int)i;
+        b.tw = parts[i].tw;
+        unsigned by = 1;
+        size_t fixed = 0;
+        for (size_t j : grp) {
+          by = std::max(by, parts[j].block.y);
+          fixed = std::max(fixed, parts[j].lds_fixed);
+        }
+        b.block = dim3(parts[i].block.x, by, 1);
+        b.lds = fixed + parts[i].lds_thread * (size_t)b.block.x * by;
+        const size_t asz = parts[i].args.size();
+        blob.resize((blob.size() + 255) / 256 * 256);
+        pd.off = blob.size();
+        int nb = 0;
+        b.map.n = 0;
+        for (size_t j : grp) {
+          const int cnt = parts[j].gx * parts[j].gy;
+          if (cnt <= 0) continue;
+          const int t = b.map.n++;
+          b.map.off[t] = nb;
+          b.map.cnt[t] = cnt;
+          b.map.gx[t] = parts[j].gx;
+          b.members.push_back((int)j);
+          nb += (cnt + 7) / 8 * 8;
+          blob.insert(blob.end(), parts[j].args.begin(), parts[j].args.end());
+          (void)asz;
+        }
+        if (b.map.n == 0) continue;
+        b.grid = dim3(nb);
+        pend[k][cls].push_back(pd);
+      }
+    }
+  }
+  if (blob.empty()) return CFD_OK;
+  if (hipMalloc(&e->dargs, blob.size()) != hipSuccess) {
+    msg = "ensemble: device memory for the member tables";
+    return CFD_ENOMEM;
+  }
+  if (hipMemcpy(e->dargs, blob.data(), blob.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    msg = "ensemble: copy of the member tables";
+    return CFD_EHIP;
+  }
+  for (int k = 0; k < 3; ++k)
+    for (int cls = 0; cls < BC_N; ++cls)
+      for (Pending& pd : pend[k][cls]) {
+        pd.b.dargs = static_cast<const char*>(e->dargs) + pd.off;
+        e->bl[k][cls].push_back(pd.b);
+      }
+  return CFD_OK;
+}
+
+// The batched substep kRK as an explicit graph.  Each batch launch and each
+// stretch of one member's own kernels (on the member's stream; the u11 solve
+// of the flow-rate correction on its side stream) is captured alone and
+// copied in as kernel nodes whose dependencies are the nodes that last
+// worked on its members, so a batch waits only for its own members and
+// members of different batches overlap as in the per-member graphs.  Order
+// per member as in op_substep (and, after kRK = 3 with flag_ctpress = 1, the
+// flow-rate integral, as cfd_step does).
+hipError_t ens_batched_graph(cfd_ensemble_s* e, int kRK, hipGraph_t* out) {
+  const int k = kRK - 1, n = (int)e->m.size();
+  hipGraph_t G = nullptr;
+  hipError_t err = hipGraphCreate(&G, 0);
+  std::vector<hipGraphNode_t> last(n, nullptr), u11(n, nullptr);
+  auto deps_of = [&](const std::vector<int>& ms) {
+    std::vector<hipGraphNode_t> d;
+    for (int i : ms)
+      if (last[i] && std::find(d.begin(), d.end(), last[i]) == d.end()) d.push_back(last[i]);
+    return d;
+  };
+  // f's launches on stream cs, captured alone, then copied into G as kernel
+  // nodes (a chain: one stream) after deps; returns the chain's last node
+  // (deps unchanged when f launches nothing).  Kernel nodes, not a child
+  // graph: the runtime runs a child graph as one unit on one queue.
+  auto add = [&](hipStream_t cs, std::vector<hipGraphNode_t> deps, auto f) -> hipGraphNode_t {
+    if (err != hipSuccess) return nullptr;
+    err = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (err != hipSuccess) return nullptr;
+    f();
+    const hipError_t le = hipGetLastError();
+    hipGraph_t sub = nullptr;
+    const hipError_t ec = hipStreamEndCapture(cs, &sub);
+    err = le != hipSuccess ? le : ec;
+    if (err == hipSuccess && !sub) err = hipErrorInvalidValue;
+    hipGraphNode_t tail = deps.size() == 1 ? deps[0] : nullptr;
+    size_t nn = 0, ne = 0;
+    if (err == hipSuccess) err = hipGraphGetNodes(sub, nullptr, &nn);
+    if (err == hipSuccess) err = hipGraphGetEdges(sub, nullptr, nullptr, &ne);
+    std::vector<hipGraphNode_t> nodes(nn), from(ne), to(ne), copy(nn, nullptr);
+    if (err == hipSuccess && nn) err = hipGraphGetNodes(sub, nodes.data(), &nn);
+    if (err == hipSuccess && ne) err = hipGraphGetEdges(sub, from.data(), to.data(), &ne);
+    // the capture of one stream is a chain: walk it from its root
+    std::vector<int> npred(nn, 0), next(nn, -1);
+    auto idx = [&](hipGraphNode_t v) { return int(std::find(nodes.begin(), nodes.end(), v) - nodes.begin()); };
+    for (size_t q = 0; q < ne && err == hipSuccess; ++q) {
+      const int a = idx(from[q]), b = idx(to[q]);
+      if (next[a] >= 0 || ++npred[b] > 1) err = hipErrorInvalidValue;  // not a chain
+      next[a] = b;
+    }
+    int v = -1;
+    for (size_t q = 0; q < nn; ++q)
+      if (!npred[q]) v = v < 0 ? int(q) : -2;
+    if (err == hipSuccess && nn && v < 0) err = hipErrorInvalidValue;
+    std::vector<hipGraphNode_t> pre = deps;
+    for (; err == hipSuccess && nn && v >= 0; v = next[v]) {
+      hipGraphNodeType t;
+      err = hipGraphNodeGetType(nodes[v], &t);
+      if (err == hipSuccess && t != hipGraphNodeTypeKernel) err = hipErrorInvalidValue;
+      hipKernelNodeParams kp{};
+      if (err == hipSuccess) err = hipGraphKernelNodeGetParams(nodes[v], &kp);
+      hipGraphNode_t nd = nullptr;
+      if (err == hipSuccess) err = hipGraphAddKernelNode(&nd, G, pre.data(), pre.size(), &kp);
+      pre.assign(1, nd);
+      tail = nd;
+    }
+    if (err == hipSuccess && !nn && deps.size() > 1) err = hipGraphAddEmptyNode(&tail, G, deps.data(), deps.size());
+    if (sub) hipGraphDestroy(sub);
+    return tail;
+  };
+  auto batches = [&](int cls) {
+    for (const BatchLaunch& b : e->bl[k][cls]) {
+      hipGraphNode_t nd = add(e->root, deps_of(b.members), [&]() {
+        if (cls == BC_RHS || cls == BC_PRESS) batch_launch_fused(e->root, b);
+        else batch_launch_fft(e->root, b);
+      });
+      for (int i : b.members) last[i] = nd;
+    }
+  };
+  auto each = [&](auto has, auto f) {
+    for (int i = 0; i < n; ++i) {
+      cfd_solver* s = e->m[i];
+      if (!has(s)) continue;
+      last[i] = add(s->stream, deps_of({i}), [&]() { f(s); });
+    }
+  };
+  auto all = [](cfd_solver*) { return true; };
+  if (kRK == 1) each(all, [&](cfd_solver* s) { launch_reset_max(*s); });
+  batches(BC_ZINV_NL);
+  batches(BC_XPASS);
+  batches(BC_ZFWD);
+  if (kRK == 1) each(all, [&](cfd_solver* s) { launch_cfl(*s); });
+  // the u11 solve needs dt only: beside the IB correction and the sweeps
+  for (int i = 0; i < n; ++i) {
+    cfd_solver* s = e->m[i];
+    if (s->P.flag_ctpress == 0)
+      u11[i] = add(s->side, deps_of({i}), [&]() { launch_mf_u11(*s, kRK, s->side); });
+  }
+  each(all, [&](cfd_solver* s) {
+    launch_ib_rhs(*s, kRK);
+    op_ib(s, 0, 3);
+  });
+  batches(BC_RHS);
+  for (int i = 0; i < n; ++i) {
+    cfd_solver* s = e->m[i];
+    if (s->P.flag_ctpress != 0) continue;
+    std::vector<hipGraphNode_t> d = deps_of({i});
+    if (u11[i]) d.push_back(u11[i]);
+    last[i] = add(s->stream, d, [&]() { launch_mf_apply(*s); });
+  }
+  batches(BC_PRESS);
+  if (kRK == 3)
+    each([](cfd_solver* s) { return s->filt || s->P.flag_ctpress == 1; },
+         [&](cfd_solver* s) {
+           if (s->filt) launch_mode_filter(*s);
+           if (s->P.flag_ctpress == 1) launch_flowrate(*s);
+         });
+  for (cfd_solver* s : e->m) ++s->nsub;
+  if (err != hipSuccess && G) {
+    hipGraphDestroy(G);
+    G = nullptr;
+  }
+  *out = G;
+  return err;
+}
+
 // Capture: the root stream records an event that every member's stream
 // waits on; each member's substep kRK (and, after kRK = 3 of a member with a
 // prescribed pressure gradient, its flow-rate integral, as cfd_step does)
 // is launched on its own stream; the root waits on an event of each member.
+// Batched: the graphs of ens_batched_graph instead.
 // The graph holds kernel nodes only.  Members must have run at least one
 // substep eagerly (so every kernel's launch attributes are set), have
 // profiling off, no closure attached and no statistics request pending.
-extern "C" int cfd_ensemble_create(cfd_handle* hs, int n, cfd_ensemble* out) {
+int ens_create(cfd_handle* hs, int n, bool batched, cfd_ensemble* out) {
   if (!hs || n < 1 || !out) return ens_fail(nullptr, CFD_EINVAL, "ensemble: bad arguments");
   *out = nullptr;
   cfd_ensemble_s* e = new cfd_ensemble_s();
+  e->batched = batched;
   auto bail = [&](int rc, const std::string& msg) {
     g_ens_err = msg;
     ens_free(e);
@@ -2250,8 +2473,23 @@ extern "C" int cfd_ensemble_create(cfd_handle* hs, int n, cfd_ensemble* out) {
   // nothing of the members may be in flight while capturing
   for (cfd_solver* s : e->m)
     if (hipStreamSynchronize(s->stream) != hipSuccess) return bail(CFD_EHIP, "ensemble: member stream");
+  if (batched) {
+    std::string msg;
+    if (int rc = ens_build_batches(e, msg)) return bail(rc, msg);
+  }
   for (int k = 0; k < 3; ++k) {
     const int kRK = k + 1;
+    if (batched) {
+      hipGraph_t g = nullptr;
+      const hipError_t err = ens_batched_graph(e, kRK, &g);
+      e->g[k] = g;
+      if (err != hipSuccess || !g)
+        return bail(CFD_EHIP, std::string("ensemble: graph of batched substep ") + std::to_string(kRK) + ": " +
+                                  hipGetErrorString(err));
+      if (hipGraphInstantiate(&e->ge[k], g, nullptr, nullptr, 0) != hipSuccess)
+        return bail(CFD_EHIP, "ensemble: graph instantiation");
+      continue;
+    }
     if (hipStreamBeginCapture(e->root, hipStreamCaptureModeThreadLocal) != hipSuccess)
       return bail(CFD_EHIP, "ensemble: begin capture");
     hipError_t err = hipEventRecord(e->fork, e->root);
@@ -2276,6 +2514,13 @@ extern "C" int cfd_ensemble_create(cfd_handle* hs, int n, cfd_ensemble* out) {
   }
   *out = e;
   return CFD_OK;
+}
+}  // namespace
+
+extern "C" int cfd_ensemble_create(cfd_handle* hs, int n, cfd_ensemble* out) { return ens_create(hs, n, false, out); }
+
+extern "C" int cfd_ensemble_create_batched(cfd_handle* hs, int n, cfd_ensemble* out) {
+  return ens_create(hs, n, true, out);
 }
 
 extern "C" int cfd_ensemble_destroy(cfd_ensemble e) {

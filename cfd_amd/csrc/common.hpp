@@ -276,6 +276,62 @@ void launch_band_gather(Solver& s);   // long columns of u1, u2, u3 -> the centr
 void launch_band_gather(Solver& s, int q0, int q1, bool with_p);
 void launch_band_scatter(Solver& s);  // central planes of the nonlinear scratch <- the central solver
 void launch_cfl_bands(Solver& s);     // dt from the wall bands' and the central band's maxima
+// Batched launches over the members of an ensemble: one launch of a kernel
+// class covers the blocks of several solvers that use the same kernel
+// variant.  Member i owns the linear blocks [off[i], off[i] + cnt[i]) (off
+// a multiple of 8, so a member's block b runs on XCD b % 8 as in its own
+// launch); its local block lb = b - off[i] is (lb % gx[i], lb / gx[i]).
+// Blocks in the padding between members leave at once.
+constexpr int kMaxBatch = 32;
+struct BMap {
+  int n;
+  int off[kMaxBatch];
+  int cnt[kMaxBatch];
+  int gx[kMaxBatch];
+};
+__device__ __forceinline__ int bmap_member(const BMap& m, int b, int& bx, int& by) {
+  int i = 0;
+  while (i + 1 < m.n && b >= m.off[i + 1]) ++i;
+  const int lb = b - m.off[i];
+  if (lb >= m.cnt[i]) return -1;
+  by = lb / m.gx[i];
+  bx = lb - by * m.gx[i];
+  return i;
+}
+// Kernel classes with batched launches.
+enum BatchClass { BC_ZINV_NL = 0, BC_XPASS, BC_ZFWD, BC_RHS, BC_PRESS, BC_N };
+// One member's part of a batched launch: `key` names the kernel variant,
+// args the bytes of its argument struct (copied to the device before
+// capture), grid its own block count (gx x gy), block its shape and
+// lds_fixed + lds_thread x (threads per block) its dynamic LDS (a batch takes
+// the largest block.y and lds_fixed of its parts; the kernels idle the
+// thread rows a member does not use).
+struct BatchPart {
+  int key = -1;
+  int gx = 0, gy = 0;
+  dim3 block;
+  size_t lds_fixed = 0, lds_thread = 0;
+  std::vector<char> args;
+  const void* tw = nullptr;  // twiddle table of the transform length
+};
+// A batch ready to launch: the parts' arguments at dargs on the device.
+struct BatchLaunch {
+  int cls = 0, key = -1;
+  int lead = 0;             // index of the batch's first member (its stream carries the launch)
+  std::vector<int> members;  // indices of the members in the batch
+  dim3 grid, block;
+  size_t lds = 0;
+  BMap map;
+  const void* dargs = nullptr;
+  const void* tw = nullptr;
+};
+// class cls of solver s at substep kRK as a batch part; false: the solver
+// has no batched form of the class (another engine or shape), message in err
+bool batch_part_fft(Solver& s, int cls, int kRK, BatchPart& part, std::string& err);
+bool batch_part_fused(Solver& s, int cls, int kRK, BatchPart& part, std::string& err);
+void batch_launch_fft(hipStream_t st, const BatchLaunch& b);
+void batch_launch_fused(hipStream_t st, const BatchLaunch& b);
+
 // no-allocation physical planes (noalloc.hip): the file-scope scratch's
 // device address is taken once per device (noalloc_init, device current);
 // four_to_phys_noalloc queues the planes [plane0, plane0 + nplanes) of field

@@ -12,6 +12,8 @@
 //             complex transform of length Ngx: a + i b.
 //  z-forward  Z plane -> spectral plane, keeping the N2 dealiased z-modes and
 //             applying the x/z derivatives (der_x/der_z, FOU3D.f90:310-414).
+#include <cstring>
+
 #include "dft.hpp"
 #include "fft.hpp"
 
@@ -464,24 +466,25 @@ __device__ __forceinline__ C from_d2(double2 v) {
 // on XCD b % 8, and XCD g takes the contiguous job range [g jper, (g+1) jper).
 // With the jobs ordered by y-plane, the jobs that read the same source
 // planes (a plane and its interpolation neighbours) then share one L2.
+// Block (bx, by) of the grid; the kernel k_zinv4 runs it for one solver,
+// k_zinv4_b for the members of a batch.
 template <typename T, int L, int P, int Q, int TI>
-__global__ void __launch_bounds__(TI * cmax(P, Q)) k_zinv4(const ZInvJob* __restrict__ jobs,
-                                                          const cplx_t<T>* __restrict__ twg, int Mx, int zs,
-                                                          int njobs, int ntiles, int jper) {
+__device__ __forceinline__ void zinv4_body(const ZInvJob* __restrict__ jobs, const cplx_t<T>* __restrict__ twg,
+                                           int Mx, int zs, int njobs, int ntiles, int jper, int bx, int by) {
   using C = cplx_t<T>;
   static_assert(P * Q == L, "P*Q");
   constexpr int N2 = Dealias<L>::N, H = Dealias<L>::H;
   __shared__ C X[P * Q * TI];
   int job, tile;
   if (jper > 0) {
-    const int g = blockIdx.x & 7, kk = blockIdx.x >> 3;
+    const int g = bx & 7, kk = bx >> 3;
     const int jl = kk / ntiles;
     job = g * jper + jl;
     tile = kk - jl * ntiles;
     if (jl >= jper || job >= njobs) return;
   } else {
-    job = blockIdx.y;
-    tile = blockIdx.x;
+    job = by;
+    tile = bx;
   }
   const ZInvJob J = jobs[job];
   const int tid = threadIdx.x, i0 = tile * TI;
@@ -569,6 +572,29 @@ __global__ void __launch_bounds__(TI * cmax(P, Q)) k_zinv4(const ZInvJob* __rest
   }
 }
 
+template <typename T, int L, int P, int Q, int TI>
+__global__ void __launch_bounds__(TI * cmax(P, Q)) k_zinv4(const ZInvJob* __restrict__ jobs,
+                                                          const cplx_t<T>* __restrict__ twg, int Mx, int zs,
+                                                          int njobs, int ntiles, int jper) {
+  zinv4_body<T, L, P, Q, TI>(jobs, twg, Mx, zs, njobs, ntiles, jper, blockIdx.x, blockIdx.y);
+}
+
+// one member's z-inverse in a batch
+struct ZInvArgs {
+  const ZInvJob* jobs;
+  int Mx, zs, njobs, ntiles, jper;
+};
+
+template <typename T, int L, int P, int Q, int TI>
+__global__ void __launch_bounds__(TI * cmax(P, Q)) k_zinv4_b(const ZInvArgs* __restrict__ as, BMap bm,
+                                                            const cplx_t<T>* __restrict__ twg) {
+  int bx = 0, by = 0;
+  const int m = bmap_member(bm, blockIdx.x, bx, by);
+  if (m < 0) return;
+  const ZInvArgs a = as[m];
+  zinv4_body<T, L, P, Q, TI>(a.jobs, twg, a.Mx, a.zs, a.njobs, a.ntiles, a.jper, bx, by);
+}
+
 // (0,k) * u
 template <typename C>
 __device__ __forceinline__ C imk(real_t<C> k, C u) {
@@ -584,11 +610,11 @@ __device__ __forceinline__ C imk(real_t<C> k, C u) {
 // accumulator per output slot is live at a time.
 template <typename T, int L, int P, int Q, int TI, bool ISU, int PR>
 __device__ __forceinline__ void zfwd_product(const ZNLArgs& a, const cplx_t<T>* __restrict__ twg, cplx_t<T>* X,
-                                             int pl, bool inter, cplx_t<T>* acc) {
+                                             int pl, bool inter, cplx_t<T>* acc, int bx) {
   using C = cplx_t<T>;
   constexpr int N2 = Dealias<L>::N, H = Dealias<L>::H;
   const int Mx = a.Mx;
-  const int tid = threadIdx.x, i0 = blockIdx.x * TI;
+  const int tid = threadIdx.x, i0 = bx * TI;
   const size_t ncol = (size_t)N2 * Mx;
   const T sc = (T)(1.0 / L);
   const C* src = (const C*)a.z[ISU ? PR : 4 + PR] + (size_t)pl * L * a.zs;
@@ -651,26 +677,41 @@ __device__ __forceinline__ void zfwd_product(const ZNLArgs& a, const cplx_t<T>* 
 }
 
 template <typename T, int L, int P, int Q, int TI>
-__global__ void __launch_bounds__(TI * cmax(P, Q)) k_zfwd4(ZNLArgs a, const cplx_t<T>* __restrict__ twg) {
+__device__ __forceinline__ void zfwd4_body(const ZNLArgs& a, const cplx_t<T>* __restrict__ twg, int bx, int by) {
   using C = cplx_t<T>;
   static_assert(P * Q == L, "P*Q");
   __shared__ C X[P * Q * TI];
   const int nue = (a.winu[1] - a.winu[0]) - (a.winu[3] - a.winu[2]);
-  const bool isu = (int)blockIdx.y < nue;
-  const int pl = isu ? win_plane(blockIdx.y, a.winu) : win_plane(blockIdx.y - nue, a.winv);
+  const bool isu = by < nue;
+  const int pl = isu ? win_plane(by, a.winu) : win_plane(by - nue, a.winv);
   const int ny = isu ? a.nyu : a.nyv;
   const bool inter = pl > 0 && (pl < ny - 1 || (a.top && pl == ny - 1));
   if (isu && !inter) return;  // ghost u planes: no product is needed there (but see a.top)
   C acc[Q];
   if (isu) {
-    zfwd_product<T, L, P, Q, TI, true, 0>(a, twg, X, pl, inter, acc);
-    zfwd_product<T, L, P, Q, TI, true, 1>(a, twg, X, pl, inter, acc);
-    zfwd_product<T, L, P, Q, TI, true, 2>(a, twg, X, pl, inter, acc);
-    zfwd_product<T, L, P, Q, TI, true, 3>(a, twg, X, pl, inter, acc);
+    zfwd_product<T, L, P, Q, TI, true, 0>(a, twg, X, pl, inter, acc, bx);
+    zfwd_product<T, L, P, Q, TI, true, 1>(a, twg, X, pl, inter, acc, bx);
+    zfwd_product<T, L, P, Q, TI, true, 2>(a, twg, X, pl, inter, acc, bx);
+    zfwd_product<T, L, P, Q, TI, true, 3>(a, twg, X, pl, inter, acc, bx);
   } else {
-    zfwd_product<T, L, P, Q, TI, false, 0>(a, twg, X, pl, inter, acc);
-    zfwd_product<T, L, P, Q, TI, false, 1>(a, twg, X, pl, inter, acc);
+    zfwd_product<T, L, P, Q, TI, false, 0>(a, twg, X, pl, inter, acc, bx);
+    zfwd_product<T, L, P, Q, TI, false, 1>(a, twg, X, pl, inter, acc, bx);
   }
+}
+
+template <typename T, int L, int P, int Q, int TI>
+__global__ void __launch_bounds__(TI * cmax(P, Q)) k_zfwd4(ZNLArgs a, const cplx_t<T>* __restrict__ twg) {
+  zfwd4_body<T, L, P, Q, TI>(a, twg, blockIdx.x, blockIdx.y);
+}
+
+template <typename T, int L, int P, int Q, int TI>
+__global__ void __launch_bounds__(TI * cmax(P, Q)) k_zfwd4_b(const ZNLArgs* __restrict__ as, BMap bm,
+                                                            const cplx_t<T>* __restrict__ twg) {
+  int bx = 0, by = 0;
+  const int m = bmap_member(bm, blockIdx.x, bx, by);
+  if (m < 0) return;
+  const ZNLArgs a = as[m];
+  zfwd4_body<T, L, P, Q, TI>(a, twg, bx, by);
 }
 
 // one Hermitian pair mode with the range of m known at compile time where
@@ -720,7 +761,7 @@ __device__ inline double wave_max(double v) {
 // physical values in place, then the forward exchange [m2][k1] (row stride
 // P+1) and the spectrum in place.
 template <typename T, int L, int P, int Q, int R>
-__global__ void __launch_bounds__(3 * xp_width(P, Q, R)) k_xpass4(XNLArgs a, const cplx_t<T>* __restrict__ twg) {
+__device__ __forceinline__ void xpass4_body(const XNLArgs& a, const cplx_t<T>* __restrict__ twg, int bx, int by) {
   using C = cplx_t<T>;
   static_assert(P * Q == L, "P*Q");
   constexpr int Mx = Dealias<L>::Mx;
@@ -729,7 +770,7 @@ __global__ void __launch_bounds__(3 * xp_width(P, Q, R)) k_xpass4(XNLArgs a, con
   __shared__ C S[R * 3 * SP];
   const int tid = threadIdx.x;
   const int tr = tid / W, lane = tid - tr * W;
-  const int q = win_plane(blockIdx.y, a.win), z0 = blockIdx.x * R, ngz = a.ngz;
+  const int q = win_plane(by, a.win), z0 = bx * R, ngz = a.ngz;
   const bool hasv = q < a.nyv;
   const bool uint = q > 0 && (q < a.nyu - 1 || (a.top && q == a.nyu - 1));
   const bool vint = hasv && q > 0 && q < a.nyv - 1;
@@ -923,6 +964,22 @@ __global__ void __launch_bounds__(3 * xp_width(P, Q, R)) k_xpass4(XNLArgs a, con
       });
     }
   });
+}
+
+template <typename T, int L, int P, int Q, int R>
+__global__ void __launch_bounds__(3 * xp_width(P, Q, R)) k_xpass4(XNLArgs a, const cplx_t<T>* __restrict__ twg) {
+  xpass4_body<T, L, P, Q, R>(a, twg, blockIdx.x, blockIdx.y);
+}
+
+template <typename T, int L, int P, int Q, int R>
+__global__ void __launch_bounds__(3 * xp_width(P, Q, R)) k_xpass4_b(const XNLArgs* __restrict__ as, BMap bm,
+                                                                    const cplx_t<T>* __restrict__ twg) {
+  int bx = 0, by = 0;
+  const int m = bmap_member(bm, blockIdx.x, bx, by);
+  if (m < 0) return;
+  // read from the table in place: a private copy of the pointer arrays
+  // would be indexed at run time and land in scratch memory
+  xpass4_body<T, L, P, Q, R>(as[m], twg, bx, by);
 }
 
 // raise the dynamic-LDS limit of a kernel once per size
@@ -1163,8 +1220,8 @@ void launch_zinv(Solver& s, const ZInvJob* jobs, int njobs, bool nl) {
   }
 }
 
-void launch_xpass_nl(Solver& s, int cfl) {
-  XNLArgs a;
+namespace {
+void xpass_args(Solver& s, int cfl, XNLArgs& a) {
   for (int c = 0; c < 6; ++c) {
     a.zi[c] = s.Z[c];
     a.zo[c] = s.Z[6 + c];
@@ -1182,17 +1239,9 @@ void launch_xpass_nl(Solver& s, int cfl) {
   a.cfv = s.d_cfv;
   for (int w = 0; w < 4; ++w) a.win[w] = s.nl_x[w];
   a.zrow = s.d_zrow;
-  if (try_xpass4(s, a)) return;
-  switch (s.px.L) {
-#define C(n) case n: xpass_nl_t<n>(s, a); break;
-    CFD_FFT_SIZES(C)
-#undef C
-    default: xpass_nl_t<0>(s, a);
-  }
 }
 
-void launch_zfwd_nl(Solver& s) {
-  ZNLArgs a;
+void zfwd_args(Solver& s, ZNLArgs& a) {
   for (int c = 0; c < 6; ++c) a.z[c] = s.Z[6 + c];
   a.nxz1 = s.nxz1;
   a.nxz3 = s.nxz3;
@@ -1212,6 +1261,24 @@ void launch_zfwd_nl(Solver& s) {
     a.winu[w] = s.nl_u[w];
     a.winv[w] = s.nl_v[w];
   }
+}
+}  // namespace
+
+void launch_xpass_nl(Solver& s, int cfl) {
+  XNLArgs a;
+  xpass_args(s, cfl, a);
+  if (try_xpass4(s, a)) return;
+  switch (s.px.L) {
+#define C(n) case n: xpass_nl_t<n>(s, a); break;
+    CFD_FFT_SIZES(C)
+#undef C
+    default: xpass_nl_t<0>(s, a);
+  }
+}
+
+void launch_zfwd_nl(Solver& s) {
+  ZNLArgs a;
+  zfwd_args(s, a);
   if (try_zfwd4(s, a)) return;
   switch (s.pz.L) {
 #define C(n) case n: zfwd_nl_t<n>(s, a); break;
@@ -1248,6 +1315,208 @@ void launch_zfwd_plain(Solver& s, const ZFwdJob* jobs, int njobs) {
     CFD_FFT_SIZES(C)
 #undef C
     default: zfwdp_t<0>(s, jobs, njobs);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// batched launches of the nonlinear transforms (ensembles)
+// ---------------------------------------------------------------------------
+namespace {
+template <class A>
+void put_args(BatchPart& p, const A& a) {
+  p.args.resize(sizeof(A));
+  std::memcpy(p.args.data(), &a, sizeof(A));
+}
+int fft_key(bool f32, int L) { return 2 * L + (f32 ? 1 : 0); }
+}  // namespace
+
+bool batch_part_fft(Solver& s, int cls, int kRK, BatchPart& part, std::string& err) {
+  part = BatchPart();
+  if (s.mb || s.compat_np > 0) {
+    err = "multi-band and last-plane replay solvers have no batched transforms";
+    return false;
+  }
+  if (cls == BC_ZINV_NL) {
+    if (!four_step_z(s)) {
+      err = "the z-inverse of this solver is not a four-step kernel";
+      return false;
+    }
+    const bool nl = true, f32 = s.mixed;
+    ZInvArgs a;
+    a.jobs = s.d_nljobs;
+    a.njobs = s.n_nljobs;
+    a.Mx = s.Mx;
+    a.zs = s.zs;
+    int TI = 0, thr = 0;
+    switch (s.pz.L) {
+#define C(L, P, Q, TIv) \
+  case L:               \
+    TI = TIv;           \
+    thr = TIv * cmax(P, Q); \
+    break;
+      CFD_ZINV4(C)
+#undef C
+      default: err = "no four-step z-inverse of this length"; return false;
+    }
+    a.ntiles = (s.Mx + TI - 1) / TI;
+    a.jper = nl ? (a.njobs + 7) / 8 : 0;
+    part.gx = nl ? 8 * a.jper * a.ntiles : a.ntiles;
+    part.gy = nl ? 1 : a.njobs;
+    part.block = dim3(thr);
+    part.key = fft_key(f32, s.pz.L);
+    part.tw = f32 ? (const void*)s.d_twzf : (const void*)s.d_twz;
+    put_args(part, a);
+    return true;
+  }
+  if (cls == BC_XPASS) {
+    XNLArgs a;
+    xpass_args(s, kRK == 1, a);
+    if ((s.fft_stockham && !s.mixed) || s.px.L != s.ngx || s.N1 != 2 * (s.ngx / 3) || s.Mx != s.N1 / 2 + 1) {
+      err = "the x-pass of this solver is not a four-step kernel";
+      return false;
+    }
+    int R = 0, thr = 0;
+    if (s.mixed) {
+      switch (s.px.L) {
+#define C(L, P, Q, Rv) \
+  case L:              \
+    R = Rv;            \
+    thr = 3 * xp_width(P, Q, Rv); \
+    break;
+        CFD_XPASS4_F32(C)
+        default: break;
+      }
+    } else {
+      switch (s.px.L) {
+        CFD_XPASS4_F64(C)
+#undef C
+        default: break;
+      }
+    }
+    if (!R) {
+      err = "no four-step x-pass of this length";
+      return false;
+    }
+    part.gx = (s.ngz + R - 1) / R;
+    part.gy = win_count(a.win);
+    part.block = dim3(thr);
+    part.key = fft_key(s.mixed, s.px.L);
+    part.tw = s.mixed ? (const void*)s.d_twxf : (const void*)s.d_twx;
+    put_args(part, a);
+    return true;
+  }
+  if (cls == BC_ZFWD) {
+    if (!four_step_z(s)) {
+      err = "the z-forward of this solver is not a four-step kernel";
+      return false;
+    }
+    ZNLArgs a;
+    zfwd_args(s, a);
+    int TI = 0, thr = 0;
+    if (s.mixed) {
+      switch (s.pz.L) {
+#define C(L, P, Q, TIv) \
+  case L:               \
+    TI = TIv;           \
+    thr = TIv * cmax(P, Q); \
+    break;
+        CFD_ZFWD4_F32(C)
+        default: break;
+      }
+    } else {
+      switch (s.pz.L) {
+        CFD_ZFWD4_F64(C)
+#undef C
+        default: break;
+      }
+    }
+    if (!TI) {
+      err = "no four-step z-forward of this length";
+      return false;
+    }
+    part.gx = (s.Mx + TI - 1) / TI;
+    part.gy = win_count(a.winu) + win_count(a.winv);
+    part.block = dim3(thr);
+    part.key = fft_key(s.mixed, s.pz.L);
+    part.tw = s.mixed ? (const void*)s.d_twzf : (const void*)s.d_twz;
+    put_args(part, a);
+    return true;
+  }
+  err = "not a transform class";
+  return false;
+}
+
+void batch_launch_fft(hipStream_t st, const BatchLaunch& b) {
+  const bool f32 = b.key & 1;
+  const int L = b.key >> 1;
+  if (b.cls == BC_ZINV_NL) {
+    const ZInvArgs* as = static_cast<const ZInvArgs*>(b.dargs);
+    if (f32) {
+      switch (L) {
+#define C(Lv, P, Q, TI)                                                                                   \
+  case Lv:                                                                                                \
+    hipLaunchKernelGGL((k_zinv4_b<float, Lv, P, Q, TI>), b.grid, b.block, 0, st, as, b.map, (const float2*)b.tw); \
+    return;
+        CFD_ZINV4(C)
+#undef C
+        default: return;
+      }
+    }
+    switch (L) {
+#define C(Lv, P, Q, TI)                                                                                     \
+  case Lv:                                                                                                  \
+    hipLaunchKernelGGL((k_zinv4_b<double, Lv, P, Q, TI>), b.grid, b.block, 0, st, as, b.map, (const double2*)b.tw); \
+    return;
+      CFD_ZINV4(C)
+#undef C
+      default: return;
+    }
+  }
+  if (b.cls == BC_XPASS) {
+    const XNLArgs* as = static_cast<const XNLArgs*>(b.dargs);
+    if (f32) {
+      switch (L) {
+#define C(Lv, P, Q, R)                                                                                   \
+  case Lv:                                                                                               \
+    hipLaunchKernelGGL((k_xpass4_b<float, Lv, P, Q, R>), b.grid, b.block, 0, st, as, b.map, (const float2*)b.tw); \
+    return;
+        CFD_XPASS4_F32(C)
+#undef C
+        default: return;
+      }
+    }
+    switch (L) {
+#define C(Lv, P, Q, R)                                                                                     \
+  case Lv:                                                                                                 \
+    hipLaunchKernelGGL((k_xpass4_b<double, Lv, P, Q, R>), b.grid, b.block, 0, st, as, b.map, (const double2*)b.tw); \
+    return;
+      CFD_XPASS4_F64(C)
+#undef C
+      default: return;
+    }
+  }
+  if (b.cls == BC_ZFWD) {
+    const ZNLArgs* as = static_cast<const ZNLArgs*>(b.dargs);
+    if (f32) {
+      switch (L) {
+#define C(Lv, P, Q, TI)                                                                                   \
+  case Lv:                                                                                                \
+    hipLaunchKernelGGL((k_zfwd4_b<float, Lv, P, Q, TI>), b.grid, b.block, 0, st, as, b.map, (const float2*)b.tw); \
+    return;
+        CFD_ZFWD4_F32(C)
+#undef C
+        default: return;
+      }
+    }
+    switch (L) {
+#define C(Lv, P, Q, TI)                                                                                     \
+  case Lv:                                                                                                  \
+    hipLaunchKernelGGL((k_zfwd4_b<double, Lv, P, Q, TI>), b.grid, b.block, 0, st, as, b.map, (const double2*)b.tw); \
+    return;
+      CFD_ZFWD4_F64(C)
+#undef C
+      default: return;
+    }
   }
 }
 

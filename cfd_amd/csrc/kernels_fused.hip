@@ -16,6 +16,7 @@
 //  k_pressure   solveP (littleharsh.f90:556-611, LUsolP tridLU_3D_IB.f90:284-318)
 //               and v_corr (littleharsh.f90:796-907) with the divergence of the
 //               corrected field, in one forward/backward sweep.
+#include <cstring>
 #include <algorithm>
 
 #include "common.hpp"
@@ -322,13 +323,15 @@ __device__ __forceinline__ RowSys seg_build(const FusedArgs& a, const FieldPtrs&
 // three components of one tile run on one XCD at about the same time and the
 // pressure rows they all read come from that XCD's L2 after the first.
 // grid 24 ceil(ntiles / 8), ntiles = ceil(Ncol / CB)
+// Block bx of the grid; k_rhs_seg runs it for one solver, k_rhs_seg_b for
+// the members of a batch (blockDim.y may exceed a member's segment count).
 template <int SEG, typename TN>
-__global__ void __launch_bounds__(256) k_rhs_seg(FusedArgs a) {
+__device__ __forceinline__ void rhs_seg_body(const FusedArgs& a, int bx) {
   extern __shared__ double sm[];
   const int CB = blockDim.x, NT = blockDim.x * blockDim.y;
   const int cl = threadIdx.x, sg = threadIdx.y, me = sg * CB + cl;
-  const int slot = blockIdx.x >> 3;
-  const int tile = (slot / 3) * 8 + (blockIdx.x & 7), f = slot % 3;
+  const int slot = bx >> 3;
+  const int tile = (slot / 3) * 8 + (bx & 7), f = slot % 3;
   if (tile * CB >= a.Ncol) return;  // the whole block: no barrier is reached
   const int col = tile * CB + cl;
   const int n = (f == 1) ? a.nyv : a.nyu;
@@ -470,6 +473,20 @@ __global__ void __launch_bounds__(256) k_rhs_seg(FusedArgs a) {
     }
     if (sg < S - 1) uo[(size_t)(b + SEG) * N + col] = xr;
   }
+}
+
+template <int SEG, typename TN>
+__global__ void __launch_bounds__(256) k_rhs_seg(FusedArgs a) {
+  rhs_seg_body<SEG, TN>(a, blockIdx.x);
+}
+
+template <int SEG, typename TN>
+__global__ void __launch_bounds__(256) k_rhs_seg_b(const FusedArgs* __restrict__ as, BMap bm) {
+  int bx = 0, by = 0;
+  const int m = bmap_member(bm, blockIdx.x, bx, by);
+  if (m < 0) return;
+  const FusedArgs a = as[m];  // a private copy, as the single-solver kernel's argument
+  rhs_seg_body<SEG, TN>(a, bx);
 }
 
 // predictor on the IB planes only: grid (ceil(Ncol/kTB), max IB planes, 3)
@@ -654,11 +671,11 @@ __global__ void __launch_bounds__(64) k_pressure(PressArgs a) {
 // CU: the first pass's u1 rows are kept for the projection too (the v faces
 // always are); a register budget matter, chosen per segment length
 template <int SEG, bool CU>
-__global__ void __launch_bounds__(256) k_press_seg(PressArgs a) {
+__device__ __forceinline__ void press_seg_body(const PressArgs& a, int bx) {
   extern __shared__ double sm[];
   const int CB = blockDim.x, NT = blockDim.x * blockDim.y;
   const int cl = threadIdx.x, sg = threadIdx.y, me = sg * CB + cl;
-  const int col = blockIdx.x * CB + cl;
+  const int col = bx * CB + cl;
   const int n = a.nyp;
   const int S = (n + SEG + 1) / (SEG + 1);
   double2* Yb = reinterpret_cast<double2*>(sm);
@@ -911,6 +928,20 @@ __global__ void __launch_bounds__(256) k_press_seg(PressArgs a) {
   a.div[(size_t)last * N + col] = gap(last) ? make_double2(0.0, 0.0)
                                             : make_double2(du.x + dw.x + m * (vf.x - vprev.x),
                                                            du.y + dw.y + m * (vf.y - vprev.y));
+}
+
+template <int SEG, bool CU>
+__global__ void __launch_bounds__(256) k_press_seg(PressArgs a) {
+  press_seg_body<SEG, CU>(a, blockIdx.x);
+}
+
+template <int SEG, bool CU>
+__global__ void __launch_bounds__(256) k_press_seg_b(const PressArgs* __restrict__ as, BMap bm) {
+  int bx = 0, by = 0;
+  const int m = bmap_member(bm, blockIdx.x, bx, by);
+  if (m < 0) return;
+  const PressArgs a = as[m];
+  press_seg_body<SEG, CU>(a, bx);
 }
 
 // (0,0) flow-rate correction staged through LDS: the column, metrics and
@@ -1278,8 +1309,8 @@ void launch_rhs_solve(Solver& s, int kRK) {
   }
 }
 
-void launch_pressure(Solver& s, int kRK) {
-  PressArgs a;
+namespace {
+void press_args(Solver& s, int kRK, PressArgs& a) {
   a.u1 = s.F[CFD_U1]; a.u2 = s.F[CFD_U2]; a.u3 = s.F[CFD_U3];
   a.p = s.F[CFD_P]; a.psi = s.F[CFD_PSI]; a.div = s.F[CFD_DIV];
   a.cs = s.cscr;
@@ -1298,6 +1329,12 @@ void launch_pressure(Solver& s, int kRK) {
   a.shortc = s.mb ? s.d_short : nullptr;
   a.P1 = s.N4[1] - s.jlo_p;      // p row N(4,1): the end of band 1
   a.P3 = s.N4[2] + 1 - s.jlo_p;  // p row N(4,2)+1: the start of band 3
+}
+}  // namespace
+
+void launch_pressure(Solver& s, int kRK) {
+  PressArgs a;
+  press_args(s, kRK, a);
   const int R = s.seg_rows;
   const int sy = (s.nyp + R + 1) / (R + 1);  // S of k_press_seg
   if (sy <= 256) {
@@ -1331,6 +1368,85 @@ void launch_meanflow_lds(Solver& s, int kRK) {
   hipLaunchKernelGGL(k_meanflow_lds, dim3(1), dim3(256), lds, s.stream, s.F[CFD_U1], (size_t)s.Ncol, s.jlo_u, s.nn,
                      (const double*)s.d_d2u, (const double*)s.d_yv, s.jlo_v, s.d_sc, s.bRK[kRK - 1],
                      s.dRK[kRK - 1], s.P.Re, s.gw[0], s.gw[1]);
+}
+
+// ---------------------------------------------------------------------------
+// batched launches of the column sweeps (ensembles)
+// ---------------------------------------------------------------------------
+namespace {
+int seg_index(int R) { return R == 5 ? 0 : R == 9 ? 1 : R == 21 ? 3 : 2; }
+constexpr int kSegOf[4] = {5, 9, 13, 21};
+}  // namespace
+
+bool batch_part_fused(Solver& s, int cls, int kRK, BatchPart& part, std::string& err) {
+  part = BatchPart();
+  if (cls != BC_RHS && cls != BC_PRESS) {
+    err = "not a column-sweep class";
+    return false;
+  }
+  const int R = seg_rows_supported(s.seg_rows) ? s.seg_rows : 13;
+  const int n = cls == BC_RHS ? s.nyu : s.nyp;
+  const int sy = (n + R + 1) / (R + 1);
+  if (sy > 256) {
+    err = "the column sweep of this solver is not segmented";
+    return false;
+  }
+  int cb = 1;
+  while (cb * 2 * sy <= s.solve_threads) cb *= 2;
+  part.block = dim3(cb, sy);
+  part.lds_thread = sizeof(double) * 15;
+  if (cls == BC_RHS) {
+    FusedArgs a;
+    fused_args(s, kRK, &a);
+    const int ntiles = (s.Ncol + cb - 1) / cb;
+    part.gx = 24 * ((ntiles + 7) / 8);
+    part.lds_fixed = sizeof(double) * 4 * (size_t)s.nyu + (size_t)s.nyu;
+    part.args.resize(sizeof(a));
+    std::memcpy(part.args.data(), &a, sizeof(a));
+  } else {
+    PressArgs a;
+    press_args(s, kRK, a);
+    part.gx = (s.Ncol + cb - 1) / cb;
+    part.lds_fixed = sizeof(double) * (5 * (size_t)s.nyp + 3);
+    part.args.resize(sizeof(a));
+    std::memcpy(part.args.data(), &a, sizeof(a));
+  }
+  part.gy = 1;
+  // variant: rows per segment, precision (column sweeps of u only), threads per column tile
+  part.key = (seg_index(R) * 2 + (cls == BC_RHS && s.mixed ? 1 : 0)) * 512 + cb;
+  return true;
+}
+
+void batch_launch_fused(hipStream_t st, const BatchLaunch& b) {
+  const int seg = kSegOf[(b.key / 512) / 2];
+  const bool f32 = ((b.key / 512) & 1) != 0;
+  if (b.cls == BC_RHS) {
+    const FusedArgs* as = static_cast<const FusedArgs*>(b.dargs);
+#define C(S)                                                                                         \
+  case S:                                                                                            \
+    if (f32)                                                                                         \
+      hipLaunchKernelGGL((k_rhs_seg_b<S, float2>), b.grid, b.block, b.lds, st, as, b.map);          \
+    else                                                                                             \
+      hipLaunchKernelGGL((k_rhs_seg_b<S, double2>), b.grid, b.block, b.lds, st, as, b.map);         \
+    return;
+    switch (seg) {
+      C(5) C(9) C(13) C(21)
+      default: return;
+    }
+#undef C
+  }
+  if (b.cls == BC_PRESS) {
+    const PressArgs* as = static_cast<const PressArgs*>(b.dargs);
+#define C(S)                                                                                      \
+  case S:                                                                                         \
+    hipLaunchKernelGGL((k_press_seg_b<S, (S <= 9)>), b.grid, b.block, b.lds, st, as, b.map);      \
+    return;
+    switch (seg) {
+      C(5) C(9) C(13) C(21)
+      default: return;
+    }
+#undef C
+  }
 }
 
 }  // namespace cfd

@@ -184,20 +184,21 @@ def _gpu_ms_per_substep(ch, nsteps):
     return sum(m for m, _ in prof.values()) / (3.0 * nsteps)
 
 
-def tune(member, nsteps=2, repeats=2):
+def tune(member, nsteps=2, repeats=2, ffts=None):
     """Time every launch-shape choice for ``member`` on scratch solvers (one
     per FFT engine) started from the member's initial field; kernel time per
     substep from the solvers' event timers.  Returns (best tuning, table of
     (tuning, ms per substep))."""
-    return tune_params(member.params, lambda ch: start(ch, member), nsteps, repeats)
+    return tune_params(member.params, lambda ch: start(ch, member), nsteps, repeats, ffts)
 
 
-def tune_params(params, load, nsteps=2, repeats=2):
+def tune_params(params, load, nsteps=2, repeats=2, ffts=None):
     """The autotuner for any solver: ``load(ch)`` puts the starting state
     into a scratch solver.  The mixed path (precision 1) runs the four-step
-    engine only, so only its column-solve shapes are tuned."""
+    engine only, so only its column-solve shapes are tuned; ``ffts`` limits
+    the FFT engines tried (the batched ensemble kernels are four-step)."""
     table = []
-    for fft in (FFT_CHOICES if params.precision == 0 else (0,)):
+    for fft in (ffts if ffts is not None else FFT_CHOICES if params.precision == 0 else (0,)):
         ch = Channel(params, tuning=dict(fft=fft))
         try:
             load(ch)
@@ -220,14 +221,21 @@ class Ensemble:
     profiling off, no closure attached and no statistics request pending.
     The channels stay usable: call ``sync`` before reading or driving one
     directly.  A member retuned after the capture keeps its captured shapes
-    here."""
+    here.
 
-    def __init__(self, channels):
+    ``batched``: the nonlinear transforms and the two column sweeps of all
+    members run as one launch per kernel variant over a member table
+    (cfd_ensemble_create_batched) instead of one launch per member; needs
+    single-resolution members on the four-step transforms."""
+
+    def __init__(self, channels, batched=False):
         self._lib = _lib.load()
         self.channels = list(channels)
+        self.batched = bool(batched)
         hs = (ctypes.c_void_p * len(self.channels))(*[c._h.value for c in self.channels])
         e = ctypes.c_void_p()
-        rc = self._lib.cfd_ensemble_create(hs, len(self.channels), ctypes.byref(e))
+        create = self._lib.cfd_ensemble_create_batched if self.batched else self._lib.cfd_ensemble_create
+        rc = create(hs, len(self.channels), ctypes.byref(e))
         if rc:
             raise _lib.CfdError("cfd_amd error %d: %s" % (rc, self._lib.cfd_ensemble_last_error(None).decode()))
         self._e = e
@@ -266,19 +274,24 @@ class Ensemble:
         self.close()
 
 
-def run(n=16, seed=1, steps=50, autotune=True, log=print):
+def run(n=16, seed=1, steps=50, autotune=True, log=print, batched=False, tunings=None):
     """The ensemble run: members drawn, each tuned (or left at the default
-    shapes), started, its first step eager (the capture needs one substep run
-    on every member), the other ``steps - 1`` steps by the captured graphs.
-    Returns a report: members, tunings, throughput, final state per member."""
+    shapes; ``tunings``: given ones), started, its first step eager (the
+    capture needs one substep run on every member), the other ``steps - 1``
+    steps by the captured graphs, batched (one launch per kernel variant over
+    the members) or one launch per member.  Returns a report: members,
+    tunings, throughput, final state per member."""
     import time
     from . import diagnostics
     ms = members(n, seed)
     rows = []
     t0 = time.perf_counter()
     for i, m in enumerate(ms):
+        if tunings is not None:
+            rows.append(dict(tuning=tunings[i], ms_default=None, ms_tuned=None))
+            continue
         if autotune:
-            best, table = tune(m)
+            best, table = tune(m, ffts=(0,) if batched else None)
             dflt = [x for t, x in table if t == dict(fft=0, seg_rows=13, solve_threads=256)][0]
             bms = min(x for _, x in table)
         else:
@@ -299,7 +312,7 @@ def run(n=16, seed=1, steps=50, autotune=True, log=print):
         for ch in chs:
             ch.sync()
         dt1 = [ch.scalars()["dt"] for ch in chs]
-        with Ensemble(chs) as ens:
+        with Ensemble(chs, batched=batched) as ens:
             t1 = time.perf_counter()
             ens.step(steps - 1)
             ens.sync()
@@ -325,7 +338,7 @@ def run(n=16, seed=1, steps=50, autotune=True, log=print):
             ch.close()
     pts = sum(m.params.ngalx * m.params.ny * m.params.ngalz for m in ms)
     sub = n * 3 * (steps - 1)
-    return dict(members=out, steps=steps, graph_steps=steps - 1, seconds=T, tuning_seconds=t_tune,
+    return dict(members=out, steps=steps, graph_steps=steps - 1, batched=batched, seconds=T, tuning_seconds=t_tune,
                 member_substeps_per_s=sub / T, ensemble_substeps_per_s=3 * (steps - 1) / T,
                 point_substeps_per_s=pts * 3 * (steps - 1) / T, points=pts, autotune=autotune,
                 device_bytes=sum(m["device_bytes"] for m in out))
@@ -339,9 +352,11 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--batched", action="store_true",
+                    help="batched launches over the member table instead of one launch per member kernel")
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
-    rep = run(a.members, a.seed, a.steps, autotune=not a.no_tune)
+    rep = run(a.members, a.seed, a.steps, autotune=not a.no_tune, batched=a.batched)
     s = json.dumps(rep, indent=1)
     if a.out:
         with open(a.out, "w") as fh:

@@ -386,6 +386,19 @@ int cfd_closure_phys(cfd_handle h, int which, double* out);
  * hold kernel nodes only. */
 typedef struct cfd_ensemble_s* cfd_ensemble;
 int cfd_ensemble_create(cfd_handle* members, int n, cfd_ensemble* out);
+/* The same substeps with batched kernels: the z-inverse, x-pass and
+ * z-forward of the nonlinear term and the two fused column sweeps run as one
+ * launch per kernel variant (transform length, precision, segment shape)
+ * over a table of the members (their arguments and block ranges, in device
+ * memory owned by the ensemble), so the members' blocks of a kernel class
+ * share a launch; each member's own kernels (CFL, IB correction, flow-rate
+ * correction, filter) run between them.  Each kRK graph is built node by
+ * node: a batch depends only on the latest work of its own members.  Every
+ * member must use the four-step transforms and the segmented column sweeps
+ * (single resolution, no last-plane replay); otherwise CFD_EINVAL names the
+ * member.  Results are bit-identical to cfd_ensemble_create's graphs and to
+ * eager stepping. */
+int cfd_ensemble_create_batched(cfd_handle* members, int n, cfd_ensemble* out);
 int cfd_ensemble_destroy(cfd_ensemble e);
 const char* cfd_ensemble_last_error(cfd_ensemble e);
 int cfd_ensemble_substep(cfd_ensemble e, int kRK);

@@ -2,9 +2,11 @@
 tuning, and the four-step FFT lengths of the ensemble grids.
 
 * graph replays against eager substeps: two copies of the 16-member
-  ensemble from the same start, one stepped by the captured graphs, one
-  member by member through cfd_step; every field and scalar bit-identical
-  after each replay (host reads of every member between replays).
+  ensemble from the same start, one stepped by the captured graphs (with
+  per-member launches, and with batched launches over the member table),
+  one member by member through cfd_step; every field and scalar
+  bit-identical after each replay (host reads of every member between
+  replays).
 * tunings: the same member under every FFT engine / segment shape gives
   the same solution to FP64 rounding (relative L2 <= 1e-11 after 2 steps).
 * the segmented y solves at every segment length against the oracle, on
@@ -45,18 +47,21 @@ def _same(a, b, what):
         assert sa[k] == sb[k], "%s: scalar %s %r != %r" % (what, k, sa[k], sb[k])
 
 
-def test_graph_stepped_members_against_oracle():
+@pytest.mark.parametrize("batched", [False, True])
+def test_graph_stepped_members_against_oracle(batched):
     """The three smallest members of the 16-member set (one of them the
     96 x 96 x 64 end member), one eager substep, then kRK = 2 and 3 and one
-    full RK3 step from the captured graphs, against the oracle's substeps
-    from the same state: relative L2 <= 1e-11 per field, the same t and
-    mean pressure gradient."""
+    full RK3 step from the captured graphs (per-member launches, or batched
+    launches over the member table), against the oracle's substeps from the
+    same state: relative L2 <= 1e-11 per field, the same t and mean pressure
+    gradient."""
     from oracle.oracle import Oracle, set_threads
     import os
     set_threads(min(16, len(os.sched_getaffinity(0))))
     ms = sorted(ensemble.members(16, seed=1),
                 key=lambda m: m.params.ngalx * m.params.ny * m.params.ngalz)[:3]
-    print("3 members, graphs vs oracle:", [(m.params.ngalx, m.params.ny, m.params.ngalz) for m in ms], flush=True)
+    print("3 members, graphs vs oracle, batched=%s:" % batched,
+          [(m.params.ngalx, m.params.ny, m.params.ngalz) for m in ms], flush=True)
     A = _started(ms)
     orcs = []
     for m, ch in zip(ms, A):
@@ -69,7 +74,7 @@ def test_graph_stepped_members_against_oracle():
         ch.substep(1)  # the eager warm-up substep the graphs need
         o.substep(1)
         ch.sync()
-    with ensemble.Ensemble(A) as ens:
+    with ensemble.Ensemble(A, batched=batched) as ens:
         for kRK in (2, 3):
             ens.substep(kRK)
         ens.step(1)
@@ -89,16 +94,19 @@ def test_graph_stepped_members_against_oracle():
         o.close()
 
 
-def test_graph_replays_equal_eager_substeps():
+@pytest.mark.parametrize("batched", [False, True])
+def test_graph_replays_equal_eager_substeps(batched):
+    """Both graph forms (per-member launches; batched launches over the member
+    table) against the same members stepped eagerly, bit for bit."""
     ms = ensemble.members(16, seed=1)
-    print("16 members, graph vs eager", flush=True)
+    print("16 members, graph vs eager, batched=%s" % batched, flush=True)
     A, B = _started(ms), _started(ms)
     for ch in A + B:
         ch.step(1)  # warm-up substeps, eager on both sides
     for ch in A + B:
         ch.sync()
     dt1 = [ch.scalars()["dt"] for ch in A]
-    with ensemble.Ensemble(A) as ens:
+    with ensemble.Ensemble(A, batched=batched) as ens:
         for rep in range(3):
             ens.step(1)
             ens.sync()
@@ -146,8 +154,19 @@ def test_ensemble_refusals():
         e.sync()
         with pytest.raises(CfdError, match="kRK"):
             e.substep(4)
+    # the batched kernels are four-step transforms: a Stockham member is
+    # refused, by name
+    c = Channel(m[1].params, tuning=dict(fft=1))
+    ensemble.start(c, m[1])
+    c.step(1)
+    with pytest.raises(CfdError, match="member 1: .*four-step"):
+        ensemble.Ensemble([a, c], batched=True)
+    with ensemble.Ensemble([a, c]) as e:
+        e.step(1)
+        e.sync()
     a.close()
     b.close()
+    c.close()
 
 
 @pytest.mark.parametrize("which", [0, 15])
