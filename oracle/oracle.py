@@ -275,6 +275,17 @@ def ref_rft(n, c, isa, m, iopt):
     load_ref().ref_rft(ctypes.c_int(n), c.ctypes.data_as(_D), ctypes.c_int(isa), ctypes.c_int(m), ctypes.c_int(iopt))
 
 
+def ref_plane_partition(N20, ngalx, ngalz, np_ranks):
+    """The compiled reference's proc_lims_planes (start.f90) in the layout of
+    plane_partition; np_ranks >= 3."""
+    lim = (ctypes.c_int * (4 * np_ranks))()
+    load_ref().ref_plane_partition((ctypes.c_int * 20)(*[int(v) for v in N20]), ctypes.c_int(ngalx),
+                                   ctypes.c_int(ngalz), ctypes.c_int(np_ranks), lim)
+    lim = list(lim)
+    return {g: [(lim[(k * 2) * np_ranks + r], lim[(k * 2 + 1) * np_ranks + r]) for r in range(np_ranks)]
+            for k, g in enumerate(("u", "v"))}
+
+
 def or_cft(n, c, ise, isa, m, iopt):
     load().or_cft(ctypes.c_int(n), c.ctypes.data_as(_D), ctypes.c_int(ise), ctypes.c_int(isa), ctypes.c_int(m),
                   ctypes.c_int(iopt))

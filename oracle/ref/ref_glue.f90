@@ -1,5 +1,6 @@
 ! ref_glue.f90 -- project-written C bindings around the MPI-free reference
-! routines (FFTPACK wrappers cft/rft, y_grid_canopy, boundary_circ_rough).
+! routines (FFTPACK wrappers cft/rft, y_grid_canopy, boundary_circ_rough,
+! proc_lims_planes).
 ! Built only into oracle/_ref/ by oracle/Makefile.ref, from the reference
 ! sources where they lie.  Test infrastructure: used to pin the oracle.
 module ref_glue
@@ -153,6 +154,56 @@ contains
       do r = 1, 4
         nout(4*b + r) = N(r, b)
       end do
+    end do
+  end subroutine
+
+  ! proc_lims_planes (start.f90) with the module state it reads set from the
+  ! arguments: the band table N(1:4,0:nband+1) (column-major, 20 values), one
+  ! x/z resolution for every band (Ngal rows 3 and 4 are N's, as y_grid_canopy
+  ! sets them), np ranks (np >= 3: the balance loop needs a central rank).
+  ! Called for a rank other than 0, so it prints nothing.  Out: per rank the
+  ! first and last planes of the u grid, then of the v grid (lim(4*np)).
+  subroutine ref_plane_partition(n20, ngalx, ngalz, npin, lim) bind(C, name="ref_plane_partition")
+    integer(c_int) :: n20(20)
+    integer(c_int), value :: ngalx, ngalz, npin
+    integer(c_int) :: lim(*)
+    integer :: r, b
+    if (associated(N)) deallocate(N)
+    if (associated(Ngal)) deallocate(Ngal)
+    if (associated(jlim)) deallocate(jlim)
+    if (associated(procs)) deallocate(procs)
+    if (associated(procs_b)) deallocate(procs_b)
+    if (associated(procs_c)) deallocate(procs_c)
+    if (associated(planelim)) deallocate(planelim)
+    if (associated(limPL_incw)) deallocate(limPL_incw)
+    if (associated(limPL_excw)) deallocate(limPL_excw)
+    if (associated(limPL_FFT)) deallocate(limPL_FFT)
+    if (associated(bandPL)) deallocate(bandPL)
+    if (associated(bandPL_FFT)) deallocate(bandPL_FFT)
+    allocate(N(4,0:nband+1), Ngal(4,0:nband+1), jlim(2,3,nband))
+    N = reshape(n20, [4, nband+2])
+    Ngal = 0
+    do b = 0, nband+1
+      Ngal(3,b) = N(3,b)
+      Ngal(4,b) = N(4,b)
+    end do
+    do b = 1, nband
+      Ngal(1,b) = ngalx
+      Ngal(2,b) = ngalz
+    end do
+    ! only the FFT-plane limits (limPL_FFT) read these; they are not compared
+    jlim = 0
+    jlim(1,ugrid,2) = N(4,1)
+    jlim(2,ugrid,2) = N(4,2)
+    physlim_bot = jlim(1,ugrid,2) + npin
+    physlim_top = jlim(2,ugrid,2) - npin
+    np = npin
+    call proc_lims_planes(1)
+    do r = 0, np-1
+      lim(1 + r) = planelim(ugrid,1,r)
+      lim(1 + np + r) = planelim(ugrid,2,r)
+      lim(1 + 2*np + r) = planelim(vgrid,1,r)
+      lim(1 + 3*np + r) = planelim(vgrid,2,r)
     end do
   end subroutine
 

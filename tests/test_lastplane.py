@@ -41,6 +41,22 @@ def test_partition_matches_oracle(name):
         assert fieldfile.plane_partition(p, n) == oracle.plane_partition(N, p.ngalx, p.ngalz, n), n
 
 
+@pytest.mark.timeout(120)
+@pytest.mark.parametrize("name", list(CONFIGS))
+def test_partition_matches_compiled_reference(name):
+    """The oracle's partition against the reference's own proc_lims_planes,
+    compiled from its text in start.f90 (oracle/Makefile.ref): every rank's
+    first and last u and v planes, np = 3..120.  This pins the split of the
+    ranks between the bands, which the properties below leave open."""
+    if oracle.load_ref() is None:
+        pytest.skip("compiled reference routines not built (oracle/Makefile.ref needs the reference sources)")
+    p = CONFIGS[name]
+    _, N = _bands(p)
+    for n in range(3, 121):
+        assert oracle.ref_plane_partition(N, p.ngalx, p.ngalz, n) == \
+            oracle.plane_partition(N, p.ngalx, p.ngalz, n), (name, n)
+
+
 @pytest.mark.parametrize("name", list(CONFIGS))
 def test_partition_properties(name):
     p = CONFIGS[name]
