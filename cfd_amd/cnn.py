@@ -144,3 +144,65 @@ class CNN:
         v = ctypes.c_double()
         self._call("cfd_cnn_device_bytes", ctypes.byref(v))
         return v.value
+
+
+def flops_per_forward(npts):
+    """Multiply-adds x 2 of one forward pass as the kernels run it: 27 taps x
+    (padded input channels) x (padded output channels) per point and layer
+    (layer 0 pads its 4 inputs to 16, layer 3 its 3 outputs to 32)."""
+    cin_pad, cout_pad = (16, 32, 32, 32), (32, 32, 32, 32)
+    return 2.0 * 27 * sum(a * b for a, b in zip(cin_pad, cout_pad)) * npts
+
+
+def bench(ny=209, nz=96, nx=192, reps=10, seed=0):
+    """Times the network on one volume of the closure's coarse grid (random
+    input and target): a forward pass (ms, wall clock over ``reps`` passes,
+    one output read at the end, less one output read alone) and a training
+    step (forward, backward with the loss read, Adam)."""
+    import time
+    net = CNN(ny, nz, nx)
+    try:
+        rng = np.random.default_rng(seed)
+        net.set_params(init_params(seed))
+        net.set_input(rng.standard_normal((ny, nz, nx, 4)).astype(np.float32))
+        net.set_target(0.1 * rng.standard_normal((ny, nz, nx, 3)).astype(np.float32))
+        net.forward()
+        net.backward()
+        net.adam(1e-4)
+        t0 = time.perf_counter()
+        net.forward()
+        t_read = time.perf_counter() - t0  # one pass + the read
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            net._call("cfd_cnn_forward")
+        net.forward()
+        t_fwd = (time.perf_counter() - t0 - t_read) / reps
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            net._call("cfd_cnn_forward")
+            net.backward()
+            net.adam(1e-4)
+        t_train = (time.perf_counter() - t0) / reps
+        fl = flops_per_forward(net.npts)
+        return {"volume": [ny, nz, nx], "points": net.npts, "params": nparams(), "reps": reps,
+                "forward_ms": 1e3 * t_fwd, "train_step_ms": 1e3 * t_train,
+                "forward_tflop": fl / 1e12, "forward_tflops_per_s": fl / t_fwd / 1e12,
+                "device_bytes": net.device_bytes()}
+    finally:
+        net.close()
+
+
+def main(argv=None):
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description="time the closure network on one coarse-grid volume")
+    ap.add_argument("--ny", type=int, default=209)
+    ap.add_argument("--nz", type=int, default=96)
+    ap.add_argument("--nx", type=int, default=192)
+    ap.add_argument("--reps", type=int, default=10)
+    a = ap.parse_args(argv)
+    print(json.dumps(bench(a.ny, a.nz, a.nx, a.reps)))
+
+
+if __name__ == "__main__":
+    main()
