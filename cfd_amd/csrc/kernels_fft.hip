@@ -982,6 +982,169 @@ __global__ void __launch_bounds__(3 * xp_width(P, Q, R)) k_xpass4_b(const XNLArg
   xpass4_body<T, L, P, Q, R>(as[m], twg, bx, by);
 }
 
+// ---------------------------------------------------------------------------
+// Four-step IB transforms (FP64): the x-pass's phases without the products.
+// ---------------------------------------------------------------------------
+// x-inverse of row pairs (r + i u): grid (ceil(Ngz/R), njobs), W lanes per
+// block; lane task (qq, r) in phase 1, (k1, r) in phase 2, which stores the
+// physical row: real part to pa, imaginary part to pb.  Rows of Mx modes.
+template <int L, int P, int Q, int R>
+__global__ void __launch_bounds__(xp_width(P, Q, R)) k_xinv4(const XJob* __restrict__ jobs,
+                                                            const double2* __restrict__ twg, int ngz,
+                                                            const double2* __restrict__ zrow) {
+  using C = double2;
+  constexpr int Mx = Dealias<L>::Mx, W = xp_width(P, Q, R), NS = (Q * R + W - 1) / W;
+  __shared__ C S[R * P * (Q + 1)];
+  const XJob J = jobs[blockIdx.y];
+  const int lane = threadIdx.x, z0 = blockIdx.x * R;
+  sfor<0, NS>([&](auto sc) {
+    constexpr int t = decltype(sc)::value;
+    const int jt = t * W + lane;
+    const int j = (t * W + W <= Q * R || jt < Q * R) ? jt : Q * R - 1;
+    const int qq = j % Q, r = j / Q, z = z0 + r;
+    const C *A = zrow, *B = zrow;
+    if (z < ngz) {
+      const size_t ro = ((size_t)J.plane * ngz + z) * Mx;
+      A = J.za + ro;
+      B = J.zb + ro;
+    }
+    C v[P];
+    sfor<0, P>([&](auto pc) {
+      constexpr int p = decltype(pc)::value;
+      v[p] = pair_mode_q<L, Q, p>(A, B, Q * p + qq);
+    });
+    dft<true, P>(v);
+    if (t * W + W <= Q * R || jt < Q * R) {
+      C* s = S + r * P * (Q + 1);
+      s[qq] = v[0];
+      sfor<1, P>([&](auto kc) {
+        constexpr int k1 = decltype(kc)::value;
+        s[k1 * (Q + 1) + qq] = mul_tw<true>(v[k1], twg[qq * k1]);
+      });
+    }
+  });
+  __syncthreads();
+  if (lane >= P * R) return;
+  const int k1 = lane % P, r = lane / P, z = z0 + r;
+  const C* s = S + r * P * (Q + 1) + k1 * (Q + 1);
+  C u[Q];
+  sfor<0, Q>([&](auto c) { u[decltype(c)::value] = s[decltype(c)::value]; });
+  dft<true, Q>(u);
+  if (z >= ngz) return;
+  const size_t o = ((size_t)J.plane * ngz + z) * L + k1;
+  sfor<0, Q>([&](auto c) {
+    constexpr int k2 = decltype(c)::value;
+    J.pa[o + P * k2] = u[k2].x;
+    J.pb[o + P * k2] = u[k2].y;
+  });
+}
+
+// x-forward of physical rows z, z+1 as one complex row (a + i b): grid
+// (ceil(Ngz/2/R), njobs); lane (k1, r) loads and transforms over k2, then
+// tasks (m2, r), then the split into the Mx modes of rows z and z+1.
+template <int L, int P, int Q, int R>
+__global__ void __launch_bounds__(xp_width(P, Q, R)) k_xfwd4(const XJob* __restrict__ jobs,
+                                                            const double2* __restrict__ twg, int ngz) {
+  using C = double2;
+  constexpr int Mx = Dealias<L>::Mx, W = xp_width(P, Q, R);
+  constexpr int SP = cmax(P * (Q + 1), Q * (P + 1));
+  __shared__ C S[R * SP];
+  const XJob J = jobs[blockIdx.y];
+  const int lane = threadIdx.x, p0 = blockIdx.x * R, npair = ngz / 2;
+  if (lane < P * R) {
+    const int k1 = lane % P, r = lane / P, zp = p0 + r;
+    C u[Q];
+    if (zp < npair) {
+      const double* a = J.pa + ((size_t)J.plane * ngz + 2 * zp) * L + k1;
+      sfor<0, Q>([&](auto c) {
+        constexpr int k2 = decltype(c)::value;
+        u[k2] = make_double2(a[P * k2], a[L + P * k2]);
+      });
+    } else {
+      sfor<0, Q>([&](auto c) { u[decltype(c)::value] = make_double2(0.0, 0.0); });
+    }
+    dft<false, Q>(u);
+    C* sf = S + r * SP;
+    sf[k1] = u[0];
+    sfor<1, Q>([&](auto c) {
+      constexpr int m2 = decltype(c)::value;
+      sf[m2 * (P + 1) + k1] = mul_tw<false>(u[m2], twg[k1 * m2]);
+    });
+  }
+  __syncthreads();
+  for (int j = lane; j < Q * R; j += W) {
+    const int m2 = j % Q, r4 = j / Q;
+    C* s4 = S + r4 * SP + m2 * (P + 1);
+    C v[P];
+    sfor<0, P>([&](auto c) { v[decltype(c)::value] = s4[decltype(c)::value]; });
+    dft<false, P>(v);
+    sfor<0, P>([&](auto c) {
+      constexpr int m1 = decltype(c)::value;
+      if constexpr (!(Q * m1 >= Mx && Q * m1 + Q - 1 <= L - Mx)) s4[m1] = v[m1];
+    });
+  }
+  __syncthreads();
+  const double hs = 0.5 / L;
+  for (int e = lane; e < Mx * R; e += W) {
+    const int m = e % Mx, r5 = e / Mx, zp = p0 + r5;
+    if (zp >= npair) continue;
+    const int m2 = m % Q, m1 = m / Q;
+    const int fa = m2 * (P + 1) + m1;
+    const int gb = m == 0 ? 0 : (m2 == 0 ? P - m1 : (Q - m2) * (P + 1) + P - 1 - m1);
+    const C* F = S + r5 * SP;
+    const C f = F[fa], g = F[gb];
+    C* o = J.zo + ((size_t)J.plane * ngz + 2 * zp) * Mx + m;
+    o[0] = make_double2(hs * (f.x + g.x), hs * (f.y - g.y));
+    o[Mx] = make_double2(hs * (f.y + g.y), -hs * (f.x - g.x));
+  }
+}
+
+// plain z-forward of a Z plane (rows of Mx) to the N2 dealiased modes,
+// scaled by 1/L: grid (ceil(Mx/TI), njobs), the z-forward's four-step
+template <int L, int P, int Q, int TI>
+__global__ void __launch_bounds__(TI * cmax(P, Q)) k_zfwdp4(const ZFwdJob* __restrict__ jobs,
+                                                           const double2* __restrict__ twg, int Mx) {
+  using C = double2;
+  constexpr int N2 = Dealias<L>::N, H = Dealias<L>::H;
+  __shared__ C X[P * Q * TI];
+  const ZFwdJob J = jobs[blockIdx.y];
+  const int tid = threadIdx.x, i0 = blockIdx.x * TI;
+  const C* src = J.src + (size_t)J.ps * L * Mx;
+  if (tid < Q * TI) {
+    const int t = tid % TI, q = tid / TI, i = i0 + t;
+    C v[P];
+    sfor<0, P>([&](auto pc) {
+      constexpr int p = decltype(pc)::value;
+      v[p] = i < Mx ? src[(size_t)(Q * p + q) * Mx + i] : make_double2(0.0, 0.0);
+    });
+    dft<false, P>(v);
+    X[q * TI + t] = v[0];
+    sfor<1, P>([&](auto kc) {
+      constexpr int kk = decltype(kc)::value;
+      X[(kk * Q + q) * TI + t] = mul_tw<false>(v[kk], twg[q * kk]);
+    });
+  }
+  __syncthreads();
+  const int t2 = tid % TI, k1 = tid / TI, i2 = i0 + t2;
+  if (tid < P * TI && i2 < Mx) {
+    C u[Q];
+    sfor<0, Q>([&](auto qc) { u[decltype(qc)::value] = X[(k1 * Q + decltype(qc)::value) * TI + t2]; });
+    dft<false, Q>(u);
+    const double sc = 1.0 / L;
+    const size_t ncol = (size_t)N2 * Mx;
+    sfor<0, Q>([&](auto c) {
+      constexpr int k2 = decltype(c)::value;
+      if constexpr (!(P * k2 >= H && P * k2 + P <= L - H)) {
+        const int sl = k1 + P * k2;
+        if (sl < H || sl >= L - H) {
+          const int k = sl < H ? sl : sl - L + N2;
+          J.dst[(size_t)J.pd * ncol + (size_t)k * Mx + i2] = make_double2(u[k2].x * sc, u[k2].y * sc);
+        }
+      }
+    });
+  }
+}
+
 // raise the dynamic-LDS limit of a kernel once per size
 void set_lds(const void* kern, size_t bytes) {
   static std::vector<std::pair<const void*, size_t>> done;
@@ -1288,8 +1451,30 @@ void launch_zfwd_nl(Solver& s) {
   }
 }
 
+namespace {
+// the four-step IB kernels serve the lengths and dealiasing of the
+// four-step x-pass and z passes (FP64 tables); the Stockham engine keeps
+// its own kernels
+bool ib_four_step_x(const Solver& s) {
+  return !s.fft_stockham && s.px.L == s.ngx && s.N1 == 2 * (s.ngx / 3) && s.Mx == s.N1 / 2 + 1;
+}
+bool ib_four_step_z(const Solver& s) { return !s.fft_stockham && s.N2 == 2 * (s.ngz / 3) && s.pz.L == s.ngz; }
+}  // namespace
+
 void launch_xpass_inv2(Solver& s, const XJob* jobs, int njobs) {
   if (njobs <= 0) return;
+  if (ib_four_step_x(s)) {
+    switch (s.px.L) {
+#define C(L, P, Q, R)                                                                                          \
+  case L:                                                                                                      \
+    hipLaunchKernelGGL((k_xinv4<L, P, Q, R>), dim3((s.ngz + R - 1) / R, njobs), dim3(xp_width(P, Q, R)), 0,    \
+                       s.stream, jobs, s.d_twx, s.ngz, (const double2*)s.d_zrow);                              \
+    return;
+      CFD_XPASS4_F64(C)
+#undef C
+      default: break;
+    }
+  }
   switch (s.px.L) {
 #define C(n) case n: xinv2_t<n>(s, jobs, njobs); break;
     CFD_FFT_SIZES(C)
@@ -1300,6 +1485,18 @@ void launch_xpass_inv2(Solver& s, const XJob* jobs, int njobs) {
 
 void launch_xpass_fwd2(Solver& s, const XJob* jobs, int njobs) {
   if (njobs <= 0) return;
+  if (ib_four_step_x(s) && s.ngz % 2 == 0) {
+    switch (s.px.L) {
+#define C(L, P, Q, R)                                                                                              \
+  case L:                                                                                                          \
+    hipLaunchKernelGGL((k_xfwd4<L, P, Q, R>), dim3((s.ngz / 2 + R - 1) / R, njobs), dim3(xp_width(P, Q, R)), 0,    \
+                       s.stream, jobs, s.d_twx, s.ngz);                                                            \
+    return;
+      CFD_XPASS4_F64(C)
+#undef C
+      default: break;
+    }
+  }
   switch (s.px.L) {
 #define C(n) case n: xfwd2_t<n>(s, jobs, njobs); break;
     CFD_FFT_SIZES(C)
@@ -1310,6 +1507,18 @@ void launch_xpass_fwd2(Solver& s, const XJob* jobs, int njobs) {
 
 void launch_zfwd_plain(Solver& s, const ZFwdJob* jobs, int njobs) {
   if (njobs <= 0) return;
+  if (ib_four_step_z(s)) {
+    switch (s.pz.L) {
+#define C(L, P, Q, TI)                                                                                     \
+  case L:                                                                                                  \
+    hipLaunchKernelGGL((k_zfwdp4<L, P, Q, TI>), dim3((s.Mx + TI - 1) / TI, njobs), dim3(TI * cmax(P, Q)), \
+                       0, s.stream, jobs, s.d_twz, s.Mx);                                                  \
+    return;
+      CFD_ZFWD4_F64(C)
+#undef C
+      default: break;
+    }
+  }
   switch (s.pz.L) {
 #define C(n) case n: zfwdp_t<n>(s, jobs, njobs); break;
     CFD_FFT_SIZES(C)

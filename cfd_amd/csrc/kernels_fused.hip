@@ -123,7 +123,9 @@ __device__ __forceinline__ void row_load(const FusedArgs& a, const FieldPtrs& F,
                                          double2& up, double2& nold, double2& nx, double2& fa, double2& fb,
                                          double2& pa, double2& pb) {
   up = F.u[(size_t)(q + 1) * N + col];
-  nold = F.nu[(size_t)q * N + col];
+  // the old Nu enters with weight C2 = -cRK, zero at kRK = 1: its read is
+  // skipped there (C2 * 0 is a zero, as C2 * Nu is)
+  nold = a.C2 != 0.0 ? F.nu[(size_t)q * N + col] : make_double2(0.0, 0.0);
   const TN* nxz = (const TN*)F.nxz;
   const TN* flux = (const TN*)F.flux;
   nx = as_d2(nxz[(size_t)q * N + col]);
