@@ -1228,9 +1228,12 @@ void zfwdp_t(Solver& s, const ZFwdJob* jobs, int njobs) {
 #define CFD_XPASS4_F64(X) X(48, 8, 6, 8) X(64, 8, 8, 8) X(72, 12, 6, 4) X(96, 8, 12, 8) X(128, 8, 16, 8) \
   X(144, 12, 12, 4) X(192, 16, 12, 4) X(256, 16, 16, 4) X(384, 16, 24, 4) X(768, 32, 24, 2)
 #endif
+// FP32 at 192 points: (8, 24, 8) fills every lane in both phases (three 8-point
+// slots, one 24-point task per lane) in 41 KB of LDS per block; in FP64 that
+// block would need 83 KB (one block per CU), so FP64 keeps (16, 12, 4)
 #ifndef CFD_XPASS4_F32
 #define CFD_XPASS4_F32(X) X(48, 8, 6, 8) X(64, 8, 8, 8) X(72, 12, 6, 4) X(96, 8, 12, 8) X(128, 8, 16, 8) \
-  X(144, 12, 12, 4) X(192, 16, 12, 4) X(256, 16, 16, 4) X(384, 16, 24, 4) X(768, 32, 24, 2)
+  X(144, 12, 12, 4) X(192, 8, 24, 8) X(256, 16, 16, 4) X(384, 16, 24, 4) X(768, 32, 24, 2)
 #endif
 
 bool four_step_z(const Solver& s) {

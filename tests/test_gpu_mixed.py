@@ -16,12 +16,12 @@ from cfd_amd import Channel, ic
 from cfd_amd._lib import CfdError
 from oracle.oracle import Oracle
 
-from .helpers import SHIPPED, SMALLPOSTS, SMOOTH, WIDE, WIDE768, rel_l2
+from .helpers import COARSE, SHIPPED, SMALLPOSTS, SMOOTH, WIDE, WIDE768, rel_l2
 from .test_gpu_parity import _pair
 
 pytestmark = pytest.mark.gpu
 
-CASES = {"shipped": SHIPPED, "smallposts": SMALLPOSTS, "wide": WIDE, "wide768": WIDE768}
+CASES = {"shipped": SHIPPED, "smallposts": SMALLPOSTS, "wide": WIDE, "wide768": WIDE768, "coarse": COARSE}
 
 
 def _mixed(p):
