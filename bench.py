@@ -179,74 +179,79 @@ def learned_leg(args, fine_value, fine_dt):
     """The learned path: the 192 x 192 x 96 coarse grid (modal filter alpha
     36, order 32, as in the closure experiment) with the trained closure
     network (cfd_amd/data/closure_rt395.npz) evaluated every ``every`` steps,
-    timed like the FP64 leg from a seeded start.  Simulated time per second
-    against the fine FP64 run's = (substeps/s x realised mean dt of the evaluated
-    coarse run) / (fine substeps/s x realised mean dt of the evaluated fine run),
-    from the closure report; the report's own wall-clock ratio of the evaluated
-    runs (speedup_vs_fine) is quoted beside it.  The dt of the timed substeps
-    (a near-laminar start, where the coarse dt is twice the fine dt) is
-    recorded but not used for the ratio."""
+    timed like the FP64 leg from a seeded start, on the FP64 path and on the
+    mixed path (keys ending in ``_mixed``).  Simulated time per second against
+    the fine FP64 run's = (substeps/s x realised mean dt of the evaluated
+    coarse run) / (fine substeps/s x realised mean dt of the evaluated fine
+    run), from the closure report of that precision (the mixed one evaluates
+    its coarse runs on the mixed path against the same FP64 fine run); the
+    report's own wall-clock ratio of the evaluated runs (speedup_vs_fine) is
+    quoted beside it.  The dt of the timed substeps (a near-laminar start,
+    where the coarse dt is twice the fine dt) is recorded but not used for
+    the ratio."""
     from cfd_amd import Channel, closure, cnn, workloads
 
     pc, _ = workloads.get("rt395_coarse_192x192x96")
     w = np.load(os.path.join(ROOT, "cfd_amd", "data", "closure_rt395.npz"))
-    rep = os.path.join(ROOT, "profiles", "closure_rt395_384x192x192_to_192x192x96.json")
-    report = None
-    if os.path.exists(rep):
-        with open(rep) as fh:
-            report = json.load(fh)
-    rdt = realised_dt(report) if report else None
     out = {"workload": "rt395_coarse_192x192x96", "grid": [pc.ngalx, pc.ny, pc.ngalz],
-           "weights": "cfd_amd/data/closure_rt395.npz", "dtype": "bf16 network (fp32 accumulation), f64 solver",
-           "realised_mean_dt": rdt}
+           "weights": "cfd_amd/data/closure_rt395.npz",
+           "dtype": "bf16 network (fp32 accumulation); f64 solver, or the mixed path (keys ending in _mixed)"}
     net = None
-    for every in (0, 1, 2, 4):
-        ch = Channel(pc)
-        closure.start_fields(ch, pc, seed=0)
-        ch.set_mode_filter(36.0, 32, 0.0)  # the coarse runs' modal filter
-        if every:
-            if net is None:
-                net = cnn.CNN(ch.nyu, pc.ngalz, pc.ngalx)
-                net.set_params(w["params"])
-                sc = np.array(w["scales"], np.float32).copy()
-                if "gain" in w.files:  # the damped correction of the closure experiment
-                    sc[3:] *= float(w["gain"])
-                net.set_scales(sc)
-            ch.closure_attach(net, every=every)
-        for s in range(args.warmup):
-            ch.substep(s % 3 + 1)
-        ch.sync()
-        t0 = time.perf_counter()
-        for s in range(args.steps):
-            ch.substep((args.warmup + s) % 3 + 1)
-        ch.sync()
-        t1 = time.perf_counter()
-        v = args.steps / (t1 - t0)
-        dt = ch.scalars()["dt"]
-        key = "closure_every_%d" % every if every else "no_closure"
-        lab = "coarse_closure_every%d" % every if every else "coarse"
-        out[key] = {"substeps_per_s": v, "substeps_per_s_vs_fine_fp64": v / fine_value,
-                    "dt_of_timed_substeps": dt, "fine_dt_of_timed_substeps": fine_dt}
-        if rdt and lab in rdt:
-            out[key]["sim_time_per_s_vs_fine_fp64"] = (v * rdt[lab]) / (fine_value * rdt["fine"])
-            out[key]["report_speedup_vs_fine"] = report[lab].get("speedup_vs_fine")
-            out[key]["report_drag_error"] = report[lab]["drag"]["drag_error"]
-        if every:
-            ch.closure_attach(None)
-        ch.close()
+    for prec, sfx in ((0, ""), (1, "_mixed")):
+        rname = "profiles/closure_rt395_384x192x192_to_192x192x96%s.json" % sfx
+        report = None
+        if os.path.exists(os.path.join(ROOT, rname)):
+            with open(os.path.join(ROOT, rname)) as fh:
+                report = json.load(fh)
+        rdt = realised_dt(report) if report else None
+        out["realised_mean_dt" + sfx] = rdt
+        for every in (0, 1, 2, 4):
+            ch = Channel(dataclasses.replace(pc, precision=prec))
+            closure.start_fields(ch, pc, seed=0)
+            ch.set_mode_filter(36.0, 32, 0.0)  # the coarse runs' modal filter
+            if every:
+                if net is None:
+                    net = cnn.CNN(ch.nyu, pc.ngalz, pc.ngalx)
+                    net.set_params(w["params"])
+                    sc = np.array(w["scales"], np.float32).copy()
+                    if "gain" in w.files:  # the damped correction of the closure experiment
+                        sc[3:] *= float(w["gain"])
+                    net.set_scales(sc)
+                ch.closure_attach(net, every=every)
+            for s in range(args.warmup):
+                ch.substep(s % 3 + 1)
+            ch.sync()
+            t0 = time.perf_counter()
+            for s in range(args.steps):
+                ch.substep((args.warmup + s) % 3 + 1)
+            ch.sync()
+            t1 = time.perf_counter()
+            v = args.steps / (t1 - t0)
+            dt = ch.scalars()["dt"]
+            key = ("closure_every_%d" % every if every else "no_closure") + sfx
+            lab = "coarse_closure_every%d" % every if every else "coarse"
+            out[key] = {"substeps_per_s": v, "substeps_per_s_vs_fine_fp64": v / fine_value,
+                        "dt_of_timed_substeps": dt, "fine_dt_of_timed_substeps": fine_dt}
+            if rdt and lab in rdt:
+                out[key]["sim_time_per_s_vs_fine_fp64"] = (v * rdt[lab]) / (fine_value * rdt["fine"])
+                out[key]["report_speedup_vs_fine"] = report[lab].get("speedup_vs_fine")
+                out[key]["report_drag_error"] = report[lab]["drag"]["drag_error"]
+            if every:
+                ch.closure_attach(None)
+            ch.close()
+        if report:
+            r = report
+            out["accuracy_from_report" + sfx] = {
+                "report": rname,
+                "held_out_gain": r.get("held_out_gain"),
+                **{lab: {k: r[lab].get(k) for k in ("completed", "t_reached", "drag")}
+                   for lab in r if lab == "coarse" or lab.startswith("coarse_closure_every")},
+                "coarse_closure_fallback": {k: r.get("coarse_closure_fallback", {}).get(k)
+                                            for k in ("closure_every", "switch_t", "completed", "t_reached", "drag")},
+                "closure_every": r.get("closure_every"), "closure_gain": r.get("closure_gain"),
+                "coarse_filter": r.get("coarse_filter"), "gain_scan": r.get("gain_scan")}
     if net is not None:
         net.close()
-    if report:
-        r = report
-        out["accuracy_from_report"] = {
-            "report": "profiles/closure_rt395_384x192x192_to_192x192x96.json",
-            "held_out_gain": r.get("held_out_gain"),
-            **{lab: {k: r[lab].get(k) for k in ("completed", "t_reached", "drag")}
-               for lab in r if lab == "coarse" or lab.startswith("coarse_closure_every")},
-            "coarse_closure_fallback": {k: r.get("coarse_closure_fallback", {}).get(k)
-                                        for k in ("closure_every", "switch_t", "completed", "t_reached", "drag")},
-            "closure_every": r.get("closure_every"), "closure_gain": r.get("closure_gain"),
-            "coarse_filter": r.get("coarse_filter"), "gain_scan": r.get("gain_scan")}
     return out
 
 

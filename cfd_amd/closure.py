@@ -21,6 +21,7 @@ pressure gradient at constant flow rate) and mean-profile errors against the
 fine run, and simulated time per wall-clock second of each run.
 """
 import argparse
+import dataclasses
 import ctypes
 import json
 import os
@@ -199,6 +200,8 @@ def main(argv=None):
                     help="evaluate this trained network (closure_weights.npz) instead of training one")
     ap.add_argument("--out", default=".")
     ap.add_argument("--seed", type=int, default=0, help="network initialisation and sample draws")
+    ap.add_argument("--coarse-precision", type=int, default=0,
+                    help="precision of the evaluated coarse runs (0 FP64, 1 mixed); samples and the fine run are FP64")
     ap.add_argument("--fine-seed", type=int, default=0,
                     help="seed of the fine run's initial field (another trajectory for choosing the gain)")
     a = ap.parse_args(argv)
@@ -209,11 +212,12 @@ def main(argv=None):
     pf, _ = workloads.get("rt395_384x192x192")
     pc, _ = workloads.get("rt395_coarse_192x192x96")
     fine, scratch = Channel(pf), Channel(pc)
+    pr = dataclasses.replace(pc, precision=a.coarse_precision)  # the evaluated coarse runs
     net = cnn.CNN(scratch.nyu, pc.ngalz, pc.ngalx)
     net.set_params(cnn.init_params(seed=a.seed))
     net.dataset(a.slots + a.held_out)
     rng = np.random.default_rng(a.seed)
-    report = {"fine_seed": a.fine_seed, "seed": a.seed, "samples": a.samples, "every": a.every, "slots": a.slots, "train_per_sample": a.train_per_sample,
+    report = {"coarse_precision": a.coarse_precision, "fine_seed": a.fine_seed, "seed": a.seed, "samples": a.samples, "every": a.every, "slots": a.slots, "train_per_sample": a.train_per_sample,
               "closure_every": every_list, "closure_gain": a.closure_gain, "coarse_filter": {"alpha": filt[0], "order": filt[1], "eta_c": filt[2]}}
     train = a.weights is None
     if train:
@@ -321,7 +325,7 @@ def main(argv=None):
     cases = [("coarse", None, 0)] + [("coarse_closure_every%d" % e, "plain", e) for e in every_list]
     cases.append(("coarse_closure_clip_every%d" % every_list[0], "clip", every_list[0]))
     for label, attach, every in cases:
-        ch = Channel(pc)
+        ch = Channel(pr)
         g = ch.ygrid()
         for kname in ("u1", "u2", "u3", "p"):
             ch.set_field(kname, fc[kname])
@@ -358,7 +362,7 @@ def main(argv=None):
         fcs = filtered_start(pc, st[0], pf)
         res = []
         for gval in gains:
-            ch = Channel(pc)
+            ch = Channel(pr)
             g = ch.ygrid()
             for kname in ("u1", "u2", "u3", "p"):
                 ch.set_field(kname, fcs[kname])
@@ -386,7 +390,7 @@ def main(argv=None):
     # the closure until its drag leaves the bound, then the fine solver from
     # its last checkpoint, zero-padded onto the fine modes, to the same t_end
     # (the network every max(closure_every) steps)
-    ch, fb = Channel(pc), Channel(pf)
+    ch, fb = Channel(pr), Channel(pf)
     g = ch.ygrid()
     for kname in ("u1", "u2", "u3", "p"):
         ch.set_field(kname, fc[kname])

@@ -1304,7 +1304,6 @@ int read_scalars(cfd_solver* s, DevScalars& d) {
 int closure_setup(cfd_solver* s) {
   auto& C = s->cl;
   if (C.ready) return CFD_OK;
-  if (s->mixed) return fail(s, CFD_EINVAL, "closure: the learned closure runs on the FP64 path (precision 0)");
   if (!cnn_dims_ok(s->nyu, s->ngz, s->ngx))
     return fail(s, CFD_EINVAL, "closure: Ngalx must be a multiple of 32 and Ngalz of 8");
   const size_t zpl = (size_t)s->ngz * s->Mx, ppl = (size_t)s->ngz * s->ngx;
@@ -1381,6 +1380,15 @@ int closure_setup(cfd_solver* s) {
   return CFD_OK;
 }
 
+// the correction (FP64) into the interior planes of the x/z part of the
+// nonlinear term, stored in the precision of the path
+void add_correction(cfd_solver* s, void* nxz, const double2* c, int ny) {
+  if (s->mixed)
+    launch_add_planes(s->stream, (float2*)nxz, c, s->Ncol, 1, ny - 1);
+  else
+    launch_add_planes(s->stream, (double2*)nxz, c, s->Ncol, 1, ny - 1);
+}
+
 // physical planes of job set f (0..2 velocity, 3..5 corrections)
 void closure_to_phys(cfd_solver* s, int f) {
   auto& C = s->cl;
@@ -1402,7 +1410,7 @@ int closure_apply(cfd_solver* s, int kRK) {
   void* nxz[3] = {s->nxz1, s->nxz2, s->nxz3};
   const int nyf[3] = {s->nyu, s->nyv, s->nyu};
   if (!fresh) {
-    for (int g = 0; g < 3; ++g) launch_add_planes(s->stream, (double2*)nxz[g], C.Cs[g], s->Ncol, 1, nyf[g] - 1);
+    for (int g = 0; g < 3; ++g) add_correction(s, nxz[g], C.Cs[g], nyf[g]);
     return CFD_OK;
   }
   for (int f = 0; f < 3; ++f) closure_to_phys(s, f);
@@ -1421,7 +1429,7 @@ int closure_apply(cfd_solver* s, int kRK) {
     // no net streamwise or spanwise force from the network (u and v planes
     // share their first index, so v face q - 1 lies below u plane q)
     if (g != 1) launch_zero_net_force(s->stream, C.Cs[g], s->Ncol, 1, nyf[g] - 1, s->d_yv);
-    launch_add_planes(s->stream, (double2*)nxz[g], C.Cs[g], s->Ncol, 1, nyf[g] - 1);
+    add_correction(s, nxz[g], C.Cs[g], nyf[g]);
   }
   return CFD_OK;
 }

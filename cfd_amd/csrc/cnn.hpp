@@ -75,6 +75,7 @@ void launch_pack(hipStream_t st, const double* p1, const double* p2, const doubl
 void launch_unpack(hipStream_t st, const float* y, int nyu, int nyv, int ngz, int ngx, const float* s, double* p1,
                    double* p2, double* p3, const double* u1, const double* u2, const double* u3);
 void launch_add_planes(hipStream_t st, double2* a, const double2* c, int ncol, int q0, int q1);
+void launch_add_planes(hipStream_t st, float2* a, const double2* c, int ncol, int q0, int q1);
 void launch_zero_net_force(hipStream_t st, double2* c, int ncol, int q0, int q1, const double* yv);
 void launch_sumsq(hipStream_t st, const double* v, size_t n, double* acc);
 }  // namespace cfd

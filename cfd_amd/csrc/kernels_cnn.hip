@@ -552,13 +552,18 @@ __global__ void k_unpack(const float* __restrict__ y, int nyu, int nyv, int ngz,
   }
 }
 
-// a[q] += c[q] on planes q0 .. q1 - 1 of Ncol complex values
-__global__ void k_add_planes(double2* __restrict__ a, const double2* __restrict__ c, int ncol, int q0, int q1) {
+// a[q] += c[q] on planes q0 .. q1 - 1 of Ncol complex values; a in FP64 or,
+// on the mixed path, FP32 (the sum is formed in FP64 and rounded once)
+template <typename C2>
+__global__ void k_add_planes(C2* __restrict__ a, const double2* __restrict__ c, int ncol, int q0, int q1) {
+  using R = decltype(a->x);
   const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t n = (size_t)(q1 - q0) * ncol;
   if (e >= n) return;
   const size_t o = (size_t)q0 * ncol + e;
-  a[o] = make_double2(a[o].x + c[o].x, a[o].y + c[o].y);
+  const C2 x = a[o];
+  a[o].x = (R)((double)x.x + c[o].x);
+  a[o].y = (R)((double)x.y + c[o].y);
 }
 
 // The (0,0) column of a correction on u planes q0 .. q1 - 1 loses its
@@ -618,7 +623,12 @@ void launch_unpack(hipStream_t st, const float* y, int nyu, int nyv, int ngz, in
 void launch_add_planes(hipStream_t st, double2* a, const double2* c, int ncol, int q0, int q1) {
   if (q1 <= q0) return;
   const size_t n = (size_t)(q1 - q0) * ncol;
-  hipLaunchKernelGGL(k_add_planes, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, c, ncol, q0, q1);
+  hipLaunchKernelGGL(k_add_planes<double2>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, c, ncol, q0, q1);
+}
+void launch_add_planes(hipStream_t st, float2* a, const double2* c, int ncol, int q0, int q1) {
+  if (q1 <= q0) return;
+  const size_t n = (size_t)(q1 - q0) * ncol;
+  hipLaunchKernelGGL(k_add_planes<float2>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, c, ncol, q0, q1);
 }
 
 void launch_zero_net_force(hipStream_t st, double2* c, int ncol, int q0, int q1, const double* yv) {

@@ -357,8 +357,10 @@ int cfd_cnn_get_scales(cfd_cnn c, float* s6);
  * `every`-th step (and in the first one after attaching) and the other
  * substeps reuse that correction; c = NULL detaches.  The (0,0) column of
  * the u1 and u3 corrections loses its mean over y weighted by the plane
- * widths, so the correction exerts no net force on the flow.  The FP64 path
- * only (precision 0); the network volume must be nyu x Ngalz x Ngalx of h.
+ * widths, so the correction exerts no net force on the flow.  Both paths:
+ * the correction is formed in FP64 and, on the mixed path (precision 1),
+ * added to its FP32 nonlinear-term storage; the network volume must be
+ * nyu x Ngalz x Ngalx of h.
  *
  * cfd_closure_phys(h, which, out): the physical planes of the last sample
  * or application: which 0..2 the velocities u1, u2, u3, 3..5 the

@@ -210,6 +210,42 @@ def test_attached_network_corrects_the_nonlinear_term():
         b.close()
 
 
+def test_attached_network_on_the_mixed_path():
+    """The mixed path (FP32 nonlinear-term storage) takes the same correction:
+    the network reads the FP64 fields through the FP64 transforms, so the
+    correction is the FP64 path's bit for bit, and the nonlinear term moves
+    by it to FP32 rounding of the stored x/z terms."""
+    # the 128 x 64 grid: the mixed path needs four-step lengths in x and z
+    print("case closure inference mixed", FINE.ngalx, flush=True)
+    pm = dataclasses.replace(FINE, precision=1)
+    f = _fields(FINE, 13)
+    chans = [Channel(FINE), Channel(FINE), Channel(pm), Channel(pm)]
+    net = cnn.CNN(chans[0].nyu, FINE.ngalz, FINE.ngalx)
+    try:
+        net.set_params(cnn.init_params(seed=3, last_scale=1.0))
+        net.set_scales(np.array([1.0, 0.3, 0.3, 0.05, 0.05, 0.05], np.float32))
+        for ch in chans:
+            _load(ch, f)
+        a64, b64, a32, b32 = chans
+        for a, b in ((a64, b64), (a32, b32)):
+            a.nonlinear(2)
+            b.closure_attach(net)
+            b.nonlinear(2)
+            b.closure_attach(None)
+        for k in range(3):
+            assert np.array_equal(b32.closure_phys(3 + k), b64.closure_phys(3 + k))
+        for n in NL:
+            d64 = b64.get_field(n) - a64.get_field(n)
+            d32 = b32.get_field(n) - a32.get_field(n)
+            scale = np.abs(a64.get_field(n)[:, 1:-1]).max()
+            assert np.abs(d32[:, 1:-1] - d64[:, 1:-1]).max() <= 1e-6 * scale, n
+            assert _rel(d32[:, 1:-1], d64[:, 1:-1]) <= 1e-3, (n, _rel(d32[:, 1:-1], d64[:, 1:-1]))
+    finally:
+        net.close()
+        for ch in chans:
+            ch.close()
+
+
 def test_training_beats_no_correction_on_a_new_sample():
     print("case closure training", FINE.ngalx, COARSE.ngalx, flush=True)
     fine, co = Channel(FINE), Channel(COARSE)
